@@ -1,0 +1,208 @@
+/*
+ * rs3d.h -- C ABI of the MI355X-native ray-cast spatial join (librs3d.so).
+ *
+ * This is the drop-in boundary under RaySpace3D's host classes. Every entry point takes
+ * plain pointers and sizes. Unless a comment says "host", pointer arguments are DEVICE
+ * pointers (HBM) and the call is asynchronous on the context's stream; calls that
+ * return counts to the host synchronise that stream. Every call returns 0 on success or
+ * a negative RS3D_E* code; rs3d_last_error() describes the last failure of the calling
+ * thread. Reference interfaces each entry replaces are cited per declaration
+ * (paths relative to the reference repository root).
+ */
+#ifndef RS3D_H
+#define RS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RS3D_OK = 0,
+    RS3D_E_ARG = -1,      /* invalid argument */
+    RS3D_E_HIP = -2,      /* HIP runtime error */
+    RS3D_E_NOMEM = -3,    /* device allocation failed */
+    RS3D_E_CAPACITY = -4, /* an output or table capacity was exceeded (retry larger) */
+    RS3D_E_DEPTH = -5,    /* BVH deeper than the traversal stack */
+};
+
+/* Wire types, byte-identical to common/include/common.h:14-37. */
+typedef struct {
+    int32_t ray_id;
+    int32_t polygon_index;
+} rs3d_ray_result; /* RayResult */
+
+typedef struct {
+    int32_t object_id_mesh1;
+    int32_t object_id_mesh2;
+} rs3d_pair; /* MeshQueryResult */
+
+typedef struct rs3d_ctx rs3d_ctx;
+typedef struct rs3d_bvh rs3d_bvh;
+
+const char* rs3d_last_error(void);
+
+/* ---- context -------------------------------------------------------------------------
+ * Replaces OptixContext (query/src/optix/OptixContext.cpp:4-8): one device, one stream. */
+int rs3d_ctx_create(int device, rs3d_ctx** out);
+int rs3d_ctx_destroy(rs3d_ctx* ctx);
+int rs3d_ctx_sync(rs3d_ctx* ctx);
+/* The hipStream_t the context launches on (as void*), for event timing by callers. */
+void* rs3d_ctx_stream(rs3d_ctx* ctx);
+/* Kernel timing: when enabled, every point-traversal launch on this context is bracketed
+ * by HIP events on the context's stream; profile_read returns the summed device time (ms)
+ * and the launch count since the last read (and resets both). Replaces the per-launch
+ * addMeasurement timings of query/src/timer.h:9-49 for the traversal kernel. */
+int rs3d_ctx_profile(rs3d_ctx* ctx, int enable);
+int rs3d_ctx_profile_read(rs3d_ctx* ctx, double* total_ms, uint64_t* launches);
+
+/* ---- BVH (replaces OptixAccelerationStructure::buildInternal,
+ *      query/src/optix/OptixAccelerationStructure.cpp:22-68) ------------------------------
+ * LBVH over an indexed fp32 triangle mesh: 30-bit Morton codes, LDS-staged radix sort,
+ * Karras hierarchy, bottom-up refit. vertices: float3[nV]; indices: uint3[nT];
+ * tri_to_obj: int32[nT] (may be NULL: object id = triangle index). */
+typedef struct {
+    int32_t leaf_size; /* 1..8, 0 = default (4) */
+    int32_t reserved[7];
+} rs3d_bvh_opts;
+
+typedef struct {
+    uint64_t num_triangles;
+    uint64_t num_nodes;
+    int32_t max_depth;  /* deepest primitive in the binary hierarchy */
+    int32_t leaf_size;
+    float bounds_min[3];
+    float bounds_max[3];
+    uint64_t device_bytes;
+} rs3d_bvh_info;
+
+int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
+                   const uint32_t* indices, uint64_t num_triangles, const int32_t* tri_to_obj,
+                   const rs3d_bvh_opts* opts, rs3d_bvh** out);
+int rs3d_bvh_free(rs3d_bvh* bvh);
+int rs3d_bvh_info_get(const rs3d_bvh* bvh, rs3d_bvh_info* info); /* info: host */
+
+/* ---- point containment (replaces __raygen__rg, query/src/cuda/raytracing.cu:9-78, and
+ *      RayLauncher::launch, query/src/raytracing/RayLauncher.cpp:44-58) ------------------
+ * One ray per point along +axis (0=+x, 1=+y, 2=+z; the reference uses +z). Odd chain
+ * length => inside; the object is that of the first chain hit. Output kinds: */
+enum {
+    RS3D_OUT_RAY_RESULT = 0, /* rs3d_ray_result[n]: {i, obj or -1} */
+    RS3D_OUT_FLAG_U8 = 1,    /* uint8[n]: 1 inside, 0 outside */
+    RS3D_OUT_OBJ_I32 = 2,    /* int32[n]: obj or -1 (as uint32: 0xFFFFFFFF = none) */
+};
+/* points: float3[n]. index_map: optional uint32[n]; when given, ray i uses point
+ * points[index_map[i]] and writes its result at out[index_map[i]] (ray_id = index_map[i]). */
+int rs3d_trace_points(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                      const uint32_t* index_map, int axis, int out_kind, void* out);
+
+/* Traversal counters of the last call with counters enabled (canonical work figures). */
+typedef struct {
+    uint64_t rays;
+    uint64_t nodes_visited; /* 64-byte node fetches */
+    uint64_t tris_tested;   /* triangle record fetches */
+    uint64_t hits;          /* candidate hits collected */
+    uint64_t restarts;      /* traversals restarted after hit-buffer overflow */
+} rs3d_trace_stats;
+int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                            int axis, rs3d_trace_stats* stats /* host */);
+
+/* ---- filter-refine (replaces FilterRefinePipeline::execute,
+ *      query/src/raytracing/FilterRefine.cpp:81-122) --------------------------------------
+ * Filter: +axis ray parity against the 12-triangle box mesh of the dataset AABB
+ * (query/src/geometry/BoundingBox.cpp:4-50). Refine: point containment of the stably
+ * compacted candidates. out_obj: int32[n] (-1 outside). candidate_count: host. */
+int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                       int axis, int32_t* out_obj, uint64_t* candidate_count,
+                       uint64_t* inside_count);
+/* Box-mesh filter flags only: flags[i] = 1 when point i passes. box: host float[6]. */
+int rs3d_filter_box(rs3d_ctx* ctx, const float* box_min_max, const float* points, uint64_t n,
+                    int axis, uint8_t* flags);
+
+/* ---- scan / compaction (replace query/src/cuda/scan_utils.cu:11-27 and
+ *      query/src/cuda/result_compaction.cu:5-61) -------------------------------------- */
+/* out[i] = sum(in[0..i)); *total (host) = sum(in). */
+int rs3d_scan_i32_i64(rs3d_ctx* ctx, const int32_t* in, int64_t* out, uint64_t n, int64_t* total);
+/* Stable compaction of rs3d_ray_result entries with polygon_index != -1. count: host. */
+int rs3d_compact_hits(rs3d_ctx* ctx, const rs3d_ray_result* in, uint64_t n, rs3d_ray_result* out,
+                      uint64_t* count);
+/* Stable compaction of indices i with flags[i] != 0 into idx_out. count: host. */
+int rs3d_compact_flags(rs3d_ctx* ctx, const uint8_t* flags, uint64_t n, uint32_t* idx_out,
+                       uint64_t* count);
+
+/* ---- sort / unique (replace deduplicate_inplace,
+ *      query/src/cuda/mesh_query_deduplication.cu:56-90) ---------------------------------
+ * LSD radix sort of u64 keys in place, then unique. unique_count: host. */
+int rs3d_sort_unique_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n, uint64_t* unique_count);
+/* Stable LSD radix sort of u32 keys with u32 values, bits [0, end_bit). In place. */
+int rs3d_sort_pairs_u32(rs3d_ctx* ctx, uint32_t* keys, uint32_t* values, uint64_t n, int end_bit);
+
+/* ---- edge rays (replace __raygen__mesh_overlap_edges,
+ *      query/src/cuda/mesh_overlap_edges.cu:106-216) ------------------------------------ */
+typedef struct {
+    const float* starts;         /* float3[num_edges] */
+    const float* ends;           /* float3[num_edges] */
+    const int32_t* src_offsets;  /* int32[num_edges] */
+    const int32_t* src_counts;   /* int32[num_edges] */
+    const int32_t* src_objects;  /* int32[sum(src_counts)] */
+    uint64_t num_edges;
+} rs3d_edges;
+
+/* Pass 1: counts[e] = chain hits * source objects of edge e. */
+int rs3d_edge_count(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int32_t* counts);
+/* Pass 2: writes raw pairs at offsets[e] (exclusive scan of counts). swap: emit (tgt, src). */
+int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
+                    const int64_t* offsets, int swap, rs3d_pair* out);
+
+/* Open-addressing pair hash table of u64 keys (a << 32 | b), empty = ~0. */
+typedef struct {
+    uint64_t* slots;
+    uint64_t size;          /* slot count */
+    int32_t bitwise;        /* 1: size is a power of two, index = hash & (size-1) */
+    int32_t max_probes;     /* 0 = unlimited (bounded by size) */
+    uint64_t* failed;       /* device u64 counter of inserts that exhausted max_probes */
+} rs3d_hash_table;
+int rs3d_ht_clear(rs3d_ctx* ctx, rs3d_hash_table* ht);
+/* Hash mode: inserts every emitted pair into ht (on-the-fly dedup). */
+int rs3d_edge_hash(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int swap,
+                   rs3d_hash_table* ht);
+/* Stable scan-based compaction of the table into keys_out (slot order). count: host. */
+int rs3d_ht_compact(rs3d_ctx* ctx, const rs3d_hash_table* ht, uint64_t* keys_out, uint64_t max_out,
+                    uint64_t* count);
+
+/* Full overlap join A->B and B->A (edge rays of each mesh against the other's BVH),
+ * result = sorted unique keys (a << 32 | b) of (mesh1 object, mesh2 object).
+ * mode 0: two-pass count/scan/write + radix sort/unique; mode 1: hash + compact + sort.
+ * keys_out capacity max_out; count: host. Returns RS3D_E_CAPACITY if it does not fit
+ * (count then holds the required size). */
+int rs3d_overlap_join(rs3d_ctx* ctx, const rs3d_bvh* bvh1, const rs3d_edges* edges1,
+                      const rs3d_bvh* bvh2, const rs3d_edges* edges2, int mode,
+                      uint64_t* keys_out, uint64_t max_out, uint64_t* count);
+
+/* ---- selectivity estimation (replaces estimateOverlapSelectivity,
+ *      query/src/cuda/estimated_overlap.cu:43-77) --------------------------------------
+ * cells: GridCell[num_cells] (device). Deterministic reduction. result: host. */
+int rs3d_estimate_raw(rs3d_ctx* ctx, const void* cells_a, const void* cells_b, uint64_t num_cells,
+                      float cell_volume, float epsilon, float gamma, double* result);
+
+/* ---- reference-named entry points (same names and signatures as the reference's
+ *      C ABI; default stream semantics: synchronous) --------------------------------- */
+/* query/src/cuda/result_compaction.h:7-16 */
+int count_hits_gpu(const rs3d_ray_result* d_results, int num_results);
+void compact_hits_gpu(const rs3d_ray_result* d_input, rs3d_ray_result* d_output, int num_results,
+                      int* h_num_hits);
+/* query/src/cuda/scan_utils.h:7 */
+long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements);
+/* query/src/cuda/mesh_query_deduplication.h:6-15 */
+long long merge_and_deduplicate_pairs_gpu(const rs3d_pair* d_results1, long long num_results1,
+                                          const rs3d_pair* d_results2, long long num_results2,
+                                          rs3d_pair* d_merged_output);
+int compact_hash_table_pairs(const unsigned long long* d_hash_table, unsigned long long table_size,
+                             rs3d_pair* d_output, int max_output_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+/*
+ * oracle.h -- CPU restatement of RaySpace3D's ray-cast spatial-join semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP product path
+ * (librs3d.so) is compared against. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. Nothing in the product links or calls it.
+ *
+ * Every function follows a reference file (cited in oracle.c). The reference itself
+ * (OptiX + CUDA) cannot be built here, so the fp32 triangle test, the closest-hit
+ * chain rules and the tie rules are the canonical ones stated in oracle.c and
+ * implemented identically, operation for operation, in the HIP kernels.
+ *
+ * Pinning (tests/test_oracle.py): closed-form triangle cases, an independent
+ * double-precision Moller-Trumbore test, analytic sphere containment and sphere-sphere
+ * overlap, brute force vs binned equality, the committed check_n_geometry.pre fixture
+ * (v1 header, closedness, grid statistics), fmix64 inverse, and hand-computed
+ * estimator cells. Parity unpinned: the edge/vertex tie behaviour of OptiX itself
+ * (only an OptiX run could pin it).
+ */
+#ifndef RS3D_ORACLE_H
+#define RS3D_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+} orc_raypre;
+
+void orc_ray_pre(const float d[3], orc_raypre* r);
+/* Returns 1 and the fp32 hit distance in *t when the (two-sided) triangle is hit. */
+int orc_tri_hit(const float o[3], const orc_raypre* r, const float v0[3], const float v1[3],
+                const float v2[3], float* t);
+
+/* A.1 point containment along +axis (0=x, 1=y, 2=z). out_obj[i] = object id of the
+ * first chain hit if the chain length is odd, else -1. use_grid=0: brute force over all
+ * triangles; 1: 2-D binned candidates (same result). Returns the number inside. */
+int64_t orc_points_contain(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                           const int32_t* tri_obj, const float* P, int64_t n, int axis,
+                           int use_grid, int32_t* out_obj);
+
+/* A.2 edge rays against a target mesh. Writes up to cap raw pair keys
+ * ((u64)a << 32 | b, a = mesh1 object) into out_keys in edge order, and the per-edge
+ * raw result count (hits * sources) into per_edge_counts when non-null. Returns the
+ * total number of raw results (may exceed cap; only cap are written). */
+int64_t orc_edge_pairs(const float* starts, const float* ends, const int32_t* src_off,
+                       const int32_t* src_cnt, const int32_t* src_obj, int64_t nE,
+                       const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                       const int32_t* tri_obj, int swap, int use_grid, uint64_t* out_keys,
+                       int64_t cap, int32_t* per_edge_counts);
+
+/* Sorts ascending and removes duplicates in place; returns the unique count. */
+int64_t orc_sort_unique_u64(uint64_t* keys, int64_t n);
+
+/* MurmurHash3 fmix64 finaliser and the reference's open-addressing insert. */
+uint64_t orc_fmix64(uint64_t k);
+/* Returns 1 if inserted or already present, 0 if the probe cap was exhausted. */
+int orc_ht_insert(uint64_t* table, uint64_t size, int bitwise, int max_probes, uint64_t key);
+
+/* A.5 selectivity estimate of one grid pair. cells are GridCell records
+ * {u32 CenterCount, u32 TouchCount, f32 AvgSizeMean, f32 VolRatio}. */
+double orc_estimate_raw(const void* cellsA, const void* cellsB, int64_t ncells, float cell_volume,
+                        float epsilon, float gamma);
+/* Full estimate including the alpha correction, as the estimated mains compute it. */
+int64_t orc_estimate_pairs(const void* cellsA, const void* cellsB, int64_t ncells,
+                           const float minb[3], const float maxb[3], const uint32_t res[3],
+                           float epsilon, float gamma, int alpha_correction);
+
+/* Grid statistics of a mesh (preprocess_dataset.cpp generateGridStats). cells_out has
+ * res^3 GridCell records; bounds_out = minBound[3], maxBound[3]. */
+void orc_grid_stats(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                    const int32_t* tri_obj, int res, float world_size, void* cells_out,
+                    float* bounds_out);
+
+/* Unique-edge extraction (EdgePreprocessor::extractEdges). Pass null outputs to get
+ * the counts: returns number of edges, *n_sources = flattened source count. */
+int64_t orc_extract_edges(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                          const int32_t* tri_obj, float* starts, float* ends, int32_t* offsets,
+                          int32_t* counts, int32_t* sources, int64_t* n_sources);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

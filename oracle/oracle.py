@@ -1,0 +1,146 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so). TEST INFRASTRUCTURE ONLY:
+imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker. The product never imports it."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_sig = {
+    "orc_ray_pre": (None, [_P, _P]),
+    "orc_tri_hit": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "orc_points_contain": (_I64, [_P, _I64, _P, _I64, _P, _P, _I64, C.c_int, C.c_int, _P]),
+    "orc_edge_pairs": (_I64, [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
+                              _P, _I64, _P]),
+    "orc_sort_unique_u64": (_I64, [_P, _I64]),
+    "orc_fmix64": (C.c_uint64, [C.c_uint64]),
+    "orc_ht_insert": (C.c_int, [_P, C.c_uint64, C.c_int, C.c_int, C.c_uint64]),
+    "orc_estimate_raw": (C.c_double, [_P, _P, _I64, C.c_float, C.c_float, C.c_float]),
+    "orc_estimate_pairs": (_I64, [_P, _P, _I64, _P, _P, _P, C.c_float, C.c_float, C.c_int]),
+    "orc_grid_stats": (None, [_P, _I64, _P, _I64, _P, C.c_int, C.c_float, _P, _P]),
+    "orc_extract_edges": (_I64, [_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(LIB)
+        for k, (r, a) in _sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+GRID_CELL = np.dtype([("center", "<u4"), ("touch", "<u4"), ("avg", "<f4"), ("ratio", "<f4")])
+
+
+def tri_hit(o, d, v0, v1, v2):
+    """(hit, t) of the canonical fp32 watertight test."""
+    rp = (C.c_byte * 32)()
+    dd = _c(d, np.float32)
+    lib().orc_ray_pre(_p(dd), C.cast(rp, C.c_void_p))
+    t = np.zeros(1, np.float32)
+    h = lib().orc_tri_hit(_p(_c(o, np.float32)), C.cast(rp, C.c_void_p), _p(_c(v0, np.float32)),
+                          _p(_c(v1, np.float32)), _p(_c(v2, np.float32)), _p(t))
+    return bool(h), float(t[0])
+
+
+def points_contain(V, I, tri_obj, P, axis=2, use_grid=True):
+    V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32); P = _c(P, np.float32)
+    out = np.empty(len(P), np.int32)
+    lib().orc_points_contain(_p(V), len(V), _p(I), len(I), _p(O), _p(P), len(P), axis,
+                             1 if use_grid else 0, _p(out))
+    return out
+
+
+def edge_pairs(edges, V, I, tri_obj, swap=False, use_grid=True, with_counts=False):
+    """Raw pair keys (a << 32 | b) in edge order, and optionally per-edge counts."""
+    S = _c(edges.starts, np.float32); E = _c(edges.ends, np.float32)
+    off = _c(edges.src_offsets, np.int32); cnt = _c(edges.src_counts, np.int32)
+    so = _c(edges.src_objects, np.int32)
+    V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32)
+    nE = len(S)
+    counts = np.empty(nE, np.int32)
+    total = lib().orc_edge_pairs(_p(S), _p(E), _p(off), _p(cnt), _p(so), nE, _p(V), len(V), _p(I),
+                                 len(I), _p(O), int(swap), int(use_grid), None, 0, _p(counts))
+    keys = np.empty(max(total, 1), np.uint64)
+    lib().orc_edge_pairs(_p(S), _p(E), _p(off), _p(cnt), _p(so), nE, _p(V), len(V), _p(I), len(I),
+                         _p(O), int(swap), int(use_grid), _p(keys), total, None)
+    keys = keys[:total]
+    return (keys, counts) if with_counts else keys
+
+
+def sort_unique(keys):
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    n = lib().orc_sort_unique_u64(_p(k), len(k))
+    return k[:n]
+
+
+def overlap_pairs(mesh1, edges1, mesh2, edges2, use_grid=True):
+    """A.2 join: sorted unique keys of edges1 -> mesh2 and edges2 -> mesh1 (swapped)."""
+    k1 = edge_pairs(edges1, mesh2.vertices, mesh2.indices, mesh2.tri_to_obj, False, use_grid)
+    k2 = edge_pairs(edges2, mesh1.vertices, mesh1.indices, mesh1.tri_to_obj, True, use_grid)
+    return sort_unique(np.concatenate([k1, k2]))
+
+
+def fmix64(k):
+    return int(lib().orc_fmix64(C.c_uint64(k)))
+
+
+def ht_insert(table, bitwise, max_probes, key):
+    return bool(lib().orc_ht_insert(_p(table), len(table), int(bitwise), int(max_probes), key))
+
+
+def estimate_raw(cells_a, cells_b, cell_volume, eps=0.001, gamma=0.8):
+    a = np.ascontiguousarray(cells_a, GRID_CELL); b = np.ascontiguousarray(cells_b, GRID_CELL)
+    return float(lib().orc_estimate_raw(_p(a), _p(b), len(a), cell_volume, eps, gamma))
+
+
+def estimate_pairs(cells_a, cells_b, minb, maxb, res, eps=0.001, gamma=0.8, alpha=True):
+    a = np.ascontiguousarray(cells_a, GRID_CELL); b = np.ascontiguousarray(cells_b, GRID_CELL)
+    return int(lib().orc_estimate_pairs(_p(a), _p(b), len(a), _p(_c(minb, np.float32)),
+                                        _p(_c(maxb, np.float32)), _p(_c(res, np.uint32)), eps,
+                                        gamma, int(alpha)))
+
+
+def grid_stats(V, I, tri_obj, res, world_size=-1.0):
+    V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32)
+    cells = np.zeros(res ** 3, GRID_CELL)
+    bounds = np.zeros(6, np.float32)
+    lib().orc_grid_stats(_p(V), len(V), _p(I), len(I), _p(O), res, world_size, _p(cells), _p(bounds))
+    return cells, bounds[:3].copy(), bounds[3:].copy()
+
+
+def extract_edges(V, I, tri_obj):
+    """Returns (starts, ends, offsets, counts, sources)."""
+    V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32)
+    ns = C.c_int64(0)
+    ne = lib().orc_extract_edges(_p(V), len(V), _p(I), len(I), _p(O), None, None, None, None, None,
+                                 C.cast(C.pointer(ns), C.c_void_p))
+    S = np.empty((ne, 3), np.float32); E = np.empty((ne, 3), np.float32)
+    off = np.empty(ne, np.int32); cnt = np.empty(ne, np.int32); src = np.empty(max(ns.value, 1), np.int32)
+    lib().orc_extract_edges(_p(V), len(V), _p(I), len(I), _p(O), _p(S), _p(E), _p(off), _p(cnt),
+                            _p(src), C.cast(C.pointer(ns), C.c_void_p))
+    return S, E, off, cnt, src[:ns.value]

@@ -1,0 +1,144 @@
+"""ctypes binding of librs3d.so (the C ABI declared in include/rs3d.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make`` in
+``rayspace3d_amd/csrc``) into ``rayspace3d_amd/lib/librs3d.so``. There is no fallback:
+if the library is missing or fails to load, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "librs3d.so")
+
+RS3D_OK = 0
+RS3D_E_ARG = -1
+RS3D_E_HIP = -2
+RS3D_E_NOMEM = -3
+RS3D_E_CAPACITY = -4
+RS3D_E_DEPTH = -5
+
+OUT_RAY_RESULT = 0
+OUT_FLAG_U8 = 1
+OUT_OBJ_I32 = 2
+
+
+class BvhOpts(C.Structure):
+    _fields_ = [("leaf_size", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [
+        ("num_triangles", C.c_uint64),
+        ("num_nodes", C.c_uint64),
+        ("max_depth", C.c_int32),
+        ("leaf_size", C.c_int32),
+        ("bounds_min", C.c_float * 3),
+        ("bounds_max", C.c_float * 3),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64),
+        ("nodes_visited", C.c_uint64),
+        ("tris_tested", C.c_uint64),
+        ("hits", C.c_uint64),
+        ("restarts", C.c_uint64),
+    ]
+
+
+class Edges(C.Structure):
+    _fields_ = [
+        ("starts", C.c_void_p),
+        ("ends", C.c_void_p),
+        ("src_offsets", C.c_void_p),
+        ("src_counts", C.c_void_p),
+        ("src_objects", C.c_void_p),
+        ("num_edges", C.c_uint64),
+    ]
+
+
+class HashTable(C.Structure):
+    _fields_ = [
+        ("slots", C.c_void_p),
+        ("size", C.c_uint64),
+        ("bitwise", C.c_int32),
+        ("max_probes", C.c_int32),
+        ("failed", C.c_void_p),
+    ]
+
+
+P = C.c_void_p
+U64 = C.c_uint64
+I32 = C.c_int
+PU64 = C.POINTER(C.c_uint64)
+PI64 = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes); every symbol declared in include/rs3d.h
+SIGNATURES = {
+    "rs3d_last_error": (C.c_char_p, []),
+    "rs3d_ctx_create": (I32, [I32, C.POINTER(P)]),
+    "rs3d_ctx_destroy": (I32, [P]),
+    "rs3d_ctx_sync": (I32, [P]),
+    "rs3d_ctx_stream": (P, [P]),
+    "rs3d_ctx_profile": (I32, [P, I32]),
+    "rs3d_ctx_profile_read": (I32, [P, C.POINTER(C.c_double), PU64]),
+    "rs3d_bvh_build": (I32, [P, P, U64, P, U64, P, C.POINTER(BvhOpts), C.POINTER(P)]),
+    "rs3d_bvh_free": (I32, [P]),
+    "rs3d_bvh_info_get": (I32, [P, C.POINTER(BvhInfo)]),
+    "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
+    "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
+    "rs3d_filter_refine": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
+    "rs3d_filter_box": (I32, [P, C.POINTER(C.c_float), P, U64, I32, P]),
+    "rs3d_scan_i32_i64": (I32, [P, P, P, U64, PI64]),
+    "rs3d_compact_hits": (I32, [P, P, U64, P, PU64]),
+    "rs3d_compact_flags": (I32, [P, P, U64, P, PU64]),
+    "rs3d_sort_unique_u64": (I32, [P, P, U64, PU64]),
+    "rs3d_sort_pairs_u32": (I32, [P, P, P, U64, I32]),
+    "rs3d_edge_count": (I32, [P, P, C.POINTER(Edges), P]),
+    "rs3d_edge_write": (I32, [P, P, C.POINTER(Edges), P, I32, P]),
+    "rs3d_ht_clear": (I32, [P, C.POINTER(HashTable)]),
+    "rs3d_edge_hash": (I32, [P, P, C.POINTER(Edges), I32, C.POINTER(HashTable)]),
+    "rs3d_ht_compact": (I32, [P, C.POINTER(HashTable), P, U64, PU64]),
+    "rs3d_overlap_join": (I32, [P, P, C.POINTER(Edges), P, C.POINTER(Edges), I32, P, U64, PU64]),
+    "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
+    "count_hits_gpu": (I32, [P, I32]),
+    "compact_hits_gpu": (None, [P, P, I32, C.POINTER(C.c_int)]),
+    "exclusive_scan_gpu": (C.c_longlong, [P, P, I32]),
+    "merge_and_deduplicate_pairs_gpu": (C.c_longlong, [P, C.c_longlong, P, C.c_longlong, P]),
+    "compact_hash_table_pairs": (I32, [P, C.c_ulonglong, P, I32]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads librs3d.so and binds every declared entry point. Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"librs3d.so not found at {path}: build it with __graft_entry__.build() "
+            "(the HIP library has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Rs3dError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = load().rs3d_last_error()
+        super().__init__(f"{where} failed with code {code}: {msg.decode() if msg else ''}")
+        self.code = code
+
+
+def check(code: int, where: str) -> None:
+    if code != RS3D_OK:
+        raise Rs3dError(code, where)

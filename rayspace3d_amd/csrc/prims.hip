@@ -1,0 +1,678 @@
+// prims.hip -- device-wide primitives: exclusive scans, stable LSD radix sort, unique,
+// stream compaction. All are HBM-bound; tiles are staged through LDS so global loads
+// and stores stay coalesced.
+#include <cstdarg>
+#include <cstring>
+
+#include "rs3d_internal.h"
+
+namespace rs3d {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes) return RS3D_OK;
+    if (ptr) {
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    size_t alloc = need < 4096 ? 4096 : need + need / 8;
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", alloc, hipGetErrorString(e));
+        return RS3D_E_NOMEM;
+    }
+    bytes = alloc;
+    return RS3D_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int grid_blocks(uint64_t n, int block, unsigned* out) {
+    uint64_t b = (n + block - 1) / block;
+    if (b > 0x7fffffffull) {
+        set_error("launch of %llu blocks exceeds the grid limit", (unsigned long long)b);
+        return RS3D_E_ARG;
+    }
+    *out = (unsigned)(b ? b : 1);
+    return RS3D_OK;
+}
+
+int prof_collect(rs3d_ctx* ctx) {
+    if (!ctx->prof_pending) return RS3D_OK;
+    RS3D_HIP(hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.0f;
+    RS3D_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ctx->prof_ms += ms;
+    ctx->prof_launches++;
+    ctx->prof_pending = false;
+    return RS3D_OK;
+}
+
+int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host) {
+    RS3D_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    *host = ctx->pinned[0];
+    return RS3D_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Exclusive scan: reduce-then-scan over tiles of SCAN_TILE elements.
+// ------------------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive prefix and
+// the block total in `total`.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T& total) {
+    __shared__ T wsum[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T inc = wave_inclusive_scan(v, lane);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    T wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        T s = wsum[w];
+        if (w < wave) wbase += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + inc - v;
+}
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const In* __restrict__ in, uint64_t n,
+                                                            Out* __restrict__ partial) {
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    Out s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint64_t i = base + (uint64_t)j * SCAN_BLOCK + threadIdx.x;
+        if (i < n) s += (Out)in[i];
+    }
+    Out total;
+    (void)block_exclusive_scan<Out>(s, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const In* in, uint64_t n, Out* out,
+                                                           const Out* __restrict__ offsets,
+                                                           Out* __restrict__ total_out) {
+    __shared__ Out tile[SCAN_TILE];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint64_t i = base + (uint64_t)j * SCAN_BLOCK + threadIdx.x;
+        tile[j * SCAN_BLOCK + threadIdx.x] = i < n ? (Out)in[i] : (Out)0;
+    }
+    __syncthreads();
+    Out v[SCAN_ITEMS];
+    Out s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        v[j] = tile[threadIdx.x * SCAN_ITEMS + j];
+        s += v[j];
+    }
+    Out total;
+    Out pre = block_exclusive_scan<Out>(s, total);
+    const Out off = offsets ? offsets[blockIdx.x] : (Out)0;
+    pre += off;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        tile[threadIdx.x * SCAN_ITEMS + j] = pre;
+        pre += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        uint64_t i = base + (uint64_t)j * SCAN_BLOCK + threadIdx.x;
+        if (i < n) out[i] = tile[j * SCAN_BLOCK + threadIdx.x];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = off + total;
+}
+
+// Recursive exclusive scan; `tmp` holds partials of every level. total_dev (optional)
+// receives the sum of all inputs.
+template <typename In, typename Out>
+static int scan_impl(rs3d_ctx* ctx, const In* in, Out* out, uint64_t n, Out* total_dev, Out* tmp,
+                     uint64_t tmp_elems) {
+    if (n == 0) {
+        if (total_dev) RS3D_HIP(hipMemsetAsync(total_dev, 0, sizeof(Out), ctx->stream));
+        return RS3D_OK;
+    }
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, SCAN_TILE, &nb));
+    if (nb == 1) {
+        k_scan_apply<In, Out><<<1, SCAN_BLOCK, 0, ctx->stream>>>(in, n, out, nullptr, total_dev);
+        RS3D_HIP(hipGetLastError());
+        return RS3D_OK;
+    }
+    if (tmp_elems < nb) {
+        set_error("scan scratch too small");
+        return RS3D_E_ARG;
+    }
+    Out* partial = tmp;
+    k_scan_reduce<In, Out><<<nb, SCAN_BLOCK, 0, ctx->stream>>>(in, n, partial);
+    RS3D_HIP(hipGetLastError());
+    RS3D_TRY((scan_impl<Out, Out>(ctx, partial, partial, nb, (Out*)nullptr, tmp + nb, tmp_elems - nb)));
+    k_scan_apply<In, Out><<<nb, SCAN_BLOCK, 0, ctx->stream>>>(in, n, out, partial, total_dev);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+static uint64_t scan_tmp_elems(uint64_t n) {
+    uint64_t tot = 0;
+    while (n > SCAN_TILE) {
+        n = (n + SCAN_TILE - 1) / SCAN_TILE;
+        tot += n;
+    }
+    return tot + 16;
+}
+
+// In-place use (in == out) is safe: each tile is read fully into LDS before it is written.
+int scan_u32_exclusive(rs3d_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n) {
+    uint64_t te = scan_tmp_elems(n);
+    RS3D_TRY(ctx->scan_tmp.ensure(te * sizeof(uint32_t)));
+    return scan_impl<uint32_t, uint32_t>(ctx, in, out, n, nullptr, ctx->scan_tmp.as<uint32_t>(), te);
+}
+
+int scan_i32_to_i64_exclusive(rs3d_ctx* ctx, const int32_t* in, int64_t* out, uint64_t n,
+                              int64_t* total_dev) {
+    uint64_t te = scan_tmp_elems(n);
+    RS3D_TRY(ctx->scan_tmp.ensure(te * sizeof(int64_t)));
+    return scan_impl<int32_t, int64_t>(ctx, in, out, n, total_dev, ctx->scan_tmp.as<int64_t>(), te);
+}
+
+int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, uint64_t n,
+                             uint32_t* total_dev) {
+    uint64_t te = scan_tmp_elems(n);
+    RS3D_TRY(ctx->scan_tmp.ensure(te * sizeof(uint32_t)));
+    return scan_impl<uint8_t, uint32_t>(ctx, in, out, n, total_dev, ctx->scan_tmp.as<uint32_t>(), te);
+}
+
+// ------------------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits. Per pass: (1) per-tile digit histograms,
+// (2) exclusive scan of the digit-major histogram table, (3) stable scatter: each tile
+// ranks its keys with wave64 ballot matching and per-wave digit counters in LDS.
+// ------------------------------------------------------------------------------------
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+
+template <typename K>
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys, uint64_t n,
+                                                      int shift, uint32_t* __restrict__ hist,
+                                                      uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K, bool HAS_VAL>
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ kin,
+                                                         const uint32_t* __restrict__ vin,
+                                                         K* __restrict__ kout,
+                                                         uint32_t* __restrict__ vout, uint64_t n,
+                                                         int shift,
+                                                         const uint32_t* __restrict__ hist,
+                                                         uint32_t nblocks) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run[tid] = hist[(uint64_t)tid * nblocks + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j = 0; j < RS_ITEMS; ++j) {
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        const uint64_t i = base + (uint64_t)j * RS_BLOCK + tid;
+        const bool valid = i < n;
+        K key = valid ? kin[i] : (K)0;
+        uint32_t val = 0;
+        if (HAS_VAL && valid) val = vin[i];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            uint32_t r = run[tid];
+#pragma unroll
+            for (int w = 0; w < RS_WAVES; ++w) {
+                uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = r;
+                r += c;
+            }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = wcnt[wave][d] + rank;
+            kout[pos] = key;
+            if (HAS_VAL) vout[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename K, bool HAS_VAL>
+static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, int begin_bit,
+                           int end_bit, K* alt_keys, uint32_t* alt_vals) {
+    if (n <= 1) return RS3D_OK;
+    if (n > 0xffffffffull) {
+        set_error("radix sort supports at most 2^32-1 keys");
+        return RS3D_E_ARG;
+    }
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, RS_TILE, &nb));
+    RS3D_TRY(ctx->sort_hist.ensure((size_t)nb * 256 * sizeof(uint32_t)));
+    uint32_t* hist = ctx->sort_hist.as<uint32_t>();
+    K* src_k = keys;
+    K* dst_k = alt_keys;
+    uint32_t* src_v = vals;
+    uint32_t* dst_v = alt_vals;
+    int passes = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        k_rs_hist<K><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, n, shift, hist, nb);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(scan_u32_exclusive(ctx, hist, hist, (uint64_t)nb * 256));
+        k_rs_scatter<K, HAS_VAL><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, src_v, dst_k, dst_v, n,
+                                                                   shift, hist, nb);
+        RS3D_HIP(hipGetLastError());
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+        ++passes;
+    }
+    if (passes & 1) {
+        RS3D_HIP(hipMemcpyAsync(keys, src_k, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
+        if (HAS_VAL)
+            RS3D_HIP(hipMemcpyAsync(vals, src_v, n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+    }
+    return RS3D_OK;
+}
+
+int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit) {
+    if (n <= 1) return RS3D_OK;
+    RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint32_t)));
+    RS3D_TRY(ctx->sort_vals.ensure(n * sizeof(uint32_t)));
+    if (end_bit < 1) end_bit = 1;
+    if (end_bit > 32) end_bit = 32;
+    return radix_sort_impl<uint32_t, true>(ctx, keys, vals, n, 0, end_bit,
+                                           ctx->sort_keys.as<uint32_t>(),
+                                           ctx->sort_vals.as<uint32_t>());
+}
+
+// Bitwise OR / AND of all keys, to skip digit positions where every key agrees.
+__global__ __launch_bounds__(256) void k_or_and_u64(const uint64_t* __restrict__ keys, uint64_t n,
+                                                    unsigned long long* __restrict__ acc) {
+    uint64_t o = 0, a = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+        o |= k;
+        a &= k;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        o |= __shfl_xor(o, off, 64);
+        a &= __shfl_xor(a, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr(&acc[0], (unsigned long long)o);
+        atomicAnd(&acc[1], (unsigned long long)a);
+    }
+}
+
+__global__ void k_init_or_and(unsigned long long* acc) {
+    acc[0] = 0ull;
+    acc[1] = ~0ull;
+}
+
+int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
+    if (n <= 1) return RS3D_OK;
+    RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint64_t)));
+    RS3D_TRY(ctx->small.ensure(64));
+    unsigned long long* acc = ctx->small.as<unsigned long long>();
+    k_init_or_and<<<1, 1, 0, ctx->stream>>>(acc);
+    unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
+    k_or_and_u64<<<nb, 256, 0, ctx->stream>>>(keys, n, acc);
+    RS3D_HIP(hipGetLastError());
+    unsigned long long h[2];
+    RS3D_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t varying = h[0] & ~h[1];
+    // Sort each varying byte, least significant first. Passes over constant bytes are
+    // skipped; the result is identical because those digits are equal for all keys.
+    uint64_t* src = keys;
+    uint64_t* dst = ctx->sort_keys.as<uint64_t>();
+    unsigned nbt;
+    RS3D_TRY(grid_blocks(n, RS_TILE, &nbt));
+    RS3D_TRY(ctx->sort_hist.ensure((size_t)nbt * 256 * sizeof(uint32_t)));
+    uint32_t* hist = ctx->sort_hist.as<uint32_t>();
+    for (int shift = 0; shift < 64; shift += 8) {
+        if (((varying >> shift) & 0xffull) == 0) continue;
+        k_rs_hist<uint64_t><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, n, shift, hist, nbt);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(scan_u32_exclusive(ctx, hist, hist, (uint64_t)nbt * 256));
+        k_rs_scatter<uint64_t, false><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, nullptr, dst, nullptr,
+                                                                         n, shift, hist, nbt);
+        RS3D_HIP(hipGetLastError());
+        std::swap(src, dst);
+    }
+    if (src != keys)
+        RS3D_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    return RS3D_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Unique of a sorted u64 array and flag-driven compactions (scan-based, stable).
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unique_flags(const uint64_t* __restrict__ in, uint64_t n,
+                                                      uint8_t* __restrict__ flags) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = (i == 0 || in[i] != in[i - 1]) ? 1 : 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_flagged(const T* __restrict__ in,
+                                                         const uint8_t* __restrict__ flags,
+                                                         const uint32_t* __restrict__ pos,
+                                                         uint64_t n, T* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) out[pos[i]] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_index(const uint8_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ pos, uint64_t n,
+                                                       uint32_t* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) out[pos[i]] = (uint32_t)i;
+}
+
+int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host) {
+    if (n == 0) {
+        *count_host = 0;
+        return RS3D_OK;
+    }
+    if (n > 0xffffffffull) {
+        set_error("unique supports at most 2^32-1 keys");
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(ctx->tmp_a.ensure(n));
+    RS3D_TRY(ctx->tmp_b.ensure(n * sizeof(uint32_t)));
+    RS3D_TRY(ctx->small.ensure(64));
+    uint8_t* flags = ctx->tmp_a.as<uint8_t>();
+    uint32_t* pos = ctx->tmp_b.as<uint32_t>();
+    uint32_t* tot = ctx->small.as<uint32_t>() + 4;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_unique_flags<<<nb, 256, 0, ctx->stream>>>(in, n, flags);
+    RS3D_HIP(hipGetLastError());
+    RS3D_TRY(scan_u8_to_u32_exclusive(ctx, flags, pos, n, tot));
+    k_scatter_flagged<uint64_t><<<nb, 256, 0, ctx->stream>>>(in, flags, pos, n, out);
+    RS3D_HIP(hipGetLastError());
+    uint32_t h = 0;
+    RS3D_HIP(hipMemcpyAsync(ctx->pinned, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(&h, ctx->pinned, sizeof h);
+    *count_host = h;
+    return RS3D_OK;
+}
+
+__global__ __launch_bounds__(256) void k_hit_flags(const rs3d_ray_result* __restrict__ in, uint64_t n,
+                                                   uint8_t* __restrict__ flags) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = in[i].polygon_index != -1 ? 1 : 0;
+}
+
+}  // namespace rs3d
+
+using namespace rs3d;
+
+extern "C" {
+
+const char* rs3d_last_error(void) { return rs3d::last_error(); }
+
+int rs3d_ctx_create(int device, rs3d_ctx** out) {
+    if (!out) {
+        set_error("rs3d_ctx_create: out is null");
+        return RS3D_E_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    RS3D_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("rs3d_ctx_create: device %d not present (%d devices)", device, ndev);
+        return RS3D_E_ARG;
+    }
+    RS3D_HIP(hipSetDevice(device));
+    rs3d_ctx* c = new rs3d_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+        return RS3D_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    e = hipHostMalloc((void**)&c->pinned, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
+        return RS3D_E_NOMEM;
+    }
+    *out = c;
+    return RS3D_OK;
+}
+
+int rs3d_ctx_destroy(rs3d_ctx* ctx) {
+    if (!ctx) return RS3D_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->sort_keys.release();
+    ctx->sort_vals.release();
+    ctx->sort_hist.release();
+    ctx->scan_tmp.release();
+    ctx->tmp_a.release();
+    ctx->tmp_b.release();
+    ctx->tmp_c.release();
+    ctx->small.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RS3D_OK;
+}
+
+int rs3d_ctx_sync(rs3d_ctx* ctx) {
+    if (!ctx) {
+        set_error("null context");
+        return RS3D_E_ARG;
+    }
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    return RS3D_OK;
+}
+
+void* rs3d_ctx_stream(rs3d_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rs3d_ctx_profile(rs3d_ctx* ctx, int enable) {
+    if (!ctx) {
+        set_error("null context");
+        return RS3D_E_ARG;
+    }
+    if (enable && !ctx->ev_start) {
+        RS3D_HIP(hipEventCreate(&ctx->ev_start));
+        RS3D_HIP(hipEventCreate(&ctx->ev_stop));
+    }
+    RS3D_TRY(prof_collect(ctx));
+    ctx->profile = enable ? 1 : 0;
+    return RS3D_OK;
+}
+
+int rs3d_ctx_profile_read(rs3d_ctx* ctx, double* total_ms, uint64_t* launches) {
+    if (!ctx) {
+        set_error("null context");
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(prof_collect(ctx));
+    if (total_ms) *total_ms = ctx->prof_ms;
+    if (launches) *launches = ctx->prof_launches;
+    ctx->prof_ms = 0.0;
+    ctx->prof_launches = 0;
+    return RS3D_OK;
+}
+
+int rs3d_scan_i32_i64(rs3d_ctx* ctx, const int32_t* in, int64_t* out, uint64_t n, int64_t* total) {
+    if (!ctx || (n && (!in || !out))) {
+        set_error("rs3d_scan_i32_i64: bad arguments");
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(ctx->small.ensure(64));
+    int64_t* tot = ctx->small.as<int64_t>() + 2;
+    RS3D_TRY(scan_i32_to_i64_exclusive(ctx, in, out, n, tot));
+    uint64_t h = 0;
+    RS3D_TRY(read_u64(ctx, tot, &h));
+    if (total) *total = (int64_t)h;
+    return RS3D_OK;
+}
+
+int rs3d_compact_flags(rs3d_ctx* ctx, const uint8_t* flags, uint64_t n, uint32_t* idx_out,
+                       uint64_t* count) {
+    if (!ctx || (n && (!flags || !idx_out)) || n > 0xffffffffull) {
+        set_error("rs3d_compact_flags: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (n == 0) {
+        if (count) *count = 0;
+        return RS3D_OK;
+    }
+    RS3D_TRY(ctx->tmp_b.ensure(n * sizeof(uint32_t)));
+    RS3D_TRY(ctx->small.ensure(64));
+    uint32_t* pos = ctx->tmp_b.as<uint32_t>();
+    uint32_t* tot = ctx->small.as<uint32_t>() + 4;
+    RS3D_TRY(scan_u8_to_u32_exclusive(ctx, flags, pos, n, tot));
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_scatter_index<<<nb, 256, 0, ctx->stream>>>(flags, pos, n, idx_out);
+    RS3D_HIP(hipGetLastError());
+    RS3D_HIP(hipMemcpyAsync(ctx->pinned, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t h;
+    memcpy(&h, ctx->pinned, sizeof h);
+    if (count) *count = h;
+    return RS3D_OK;
+}
+
+int rs3d_compact_hits(rs3d_ctx* ctx, const rs3d_ray_result* in, uint64_t n, rs3d_ray_result* out,
+                      uint64_t* count) {
+    if (!ctx || (n && (!in || !out)) || n > 0xffffffffull) {
+        set_error("rs3d_compact_hits: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (n == 0) {
+        if (count) *count = 0;
+        return RS3D_OK;
+    }
+    RS3D_TRY(ctx->tmp_a.ensure(n));
+    RS3D_TRY(ctx->tmp_b.ensure(n * sizeof(uint32_t)));
+    RS3D_TRY(ctx->small.ensure(64));
+    uint8_t* flags = ctx->tmp_a.as<uint8_t>();
+    uint32_t* pos = ctx->tmp_b.as<uint32_t>();
+    uint32_t* tot = ctx->small.as<uint32_t>() + 4;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_hit_flags<<<nb, 256, 0, ctx->stream>>>(in, n, flags);
+    RS3D_HIP(hipGetLastError());
+    RS3D_TRY(scan_u8_to_u32_exclusive(ctx, flags, pos, n, tot));
+    k_scatter_flagged<rs3d_ray_result><<<nb, 256, 0, ctx->stream>>>(in, flags, pos, n, out);
+    RS3D_HIP(hipGetLastError());
+    RS3D_HIP(hipMemcpyAsync(ctx->pinned, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t h;
+    memcpy(&h, ctx->pinned, sizeof h);
+    if (count) *count = h;
+    return RS3D_OK;
+}
+
+int rs3d_sort_unique_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n, uint64_t* unique_count) {
+    if (!ctx || (n && !keys)) {
+        set_error("rs3d_sort_unique_u64: bad arguments");
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(radix_sort_u64(ctx, keys, n));
+    if (n == 0) {
+        if (unique_count) *unique_count = 0;
+        return RS3D_OK;
+    }
+    // unique into scratch, then copy back
+    RS3D_TRY(ctx->tmp_c.ensure(n * sizeof(uint64_t)));
+    uint64_t cnt = 0;
+    RS3D_TRY(unique_u64(ctx, keys, n, ctx->tmp_c.as<uint64_t>(), &cnt));
+    if (cnt)
+        RS3D_HIP(hipMemcpyAsync(keys, ctx->tmp_c.ptr, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    if (unique_count) *unique_count = cnt;
+    return RS3D_OK;
+}
+
+int rs3d_sort_pairs_u32(rs3d_ctx* ctx, uint32_t* keys, uint32_t* values, uint64_t n, int end_bit) {
+    if (!ctx || (n && (!keys || !values))) {
+        set_error("rs3d_sort_pairs_u32: bad arguments");
+        return RS3D_E_ARG;
+    }
+    return radix_sort_u32_pairs(ctx, keys, values, n, end_bit);
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// rs3d_internal.h -- shared host/device definitions of librs3d (MI355X, gfx950).
+//
+// HBM layouts:
+//   BNode (64 B, one per internal node of the binary LBVH): the boxes of both children
+//   and their links, so one node fetch decides both children.
+//   TriRec (48 B, one per triangle in Morton order): three vertices; v0.w carries the
+//   original triangle index (tie-break key and reported primitive id), v1.w the object id.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/rs3d.h"
+
+namespace rs3d {
+
+// ------------------------------------------------------------------------------------
+// Device data layouts
+// ------------------------------------------------------------------------------------
+struct alignas(16) BNode {
+    float4 a;  // lo0.x hi0.x lo0.y hi0.y
+    float4 b;  // lo0.z hi0.z lo1.x hi1.x
+    float4 c;  // lo1.y hi1.y lo1.z hi1.z
+    int4 link; // child0, child1 (node index, or first triangle), count0, count1
+               // count > 0: leaf of `count` triangles starting at the link;
+               // count == 0 and link >= 0: internal node; link < 0: empty child.
+};
+static_assert(sizeof(BNode) == 64, "BNode must be 64 bytes");
+
+struct alignas(16) TriRec {
+    float4 v0;  // .w = original triangle index (bits)
+    float4 v1;  // .w = object id (bits)
+    float4 v2;  // .w unused
+};
+static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
+
+struct GridCellRec {
+    uint32_t center;
+    uint32_t touch;
+    float avg;
+    float ratio;
+};
+
+// ------------------------------------------------------------------------------------
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), two-sided.
+// Every fp32 operation is written out in a fixed order and the library is compiled with
+// -ffp-contract=off, so results match the CPU oracle bit for bit.
+// ------------------------------------------------------------------------------------
+struct RayPre {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__host__ __device__ inline float get3(const float3& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : v.z);
+}
+
+__host__ __device__ inline RayPre ray_pre(float3 d) {
+    RayPre r;
+    int kz = 0;
+    float m = fabsf(d.x);
+    if (fabsf(d.y) > m) { kz = 1; m = fabsf(d.y); }
+    if (fabsf(d.z) > m) { kz = 2; }
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    const float dz = get3(d, kz);
+    if (dz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sx = get3(d, kx) / dz;
+    r.Sy = get3(d, ky) / dz;
+    r.Sz = 1.0f / dz;
+    return r;
+}
+
+// Returns true and t (unclipped) when the triangle is hit.
+__device__ __forceinline__ bool tri_hit(const float3 o, const RayPre& r, const float3 v0,
+                                        const float3 v1, const float3 v2, float& tout) {
+    const float3 A = make_float3(v0.x - o.x, v0.y - o.y, v0.z - o.z);
+    const float3 B = make_float3(v1.x - o.x, v1.y - o.y, v1.z - o.z);
+    const float3 C = make_float3(v2.x - o.x, v2.y - o.y, v2.z - o.z);
+    const float Akz = get3(A, r.kz), Bkz = get3(B, r.kz), Ckz = get3(C, r.kz);
+    const float Ax = get3(A, r.kx) - r.Sx * Akz;
+    const float Ay = get3(A, r.ky) - r.Sy * Akz;
+    const float Bx = get3(B, r.kx) - r.Sx * Bkz;
+    const float By = get3(B, r.ky) - r.Sy * Bkz;
+    const float Cx = get3(C, r.kx) - r.Sx * Ckz;
+    const float Cy = get3(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Akz;
+    const float Bz = r.Sz * Bkz;
+    const float Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    tout = T / det;
+    return true;
+}
+
+// Outward padding of primitive boxes, relative to coordinate magnitude and extent, so
+// box culling can never remove a triangle the fp32 test would report (pruning only).
+__host__ __device__ inline float box_pad(float mag, float ext) {
+    return (mag + ext) * 3.814697265625e-6f /* 2^-18 */ + 1e-35f;
+}
+
+// ------------------------------------------------------------------------------------
+// Host-side objects
+// ------------------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+
+#define RS3D_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::rs3d::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                              __FILE__, __LINE__);                                      \
+            return e_ == hipErrorOutOfMemory ? RS3D_E_NOMEM : RS3D_E_HIP;               \
+        }                                                                               \
+    } while (0)
+
+#define RS3D_TRY(call)                       \
+    do {                                     \
+        int r_ = (call);                     \
+        if (r_ != RS3D_OK) return r_;        \
+    } while (0)
+
+// Grow-only device buffer owned by a context (scratch that persists across calls).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace rs3d
+
+struct rs3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    // Scratch, reused across calls on this context (calls on one context are serial).
+    rs3d::DevBuf sort_keys, sort_vals, sort_hist, scan_tmp, tmp_a, tmp_b, tmp_c, small;
+    uint64_t* pinned = nullptr;  // small pinned host buffer for counts
+    // traversal-kernel timing (rs3d_ctx_profile)
+    int profile = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool prof_pending = false;
+    double prof_ms = 0.0;
+    uint64_t prof_launches = 0;
+};
+
+struct rs3d_bvh {
+    rs3d_ctx* ctx = nullptr;
+    uint64_t num_tris = 0;
+    uint64_t num_nodes = 0;
+    int leaf_size = 4;
+    int max_depth = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // padded box bounds
+    float vmin[3] = {0, 0, 0}, vmax[3] = {0, 0, 0};  // exact bounds of all vertices
+    rs3d::BNode* nodes = nullptr;  // device
+    rs3d::TriRec* tris = nullptr;  // device, Morton order
+    size_t device_bytes = 0;
+};
+
+namespace rs3d {
+// Internal primitives used across translation units (all asynchronous on ctx->stream
+// unless they return a host value).
+int scan_u32_exclusive(rs3d_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n);
+int scan_i32_to_i64_exclusive(rs3d_ctx* ctx, const int32_t* in, int64_t* out, uint64_t n,
+                              int64_t* total_dev);
+int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, uint64_t n,
+                             uint32_t* total_dev);
+int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit);
+int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
+int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
+int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
+int grid_blocks(uint64_t n, int block, unsigned* out);
+int prof_collect(rs3d_ctx* ctx);  // folds a finished timed launch into the totals
+}  // namespace rs3d

@@ -1,0 +1,763 @@
+// trace.hip -- wavefront BVH traversal kernels (replace the OptiX raygen/closest-hit
+// programs of query/src/cuda/raytracing.cu and query/src/cuda/mesh_overlap_edges.cu).
+//
+// One lane per ray, 64-lane workgroups, per-lane traversal stack in LDS (stride 64 so the
+// lanes of a wave hit distinct banks). Each traversal collects the KBUF smallest
+// (t, triangle) hits at or beyond the current chain threshold in a register-resident
+// sorted buffer; the closest-hit chain of the reference is then walked inside that
+// buffer. When more than KBUF hits were seen the traversal restarts from the threshold,
+// which reproduces the chain exactly for any hit count. Results therefore do not depend
+// on the BVH shape, leaf size or stack depth.
+#include <cstring>
+
+#include "rs3d_internal.h"
+
+namespace rs3d {
+
+constexpr int TR_BLOCK = 64;
+constexpr int KBUF = 8;
+
+__device__ __forceinline__ float3 ld3f(const float* p, uint64_t i) {
+    return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
+// Sorted insertion of (t, tri, obj) into the KBUF smallest, ordered by (t, tri).
+__device__ __forceinline__ void kbuf_insert(float (&bt)[KBUF], uint32_t (&bi)[KBUF],
+                                            int (&bo)[KBUF], float t, uint32_t tri, int obj) {
+    float ct = t;
+    uint32_t ci = tri;
+    int co = obj;
+#pragma unroll
+    for (int s = 0; s < KBUF; ++s) {
+        const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
+        if (less) {
+            const float tt = bt[s];
+            const uint32_t ti = bi[s];
+            const int to = bo[s];
+            bt[s] = ct;
+            bi[s] = ci;
+            bo[s] = co;
+            ct = tt;
+            ci = ti;
+            co = to;
+        }
+    }
+}
+
+__device__ __forceinline__ void kbuf_clear(float (&bt)[KBUF], uint32_t (&bi)[KBUF], int (&bo)[KBUF]) {
+#pragma unroll
+    for (int s = 0; s < KBUF; ++s) {
+        bt[s] = INFINITY;
+        bi[s] = 0xffffffffu;
+        bo[s] = -1;
+    }
+}
+
+struct NodeBoxes {
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    int l0, l1, n0, n1;
+};
+
+__device__ __forceinline__ NodeBoxes load_node(const BNode* __restrict__ nodes, int idx) {
+    const BNode nd = nodes[idx];
+    NodeBoxes b;
+    b.lo0[0] = nd.a.x; b.hi0[0] = nd.a.y; b.lo0[1] = nd.a.z; b.hi0[1] = nd.a.w;
+    b.lo0[2] = nd.b.x; b.hi0[2] = nd.b.y; b.lo1[0] = nd.b.z; b.hi1[0] = nd.b.w;
+    b.lo1[1] = nd.c.x; b.hi1[1] = nd.c.y; b.lo1[2] = nd.c.z; b.hi1[2] = nd.c.w;
+    b.l0 = nd.link.x; b.l1 = nd.link.y; b.n0 = nd.link.z; b.n1 = nd.link.w;
+    return b;
+}
+
+__device__ __forceinline__ void load_tri(const TriRec* __restrict__ tris, int j, float3& v0, float3& v1,
+                                         float3& v2, uint32_t& orig, int& obj) {
+    const TriRec r = tris[j];
+    v0 = make_float3(r.v0.x, r.v0.y, r.v0.z);
+    v1 = make_float3(r.v1.x, r.v1.y, r.v1.z);
+    v2 = make_float3(r.v2.x, r.v2.y, r.v2.z);
+    orig = (uint32_t)__float_as_int(r.v0.w);
+    obj = __float_as_int(r.v1.w);
+}
+
+struct TraceCounters {
+    uint64_t nodes = 0, tris = 0, hits = 0, restarts = 0;
+};
+
+// ------------------------------------------------------------------------------------
+// All-hits traversal for an axis-aligned ray (+AX). Hits with thr <= t < tmax_excl are
+// offered to the buffer; returns the number of such hits.
+// ------------------------------------------------------------------------------------
+template <int AX, int DEPTH, bool STATS>
+__device__ __forceinline__ uint32_t collect_axis(const BNode* __restrict__ nodes,
+                                                 const TriRec* __restrict__ tris, const float3 o,
+                                                 float thr, float tmax_excl, int* stack,
+                                                 float (&bt)[KBUF], uint32_t (&bi)[KBUF],
+                                                 int (&bo)[KBUF], TraceCounters& ctr) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    uint32_t total = 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const NodeBoxes b = load_node(nodes, node);
+        if (STATS) ctr.nodes++;
+        // A triangle entirely behind the origin (all vertices <= o along AX) yields t <= 0,
+        // so boxes are culled only on the perpendicular slab and on hi < o along AX.
+        const bool h0 = ou >= b.lo0[U] && ou <= b.hi0[U] && ow >= b.lo0[W] && ow <= b.hi0[W] &&
+                        b.hi0[AX] >= oa;
+        const bool h1 = ou >= b.lo1[U] && ou <= b.hi1[U] && ow >= b.lo1[W] && ow <= b.hi1[W] &&
+                        b.hi1[AX] >= oa;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bool hit = c == 0 ? h0 : h1;
+            const int cnt = c == 0 ? b.n0 : b.n1;
+            const int first = c == 0 ? b.l0 : b.l1;
+            if (hit && cnt > 0) {
+                for (int j = first; j < first + cnt; ++j) {
+                    float3 v0, v1, v2;
+                    uint32_t orig;
+                    int obj;
+                    load_tri(tris, j, v0, v1, v2, orig, obj);
+                    if (STATS) ctr.tris++;
+                    float t;
+                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t < tmax_excl) {
+                        ++total;
+                        if (STATS) ctr.hits++;
+                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, obj);
+                    }
+                }
+            }
+        }
+        const bool in0 = h0 && b.n0 == 0;
+        const bool in1 = h1 && b.n1 == 0;
+        if (in0 && in1) {
+            const bool first0 = b.lo0[AX] <= b.lo1[AX];
+            stack[sp * TR_BLOCK] = first0 ? b.l1 : b.l0;
+            ++sp;
+            node = first0 ? b.l0 : b.l1;
+        } else if (in0) {
+            node = b.l0;
+        } else if (in1) {
+            node = b.l1;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * TR_BLOCK];
+        }
+    }
+    return total;
+}
+
+__device__ __forceinline__ void flush_counters(const TraceCounters& c, unsigned long long* stats) {
+    uint64_t v[4] = {c.nodes, c.tris, c.hits, c.restarts};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint64_t x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        v[k] = x;
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&stats[k], (unsigned long long)v[k]);
+    }
+}
+
+// A.1 point containment (query/src/cuda/raytracing.cu:22-77 with the canonical chain).
+template <int AX, int DEPTH, bool STATS>
+__global__ __launch_bounds__(TR_BLOCK) void k_points(const BNode* __restrict__ nodes,
+                                                     const TriRec* __restrict__ tris,
+                                                     const float* __restrict__ pts, uint64_t n,
+                                                     const uint32_t* __restrict__ imap, int out_kind,
+                                                     void* __restrict__ out,
+                                                     unsigned long long* __restrict__ stats) {
+    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    const uint64_t i = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
+    TraceCounters ctr;
+    const bool active = i < n;
+    if (active) {
+        const uint64_t pi = imap ? (uint64_t)imap[i] : i;
+        const float3 o = ld3f(pts, pi);
+        const float eps = 1e-6f, tmax = 1e16f;
+        float thr = eps;
+        int count = 0;
+        int first_obj = -1;
+        float bt[KBUF];
+        uint32_t bi[KBUF];
+        int bo[KBUF];
+        while (true) {
+            kbuf_clear(bt, bi, bo);
+            const uint32_t total = collect_axis<AX, DEPTH, STATS>(nodes, tris, o, thr, tmax,
+                                                                  stack_mem + threadIdx.x, bt, bi,
+                                                                  bo, ctr);
+#pragma unroll
+            for (int k = 0; k < KBUF; ++k) {
+                if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                    if (count == 0) first_obj = bo[k];
+                    ++count;
+                    float nthr = bt[k] + eps;
+                    if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                    thr = nthr;
+                }
+            }
+            if (total <= (uint32_t)KBUF || count >= 10000) break;
+            if (STATS) ctr.restarts++;
+        }
+        const bool inside = (count & 1) != 0;
+        const int obj = inside ? first_obj : -1;
+        if (out_kind == RS3D_OUT_RAY_RESULT) {
+            rs3d_ray_result r;
+            r.ray_id = (int32_t)pi;
+            r.polygon_index = obj;
+            static_cast<rs3d_ray_result*>(out)[pi] = r;
+        } else if (out_kind == RS3D_OUT_FLAG_U8) {
+            static_cast<uint8_t*>(out)[pi] = inside ? 1 : 0;
+        } else {
+            static_cast<int32_t*>(out)[pi] = obj;
+        }
+    }
+    if (STATS) flush_counters(ctr, stats);
+}
+
+// ------------------------------------------------------------------------------------
+// Filter: parity of the +AX ray against the 12-triangle box mesh of
+// query/src/geometry/BoundingBox.cpp:26-49 (same vertex order and triangulation).
+// ------------------------------------------------------------------------------------
+template <int AX>
+__global__ __launch_bounds__(256) void k_filter_box(const float* __restrict__ pts, uint64_t n,
+                                                    float x0, float y0, float z0, float x1,
+                                                    float y1, float z1, uint8_t* __restrict__ flags) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float3 o = ld3f(pts, i);
+    const float3 bv[8] = {make_float3(x0, y0, z0), make_float3(x1, y0, z0), make_float3(x1, y1, z0),
+                          make_float3(x0, y1, z0), make_float3(x0, y0, z1), make_float3(x1, y0, z1),
+                          make_float3(x1, y1, z1), make_float3(x0, y1, z1)};
+    const int tri[12][3] = {{0, 2, 1}, {0, 3, 2}, {4, 5, 6}, {4, 6, 7}, {0, 1, 5}, {0, 5, 4},
+                            {3, 6, 2}, {3, 7, 6}, {0, 4, 7}, {0, 7, 3}, {1, 2, 6}, {1, 6, 5}};
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    constexpr int NB = 12;
+    float bt[NB];
+    uint32_t bi[NB];
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        bt[s] = INFINITY;
+        bi[s] = 0xffffffffu;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        float t;
+        if (tri_hit(o, rp, bv[tri[k][0]], bv[tri[k][1]], bv[tri[k][2]], t) && t >= 1e-6f &&
+            t < 1e16f) {
+            float ct = t;
+            uint32_t ci = (uint32_t)k;
+#pragma unroll
+            for (int s = 0; s < NB; ++s) {
+                const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
+                if (less) {
+                    const float tt = bt[s];
+                    const uint32_t ti = bi[s];
+                    bt[s] = ct;
+                    bi[s] = ci;
+                    ct = tt;
+                    ci = ti;
+                }
+            }
+        }
+    }
+    float thr = 1e-6f;
+    int count = 0;
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        if (bt[s] < INFINITY && bt[s] >= thr) {
+            ++count;
+            float nthr = bt[s] + 1e-6f;
+            if (nthr <= bt[s]) nthr = nextafterf(bt[s], INFINITY);
+            thr = nthr;
+        }
+    }
+    flags[i] = (count & 1) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_count_nonneg(const int32_t* __restrict__ p, uint64_t n,
+                                                      unsigned long long* __restrict__ cnt) {
+    uint64_t c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        c += p[i] >= 0 ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, uint64_t n, int32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------
+// Edge rays (A.2): segment from start towards end, hits in [thr, len + 1e-6].
+// ------------------------------------------------------------------------------------
+enum { EMODE_COUNT = 0, EMODE_WRITE_PAIRS = 1, EMODE_WRITE_KEYS = 2, EMODE_HASH = 3 };
+
+struct EdgeArgs {
+    const float* starts;
+    const float* ends;
+    const int32_t* src_off;
+    const int32_t* src_cnt;
+    const int32_t* src_obj;
+    uint64_t num_edges;
+};
+
+struct HashArgs {
+    unsigned long long* slots;
+    uint64_t size;
+    int bitwise;
+    int max_probes;
+    unsigned long long* failed;
+};
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Insert of the reference's open-addressing table (query/src/cuda/mesh_overlap_edges.cu:62-104):
+// fmix64 hash, linear probing, CAS from the all-ones empty key. A read before the CAS
+// skips the atomic when the slot already holds the key or another key.
+__device__ __forceinline__ void ht_insert(const HashArgs& h, uint64_t key) {
+    const uint64_t k = fmix64(key);
+    uint64_t s = h.bitwise ? (k & (h.size - 1)) : (k % h.size);
+    const uint64_t probes = h.max_probes > 0 ? (uint64_t)h.max_probes : h.size;
+    for (uint64_t p = 0; p < probes; ++p) {
+        const unsigned long long cur = __hip_atomic_load(&h.slots[s], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == (unsigned long long)key) return;
+        if (cur == ~0ull) {
+            const unsigned long long old = atomicCAS(&h.slots[s], ~0ull, (unsigned long long)key);
+            if (old == ~0ull || old == (unsigned long long)key) return;
+        }
+        s = (s + 1 == h.size) ? 0 : s + 1;
+    }
+    atomicAdd(h.failed, 1ull);
+}
+
+template <int DEPTH, bool STATS>
+__device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ nodes,
+                                                    const TriRec* __restrict__ tris, const float3 o,
+                                                    const float3 inv, const RayPre& rp, float thr,
+                                                    float tmax, int* stack, float (&bt)[KBUF],
+                                                    uint32_t (&bi)[KBUF], int (&bo)[KBUF],
+                                                    TraceCounters& ctr) {
+    // The computed t of any reported hit lies within the kz extent of its triangle's
+    // vertices (scaled by Sz) up to a few ulps; boxes are tested against [thr, tmax] on
+    // that slab with a relative margin, and against the full line on all three slabs.
+    const float okz = get3(o, rp.kz);
+    const float lo_t = thr - fabsf(thr) * 9.5367431640625e-7f;    // 2^-20
+    const float hi_t = tmax + fabsf(tmax) * 9.5367431640625e-7f;
+    uint32_t total = 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const NodeBoxes b = load_node(nodes, node);
+        if (STATS) ctr.nodes++;
+        bool h[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float* lo = c == 0 ? b.lo0 : b.lo1;
+            const float* hi = c == 0 ? b.hi0 : b.hi1;
+            const float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+            const float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+            const float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+            const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+            const float klo = rp.kz == 0 ? lo[0] : (rp.kz == 1 ? lo[1] : lo[2]);
+            const float khi = rp.kz == 0 ? hi[0] : (rp.kz == 1 ? hi[1] : hi[2]);
+            const float k0 = (klo - okz) * rp.Sz, k1 = (khi - okz) * rp.Sz;
+            const float kmin = fminf(k0, k1), kmax = fmaxf(k0, k1);
+            h[c] = (tn <= tf + fabsf(tf) * 9.5367431640625e-7f) && kmax >= lo_t && kmin <= hi_t &&
+                   lo[0] <= hi[0];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int cnt = c == 0 ? b.n0 : b.n1;
+            const int first = c == 0 ? b.l0 : b.l1;
+            if (h[c] && cnt > 0) {
+                for (int j = first; j < first + cnt; ++j) {
+                    float3 v0, v1, v2;
+                    uint32_t orig;
+                    int obj;
+                    load_tri(tris, j, v0, v1, v2, orig, obj);
+                    if (STATS) ctr.tris++;
+                    float t;
+                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t <= tmax) {
+                        ++total;
+                        if (STATS) ctr.hits++;
+                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, obj);
+                    }
+                }
+            }
+        }
+        const bool in0 = h[0] && b.n0 == 0;
+        const bool in1 = h[1] && b.n1 == 0;
+        if (in0 && in1) {
+            stack[sp * TR_BLOCK] = b.l1;
+            ++sp;
+            node = b.l0;
+        } else if (in0) {
+            node = b.l0;
+        } else if (in1) {
+            node = b.l1;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * TR_BLOCK];
+        }
+    }
+    return total;
+}
+
+template <int MODE, int DEPTH, bool STATS>
+__global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ nodes,
+                                                    const TriRec* __restrict__ tris, EdgeArgs E,
+                                                    int swap, int32_t* __restrict__ counts,
+                                                    const int64_t* __restrict__ offsets,
+                                                    int64_t offset_bias, void* __restrict__ out,
+                                                    HashArgs H, unsigned long long* __restrict__ stats) {
+    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    const uint64_t e = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
+    TraceCounters ctr;
+    if (e < E.num_edges) {
+        const float3 s = ld3f(E.starts, e);
+        const float3 en = ld3f(E.ends, e);
+        const float eps = 1e-6f;
+        const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
+        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        const int nsrc = E.src_cnt[e];
+        const int soff = E.src_off[e];
+        int results = 0;
+        int64_t cursor = 0;
+        if (MODE == EMODE_WRITE_PAIRS || MODE == EMODE_WRITE_KEYS) cursor = offsets[e] + offset_bias;
+        if (len >= eps) {
+            const float3 d = make_float3(dx / len, dy / len, dz / len);
+            const RayPre rp = ray_pre(d);
+            const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            const float tmax = len + eps;
+            float thr = eps;
+            int iter = 0;
+            float bt[KBUF];
+            uint32_t bi[KBUF];
+            int bo[KBUF];
+            while (true) {
+                kbuf_clear(bt, bi, bo);
+                const uint32_t total = collect_segment<DEPTH, STATS>(
+                    nodes, tris, s, inv, rp, thr, tmax, stack_mem + threadIdx.x, bt, bi, bo, ctr);
+#pragma unroll
+                for (int k = 0; k < KBUF; ++k) {
+                    if ((uint32_t)k < total && iter < 100 && thr <= tmax && bt[k] >= thr) {
+                        const int tgt = bo[k];
+                        for (int q = 0; q < nsrc; ++q) {
+                            const int src = E.src_obj[soff + q];
+                            const uint32_t a = (uint32_t)(swap ? tgt : src);
+                            const uint32_t bb = (uint32_t)(swap ? src : tgt);
+                            if (MODE == EMODE_WRITE_PAIRS) {
+                                rs3d_pair pr;
+                                pr.object_id_mesh1 = (int32_t)a;
+                                pr.object_id_mesh2 = (int32_t)bb;
+                                static_cast<rs3d_pair*>(out)[cursor++] = pr;
+                            } else if (MODE == EMODE_WRITE_KEYS) {
+                                static_cast<uint64_t*>(out)[cursor++] = ((uint64_t)a << 32) | bb;
+                            } else if (MODE == EMODE_HASH) {
+                                ht_insert(H, ((uint64_t)a << 32) | bb);
+                            }
+                        }
+                        results += nsrc;
+                        ++iter;
+                        float nthr = bt[k] + eps;
+                        if (nthr <= bt[k]) nthr = bt[k] + 2e-4f;
+                        thr = nthr;
+                    }
+                }
+                if (total <= (uint32_t)KBUF || iter >= 100 || thr > tmax) break;
+                if (STATS) ctr.restarts++;
+            }
+        }
+        if (MODE == EMODE_COUNT) counts[e] = results;
+    }
+    if (STATS) flush_counters(ctr, stats);
+}
+
+// ------------------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------------------
+static int pick_depth(const rs3d_bvh* bvh, int* depth) {
+    const int need = bvh->max_depth + 1;
+    if (need <= 24) *depth = 24;
+    else if (need <= 32) *depth = 32;
+    else if (need <= 48) *depth = 48;
+    else if (need <= 64) *depth = 64;
+    else {
+        set_error("BVH depth %d exceeds the traversal stack (64)", bvh->max_depth);
+        return RS3D_E_DEPTH;
+    }
+    return RS3D_OK;
+}
+
+template <int AX, bool STATS>
+static int launch_points_ax(rs3d_ctx* ctx, const rs3d_bvh* bvh, int depth, const float* pts,
+                            uint64_t n, const uint32_t* imap, int out_kind, void* out,
+                            unsigned long long* stats) {
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, TR_BLOCK, &nb));
+    switch (depth) {
+        case 24: k_points<AX, 24, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, stats); break;
+        case 32: k_points<AX, 32, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, stats); break;
+        case 48: k_points<AX, 48, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, stats); break;
+        default: k_points<AX, 64, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, stats); break;
+    }
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+template <bool STATS>
+static int launch_points(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, uint64_t n,
+                         const uint32_t* imap, int axis, int out_kind, void* out,
+                         unsigned long long* stats) {
+    int depth;
+    RS3D_TRY(pick_depth(bvh, &depth));
+    switch (axis) {
+        case 0: return launch_points_ax<0, STATS>(ctx, bvh, depth, pts, n, imap, out_kind, out, stats);
+        case 1: return launch_points_ax<1, STATS>(ctx, bvh, depth, pts, n, imap, out_kind, out, stats);
+        default: return launch_points_ax<2, STATS>(ctx, bvh, depth, pts, n, imap, out_kind, out, stats);
+    }
+}
+
+__global__ void k_fill_results_miss(void* out, uint64_t n, int out_kind, const uint32_t* imap) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t pi = imap ? imap[i] : i;
+    if (out_kind == RS3D_OUT_RAY_RESULT) {
+        rs3d_ray_result r;
+        r.ray_id = (int32_t)pi;
+        r.polygon_index = -1;
+        static_cast<rs3d_ray_result*>(out)[pi] = r;
+    } else if (out_kind == RS3D_OUT_FLAG_U8) {
+        static_cast<uint8_t*>(out)[pi] = 0;
+    } else {
+        static_cast<int32_t*>(out)[pi] = -1;
+    }
+}
+
+int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, uint64_t n,
+                          const uint32_t* imap, int axis, int out_kind, void* out,
+                          unsigned long long* stats) {
+    if (n == 0) return RS3D_OK;
+    if (bvh->num_tris == 0) {
+        unsigned nb;
+        RS3D_TRY(grid_blocks(n, 256, &nb));
+        k_fill_results_miss<<<nb, 256, 0, ctx->stream>>>(out, n, out_kind, imap);
+        RS3D_HIP(hipGetLastError());
+        return RS3D_OK;
+    }
+    if (stats) return launch_points<true>(ctx, bvh, pts, n, imap, axis, out_kind, out, stats);
+    if (ctx->profile) {
+        RS3D_TRY(prof_collect(ctx));
+        RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    }
+    RS3D_TRY(launch_points<false>(ctx, bvh, pts, n, imap, axis, out_kind, out, nullptr));
+    if (ctx->profile) {
+        RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        ctx->prof_pending = true;
+    }
+    return RS3D_OK;
+}
+
+template <int MODE>
+static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* edges, int swap,
+                        int32_t* counts, const int64_t* offsets, int64_t bias, void* out,
+                        const HashArgs& H) {
+    if (edges->num_edges == 0) return RS3D_OK;
+    EdgeArgs E{edges->starts, edges->ends, edges->src_offsets, edges->src_counts, edges->src_objects,
+               edges->num_edges};
+    unsigned nb;
+    RS3D_TRY(grid_blocks(edges->num_edges, TR_BLOCK, &nb));
+    if (bvh->num_tris == 0) {
+        if (MODE == EMODE_COUNT)
+            RS3D_HIP(hipMemsetAsync(counts, 0, edges->num_edges * sizeof(int32_t), ctx->stream));
+        return RS3D_OK;
+    }
+    int depth;
+    RS3D_TRY(pick_depth(bvh, &depth));
+    switch (depth) {
+        case 24: k_edges<MODE, 24, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, counts, offsets, bias, out, H, nullptr); break;
+        case 32: k_edges<MODE, 32, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, counts, offsets, bias, out, H, nullptr); break;
+        case 48: k_edges<MODE, 48, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, counts, offsets, bias, out, H, nullptr); break;
+        default: k_edges<MODE, 64, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, counts, offsets, bias, out, H, nullptr); break;
+    }
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+static bool edges_ok(const rs3d_edges* e) {
+    return e && (e->num_edges == 0 ||
+                 (e->starts && e->ends && e->src_offsets && e->src_counts && e->src_objects));
+}
+
+int edge_count_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, int32_t* counts) {
+    HashArgs H{};
+    return launch_edges<EMODE_COUNT>(ctx, t, e, 0, counts, nullptr, 0, nullptr, H);
+}
+int edge_write_keys_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e,
+                             const int64_t* offsets, int64_t bias, int swap, uint64_t* out) {
+    HashArgs H{};
+    return launch_edges<EMODE_WRITE_KEYS>(ctx, t, e, swap, nullptr, offsets, bias, out, H);
+}
+int edge_hash_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, int swap,
+                       const rs3d_hash_table* ht) {
+    HashArgs H{reinterpret_cast<unsigned long long*>(ht->slots), ht->size, ht->bitwise,
+               ht->max_probes, reinterpret_cast<unsigned long long*>(ht->failed)};
+    return launch_edges<EMODE_HASH>(ctx, t, e, swap, nullptr, nullptr, 0, nullptr, H);
+}
+
+}  // namespace rs3d
+
+using namespace rs3d;
+
+extern "C" {
+
+int rs3d_trace_points(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                      const uint32_t* index_map, int axis, int out_kind, void* out) {
+    if (!ctx || !bvh || (n && (!points || !out)) || axis < 0 || axis > 2 || out_kind < 0 ||
+        out_kind > 2) {
+        set_error("rs3d_trace_points: bad arguments");
+        return RS3D_E_ARG;
+    }
+    return trace_points_internal(ctx, bvh, points, n, index_map, axis, out_kind, out, nullptr);
+}
+
+int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                            int axis, rs3d_trace_stats* stats) {
+    if (!ctx || !bvh || !stats || (n && !points) || axis < 0 || axis > 2) {
+        set_error("rs3d_trace_points_stats: bad arguments");
+        return RS3D_E_ARG;
+    }
+    memset(stats, 0, sizeof *stats);
+    stats->rays = n;
+    if (n == 0 || bvh->num_tris == 0) return RS3D_OK;
+    RS3D_TRY(ctx->small.ensure(256));
+    RS3D_TRY(ctx->tmp_a.ensure(n));
+    unsigned long long* dstats = ctx->small.as<unsigned long long>() + 16;
+    RS3D_HIP(hipMemsetAsync(dstats, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    RS3D_TRY(trace_points_internal(ctx, bvh, points, n, nullptr, axis, RS3D_OUT_FLAG_U8,
+                                   ctx->tmp_a.ptr, dstats));
+    unsigned long long h[4];
+    RS3D_HIP(hipMemcpyAsync(h, dstats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    stats->nodes_visited = h[0];
+    stats->tris_tested = h[1];
+    stats->hits = h[2];
+    stats->restarts = h[3];
+    return RS3D_OK;
+}
+
+int rs3d_filter_box(rs3d_ctx* ctx, const float* box, const float* points, uint64_t n, int axis,
+                    uint8_t* flags) {
+    if (!ctx || !box || (n && (!points || !flags)) || axis < 0 || axis > 2) {
+        set_error("rs3d_filter_box: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (n == 0) return RS3D_OK;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    if (axis == 0)
+        k_filter_box<0><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+    else if (axis == 1)
+        k_filter_box<1><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+    else
+        k_filter_box<2><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                       int axis, int32_t* out_obj, uint64_t* candidate_count,
+                       uint64_t* inside_count) {
+    if (!ctx || !bvh || (n && (!points || !out_obj)) || axis < 0 || axis > 2 ||
+        n > 0xffffffffull) {
+        set_error("rs3d_filter_refine: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (candidate_count) *candidate_count = 0;
+    if (inside_count) *inside_count = 0;
+    if (n == 0) return RS3D_OK;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_fill_i32<<<nb, 256, 0, ctx->stream>>>(out_obj, n, -1);
+    RS3D_HIP(hipGetLastError());
+    if (bvh->num_tris == 0) return RS3D_OK;
+    // filter flags live in tmp_c; compact_flags uses tmp_b for positions
+    RS3D_TRY(ctx->tmp_c.ensure(n + n * sizeof(uint32_t) + 256));
+    uint8_t* flags = ctx->tmp_c.as<uint8_t>();
+    uint32_t* cand = reinterpret_cast<uint32_t*>(ctx->tmp_c.as<uint8_t>() + ((n + 255) & ~255ull));
+    const float box[6] = {bvh->vmin[0], bvh->vmin[1], bvh->vmin[2],
+                          bvh->vmax[0], bvh->vmax[1], bvh->vmax[2]};
+    RS3D_TRY(rs3d_filter_box(ctx, box, points, n, axis, flags));
+    uint64_t nc = 0;
+    RS3D_TRY(rs3d_compact_flags(ctx, flags, n, cand, &nc));
+    if (candidate_count) *candidate_count = nc;
+    if (nc == 0) return RS3D_OK;
+    RS3D_TRY(trace_points_internal(ctx, bvh, points, nc, cand, axis, RS3D_OUT_OBJ_I32, out_obj, nullptr));
+    if (inside_count) {
+        RS3D_TRY(ctx->small.ensure(256));
+        unsigned long long* cnt = ctx->small.as<unsigned long long>() + 24;
+        RS3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
+        k_count_nonneg<<<std::min(nb, 4096u), 256, 0, ctx->stream>>>(out_obj, n, cnt);
+        RS3D_HIP(hipGetLastError());
+        uint64_t h = 0;
+        RS3D_TRY(read_u64(ctx, cnt, &h));
+        *inside_count = h;
+    }
+    return RS3D_OK;
+}
+
+int rs3d_edge_count(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int32_t* counts) {
+    if (!ctx || !target || !edges_ok(edges) || (edges->num_edges && !counts)) {
+        set_error("rs3d_edge_count: bad arguments");
+        return RS3D_E_ARG;
+    }
+    return edge_count_internal(ctx, target, edges, counts);
+}
+
+int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
+                    const int64_t* offsets, int swap, rs3d_pair* out) {
+    if (!ctx || !target || !edges_ok(edges) || (edges->num_edges && (!offsets || !out))) {
+        set_error("rs3d_edge_write: bad arguments");
+        return RS3D_E_ARG;
+    }
+    HashArgs H{};
+    return launch_edges<EMODE_WRITE_PAIRS>(ctx, target, edges, swap, nullptr, offsets, 0, out, H);
+}
+
+int rs3d_edge_hash(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int swap,
+                   rs3d_hash_table* ht) {
+    if (!ctx || !target || !edges_ok(edges) || !ht || !ht->slots || ht->size == 0 || !ht->failed ||
+        (ht->bitwise && (ht->size & (ht->size - 1)))) {
+        set_error("rs3d_edge_hash: bad arguments");
+        return RS3D_E_ARG;
+    }
+    return edge_hash_internal(ctx, target, edges, swap, ht);
+}
+
+}  // extern "C"

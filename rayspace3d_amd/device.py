@@ -1,0 +1,216 @@
+"""Python mirror of the join's host interfaces over librs3d (device arrays are torch
+tensors on a ROCm device; torch only provides HBM allocations and host<->device copies).
+
+Names follow the reference's host classes:
+  Context                <- OptixContext          (query/src/optix/OptixContext.cpp:4-8)
+  Bvh                    <- OptixAccelerationStructure (query/src/optix/OptixAccelerationStructure.h:9-34)
+  RayLauncher            <- RayLauncher           (query/src/raytracing/RayLauncher.h:9-40)
+  FilterRefinePipeline   <- FilterRefinePipeline  (query/src/raytracing/FilterRefine.h:27-48)
+  overlap_join           <- executeTwoPassQueryEdgesOptimized / the hash query
+                            (query/src/applications/rayspace_mesh_overlap_main.cpp:42-146)
+Every call goes to the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+_TORCH_OF = {
+    np.dtype(np.float32): torch.float32,
+    np.dtype(np.int32): torch.int32,
+    np.dtype(np.uint32): torch.int32,
+    np.dtype(np.int64): torch.int64,
+    np.dtype(np.uint64): torch.int64,
+    np.dtype(np.uint8): torch.uint8,
+}
+
+
+def to_dev(a: np.ndarray, device="cuda") -> torch.Tensor:
+    """Copies a numpy array to HBM (unsigned types travel as their signed bit patterns)."""
+    a = np.ascontiguousarray(a)
+    view = a
+    if a.dtype == np.uint32:
+        view = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        view = a.view(np.int64)
+    return torch.from_numpy(view).to(device)
+
+
+def to_host(t: torch.Tensor, dtype=None) -> np.ndarray:
+    a = t.detach().cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.handle = C.c_void_p()
+        torch.cuda.init()
+        check(self.lib.rs3d_ctx_create(device, C.byref(self.handle)), "rs3d_ctx_create")
+        self.device = device
+
+    def sync(self):
+        check(self.lib.rs3d_ctx_sync(self.handle), "rs3d_ctx_sync")
+
+    def profile(self, enable: bool):
+        check(self.lib.rs3d_ctx_profile(self.handle, int(enable)), "rs3d_ctx_profile")
+
+    def profile_read(self):
+        ms = C.c_double()
+        n = C.c_uint64()
+        check(self.lib.rs3d_ctx_profile_read(self.handle, C.byref(ms), C.byref(n)),
+              "rs3d_ctx_profile_read")
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            self.lib.rs3d_ctx_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Bvh:
+    """Linear BVH over a device-resident indexed mesh."""
+
+    def __init__(self, ctx: Context, vertices: torch.Tensor, indices: torch.Tensor,
+                 tri_to_obj: torch.Tensor | None, leaf_size: int = 4):
+        self.ctx = ctx
+        self.handle = C.c_void_p()
+        opts = _lib.BvhOpts()
+        opts.leaf_size = leaf_size
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_bvh_build(ctx.handle, _ptr(vertices), vertices.shape[0], _ptr(indices),
+                                     indices.shape[0], _ptr(tri_to_obj), C.byref(opts),
+                                     C.byref(self.handle)), "rs3d_bvh_build")
+        # keep the source arrays alive with the structure
+        self._src = (vertices, indices, tri_to_obj)
+
+    @classmethod
+    def from_mesh(cls, ctx: Context, mesh, leaf_size: int = 4) -> "Bvh":
+        return cls(ctx, to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj),
+                   leaf_size)
+
+    def info(self) -> _lib.BvhInfo:
+        inf = _lib.BvhInfo()
+        check(self.ctx.lib.rs3d_bvh_info_get(self.handle, C.byref(inf)), "rs3d_bvh_info_get")
+        return inf
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.rs3d_bvh_free(self.handle)
+            self.handle = C.c_void_p()
+
+
+def trace_points(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2,
+                 out_kind: int = _lib.OUT_OBJ_I32, index_map: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None, sync: bool = True) -> torch.Tensor:
+    n = points.shape[0]
+    if out is None:
+        if out_kind == _lib.OUT_FLAG_U8:
+            out = torch.empty(n, dtype=torch.uint8, device=points.device)
+        elif out_kind == _lib.OUT_OBJ_I32:
+            out = torch.empty(n, dtype=torch.int32, device=points.device)
+        else:
+            out = torch.empty((n, 2), dtype=torch.int32, device=points.device)
+    nrays = index_map.shape[0] if index_map is not None else n
+    if sync:
+        torch.cuda.synchronize()
+    check(ctx.lib.rs3d_trace_points(ctx.handle, bvh.handle, _ptr(points), nrays, _ptr(index_map),
+                                    axis, out_kind, _ptr(out)), "rs3d_trace_points")
+    if sync:
+        ctx.sync()
+    return out
+
+
+def trace_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> dict:
+    st = _lib.TraceStats()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_trace_points_stats(ctx.handle, bvh.handle, _ptr(points), points.shape[0],
+                                          axis, C.byref(st)), "rs3d_trace_points_stats")
+    return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+
+@dataclass
+class FilterRefineResult:
+    final_results: torch.Tensor   # int32 per point, -1 outside
+    candidate_count: int
+    inside_count: int
+
+
+class FilterRefinePipeline:
+    """Dataset-box filter then exact containment (query/src/raytracing/FilterRefine.cpp:81-122)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def execute(self, bvh: Bvh, points: torch.Tensor, axis: int = 2,
+                out: torch.Tensor | None = None, sync: bool = True) -> FilterRefineResult:
+        n = points.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=points.device)
+        cand = C.c_uint64()
+        inside = C.c_uint64()
+        if sync:
+            torch.cuda.synchronize()
+        check(self.ctx.lib.rs3d_filter_refine(self.ctx.handle, bvh.handle, _ptr(points), n, axis,
+                                              _ptr(out), C.byref(cand), C.byref(inside)),
+              "rs3d_filter_refine")
+        return FilterRefineResult(out, cand.value, inside.value)
+
+
+class DevEdges:
+    """Device copy of an EdgeSet plus the C struct that points into it."""
+
+    def __init__(self, edges):
+        self.t = [to_dev(edges.starts), to_dev(edges.ends), to_dev(edges.src_offsets),
+                  to_dev(edges.src_counts), to_dev(edges.src_objects if len(edges.src_objects)
+                                                    else np.zeros(1, np.int32))]
+        self.s = _lib.Edges(self.t[0].data_ptr(), self.t[1].data_ptr(), self.t[2].data_ptr(),
+                            self.t[3].data_ptr(), self.t[4].data_ptr(), edges.num_edges)
+
+
+def edge_count(ctx, bvh, dedges: DevEdges) -> torch.Tensor:
+    counts = torch.empty(max(dedges.s.num_edges, 1), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_edge_count(ctx.handle, bvh.handle, C.byref(dedges.s), _ptr(counts)),
+          "rs3d_edge_count")
+    ctx.sync()
+    return counts[:dedges.s.num_edges]
+
+
+def overlap_join(ctx, bvh1, dedges1, bvh2, dedges2, mode: int = 0, max_out: int = 1 << 22) -> np.ndarray:
+    """Sorted unique (mesh1, mesh2) keys of both edge directions (uint64 a << 32 | b)."""
+    while True:
+        out = torch.empty(max(max_out, 1), dtype=torch.int64, device="cuda")
+        cnt = C.c_uint64()
+        torch.cuda.synchronize()
+        rc = ctx.lib.rs3d_overlap_join(ctx.handle, bvh1.handle, C.byref(dedges1.s), bvh2.handle,
+                                       C.byref(dedges2.s), mode, _ptr(out), max_out, C.byref(cnt))
+        if rc == _lib.RS3D_E_CAPACITY and cnt.value > max_out:
+            max_out = int(cnt.value)
+            continue
+        check(rc, "rs3d_overlap_join")
+        return to_host(out[:cnt.value], np.uint64)
+
+
+def sort_unique_u64(ctx, keys: torch.Tensor) -> torch.Tensor:
+    cnt = C.c_uint64()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_sort_unique_u64(ctx.handle, _ptr(keys), keys.shape[0], C.byref(cnt)),
+          "rs3d_sort_unique_u64")
+    return keys[:cnt.value]

@@ -1,0 +1,223 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle, bit-exact for
+all integer outputs (flags, object ids, pair sets, counts)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+from rayspace3d_amd import _lib, gen
+from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, edge_count, overlap_join,
+                                   sort_unique_u64, to_dev, to_host, trace_points, trace_stats)
+
+pytestmark = pytest.mark.gpu
+
+
+def _contain(ctx, mesh, pts, axis, leaf=4, kind=_lib.OUT_OBJ_I32):
+    bvh = Bvh.from_mesh(ctx, mesh, leaf_size=leaf)
+    out = to_host(trace_points(ctx, bvh, to_dev(pts), axis=axis, out_kind=kind))
+    info = bvh.info()
+    bvh.free()
+    return out, info
+
+
+@pytest.mark.parametrize("axis", [2, 0, 1])
+def test_single_sphere_points(ctx, axis):
+    mesh, pts = gen.single_sphere_workload(seed=11, n_points=200_000)
+    print(f"single sphere L5, 200k points, axis {axis}", flush=True)
+    out, info = _contain(ctx, mesh, pts, axis)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=axis)
+    assert info.num_triangles == 20480 and info.num_nodes == 20479
+    assert np.array_equal(out, ref), f"{np.count_nonzero(out != ref)} mismatches"
+    frac = (out >= 0).mean()
+    assert 0.12 < frac < 0.19
+
+
+def test_output_kinds_agree(ctx):
+    mesh, pts = gen.single_sphere_workload(seed=12, n_points=50_000, level=4)
+    obj, _ = _contain(ctx, mesh, pts, 2, kind=_lib.OUT_OBJ_I32)
+    flag, _ = _contain(ctx, mesh, pts, 2, kind=_lib.OUT_FLAG_U8)
+    rr, _ = _contain(ctx, mesh, pts, 2, kind=_lib.OUT_RAY_RESULT)
+    assert np.array_equal(flag.astype(bool), obj >= 0)
+    assert np.array_equal(rr[:, 0], np.arange(len(pts)))
+    assert np.array_equal(rr[:, 1], obj)
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_many_spheres_points_leaf_invariant(ctx, leaf):
+    mesh, pts = gen.many_spheres_workload(seed=21, n_objects=2000, n_points=300_000, world=200.0)
+    print(f"2000 spheres in 200^3, 300k points, leaf {leaf}", flush=True)
+    out, _ = _contain(ctx, mesh, pts, 2, leaf=leaf)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    assert np.array_equal(out, ref), f"{np.count_nonzero(out != ref)} mismatches"
+    assert (out >= 0).sum() > 100
+
+
+def test_hit_buffer_restart(ctx):
+    # 40 concentric spheres: +z rays cross up to 80 surfaces (> the 8-entry hit buffer)
+    n = 40
+    mesh = gen.spheres(2, np.zeros((n, 3)), np.linspace(1.0, 3.0, n))
+    rng = np.random.Generator(np.random.PCG64(5))
+    pts = gen.uniform_points(rng, 20000, -3.2, 3.2)
+    out, _ = _contain(ctx, mesh, pts, 2)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    assert np.array_equal(out, ref)
+    st = trace_stats(ctx, Bvh.from_mesh(ctx, mesh), to_dev(pts), 2)
+    assert st["restarts"] > 0
+
+
+def test_degenerate_sizes(ctx):
+    # one triangle, zero points, empty mesh
+    V = np.array([[0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
+    m = gen.Mesh(V, np.array([[0, 1, 2]], np.uint32), np.array([7], np.int32), 1)
+    pts = np.array([[0.2, 0.2, 0.0], [0.9, 0.9, 0.0], [0.2, 0.2, 2.0]], np.float32)
+    out, info = _contain(ctx, m, pts, 2)
+    assert list(out) == [7, -1, -1] and info.num_triangles == 1
+    out0, _ = _contain(ctx, m, np.zeros((0, 3), np.float32), 2)
+    assert out0.shape == (0,)
+    empty = gen.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.int32))
+    oute, _ = _contain(ctx, empty, pts, 2)
+    assert list(oute) == [-1, -1, -1]
+
+
+def test_filter_refine(ctx):
+    mesh, pts = gen.many_spheres_workload(seed=31, n_objects=1500, n_points=200_000, world=150.0)
+    # points beyond the dataset box on every side
+    rng = np.random.Generator(np.random.PCG64(3))
+    pts = np.concatenate([pts, gen.uniform_points(rng, 20000, -30.0, 180.0)])
+    bvh = Bvh.from_mesh(ctx, mesh)
+    res = FilterRefinePipeline(ctx).execute(bvh, to_dev(pts), axis=2)
+    out = to_host(res.final_results)
+    bvh.free()
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    assert np.array_equal(out, ref)
+    assert res.inside_count == int((ref >= 0).sum())
+    vmin, vmax = mesh.vertices.min(0), mesh.vertices.max(0)
+    in_box = np.all((pts > vmin) & (pts < vmax), axis=1)
+    assert res.candidate_count == int(in_box.sum())
+
+
+def test_scan_compact_sort(ctx):
+    L = ctx.lib
+    rng = np.random.Generator(np.random.PCG64(9))
+    for n in [0, 1, 5, 2047, 2048, 2049, 3_000_001]:
+        x = rng.integers(0, 5, size=n).astype(np.int32)
+        dx = to_dev(x) if n else torch.zeros(1, dtype=torch.int32, device="cuda")
+        out = torch.zeros(max(n, 1), dtype=torch.int64, device="cuda")
+        tot = C.c_int64()
+        torch.cuda.synchronize()
+        _lib.check(L.rs3d_scan_i32_i64(ctx.handle, C.c_void_p(dx.data_ptr()),
+                                       C.c_void_p(out.data_ptr()), n, C.byref(tot)), "scan")
+        exp = np.concatenate([[0], np.cumsum(x, dtype=np.int64)])
+        assert tot.value == exp[-1]
+        assert np.array_equal(to_host(out)[:n], exp[:n])
+        f = (x > 2).astype(np.uint8)
+        df = to_dev(f) if n else torch.zeros(1, dtype=torch.uint8, device="cuda")
+        idx = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+        cnt = C.c_uint64()
+        torch.cuda.synchronize()
+        _lib.check(L.rs3d_compact_flags(ctx.handle, C.c_void_p(df.data_ptr()), n,
+                                        C.c_void_p(idx.data_ptr()), C.byref(cnt)), "compact")
+        assert np.array_equal(to_host(idx)[:cnt.value], np.nonzero(f)[0].astype(np.int32))
+    for n in [1, 100, 70_000, 2_500_000]:
+        keys = rng.integers(0, 1 << 40, size=n, dtype=np.uint64)
+        keys[: n // 3] = keys[n // 3: 2 * (n // 3)]  # duplicates
+        got = to_host(sort_unique_u64(ctx, to_dev(keys)), np.uint64)
+        assert np.array_equal(got, np.unique(keys))
+        k32 = rng.integers(0, 1 << 30, size=n, dtype=np.uint32)
+        v32 = np.arange(n, dtype=np.uint32)
+        dk, dv = to_dev(k32), to_dev(v32)
+        torch.cuda.synchronize()
+        _lib.check(L.rs3d_sort_pairs_u32(ctx.handle, C.c_void_p(dk.data_ptr()),
+                                         C.c_void_p(dv.data_ptr()), n, 30), "sort_pairs")
+        ctx.sync()
+        perm = np.argsort(k32, kind="stable")
+        assert np.array_equal(to_host(dk, np.uint32), k32[perm])
+        assert np.array_equal(to_host(dv, np.uint32), v32[perm])
+
+
+def _overlap_case(seed, n, world):
+    a = gen.overlap_dataset(seed, n_objects=n, world=world, levels=(2, 3))
+    b = gen.overlap_dataset(seed + 1, n_objects=n, world=world, levels=(2, 3))
+    a.edges = gen.extract_edges(a)
+    b.edges = gen.extract_edges(b)
+    return a, b
+
+
+def test_edge_counts_and_overlap_join(ctx):
+    a, b = _overlap_case(41, 300, 120.0)
+    print(f"overlap 300 x 300 objects, {a.edges.num_edges} + {b.edges.num_edges} edges", flush=True)
+    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    ea, eb = DevEdges(a.edges), DevEdges(b.edges)
+    keys, counts = orc.edge_pairs(a.edges, b.vertices, b.indices, b.tri_to_obj, with_counts=True)
+    got_counts = to_host(edge_count(ctx, bb, ea))
+    assert np.array_equal(got_counts, counts)
+    ref = orc.overlap_pairs(a, a.edges, b, b.edges)
+    assert len(ref) > 50
+    for mode in (0, 1):
+        got = overlap_join(ctx, ba, ea, bb, eb, mode=mode)
+        assert np.array_equal(got, ref), f"mode {mode}: {len(got)} vs {len(ref)} pairs"
+    ba.free()
+    bb.free()
+
+
+def test_reference_named_dedup(ctx):
+    L = ctx.lib
+    rng = np.random.Generator(np.random.PCG64(4))
+    p1 = rng.integers(0, 50, size=(1000, 2)).astype(np.int32)
+    p2 = rng.integers(0, 50, size=(700, 2)).astype(np.int32)
+    d1, d2 = to_dev(p1), to_dev(p2)
+    out = torch.zeros((1700, 2), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    n = L.merge_and_deduplicate_pairs_gpu(C.c_void_p(d1.data_ptr()), 1000, C.c_void_p(d2.data_ptr()),
+                                          700, C.c_void_p(out.data_ptr()))
+    got = to_host(out)[:n]
+    allp = np.concatenate([p1, p2])
+    # reference order: u64 of the little-endian struct = (b << 32 | a)
+    key = (allp[:, 1].astype(np.uint64) << np.uint64(32)) | allp[:, 0].astype(np.uint64)
+    uk = np.unique(key)
+    exp = np.stack([(uk & np.uint64(0xffffffff)).astype(np.int32), (uk >> np.uint64(32)).astype(np.int32)], 1)
+    assert np.array_equal(got, exp)
+    x = rng.integers(0, 9, size=5000).astype(np.int32)
+    dx = to_dev(x)
+    dout = torch.zeros(5000, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    tot = L.exclusive_scan_gpu(C.c_void_p(dx.data_ptr()), C.c_void_p(dout.data_ptr()), 5000)
+    assert tot == int(x.sum())
+    assert np.array_equal(to_host(dout)[1:], np.cumsum(x)[:-1])
+
+
+def test_hash_table_compact(ctx):
+    L = ctx.lib
+    size = 1 << 12
+    slots = torch.full((size,), -1, dtype=torch.int64, device="cuda")
+    keys = np.unique(np.random.Generator(np.random.PCG64(8)).integers(0, 1 << 40, 900, dtype=np.uint64))
+    table = np.full(size, np.uint64(0xFFFFFFFFFFFFFFFF))
+    for k in keys:
+        assert orc.ht_insert(table, True, 5000, int(k))
+    slots.copy_(to_dev(table))
+    out = torch.zeros((size, 2), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    n = L.compact_hash_table_pairs(C.c_void_p(slots.data_ptr()), size, C.c_void_p(out.data_ptr()), size)
+    got = to_host(out)[:n]
+    gk = (got[:, 0].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | got[:, 1].astype(np.uint32).astype(np.uint64)
+    # slot order: exactly the occupied slots of the oracle table, in slot order
+    assert np.array_equal(gk, table[table != np.uint64(0xFFFFFFFFFFFFFFFF)])
+    m = L.compact_hash_table_pairs(C.c_void_p(slots.data_ptr()), size, C.c_void_p(out.data_ptr()), 10)
+    assert m == 10
+
+
+def test_estimate_matches_oracle(ctx):
+    a = gen.overlap_dataset(3, n_objects=400, world=100.0, levels=(2,))
+    b = gen.overlap_dataset(4, n_objects=400, world=100.0, levels=(2,))
+    ca, mn, mx = orc.grid_stats(a.vertices, a.indices, a.tri_to_obj, 16, world_size=100.0)
+    cb, _, _ = orc.grid_stats(b.vertices, b.indices, b.tri_to_obj, 16, world_size=100.0)
+    cv = float(np.prod((mx - mn) / 16))
+    ref = orc.estimate_raw(ca, cb, cv)
+    dA, dB = to_dev(ca.view(np.uint8)), to_dev(cb.view(np.uint8))
+    r = C.c_double()
+    torch.cuda.synchronize()
+    _lib.check(ctx.lib.rs3d_estimate_raw(ctx.handle, C.c_void_p(dA.data_ptr()), C.c_void_p(dB.data_ptr()),
+                                         len(ca), cv, 0.001, 0.8, C.byref(r)), "estimate")
+    assert ref > 0 and abs(r.value - ref) <= 1e-5 * ref
