@@ -181,6 +181,61 @@ int rs3d_overlap_join(rs3d_ctx* ctx, const rs3d_bvh* bvh1, const rs3d_edges* edg
                       const rs3d_bvh* bvh2, const rs3d_edges* edges2, int mode,
                       uint64_t* keys_out, uint64_t max_out, uint64_t* count);
 
+/* Edge-ray hash insert with an explicit chain rule and iteration cap:
+ * RS3D_CHAIN_EPS (A->B overlap, query/src/cuda/mesh_overlap_edges.cu:128-178 and the
+ * intersection module's trace_edge_multi_hits, query/src/cuda/mesh_intersection.cu:16-91);
+ * RS3D_CHAIN_NEXTAFTER (containment join edge check, query/src/cuda/mesh_containment.cu:76-172). */
+enum { RS3D_CHAIN_EPS = 0, RS3D_CHAIN_NEXTAFTER = 1 };
+int rs3d_edge_hash_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int swap,
+                      int chain_rule, int max_iterations, rs3d_hash_table* ht);
+
+/* A mesh as the device arrays of the .pre payload (needed where rays start on the
+ * source mesh itself). num_objects: dense object ids 0..num_objects-1. */
+typedef struct {
+    const float* vertices;     /* float3[num_vertices] */
+    uint64_t num_vertices;
+    const uint32_t* indices;   /* uint3[num_triangles] */
+    uint64_t num_triangles;
+    const int32_t* tri_to_obj; /* int32[num_triangles] */
+    uint64_t num_objects;
+} rs3d_mesh_view;
+
+/* Containment fallback of the intersection join (replaces __raygen__mesh_containment_anyhit,
+ * query/src/cuda/mesh_intersection_anyhit.cu:46-186): one +z ray per source object from
+ * the first vertex of its first triangle; every target object crossed an odd number of
+ * times yields (src, tgt), or (tgt, src) when swap, inserted into ht. max_targets: distinct
+ * targets tracked per ray (reference: 256); *overflow (host) counts targets beyond it. */
+int rs3d_containment_fallback(rs3d_ctx* ctx, const rs3d_mesh_view* src, const rs3d_bvh* target,
+                              int swap, int max_targets, rs3d_hash_table* ht, uint64_t* overflow);
+
+/* Point-in-mesh phase of the containment join (replaces __raygen__point_in_mesh,
+ * query/src/cuda/mesh_containment.cu:258-377): for each B object, A objects containing its
+ * first vertex give (a, b), kept when (a, b) is not in overlap_ht. Reference max_targets: 512. */
+int rs3d_point_in_mesh_join(rs3d_ctx* ctx, const rs3d_mesh_view* src_b, const rs3d_bvh* target_a,
+                            int max_targets, const rs3d_hash_table* overlap_ht,
+                            rs3d_hash_table* out_ht, uint64_t* overflow);
+
+/* Whole intersection join (raytracer_intersection_estimated / raytracer_mesh_intersection,
+ * query/src/applications/raytracer_intersection_estimated_main.cpp:77-176): overlap edge
+ * rays 1->2 and 2->1 (cap overlap_max_iterations, reference default 100) plus the
+ * containment fallback both ways, deduplicated in one table (table_slots 0 = automatic;
+ * grown and rerun whenever an insert fails), sorted unique keys (a << 32 | b). */
+int rs3d_intersection_join(rs3d_ctx* ctx, const rs3d_mesh_view* m1, const rs3d_bvh* b1,
+                           const rs3d_edges* e1, const rs3d_mesh_view* m2, const rs3d_bvh* b2,
+                           const rs3d_edges* e2, int overlap_max_iterations, uint64_t table_slots,
+                           uint64_t* keys_out, uint64_t max_out, uint64_t* count,
+                           uint64_t* overflow);
+
+/* Whole containment join (raytracer_containment, query/src/applications/
+ * raytracer_containment_main.cpp:247-357): overlap set from B->A and A->B edge rays
+ * (nextafter chain, cap 256), then strict containment pairs (a, b) from B's first vertices.
+ * contain_out / overlap_out: sorted unique keys (a << 32 | b); overlap_out may be NULL. */
+int rs3d_containment_join(rs3d_ctx* ctx, const rs3d_mesh_view* ma, const rs3d_bvh* ba,
+                          const rs3d_edges* ea, const rs3d_mesh_view* mb, const rs3d_bvh* bb,
+                          const rs3d_edges* eb, uint64_t table_slots, uint64_t* contain_out,
+                          uint64_t max_contain, uint64_t* n_contain, uint64_t* overlap_out,
+                          uint64_t max_overlap, uint64_t* n_overlap, uint64_t* overflow);
+
 /* ---- selectivity estimation (replaces estimateOverlapSelectivity,
  *      query/src/cuda/estimated_overlap.cu:43-77) --------------------------------------
  * cells: GridCell[num_cells] (device). Deterministic reduction. result: host. */

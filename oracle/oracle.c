@@ -316,6 +316,17 @@ int64_t orc_edge_pairs(const float* starts, const float* ends, const int32_t* sr
                        const float* V, int64_t nV, const uint32_t* I, int64_t nT,
                        const int32_t* tri_obj, int swap, int use_grid, uint64_t* out_keys,
                        int64_t cap, int32_t* per_edge_counts) {
+    return orc_edge_pairs_ex(starts, ends, src_off, src_cnt, src_obj, nE, V, nV, I, nT, tri_obj,
+                             swap, use_grid, 0, 100, out_keys, cap, per_edge_counts);
+}
+
+/* chain_rule 0: fl(t + 1e-6), or t + 2e-4 when absorbed (mesh_overlap_edges.cu:173-177).
+ * chain_rule 1: nextafterf(t, tmax) (mesh_containment.cu:150-160). max_iter: chain cap. */
+int64_t orc_edge_pairs_ex(const float* starts, const float* ends, const int32_t* src_off,
+                          const int32_t* src_cnt, const int32_t* src_obj, int64_t nE,
+                          const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                          const int32_t* tri_obj, int swap, int use_grid, int chain_rule,
+                          int max_iter, uint64_t* out_keys, int64_t cap, int32_t* per_edge_counts) {
     (void)nV;
     const float eps = 1e-6f;
     grid_t G;
@@ -374,7 +385,7 @@ int64_t orc_edge_pairs(const float* starts, const float* ends, const int32_t* sr
             qsort(hv.h, (size_t)hv.n, sizeof(hit_t), hit_cmp);
             float thr = eps;
             int64_t j = 0;
-            for (int iter = 0; iter < 100; ++iter) {
+            for (int iter = 0; iter < max_iter; ++iter) {
                 if (thr > tmax) break;
                 while (j < hv.n && hv.h[j].t < thr) ++j;
                 if (j == hv.n) break;
@@ -388,8 +399,14 @@ int64_t orc_edge_pairs(const float* starts, const float* ends, const int32_t* sr
                     ++total;
                     ++nres;
                 }
-                float nthr = t + eps;
-                if (nthr <= t) nthr = t + 2e-4f;
+                float nthr;
+                if (chain_rule == 0) {
+                    nthr = t + eps;
+                    if (nthr <= t) nthr = t + 2e-4f;
+                } else {
+                    nthr = nextafterf(t, tmax);
+                    if (nthr <= t) nthr = t + eps;
+                }
                 thr = nthr;
                 ++j;
             }
@@ -708,4 +725,83 @@ int64_t orc_extract_edges(const float* V, int64_t nV, const uint32_t* I, int64_t
     free(recs);
     if (n_sources) *n_sources = ns;
     return ne;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* A.3 / A.4 per-object containment rays (query/src/cuda/mesh_intersection_anyhit.cu:      */
+/* 46-240; query/src/cuda/mesh_containment.cu:196-377)                                   */
+/* ------------------------------------------------------------------------------------ */
+
+int64_t orc_object_parity(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
+                          int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
+                          const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
+                          int swap, uint64_t* out_keys, int64_t cap, int64_t* overflow) {
+    (void)nV;
+    const float d[3] = {0.0f, 0.0f, 1.0f};
+    orc_raypre rp;
+    orc_ray_pre(d, &rp);
+    int32_t* first = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_src_obj > 0 ? n_src_obj : 1));
+    for (int64_t o = 0; o < n_src_obj; ++o) first[o] = -1;
+    for (int64_t t = 0; t < src_nT; ++t) {
+        const int32_t o = src_tri_obj[t];
+        if (o >= 0 && o < n_src_obj && first[o] < 0) first[o] = (int32_t)t;
+    }
+    int32_t* ids = (int32_t*)malloc(sizeof(int32_t) * (size_t)(max_targets > 0 ? max_targets : 1));
+    unsigned char* par = (unsigned char*)malloc((size_t)(max_targets > 0 ? max_targets : 1));
+    grid_t G;
+    const int axes[2] = {0, 1};
+    if (nT > 0) grid_build(&G, 2, axes, V, I, nT, 1.0);
+    hitvec hv = {0, 0, 0};
+    int64_t total = 0, over = 0;
+    for (int64_t s = 0; s < n_src_obj; ++s) {
+        if (first[s] < 0 || nT == 0) continue;
+        const float* o = srcV + 3 * (int64_t)srcI[3 * (int64_t)first[s]];
+        hv.n = 0;
+        const int cu = cell_coord(&G, 0, o[0]), cv = cell_coord(&G, 1, o[1]);
+        if (cu >= 0 && cu < G.g[0] && cv >= 0 && cv < G.g[1]) {
+            const int64_t c = cu + (int64_t)G.g[0] * cv;
+            for (int64_t k = G.start[c]; k < G.start[c + 1]; ++k)
+                test_tri_collect(V, I, G.items[k], o, &rp, 1e-4f, 1e10f, 1, &hv);
+        }
+        qsort(hv.h, (size_t)hv.n, sizeof(hit_t), hit_cmp);
+        int count = 0, has_last = 0, last_obj = -1;
+        float last_t = 0.0f;
+        for (int64_t j = 0; j < hv.n; ++j) {
+            const float t = hv.h[j].t;
+            const int obj = tri_obj[hv.h[j].tri];
+            const int same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
+            has_last = 1;
+            last_obj = obj;
+            last_t = t;
+            if (same) continue;
+            int pos = -1;
+            for (int q = 0; q < count; ++q)
+                if (ids[q] == obj) {
+                    pos = q;
+                    break;
+                }
+            if (pos >= 0)
+                par[pos] ^= 1;
+            else if (count < max_targets) {
+                ids[count] = obj;
+                par[count] = 1;
+                ++count;
+            } else
+                ++over;
+        }
+        for (int q = 0; q < count; ++q) {
+            if (!(par[q] & 1)) continue;
+            const uint32_t a = (uint32_t)(swap ? ids[q] : (int32_t)s);
+            const uint32_t b = (uint32_t)(swap ? (int32_t)s : ids[q]);
+            if (total < cap) out_keys[total] = ((uint64_t)a << 32) | b;
+            ++total;
+        }
+    }
+    free(hv.h);
+    free(first);
+    free(ids);
+    free(par);
+    if (nT > 0) grid_free(&G);
+    if (overflow) *overflow = over;
+    return total;
 }

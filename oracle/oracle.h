@@ -52,6 +52,23 @@ int64_t orc_edge_pairs(const float* starts, const float* ends, const int32_t* sr
                        const int32_t* tri_obj, int swap, int use_grid, uint64_t* out_keys,
                        int64_t cap, int32_t* per_edge_counts);
 
+/* As orc_edge_pairs with an explicit chain rule (0: t + 1e-6 / 2e-4, 1: nextafterf) and
+ * iteration cap. */
+int64_t orc_edge_pairs_ex(const float* starts, const float* ends, const int32_t* src_off,
+                          const int32_t* src_cnt, const int32_t* src_obj, int64_t nE,
+                          const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                          const int32_t* tri_obj, int swap, int use_grid, int chain_rule,
+                          int max_iter, uint64_t* out_keys, int64_t cap, int32_t* per_edge_counts);
+
+/* A.3/A.4 per-object +z rays over [1e-4, 1e10] from the first vertex of each source
+ * object's first triangle; hits walked in (t, tri) order, same-object hits within 1e-5 of
+ * the previous hit skipped, per-target parity over at most max_targets targets. Emits
+ * (src << 32 | tgt), or (tgt << 32 | src) when swap, for odd targets. */
+int64_t orc_object_parity(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
+                          int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
+                          const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
+                          int swap, uint64_t* out_keys, int64_t cap, int64_t* overflow);
+
 /* Sorts ascending and removes duplicates in place; returns the unique count. */
 int64_t orc_sort_unique_u64(uint64_t* keys, int64_t n);
 

@@ -19,6 +19,10 @@ _sig = {
     "orc_points_contain": (_I64, [_P, _I64, _P, _I64, _P, _P, _I64, C.c_int, C.c_int, _P]),
     "orc_edge_pairs": (_I64, [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
                               _P, _I64, _P]),
+    "orc_edge_pairs_ex": (_I64, [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, _P, _I64, _P]),
+    "orc_object_parity": (_I64, [_P, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
+                                 _P, _I64, _P]),
     "orc_sort_unique_u64": (_I64, [_P, _I64]),
     "orc_fmix64": (C.c_uint64, [C.c_uint64]),
     "orc_ht_insert": (C.c_int, [_P, C.c_uint64, C.c_int, C.c_int, C.c_uint64]),
@@ -75,7 +79,8 @@ def points_contain(V, I, tri_obj, P, axis=2, use_grid=True):
     return out
 
 
-def edge_pairs(edges, V, I, tri_obj, swap=False, use_grid=True, with_counts=False):
+def edge_pairs(edges, V, I, tri_obj, swap=False, use_grid=True, with_counts=False,
+               chain_rule=0, max_iter=100):
     """Raw pair keys (a << 32 | b) in edge order, and optionally per-edge counts."""
     S = _c(edges.starts, np.float32); E = _c(edges.ends, np.float32)
     off = _c(edges.src_offsets, np.int32); cnt = _c(edges.src_counts, np.int32)
@@ -83,13 +88,47 @@ def edge_pairs(edges, V, I, tri_obj, swap=False, use_grid=True, with_counts=Fals
     V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32)
     nE = len(S)
     counts = np.empty(nE, np.int32)
-    total = lib().orc_edge_pairs(_p(S), _p(E), _p(off), _p(cnt), _p(so), nE, _p(V), len(V), _p(I),
-                                 len(I), _p(O), int(swap), int(use_grid), None, 0, _p(counts))
+    args = (_p(S), _p(E), _p(off), _p(cnt), _p(so), nE, _p(V), len(V), _p(I), len(I), _p(O),
+            int(swap), int(use_grid), int(chain_rule), int(max_iter))
+    total = lib().orc_edge_pairs_ex(*args, None, 0, _p(counts))
     keys = np.empty(max(total, 1), np.uint64)
-    lib().orc_edge_pairs(_p(S), _p(E), _p(off), _p(cnt), _p(so), nE, _p(V), len(V), _p(I), len(I),
-                         _p(O), int(swap), int(use_grid), _p(keys), total, None)
+    lib().orc_edge_pairs_ex(*args, _p(keys), total, None)
     keys = keys[:total]
     return (keys, counts) if with_counts else keys
+
+
+def object_parity(src_mesh, mesh, max_targets=256, swap=False):
+    """A.3/A.4 per-object +z containment rays: odd (src, tgt) keys (swapped when swap)."""
+    sV = _c(src_mesh.vertices, np.float32); sI = _c(src_mesh.indices, np.uint32)
+    sO = _c(src_mesh.tri_to_obj, np.int32)
+    nobj = int(sO.max()) + 1 if len(sO) else 0
+    V = _c(mesh.vertices, np.float32); I = _c(mesh.indices, np.uint32); O = _c(mesh.tri_to_obj, np.int32)
+    over = C.c_int64(0)
+    args = (_p(sV), _p(sI), _p(sO), len(sI), nobj, _p(V), len(V), _p(I), len(I), _p(O),
+            int(max_targets), int(swap))
+    total = lib().orc_object_parity(*args, None, 0, C.cast(C.pointer(over), C.c_void_p))
+    keys = np.empty(max(total, 1), np.uint64)
+    lib().orc_object_parity(*args, _p(keys), total, C.cast(C.pointer(over), C.c_void_p))
+    return keys[:total], over.value
+
+
+def intersection_pairs(m1, m2, max_iter=100):
+    """Intersection join: overlap edge rays both ways plus the containment fallback both ways."""
+    k1 = edge_pairs(m1.edges, m2.vertices, m2.indices, m2.tri_to_obj, False, max_iter=max_iter)
+    k2 = edge_pairs(m2.edges, m1.vertices, m1.indices, m1.tri_to_obj, True, max_iter=max_iter)
+    c1, _ = object_parity(m1, m2, 256, False)
+    c2, _ = object_parity(m2, m1, 256, True)
+    return sort_unique(np.concatenate([k1, k2, c1, c2]))
+
+
+def containment_pairs(ma, mb):
+    """Containment join: (overlap keys, strict containment keys), both (a << 32 | b)."""
+    o1 = edge_pairs(mb.edges, ma.vertices, ma.indices, ma.tri_to_obj, True, chain_rule=1, max_iter=256)
+    o2 = edge_pairs(ma.edges, mb.vertices, mb.indices, mb.tri_to_obj, False, chain_rule=1, max_iter=256)
+    ov = sort_unique(np.concatenate([o1, o2]))
+    cand, _ = object_parity(mb, ma, 512, True)  # (a << 32 | b)
+    cand = sort_unique(cand)
+    return ov, cand[~np.isin(cand, ov)]
 
 
 def sort_unique(keys):

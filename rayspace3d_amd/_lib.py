@@ -60,6 +60,17 @@ class Edges(C.Structure):
     ]
 
 
+class MeshView(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_void_p),
+        ("num_vertices", C.c_uint64),
+        ("indices", C.c_void_p),
+        ("num_triangles", C.c_uint64),
+        ("tri_to_obj", C.c_void_p),
+        ("num_objects", C.c_uint64),
+    ]
+
+
 class HashTable(C.Structure):
     _fields_ = [
         ("slots", C.c_void_p),
@@ -103,6 +114,14 @@ SIGNATURES = {
     "rs3d_edge_hash": (I32, [P, P, C.POINTER(Edges), I32, C.POINTER(HashTable)]),
     "rs3d_ht_compact": (I32, [P, C.POINTER(HashTable), P, U64, PU64]),
     "rs3d_overlap_join": (I32, [P, P, C.POINTER(Edges), P, C.POINTER(Edges), I32, P, U64, PU64]),
+    "rs3d_edge_hash_ex": (I32, [P, P, C.POINTER(Edges), I32, I32, I32, C.POINTER(HashTable)]),
+    "rs3d_containment_fallback": (I32, [P, C.POINTER(MeshView), P, I32, I32, C.POINTER(HashTable), PU64]),
+    "rs3d_point_in_mesh_join": (I32, [P, C.POINTER(MeshView), P, I32, C.POINTER(HashTable),
+                                      C.POINTER(HashTable), PU64]),
+    "rs3d_intersection_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
+                                     P, C.POINTER(Edges), I32, U64, P, U64, PU64, PU64]),
+    "rs3d_containment_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
+                                    P, C.POINTER(Edges), U64, P, U64, PU64, P, U64, PU64, PU64]),
     "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
     "count_hits_gpu": (I32, [P, I32]),
     "compact_hits_gpu": (None, [P, P, I32, C.POINTER(C.c_int)]),

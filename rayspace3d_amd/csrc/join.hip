@@ -10,8 +10,6 @@ namespace rs3d {
 int edge_count_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, int32_t* counts);
 int edge_write_keys_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e,
                              const int64_t* offsets, int64_t bias, int swap, uint64_t* out);
-int edge_hash_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, int swap,
-                       const rs3d_hash_table* ht);
 
 __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n,
                                                   unsigned long long v) {
@@ -226,8 +224,8 @@ int rs3d_overlap_join(rs3d_ctx* ctx, const rs3d_bvh* bvh1, const rs3d_edges* edg
             t.max_probes = 0;
             t.failed = reinterpret_cast<uint64_t*>(slots_buf.as<uint8_t>() + size * sizeof(uint64_t));
             RS3D_TRY(ht_clear_internal(ctx, t.slots, t.size, t.failed));
-            RS3D_TRY(edge_hash_internal(ctx, bvh2, edges1, 0, &t));
-            RS3D_TRY(edge_hash_internal(ctx, bvh1, edges2, 1, &t));
+            RS3D_TRY(edge_hash_internal(ctx, bvh2, edges1, 0, &t, 0, 100));
+            RS3D_TRY(edge_hash_internal(ctx, bvh1, edges2, 1, &t, 0, 100));
             uint64_t failed = 0;
             RS3D_TRY(read_u64(ctx, t.failed, &failed));
             uint64_t occ = 0;

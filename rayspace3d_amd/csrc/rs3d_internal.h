@@ -197,4 +197,10 @@ int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uin
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
 int grid_blocks(uint64_t n, int block, unsigned* out);
 int prof_collect(rs3d_ctx* ctx);  // folds a finished timed launch into the totals
+int pick_depth(const rs3d_bvh* bvh, int* depth);  // LDS stack size for a BVH
+int ht_clear_internal(rs3d_ctx* ctx, uint64_t* slots, uint64_t size, uint64_t* failed);
+int ht_compact_internal(rs3d_ctx* ctx, const uint64_t* slots, uint64_t size, uint64_t* out,
+                        uint64_t max_out, uint64_t* count);
+int edge_hash_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, int swap,
+                       const rs3d_hash_table* ht, int chain_rule, int max_iter);
 }  // namespace rs3d

@@ -1,0 +1,304 @@
+// traverse.h -- device-side traversal building blocks shared by the ray kernels:
+// hit buffer, node/triangle fetch, all-hits traversal for axis rays and segments, and
+// the pair hash insert of the reference's open-addressing table.
+#pragma once
+
+#include "rs3d_internal.h"
+
+namespace rs3d {
+
+constexpr int TR_BLOCK = 64;
+constexpr int KBUF = 8;
+
+__device__ __forceinline__ float3 ld3f(const float* p, uint64_t i) {
+    return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
+// Sorted insertion of (t, tri, obj) into the KBUF smallest, ordered by (t, tri).
+__device__ __forceinline__ void kbuf_insert(float (&bt)[KBUF], uint32_t (&bi)[KBUF],
+                                            int (&bo)[KBUF], float t, uint32_t tri, int obj) {
+    float ct = t;
+    uint32_t ci = tri;
+    int co = obj;
+#pragma unroll
+    for (int s = 0; s < KBUF; ++s) {
+        const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
+        if (less) {
+            const float tt = bt[s];
+            const uint32_t ti = bi[s];
+            const int to = bo[s];
+            bt[s] = ct;
+            bi[s] = ci;
+            bo[s] = co;
+            ct = tt;
+            ci = ti;
+            co = to;
+        }
+    }
+}
+
+__device__ __forceinline__ void kbuf_clear(float (&bt)[KBUF], uint32_t (&bi)[KBUF], int (&bo)[KBUF]) {
+#pragma unroll
+    for (int s = 0; s < KBUF; ++s) {
+        bt[s] = INFINITY;
+        bi[s] = 0xffffffffu;
+        bo[s] = -1;
+    }
+}
+
+struct NodeBoxes {
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    int l0, l1, n0, n1;
+};
+
+__device__ __forceinline__ NodeBoxes load_node(const BNode* __restrict__ nodes, int idx) {
+    const BNode nd = nodes[idx];
+    NodeBoxes b;
+    b.lo0[0] = nd.a.x; b.hi0[0] = nd.a.y; b.lo0[1] = nd.a.z; b.hi0[1] = nd.a.w;
+    b.lo0[2] = nd.b.x; b.hi0[2] = nd.b.y; b.lo1[0] = nd.b.z; b.hi1[0] = nd.b.w;
+    b.lo1[1] = nd.c.x; b.hi1[1] = nd.c.y; b.lo1[2] = nd.c.z; b.hi1[2] = nd.c.w;
+    b.l0 = nd.link.x; b.l1 = nd.link.y; b.n0 = nd.link.z; b.n1 = nd.link.w;
+    return b;
+}
+
+__device__ __forceinline__ void load_tri(const TriRec* __restrict__ tris, int j, float3& v0, float3& v1,
+                                         float3& v2, uint32_t& orig, int& obj) {
+    const TriRec r = tris[j];
+    v0 = make_float3(r.v0.x, r.v0.y, r.v0.z);
+    v1 = make_float3(r.v1.x, r.v1.y, r.v1.z);
+    v2 = make_float3(r.v2.x, r.v2.y, r.v2.z);
+    orig = (uint32_t)__float_as_int(r.v0.w);
+    obj = __float_as_int(r.v1.w);
+}
+
+struct TraceCounters {
+    uint64_t nodes = 0, tris = 0, hits = 0, restarts = 0;
+};
+
+// ------------------------------------------------------------------------------------
+// All-hits traversal for an axis-aligned ray (+AX). Hits with thr <= t < tmax_excl are
+// offered to the buffer; returns the number of such hits.
+// ------------------------------------------------------------------------------------
+template <int AX, int DEPTH, bool STATS>
+__device__ __forceinline__ uint32_t collect_axis(const BNode* __restrict__ nodes,
+                                                 const TriRec* __restrict__ tris, const float3 o,
+                                                 float thr, uint32_t thr_tri, float tmax_excl,
+                                                 int* stack,
+                                                 float (&bt)[KBUF], uint32_t (&bi)[KBUF],
+                                                 int (&bo)[KBUF], TraceCounters& ctr) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    uint32_t total = 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const NodeBoxes b = load_node(nodes, node);
+        if (STATS) ctr.nodes++;
+        // A triangle entirely behind the origin (all vertices <= o along AX) yields t <= 0,
+        // so boxes are culled only on the perpendicular slab and on hi < o along AX.
+        const bool h0 = ou >= b.lo0[U] && ou <= b.hi0[U] && ow >= b.lo0[W] && ow <= b.hi0[W] &&
+                        b.hi0[AX] >= oa;
+        const bool h1 = ou >= b.lo1[U] && ou <= b.hi1[U] && ow >= b.lo1[W] && ow <= b.hi1[W] &&
+                        b.hi1[AX] >= oa;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bool hit = c == 0 ? h0 : h1;
+            const int cnt = c == 0 ? b.n0 : b.n1;
+            const int first = c == 0 ? b.l0 : b.l1;
+            if (hit && cnt > 0) {
+                for (int j = first; j < first + cnt; ++j) {
+                    float3 v0, v1, v2;
+                    uint32_t orig;
+                    int obj;
+                    load_tri(tris, j, v0, v1, v2, orig, obj);
+                    if (STATS) ctr.tris++;
+                    float t;
+                    if (tri_hit(o, rp, v0, v1, v2, t) && (t > thr || (t == thr && orig >= thr_tri)) &&
+                        t < tmax_excl) {
+                        ++total;
+                        if (STATS) ctr.hits++;
+                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, obj);
+                    }
+                }
+            }
+        }
+        const bool in0 = h0 && b.n0 == 0;
+        const bool in1 = h1 && b.n1 == 0;
+        if (in0 && in1) {
+            const bool first0 = b.lo0[AX] <= b.lo1[AX];
+            stack[sp * TR_BLOCK] = first0 ? b.l1 : b.l0;
+            ++sp;
+            node = first0 ? b.l0 : b.l1;
+        } else if (in0) {
+            node = b.l0;
+        } else if (in1) {
+            node = b.l1;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * TR_BLOCK];
+        }
+    }
+    return total;
+}
+
+__device__ __forceinline__ void flush_counters(const TraceCounters& c, unsigned long long* stats) {
+    uint64_t v[4] = {c.nodes, c.tris, c.hits, c.restarts};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint64_t x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        v[k] = x;
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&stats[k], (unsigned long long)v[k]);
+    }
+}
+
+enum { EMODE_COUNT = 0, EMODE_WRITE_PAIRS = 1, EMODE_WRITE_KEYS = 2, EMODE_HASH = 3 };
+
+struct EdgeArgs {
+    const float* starts;
+    const float* ends;
+    const int32_t* src_off;
+    const int32_t* src_cnt;
+    const int32_t* src_obj;
+    uint64_t num_edges;
+};
+
+struct HashArgs {
+    unsigned long long* slots;
+    uint64_t size;
+    int bitwise;
+    int max_probes;
+    unsigned long long* failed;
+};
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Insert of the reference's open-addressing table (query/src/cuda/mesh_overlap_edges.cu:62-104):
+// fmix64 hash, linear probing, CAS from the all-ones empty key. A read before the CAS
+// skips the atomic when the slot already holds the key or another key.
+__device__ __forceinline__ void ht_insert(const HashArgs& h, uint64_t key) {
+    const uint64_t k = fmix64(key);
+    uint64_t s = h.bitwise ? (k & (h.size - 1)) : (k % h.size);
+    const uint64_t probes = h.max_probes > 0 ? (uint64_t)h.max_probes : h.size;
+    for (uint64_t p = 0; p < probes; ++p) {
+        const unsigned long long cur = __hip_atomic_load(&h.slots[s], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == (unsigned long long)key) return;
+        if (cur == ~0ull) {
+            const unsigned long long old = atomicCAS(&h.slots[s], ~0ull, (unsigned long long)key);
+            if (old == ~0ull || old == (unsigned long long)key) return;
+        }
+        s = (s + 1 == h.size) ? 0 : s + 1;
+    }
+    atomicAdd(h.failed, 1ull);
+}
+
+// Membership test in a table filled by ht_insert (the lookup of
+// query/src/cuda/mesh_containment.cu:56-68, without its 1000-probe cut-off).
+__device__ __forceinline__ bool ht_contains(const HashArgs& h, uint64_t key) {
+    const uint64_t k = fmix64(key);
+    uint64_t s = h.bitwise ? (k & (h.size - 1)) : (k % h.size);
+    for (uint64_t p = 0; p < h.size; ++p) {
+        const unsigned long long cur = h.slots[s];
+        if (cur == (unsigned long long)key) return true;
+        if (cur == ~0ull) return false;
+        s = (s + 1 == h.size) ? 0 : s + 1;
+    }
+    return false;
+}
+
+template <int DEPTH, bool STATS>
+__device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ nodes,
+                                                    const TriRec* __restrict__ tris, const float3 o,
+                                                    const float3 inv, const RayPre& rp, float thr,
+                                                    float tmax, int* stack, float (&bt)[KBUF],
+                                                    uint32_t (&bi)[KBUF], int (&bo)[KBUF],
+                                                    TraceCounters& ctr) {
+    // The computed t of any reported hit lies within the kz extent of its triangle's
+    // vertices (scaled by Sz) up to a few ulps; boxes are tested against [thr, tmax] on
+    // that slab with a relative margin, and against the full line on all three slabs.
+    const float okz = get3(o, rp.kz);
+    const float lo_t = thr - fabsf(thr) * 9.5367431640625e-7f;    // 2^-20
+    const float hi_t = tmax + fabsf(tmax) * 9.5367431640625e-7f;
+    uint32_t total = 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const NodeBoxes b = load_node(nodes, node);
+        if (STATS) ctr.nodes++;
+        bool h[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float* lo = c == 0 ? b.lo0 : b.lo1;
+            const float* hi = c == 0 ? b.hi0 : b.hi1;
+            const float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+            const float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+            const float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+            const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+            const float klo = rp.kz == 0 ? lo[0] : (rp.kz == 1 ? lo[1] : lo[2]);
+            const float khi = rp.kz == 0 ? hi[0] : (rp.kz == 1 ? hi[1] : hi[2]);
+            const float k0 = (klo - okz) * rp.Sz, k1 = (khi - okz) * rp.Sz;
+            const float kmin = fminf(k0, k1), kmax = fmaxf(k0, k1);
+            h[c] = (tn <= tf + fabsf(tf) * 9.5367431640625e-7f) && kmax >= lo_t && kmin <= hi_t &&
+                   lo[0] <= hi[0];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int cnt = c == 0 ? b.n0 : b.n1;
+            const int first = c == 0 ? b.l0 : b.l1;
+            if (h[c] && cnt > 0) {
+                for (int j = first; j < first + cnt; ++j) {
+                    float3 v0, v1, v2;
+                    uint32_t orig;
+                    int obj;
+                    load_tri(tris, j, v0, v1, v2, orig, obj);
+                    if (STATS) ctr.tris++;
+                    float t;
+                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t <= tmax) {
+                        ++total;
+                        if (STATS) ctr.hits++;
+                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, obj);
+                    }
+                }
+            }
+        }
+        const bool in0 = h[0] && b.n0 == 0;
+        const bool in1 = h[1] && b.n1 == 0;
+        if (in0 && in1) {
+            stack[sp * TR_BLOCK] = b.l1;
+            ++sp;
+            node = b.l0;
+        } else if (in0) {
+            node = b.l0;
+        } else if (in1) {
+            node = b.l1;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = stack[sp * TR_BLOCK];
+        }
+    }
+    return total;
+}
+
+
+}  // namespace rs3d

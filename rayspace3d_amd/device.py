@@ -102,8 +102,19 @@ class Bvh:
 
     @classmethod
     def from_mesh(cls, ctx: Context, mesh, leaf_size: int = 4) -> "Bvh":
-        return cls(ctx, to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj),
-                   leaf_size)
+        b = cls(ctx, to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj),
+                leaf_size)
+        b.num_objects = int(mesh.tri_to_obj.max()) + 1 if len(mesh.tri_to_obj) else 0
+        return b
+
+    def view(self) -> _lib.MeshView:
+        """The source mesh as an rs3d_mesh_view (rays that start on this mesh)."""
+        v, i, o = self._src
+        n_obj = getattr(self, "num_objects", None)
+        if n_obj is None:
+            n_obj = int(o.max().item()) + 1 if o is not None and o.numel() else 0
+        return _lib.MeshView(v.data_ptr(), v.shape[0], i.data_ptr(), i.shape[0],
+                             o.data_ptr() if o is not None else None, n_obj)
 
     def info(self) -> _lib.BvhInfo:
         inf = _lib.BvhInfo()
@@ -214,3 +225,50 @@ def sort_unique_u64(ctx, keys: torch.Tensor) -> torch.Tensor:
     check(ctx.lib.rs3d_sort_unique_u64(ctx.handle, _ptr(keys), keys.shape[0], C.byref(cnt)),
           "rs3d_sort_unique_u64")
     return keys[:cnt.value]
+
+
+def _join_out(fn, max_out):
+    while True:
+        out = torch.empty(max(max_out, 1), dtype=torch.int64, device="cuda")
+        cnt = C.c_uint64()
+        rc, extra = fn(out, max_out, cnt)
+        if rc == _lib.RS3D_E_CAPACITY and cnt.value > max_out:
+            max_out = int(cnt.value)
+            continue
+        check(rc, "join")
+        return to_host(out[:cnt.value], np.uint64), extra
+
+
+def intersection_join(ctx, bvh1, dedges1, bvh2, dedges2, overlap_max_iterations=100,
+                      max_out=1 << 20):
+    """Sorted unique keys of the intersection join (overlap + containment fallback)."""
+    v1, v2 = bvh1.view(), bvh2.view()
+    torch.cuda.synchronize()
+
+    def call(out, cap, cnt):
+        over = C.c_uint64()
+        rc = ctx.lib.rs3d_intersection_join(ctx.handle, C.byref(v1), bvh1.handle, C.byref(dedges1.s),
+                                            C.byref(v2), bvh2.handle, C.byref(dedges2.s),
+                                            overlap_max_iterations, 0, _ptr(out), cap, C.byref(cnt),
+                                            C.byref(over))
+        return rc, over.value
+    return _join_out(call, max_out)
+
+
+def containment_join(ctx, bvha, dedgesa, bvhb, dedgesb, max_out=1 << 20):
+    """(containment keys, overlap keys, overflow) of the containment join, (a << 32 | b)."""
+    va, vb = bvha.view(), bvhb.view()
+    torch.cuda.synchronize()
+    while True:
+        cont = torch.empty(max_out, dtype=torch.int64, device="cuda")
+        ov = torch.empty(max_out, dtype=torch.int64, device="cuda")
+        nc, no, over = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        rc = ctx.lib.rs3d_containment_join(ctx.handle, C.byref(va), bvha.handle, C.byref(dedgesa.s),
+                                           C.byref(vb), bvhb.handle, C.byref(dedgesb.s), 0,
+                                           _ptr(cont), max_out, C.byref(nc), _ptr(ov), max_out,
+                                           C.byref(no), C.byref(over))
+        if rc == _lib.RS3D_E_CAPACITY:
+            max_out = int(max(nc.value, no.value))
+            continue
+        check(rc, "rs3d_containment_join")
+        return (to_host(cont[:nc.value], np.uint64), to_host(ov[:no.value], np.uint64), over.value)

@@ -8,8 +8,9 @@ import torch
 
 import oracle as orc
 from rayspace3d_amd import _lib, gen
-from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, edge_count, overlap_join,
-                                   sort_unique_u64, to_dev, to_host, trace_points, trace_stats)
+from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, containment_join,
+                                   edge_count, intersection_join, overlap_join, sort_unique_u64,
+                                   to_dev, to_host, trace_points, trace_stats)
 
 pytestmark = pytest.mark.gpu
 
@@ -221,3 +222,47 @@ def test_estimate_matches_oracle(ctx):
     _lib.check(ctx.lib.rs3d_estimate_raw(ctx.handle, C.c_void_p(dA.data_ptr()), C.c_void_p(dB.data_ptr()),
                                          len(ca), cv, 0.001, 0.8, C.byref(r)), "estimate")
     assert ref > 0 and abs(r.value - ref) <= 1e-5 * ref
+
+
+def _nested_case(seed, n=200, world=200.0):
+    """A: radius-5 spheres; B: half nested inside A objects (radius 2 at A centres), half
+    random radius-3 spheres, so the joins see overlap, containment and disjoint pairs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ca = rng.uniform(10, world - 10, size=(n, 3))
+    a = gen.spheres(rng.choice([2, 3], size=n), ca, np.full(n, 5.0),
+                    rotations=gen.random_rotations(rng, n))
+    cb = np.concatenate([ca[: n // 2] + rng.uniform(-1, 1, size=(n // 2, 3)),
+                         rng.uniform(10, world - 10, size=(n - n // 2, 3))])
+    rb = np.concatenate([np.full(n // 2, 2.0), np.full(n - n // 2, 3.0)])
+    b = gen.spheres(2, cb, rb, rotations=gen.random_rotations(rng, n))
+    a.edges, b.edges = gen.extract_edges(a), gen.extract_edges(b)
+    return a, b
+
+
+def test_intersection_join(ctx):
+    a, b = _nested_case(51)
+    ref = orc.intersection_pairs(a, b)
+    ov = orc.overlap_pairs(a, a.edges, b, b.edges)
+    assert len(ref) > len(ov) > 10  # the fallback adds the nested pairs
+    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    got, over = intersection_join(ctx, ba, DevEdges(a.edges), bb, DevEdges(b.edges))
+    assert over == 0
+    assert np.array_equal(got, ref), f"{len(got)} vs {len(ref)}"
+    ba.free()
+    bb.free()
+
+
+def test_containment_join(ctx):
+    a, b = _nested_case(61)
+    ov_ref, ct_ref = orc.containment_pairs(a, b)
+    assert len(ct_ref) >= 50 and len(ov_ref) > 5
+    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    ct, ov, over = containment_join(ctx, ba, DevEdges(a.edges), bb, DevEdges(b.edges))
+    assert over == 0
+    assert np.array_equal(ov, ov_ref)
+    assert np.array_equal(ct, ct_ref)
+    # every nested B object (first half) is contained in the A object it was placed in
+    pairs = {(int(k >> np.uint64(32)), int(k & np.uint64(0xffffffff))) for k in ct}
+    assert sum((i, i) in pairs for i in range(100)) >= 95
+    ba.free()
+    bb.free()
