@@ -242,6 +242,22 @@ int rs3d_containment_join(rs3d_ctx* ctx, const rs3d_mesh_view* ma, const rs3d_bv
 int rs3d_estimate_raw(rs3d_ctx* ctx, const void* cells_a, const void* cells_b, uint64_t num_cells,
                       float cell_volume, float epsilon, float gamma, double* result);
 
+/* Grid statistics of a mesh on the GPU (replaces generateGridStats,
+ * preprocess/src/preprocess_dataset.cpp:37-179): cells = GridCell[resolution^3] (device),
+ * bounds_out (host float[6]) = grid min/max: the vertex bounds, or [0, world_size]^3 when
+ * world_size > 0. */
+int rs3d_grid_stats(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
+                    const uint32_t* indices, uint64_t num_triangles, const int32_t* tri_to_obj,
+                    uint64_t num_objects, int resolution, float world_size, void* cells,
+                    float* bounds_out);
+
+/* Estimated overlap pairs from two grids (host sizing of
+ * query/src/applications/raytracer_overlap_estimated_main.cpp:133-195): raw estimate / alpha
+ * (alpha_correction = 0: raw). grid_min/max/resolution: host arrays of grid A. */
+int rs3d_estimate_pairs(rs3d_ctx* ctx, const void* cells_a, const void* cells_b, uint64_t num_cells,
+                        const float* grid_min, const float* grid_max, const uint32_t* resolution,
+                        float epsilon, float gamma, int alpha_correction, int64_t* estimate);
+
 /* ---- reference-named entry points (same names and signatures as the reference's
  *      C ABI; default stream semantics: synchronous) --------------------------------- */
 /* query/src/cuda/result_compaction.h:7-16 */

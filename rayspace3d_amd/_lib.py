@@ -123,6 +123,9 @@ SIGNATURES = {
     "rs3d_containment_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
                                     P, C.POINTER(Edges), U64, P, U64, PU64, P, U64, PU64, PU64]),
     "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
+    "rs3d_grid_stats": (I32, [P, P, U64, P, U64, P, U64, I32, C.c_float, P, C.POINTER(C.c_float)]),
+    "rs3d_estimate_pairs": (I32, [P, P, P, U64, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_uint32), C.c_float, C.c_float, I32, PI64]),
     "count_hits_gpu": (I32, [P, I32]),
     "compact_hits_gpu": (None, [P, P, I32, C.POINTER(C.c_int)]),
     "exclusive_scan_gpu": (C.c_longlong, [P, P, I32]),

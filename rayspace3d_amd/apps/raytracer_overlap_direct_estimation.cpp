@@ -1,0 +1,4 @@
+// raytracer_overlap_direct_estimation
+// (query/src/applications/raytracer_overlap_direct_estimation_main.cpp).
+#include "overlap_hash_app.hpp"
+int main(int argc, char** argv) { return rs3d_cli::runOverlapHash(argc, argv, true); }

@@ -1,0 +1,41 @@
+"""Preprocessing of generated meshes into .pre files (the role of
+preprocess/src/preprocess_dataset.cpp:183-385 for this build's synthetic inputs):
+unique-edge payload and, optionally, the grid histogram computed on the GPU."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib, gen, preio
+from .device import Context, to_dev, to_host
+
+
+def grid_stats_gpu(ctx: Context, mesh: gen.Mesh, res: int, world_size: float = -1.0) -> preio.Grid:
+    n_obj = int(mesh.tri_to_obj.max()) + 1 if len(mesh.tri_to_obj) else 0
+    cells = torch.zeros(res ** 3 * 16, dtype=torch.uint8, device="cuda")
+    dV, dI, dO = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
+    b = (C.c_float * 6)()
+    torch.cuda.synchronize()
+    _lib.check(ctx.lib.rs3d_grid_stats(ctx.handle, C.c_void_p(dV.data_ptr()), len(mesh.vertices),
+                                       C.c_void_p(dI.data_ptr()), len(mesh.indices),
+                                       C.c_void_p(dO.data_ptr()), n_obj, res, world_size,
+                                       C.c_void_p(cells.data_ptr()), b), "rs3d_grid_stats")
+    c = to_host(cells).view(preio.GRID_CELL).copy()
+    return preio.Grid(np.array(b[:3], np.float32), np.array(b[3:], np.float32), (res, res, res), c)
+
+
+def write_dataset(path: str, mesh: gen.Mesh, ctx: Context | None = None, grid_res: int = 0,
+                  world_size: float = -1.0) -> None:
+    """Writes mesh (+ unique edges, + grid histogram when grid_res > 0) as .pre v2."""
+    if mesh.edges is None:
+        mesh.edges = gen.extract_edges(mesh)
+    grid = grid_stats_gpu(ctx, mesh, grid_res, world_size) if grid_res > 0 else None
+    preio.write_pre(path, mesh, grid)
+
+
+def write_points_wkt(path: str, pts: np.ndarray) -> None:
+    with open(path, "w") as f:
+        for p in pts.astype(np.float64):
+            f.write(f"POINT Z ({p[0]!r} {p[1]!r} {p[2]!r})\n")

@@ -266,3 +266,39 @@ def test_containment_join(ctx):
     assert sum((i, i) in pairs for i in range(100)) >= 95
     ba.free()
     bb.free()
+
+
+def test_grid_stats_and_estimate(ctx):
+    from rayspace3d_amd import preio
+    import os
+    golden = os.path.join(os.path.dirname(__file__), "golden", "check_n_geometry.pre")
+    for mesh, res, world in [(preio.read_pre(golden).mesh, 5, -1.0),
+                             (gen.overlap_dataset(3, n_objects=3000, world=100.0, levels=(2,)), 16, 100.0)]:
+        ref, mn, mx = orc.grid_stats(mesh.vertices, mesh.indices, mesh.tri_to_obj, res, world)
+        cells = torch.zeros(res ** 3 * 16, dtype=torch.uint8, device="cuda")
+        dV, dI, dO = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
+        b = (C.c_float * 6)()
+        torch.cuda.synchronize()
+        _lib.check(ctx.lib.rs3d_grid_stats(ctx.handle, C.c_void_p(dV.data_ptr()), len(mesh.vertices),
+                                           C.c_void_p(dI.data_ptr()), len(mesh.indices),
+                                           C.c_void_p(dO.data_ptr()), int(mesh.tri_to_obj.max()) + 1,
+                                           res, world, C.c_void_p(cells.data_ptr()), b), "grid_stats")
+        got = to_host(cells).view(orc.GRID_CELL)
+        assert np.allclose(list(b)[:3], mn) and np.allclose(list(b)[3:], mx)
+        assert np.array_equal(got["center"], ref["center"]) and np.array_equal(got["touch"], ref["touch"])
+        assert np.allclose(got["avg"], ref["avg"], rtol=1e-5)
+        assert np.allclose(got["ratio"], ref["ratio"], rtol=1e-4)
+    # estimate with the alpha correction against the oracle (sizes tables only: tolerance)
+    a = gen.overlap_dataset(3, n_objects=3000, world=100.0, levels=(2,))
+    bm = gen.overlap_dataset(4, n_objects=3000, world=100.0, levels=(2,))
+    ca, mn, mx = orc.grid_stats(a.vertices, a.indices, a.tri_to_obj, 16, 100.0)
+    cb, _, _ = orc.grid_stats(bm.vertices, bm.indices, bm.tri_to_obj, 16, 100.0)
+    ref = orc.estimate_pairs(ca, cb, mn, mx, [16, 16, 16])
+    dA, dB = to_dev(ca.view(np.uint8)), to_dev(cb.view(np.uint8))
+    est = C.c_int64()
+    torch.cuda.synchronize()
+    _lib.check(ctx.lib.rs3d_estimate_pairs(ctx.handle, C.c_void_p(dA.data_ptr()), C.c_void_p(dB.data_ptr()),
+                                           len(ca), (C.c_float * 3)(*mn), (C.c_float * 3)(*mx),
+                                           (C.c_uint32 * 3)(16, 16, 16), 0.001, 0.8, 1, C.byref(est)),
+               "estimate_pairs")
+    assert ref > 0 and abs(est.value - ref) <= max(1, 1e-4 * ref)
