@@ -51,6 +51,13 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx);
 int rs3d_ctx_sync(rs3d_ctx* ctx);
 /* The hipStream_t the context launches on (as void*), for event timing by callers. */
 void* rs3d_ctx_stream(rs3d_ctx* ctx);
+/* Context options. RS3D_OPT_RAY_SORT_MIN: point launches with at least this many rays
+ * are first reordered along a 2-D Morton curve of the ray origins (perpendicular to the ray
+ * axis) so that the lanes of a wave walk the same BVH nodes; results are unchanged (each
+ * ray still writes its own slot). 0 disables; default 65536. */
+enum { RS3D_OPT_RAY_SORT_MIN = 1 };
+int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value);
+
 /* Kernel timing: when enabled, every point-traversal launch on this context is bracketed
  * by HIP events on the context's stream; profile_read returns the summed device time (ms)
  * and the launch count since the last read (and resets both). Replaces the per-launch

@@ -21,6 +21,7 @@ RS3D_E_DEPTH = -5
 OUT_RAY_RESULT = 0
 OUT_FLAG_U8 = 1
 OUT_OBJ_I32 = 2
+OPT_RAY_SORT_MIN = 1
 
 
 class BvhOpts(C.Structure):
@@ -94,6 +95,7 @@ SIGNATURES = {
     "rs3d_ctx_destroy": (I32, [P]),
     "rs3d_ctx_sync": (I32, [P]),
     "rs3d_ctx_stream": (P, [P]),
+    "rs3d_ctx_set_option": (I32, [P, I32, C.c_int64]),
     "rs3d_ctx_profile": (I32, [P, I32]),
     "rs3d_ctx_profile_read": (I32, [P, C.POINTER(C.c_double), PU64]),
     "rs3d_bvh_build": (I32, [P, P, U64, P, U64, P, C.POINTER(BvhOpts), C.POINTER(P)]),

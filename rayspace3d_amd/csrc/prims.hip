@@ -527,6 +527,8 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx) {
     ctx->tmp_b.release();
     ctx->tmp_c.release();
     ctx->small.release();
+    ctx->ray_keys.release();
+    ctx->ray_vals.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
@@ -545,6 +547,19 @@ int rs3d_ctx_sync(rs3d_ctx* ctx) {
 }
 
 void* rs3d_ctx_stream(rs3d_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value) {
+    if (!ctx) {
+        set_error("null context");
+        return RS3D_E_ARG;
+    }
+    if (option == RS3D_OPT_RAY_SORT_MIN) {
+        ctx->opt_ray_sort_min = value;
+        return RS3D_OK;
+    }
+    set_error("rs3d_ctx_set_option: unknown option %d", option);
+    return RS3D_E_ARG;
+}
 
 int rs3d_ctx_profile(rs3d_ctx* ctx, int enable) {
     if (!ctx) {

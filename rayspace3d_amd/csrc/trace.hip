@@ -296,6 +296,33 @@ __global__ void k_fill_results_miss(void* out, uint64_t n, int out_kind, const u
     }
 }
 
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// 2-D Morton key of each ray origin in the plane perpendicular to the ray axis.
+__global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ pts, uint64_t n,
+                                                  const uint32_t* __restrict__ imap, int axis,
+                                                  float lo_u, float lo_w, float sc_u, float sc_w,
+                                                  uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pi = imap ? imap[i] : (uint32_t)i;
+    const int U = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
+    const int W = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
+    const float u = pts[3 * (uint64_t)pi + U], w = pts[3 * (uint64_t)pi + W];
+    const float fu = fminf(fmaxf((u - lo_u) * sc_u, 0.0f), 65535.0f);
+    const float fw = fminf(fmaxf((w - lo_w) * sc_w, 0.0f), 65535.0f);
+    keys[i] = (spread16((uint32_t)fu) << 1) | spread16((uint32_t)fw);
+    vals[i] = pi;
+}
+
 int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, uint64_t n,
                           const uint32_t* imap, int axis, int out_kind, void* out,
                           unsigned long long* stats) {
@@ -308,6 +335,23 @@ int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, 
         return RS3D_OK;
     }
     if (stats) return launch_points<true>(ctx, bvh, pts, n, imap, axis, out_kind, out, stats);
+    if (ctx->opt_ray_sort_min > 0 && (int64_t)n >= ctx->opt_ray_sort_min && n <= 0xffffffffull) {
+        // reorder rays along a Morton curve of their origins (identical results, coherent waves)
+        RS3D_TRY(ctx->ray_keys.ensure(n * sizeof(uint32_t)));
+        RS3D_TRY(ctx->ray_vals.ensure(n * sizeof(uint32_t)));
+        const int U = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
+        const int W = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
+        const float lo_u = bvh->bmin[U], lo_w = bvh->bmin[W];
+        const float eu = bvh->bmax[U] - lo_u, ew = bvh->bmax[W] - lo_w;
+        const float sc_u = eu > 0.0f ? 65536.0f / eu : 0.0f, sc_w = ew > 0.0f ? 65536.0f / ew : 0.0f;
+        unsigned nb;
+        RS3D_TRY(grid_blocks(n, 256, &nb));
+        k_ray_keys<<<nb, 256, 0, ctx->stream>>>(pts, n, imap, axis, lo_u, lo_w, sc_u, sc_w,
+                                                 ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(), n, 32));
+        imap = ctx->ray_vals.as<uint32_t>();
+    }
     if (ctx->profile) {
         RS3D_TRY(prof_collect(ctx));
         RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));

@@ -62,6 +62,9 @@ class Context:
     def sync(self):
         check(self.lib.rs3d_ctx_sync(self.handle), "rs3d_ctx_sync")
 
+    def set_option(self, option: int, value: int):
+        check(self.lib.rs3d_ctx_set_option(self.handle, option, value), "rs3d_ctx_set_option")
+
     def profile(self, enable: bool):
         check(self.lib.rs3d_ctx_profile(self.handle, int(enable)), "rs3d_ctx_profile")
 

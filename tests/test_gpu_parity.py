@@ -302,3 +302,17 @@ def test_grid_stats_and_estimate(ctx):
                                            (C.c_uint32 * 3)(16, 16, 16), 0.001, 0.8, 1, C.byref(est)),
                "estimate_pairs")
     assert ref > 0 and abs(est.value - ref) <= max(1, 1e-4 * ref)
+
+
+def test_ray_reordering_is_invisible(ctx):
+    mesh, pts = gen.many_spheres_workload(seed=23, n_objects=1000, n_points=200_000, world=120.0)
+    bvh = Bvh.from_mesh(ctx, mesh)
+    dp = to_dev(pts)
+    outs = []
+    for thresh in (0, 1):  # 0: launch order; 1: every launch reordered
+        ctx.set_option(_lib.OPT_RAY_SORT_MIN, thresh)
+        outs.append(to_host(trace_points(ctx, bvh, dp, axis=2, out_kind=_lib.OUT_RAY_RESULT)))
+    ctx.set_option(_lib.OPT_RAY_SORT_MIN, 1 << 16)
+    bvh.free()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0][:, 0], np.arange(len(pts)))
