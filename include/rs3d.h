@@ -243,6 +243,30 @@ int rs3d_containment_join(rs3d_ctx* ctx, const rs3d_mesh_view* ma, const rs3d_bv
                           uint64_t max_contain, uint64_t* n_contain, uint64_t* overlap_out,
                           uint64_t max_overlap, uint64_t* n_overlap, uint64_t* overflow);
 
+/* ---- batched form: many independent queries in one persistent launch --------------
+ * No reference counterpart (the reference runs one query per launch); used for the
+ * batched workload. Containment queries write out_obj like rs3d_trace_points; overlap
+ * queries produce the same sorted unique pair keys as rs3d_overlap_join (both edge
+ * directions). Object ids of overlap queries must be < 2^24; at most 1023 containment
+ * and 32767 queries in total per batch. */
+enum { RS3D_BATCH_CONTAIN = 0, RS3D_BATCH_OVERLAP = 1 };
+typedef struct {
+    int32_t kind;
+    int32_t axis;                /* containment ray axis (0,1,2) */
+    const rs3d_bvh* bvh1;        /* containment: target; overlap: mesh1 */
+    const float* points;         /* containment: float3[num_points] */
+    uint64_t num_points;
+    int32_t* out_obj;            /* containment: int32[num_points] */
+    const rs3d_edges* edges1;    /* overlap: mesh1 edges, traced against bvh2 */
+    const rs3d_bvh* bvh2;        /* overlap: mesh2 */
+    const rs3d_edges* edges2;    /* overlap: mesh2 edges, traced against bvh1 (swapped) */
+} rs3d_batch_query;
+/* pair_keys_out: all overlap pairs, query by query; pair_offsets (host, num_queries + 1):
+ * query q owns [pair_offsets[q], pair_offsets[q+1]). */
+int rs3d_batch_execute(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_queries,
+                       uint64_t* pair_keys_out, uint64_t max_pairs, uint64_t* pair_offsets,
+                       uint64_t* total_pairs);
+
 /* ---- selectivity estimation (replaces estimateOverlapSelectivity,
  *      query/src/cuda/estimated_overlap.cu:43-77) --------------------------------------
  * cells: GridCell[num_cells] (device). Deterministic reduction. result: host. */

@@ -82,6 +82,20 @@ class HashTable(C.Structure):
     ]
 
 
+class BatchQuery(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("axis", C.c_int32),
+        ("bvh1", C.c_void_p),
+        ("points", C.c_void_p),
+        ("num_points", C.c_uint64),
+        ("out_obj", C.c_void_p),
+        ("edges1", C.c_void_p),
+        ("bvh2", C.c_void_p),
+        ("edges2", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 U64 = C.c_uint64
 I32 = C.c_int
@@ -124,6 +138,7 @@ SIGNATURES = {
                                      P, C.POINTER(Edges), I32, U64, P, U64, PU64, PU64]),
     "rs3d_containment_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
                                     P, C.POINTER(Edges), U64, P, U64, PU64, P, U64, PU64, PU64]),
+    "rs3d_batch_execute": (I32, [P, C.POINTER(BatchQuery), I32, P, U64, PU64, PU64]),
     "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
     "rs3d_grid_stats": (I32, [P, P, U64, P, U64, P, U64, I32, C.c_float, P, C.POINTER(C.c_float)]),
     "rs3d_estimate_pairs": (I32, [P, P, P, U64, C.POINTER(C.c_float), C.POINTER(C.c_float),

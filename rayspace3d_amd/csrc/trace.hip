@@ -26,44 +26,16 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points(const BNode* __restrict__ n
     __shared__ int stack_mem[DEPTH * TR_BLOCK];
     const uint64_t i = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
     TraceCounters ctr;
-    const bool active = i < n;
-    if (active) {
+    if (i < n) {
         const uint64_t pi = imap ? (uint64_t)imap[i] : i;
-        const float3 o = ld3f(pts, pi);
-        const float eps = 1e-6f, tmax = 1e16f;
-        float thr = eps;
-        int count = 0;
-        int first_obj = -1;
-        float bt[KBUF];
-        uint32_t bi[KBUF];
-        int bo[KBUF];
-        while (true) {
-            kbuf_clear(bt, bi, bo);
-            const uint32_t total = collect_axis<AX, DEPTH, STATS>(nodes, tris, o, thr, 0u, tmax,
-                                                                  stack_mem + threadIdx.x, bt, bi,
-                                                                  bo, ctr);
-#pragma unroll
-            for (int k = 0; k < KBUF; ++k) {
-                if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
-                    if (count == 0) first_obj = bo[k];
-                    ++count;
-                    float nthr = bt[k] + eps;
-                    if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
-                    thr = nthr;
-                }
-            }
-            if (total <= (uint32_t)KBUF || count >= 10000) break;
-            if (STATS) ctr.restarts++;
-        }
-        const bool inside = (count & 1) != 0;
-        const int obj = inside ? first_obj : -1;
+        const int obj = point_ray<AX, DEPTH, STATS>(nodes, tris, ld3f(pts, pi), stack_mem + threadIdx.x, ctr);
         if (out_kind == RS3D_OUT_RAY_RESULT) {
             rs3d_ray_result r;
             r.ray_id = (int32_t)pi;
             r.polygon_index = obj;
             static_cast<rs3d_ray_result*>(out)[pi] = r;
         } else if (out_kind == RS3D_OUT_FLAG_U8) {
-            static_cast<uint8_t*>(out)[pi] = inside ? 1 : 0;
+            static_cast<uint8_t*>(out)[pi] = obj >= 0 ? 1 : 0;
         } else {
             static_cast<int32_t*>(out)[pi] = obj;
         }
@@ -153,11 +125,6 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, uint6
 // ------------------------------------------------------------------------------------
 // Edge rays (A.2): segment from start towards end, hits in [thr, len + 1e-6].
 // ------------------------------------------------------------------------------------
-// Chain rules: CHAIN_EPS (query/src/cuda/mesh_overlap_edges.cu:173-177): next threshold
-// fl(t + 1e-6), or t + 2e-4 when the add is absorbed. CHAIN_NEXT
-// (query/src/cuda/mesh_containment.cu:150-160): next threshold nextafterf(t, tmax).
-enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
-
 template <int MODE, int DEPTH, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, EdgeArgs E,
@@ -170,67 +137,30 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
     const uint64_t e = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
     TraceCounters ctr;
     if (e < E.num_edges) {
-        const float3 s = ld3f(E.starts, e);
-        const float3 en = ld3f(E.ends, e);
-        const float eps = 1e-6f;
-        const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
-        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
         const int nsrc = E.src_cnt[e];
         const int soff = E.src_off[e];
-        int results = 0;
         int64_t cursor = 0;
         if (MODE == EMODE_WRITE_PAIRS || MODE == EMODE_WRITE_KEYS) cursor = offsets[e] + offset_bias;
-        if (len >= eps) {
-            const float3 d = make_float3(dx / len, dy / len, dz / len);
-            const RayPre rp = ray_pre(d);
-            const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-            const float tmax = len + eps;
-            float thr = eps;
-            int iter = 0;
-            float bt[KBUF];
-            uint32_t bi[KBUF];
-            int bo[KBUF];
-            while (true) {
-                kbuf_clear(bt, bi, bo);
-                const uint32_t total = collect_segment<DEPTH, STATS>(
-                    nodes, tris, s, inv, rp, thr, tmax, stack_mem + threadIdx.x, bt, bi, bo, ctr);
-#pragma unroll
-                for (int k = 0; k < KBUF; ++k) {
-                    if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
-                        const int tgt = bo[k];
-                        for (int q = 0; q < nsrc; ++q) {
-                            const int src = E.src_obj[soff + q];
-                            const uint32_t a = (uint32_t)(swap ? tgt : src);
-                            const uint32_t bb = (uint32_t)(swap ? src : tgt);
-                            if (MODE == EMODE_WRITE_PAIRS) {
-                                rs3d_pair pr;
-                                pr.object_id_mesh1 = (int32_t)a;
-                                pr.object_id_mesh2 = (int32_t)bb;
-                                static_cast<rs3d_pair*>(out)[cursor++] = pr;
-                            } else if (MODE == EMODE_WRITE_KEYS) {
-                                static_cast<uint64_t*>(out)[cursor++] = ((uint64_t)a << 32) | bb;
-                            } else if (MODE == EMODE_HASH) {
-                                ht_insert(H, ((uint64_t)a << 32) | bb);
-                            }
-                        }
-                        results += nsrc;
-                        ++iter;
-                        float nthr;
-                        if (chain_rule == CHAIN_EPS) {
-                            nthr = bt[k] + eps;
-                            if (nthr <= bt[k]) nthr = bt[k] + 2e-4f;
-                        } else {
-                            nthr = nextafterf(bt[k], tmax);
-                            if (nthr <= bt[k]) nthr = bt[k] + eps;
-                        }
-                        thr = nthr;
+        const int hits = edge_ray<DEPTH, STATS>(
+            nodes, tris, ld3f(E.starts, e), ld3f(E.ends, e), chain_rule, max_iter,
+            stack_mem + threadIdx.x, ctr, [&](int tgt) {
+                for (int q = 0; q < nsrc; ++q) {
+                    const int src = E.src_obj[soff + q];
+                    const uint32_t a = (uint32_t)(swap ? tgt : src);
+                    const uint32_t b = (uint32_t)(swap ? src : tgt);
+                    if (MODE == EMODE_WRITE_PAIRS) {
+                        rs3d_pair pr;
+                        pr.object_id_mesh1 = (int32_t)a;
+                        pr.object_id_mesh2 = (int32_t)b;
+                        static_cast<rs3d_pair*>(out)[cursor++] = pr;
+                    } else if (MODE == EMODE_WRITE_KEYS) {
+                        static_cast<uint64_t*>(out)[cursor++] = ((uint64_t)a << 32) | b;
+                    } else if (MODE == EMODE_HASH) {
+                        ht_insert(H, ((uint64_t)a << 32) | b);
                     }
                 }
-                if (total <= (uint32_t)KBUF || iter >= max_iter || thr > tmax) break;
-                if (STATS) ctr.restarts++;
-            }
-        }
-        if (MODE == EMODE_COUNT) counts[e] = results;
+            });
+        if (MODE == EMODE_COUNT) counts[e] = hits * nsrc;
     }
     if (STATS) flush_counters(ctr, stats);
 }

@@ -301,4 +301,92 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
 }
 
 
+// ------------------------------------------------------------------------------------
+// Per-ray programs shared by the single-query kernels and the batched persistent kernel.
+// ------------------------------------------------------------------------------------
+
+// A.1 point containment (query/src/cuda/raytracing.cu:22-77 with the canonical chain):
+// returns the object of the first chain hit when the chain length is odd, else -1.
+template <int AX, int DEPTH, bool STATS>
+__device__ __forceinline__ int point_ray(const BNode* __restrict__ nodes,
+                                         const TriRec* __restrict__ tris, const float3 o,
+                                         int* stack, TraceCounters& ctr) {
+    const float eps = 1e-6f, tmax = 1e16f;
+    float thr = eps;
+    int count = 0;
+    int first_obj = -1;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    while (true) {
+        kbuf_clear(bt, bi, bo);
+        const uint32_t total =
+            collect_axis<AX, DEPTH, STATS>(nodes, tris, o, thr, 0u, tmax, stack, bt, bi, bo, ctr);
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                if (count == 0) first_obj = bo[k];
+                ++count;
+                float nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                thr = nthr;
+            }
+        }
+        if (total <= (uint32_t)KBUF || count >= 10000) break;
+        if (STATS) ctr.restarts++;
+    }
+    return (count & 1) ? first_obj : -1;
+}
+
+// Chain rules: CHAIN_EPS (query/src/cuda/mesh_overlap_edges.cu:173-177): next threshold
+// fl(t + 1e-6), or t + 2e-4 when the add is absorbed. CHAIN_NEXT
+// (query/src/cuda/mesh_containment.cu:150-160): next threshold nextafterf(t, tmax).
+enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
+
+// A.2 edge ray from s towards en: calls emit(target object) once per chain hit; returns
+// the number of chain hits.
+template <int DEPTH, bool STATS, typename Emit>
+__device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
+                                        const TriRec* __restrict__ tris, const float3 s,
+                                        const float3 en, int chain_rule, int max_iter, int* stack,
+                                        TraceCounters& ctr, Emit&& emit) {
+    const float eps = 1e-6f;
+    const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (!(len >= eps)) return 0;
+    const float3 d = make_float3(dx / len, dy / len, dz / len);
+    const RayPre rp = ray_pre(d);
+    const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const float tmax = len + eps;
+    float thr = eps;
+    int iter = 0;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    while (true) {
+        kbuf_clear(bt, bi, bo);
+        const uint32_t total =
+            collect_segment<DEPTH, STATS>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr);
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
+                emit(bo[k]);
+                ++iter;
+                float nthr;
+                if (chain_rule == CHAIN_EPS) {
+                    nthr = bt[k] + eps;
+                    if (nthr <= bt[k]) nthr = bt[k] + 2e-4f;
+                } else {
+                    nthr = nextafterf(bt[k], tmax);
+                    if (nthr <= bt[k]) nthr = bt[k] + eps;
+                }
+                thr = nthr;
+            }
+        }
+        if (total <= (uint32_t)KBUF || iter >= max_iter || thr > tmax) break;
+        if (STATS) ctr.restarts++;
+    }
+    return iter;
+}
+
 }  // namespace rs3d
