@@ -31,14 +31,21 @@ def main():
     ap.add_argument("--objects", type=int, default=10_000)
     ap.add_argument("--points", type=int, default=10_000_000)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=400_000,
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
+    ap.add_argument("--workload", choices=["containment", "batched"], default="containment",
+                    help="containment: the headline filter-refine workload; batched: many "
+                         "independent queries, one persistent launch vs one launch per query")
+    ap.add_argument("--queries", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the join shards by point or edge range across GPUs, see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        return run_batched(args)
 
     import numpy as np
     import torch
@@ -163,6 +170,100 @@ def main():
     }
     print(json.dumps(line), flush=True)
     bvh.free()
+    ctx.close()
+
+
+def run_batched(args):
+    """Independent containment and overlap queries (mesh sizes log-uniform in [1e3, 5e5]
+    triangles, point counts log-uniform in [1e3, 1e7]): all queries through one persistent
+    launch versus one launch sequence per query, with identical results required."""
+    import numpy as np
+    import torch
+
+    from rayspace3d_amd import gen
+    from rayspace3d_amd.device import (Bvh, Context, DevEdges, batch_execute, overlap_join,
+                                       to_dev, to_host, trace_points)
+
+    t0 = time.time()
+    work = gen.batched_workload(seed=args.seed + 2, n_queries=args.queries)
+    print(f"[bench] generated {len(work)} queries in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = Context(0)
+    queries, rays, keep = [], 0, []
+    for w in work:
+        if w["kind"] == "contain":
+            bvh = Bvh.from_mesh(ctx, w["mesh"])
+            dp = to_dev(w["points"])
+            q = {"kind": "contain", "bvh": bvh, "points": dp, "axis": 2,
+                 "out": torch.empty(len(w["points"]), dtype=torch.int32, device="cuda"),
+                 "out_single": torch.empty(len(w["points"]), dtype=torch.int32, device="cuda")}
+            rays += len(w["points"])
+        else:
+            ba, bb = Bvh.from_mesh(ctx, w["a"]), Bvh.from_mesh(ctx, w["b"])
+            q = {"kind": "overlap", "bvh1": ba, "edges1": DevEdges(w["a"].edges), "bvh2": bb,
+                 "edges2": DevEdges(w["b"].edges)}
+            rays += w["a"].edges.num_edges + w["b"].edges.num_edges
+        queries.append(q)
+    torch.cuda.synchronize()
+    n_contain = sum(q["kind"] == "contain" for q in queries)
+    print(f"[bench] {len(queries)} queries ({n_contain} containment), {rays} rays, BVHs resident",
+          file=sys.stderr, flush=True)
+
+    def per_query():
+        res = []
+        for q in queries:
+            if q["kind"] == "contain":
+                trace_points(ctx, q["bvh"], q["points"], axis=2, out=q["out_single"])
+                res.append(None)
+            else:
+                res.append(overlap_join(ctx, q["bvh1"], q["edges1"], q["bvh2"], q["edges2"], mode=1))
+        torch.cuda.synchronize()
+        return res
+
+    def batched():
+        r = batch_execute(ctx, queries)
+        torch.cuda.synchronize()
+        return r
+
+    for _ in range(max(args.warmup, 1)):
+        single = per_query()
+        multi = batched()
+    steps = max(1, min(args.steps, 5))
+    ts = time.perf_counter()
+    for _ in range(steps):
+        single = per_query()
+    t_single = (time.perf_counter() - ts) / steps
+    ts = time.perf_counter()
+    for _ in range(steps):
+        multi = batched()
+    t_batch = (time.perf_counter() - ts) / steps
+    same = True
+    pairs = 0
+    for q, a, b in zip(queries, single, multi):
+        if q["kind"] == "contain":
+            same &= bool(torch.equal(q["out"], q["out_single"]))
+        else:
+            same &= bool(np.array_equal(a, b))
+            pairs += len(b)
+    line = {
+        "metric": "Mrays/s + join pairs/s, batched vs per-query on MI355X (exact); p50/p99 ms",
+        "value": round(rays / t_batch / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched: {len(queries)} independent queries ({n_contain} containment, "
+                               f"{len(queries) - n_contain} overlap), mesh tris log-uniform [1e3,5e5], "
+                               "points log-uniform [1e3,1e7], one persistent launch",
+                   "rays": rays, "seed": args.seed + 2},
+        "per_query_ms": round(t_single * 1e3, 3),
+        "per_query_mrays_s": round(rays / t_single / 1e6, 3),
+        "batched_speedup": round(t_single / t_batch, 3),
+        "pairs": pairs, "pairs_per_s": round(pairs / t_batch, 1),
+        "identical_results": same,
+    }
+    print(json.dumps(line), flush=True)
+    for q in queries:
+        for k in ("bvh", "bvh1", "bvh2"):
+            if k in q:
+                q[k].free()
     ctx.close()
 
 

@@ -47,54 +47,9 @@ __global__ __launch_bounds__(TR_BLOCK) void k_object_rays(
     const float3 o = ld3f(src_v, src_i[3 * (uint64_t)ft]);
     int* ids = cand_ids + s * (uint64_t)max_targets;
     uint8_t* par = cand_par + s * (uint64_t)max_targets;
-    int count = 0;
     unsigned long long over = 0;
-    float lo_t = 1e-4f;
-    uint32_t lo_tri = 0;
-    const float tmax_excl = nextafterf(1e10f, INFINITY);
-    bool has_last = false;
-    int last_obj = -1;
-    float last_t = 0.0f;
-    float bt[KBUF];
-    uint32_t bi[KBUF];
-    int bo[KBUF];
-    TraceCounters ctr;
-    while (true) {
-        kbuf_clear(bt, bi, bo);
-        const uint32_t total = collect_axis<2, DEPTH, false>(nodes, tris, o, lo_t, lo_tri, tmax_excl,
-                                                             stack_mem + threadIdx.x, bt, bi, bo, ctr);
-#pragma unroll
-        for (int k = 0; k < KBUF; ++k) {
-            if ((uint32_t)k < total) {
-                const float t = bt[k];
-                const int obj = bo[k];
-                const bool same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
-                has_last = true;
-                last_obj = obj;
-                last_t = t;
-                if (!same) {
-                    int pos = -1;
-                    for (int j = 0; j < count; ++j)
-                        if (ids[j] == obj) {
-                            pos = j;
-                            break;
-                        }
-                    if (pos >= 0) {
-                        par[pos] ^= 1;
-                    } else if (count < max_targets) {
-                        ids[count] = obj;
-                        par[count] = 1;
-                        ++count;
-                    } else {
-                        ++over;
-                    }
-                }
-            }
-        }
-        if (total <= (uint32_t)KBUF) break;
-        lo_t = bt[KBUF - 1];
-        lo_tri = bi[KBUF - 1] + 1;
-    }
+    const int count = object_ray_parity<DEPTH>(nodes, tris, o, stack_mem + threadIdx.x, ids, par,
+                                               max_targets, over);
     if (over) atomicAdd(overflow, over);
     for (int j = 0; j < count; ++j) {
         if (!(par[j] & 1)) continue;

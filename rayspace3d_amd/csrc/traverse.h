@@ -389,4 +389,63 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
     return iter;
 }
 
+// A.3/A.4 object ray: +z from o over [1e-4, 1e10]; all hits in (t, triangle) order; a hit
+// of the same object as the previous hit within 1e-5 is skipped, every other hit toggles
+// its object's parity in (ids, par) (at most max_targets distinct objects, further new
+// objects counted in `over`). Returns the number of distinct objects recorded.
+template <int DEPTH>
+__device__ __forceinline__ int object_ray_parity(const BNode* __restrict__ nodes,
+                                                 const TriRec* __restrict__ tris, const float3 o,
+                                                 int* stack, int* ids, uint8_t* par,
+                                                 int max_targets, unsigned long long& over) {
+    int count = 0;
+    float lo_t = 1e-4f;
+    uint32_t lo_tri = 0;
+    const float tmax_excl = nextafterf(1e10f, INFINITY);
+    bool has_last = false;
+    int last_obj = -1;
+    float last_t = 0.0f;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    TraceCounters ctr;
+    while (true) {
+        kbuf_clear(bt, bi, bo);
+        const uint32_t total = collect_axis<2, DEPTH, false>(nodes, tris, o, lo_t, lo_tri, tmax_excl,
+                                                             stack, bt, bi, bo, ctr);
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total) {
+                const float t = bt[k];
+                const int obj = bo[k];
+                const bool same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
+                has_last = true;
+                last_obj = obj;
+                last_t = t;
+                if (!same) {
+                    int pos = -1;
+                    for (int j = 0; j < count; ++j)
+                        if (ids[j] == obj) {
+                            pos = j;
+                            break;
+                        }
+                    if (pos >= 0) {
+                        par[pos] ^= 1;
+                    } else if (count < max_targets) {
+                        ids[count] = obj;
+                        par[count] = 1;
+                        ++count;
+                    } else {
+                        ++over;
+                    }
+                }
+            }
+        }
+        if (total <= (uint32_t)KBUF) break;
+        lo_t = bt[KBUF - 1];
+        lo_tri = bi[KBUF - 1] + 1;
+    }
+    return count;
+}
+
 }  // namespace rs3d

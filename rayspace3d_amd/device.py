@@ -275,3 +275,40 @@ def containment_join(ctx, bvha, dedgesa, bvhb, dedgesb, max_out=1 << 20):
             continue
         check(rc, "rs3d_containment_join")
         return (to_host(cont[:nc.value], np.uint64), to_host(ov[:no.value], np.uint64), over.value)
+
+
+def batch_execute(ctx, queries, max_pairs=1 << 22):
+    """Runs many queries in one persistent launch. Each query is a dict:
+    {"kind": "contain", "bvh": Bvh, "points": tensor (n,3), "out": int32 tensor (n,), "axis": 2}
+    {"kind": "overlap", "bvh1": Bvh, "edges1": DevEdges, "bvh2": Bvh, "edges2": DevEdges}
+    Returns a list with the sorted pair keys (uint64 numpy) of each query (empty for
+    containment queries, whose ids land in their "out" tensors)."""
+    n = len(queries)
+    arr = (_lib.BatchQuery * max(n, 1))()
+    for i, q in enumerate(queries):
+        b = arr[i]
+        if q["kind"] == "contain":
+            b.kind = 0
+            b.axis = q.get("axis", 2)
+            b.bvh1 = q["bvh"].handle.value
+            b.points = q["points"].data_ptr()
+            b.num_points = q["points"].shape[0]
+            b.out_obj = q["out"].data_ptr()
+        else:
+            b.kind = 1
+            b.bvh1 = q["bvh1"].handle.value
+            b.bvh2 = q["bvh2"].handle.value
+            b.edges1 = C.cast(C.pointer(q["edges1"].s), C.c_void_p).value
+            b.edges2 = C.cast(C.pointer(q["edges2"].s), C.c_void_p).value
+    offs = (C.c_uint64 * (n + 1))()
+    torch.cuda.synchronize()
+    while True:
+        out = torch.empty(max(max_pairs, 1), dtype=torch.int64, device="cuda")
+        total = C.c_uint64()
+        rc = ctx.lib.rs3d_batch_execute(ctx.handle, arr, n, _ptr(out), max_pairs, offs, C.byref(total))
+        if rc == _lib.RS3D_E_CAPACITY and total.value > max_pairs:
+            max_pairs = int(total.value)
+            continue
+        check(rc, "rs3d_batch_execute")
+        keys = to_host(out[:total.value], np.uint64)
+        return [keys[offs[i]:offs[i + 1]].copy() for i in range(n)]

@@ -214,3 +214,45 @@ def extract_edges(mesh: Mesh) -> EdgeSet:
     offsets[1:] = np.cumsum(counts)[:-1]
     return EdgeSet(np.ascontiguousarray(starts, np.float32), np.ascontiguousarray(ends, np.float32),
                    offsets, counts, np.ascontiguousarray(sobj))
+
+
+# ---- batched and serving workloads -----------------------------------------------------
+
+def log_uniform_int(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=size)).astype(np.int64)
+
+
+def sphere_field(rng, n_objects: int, density_world=1000.0, density_objects=10_000, level=2):
+    """n_objects level-`level` icospheres (radii log-uniform [0.5, 5]) in a cube sized so
+    the object density equals that of density_objects spheres in density_world^3."""
+    side = density_world * (n_objects / density_objects) ** (1.0 / 3.0)
+    centers = rng.uniform(0.0, side, size=(n_objects, 3))
+    radii = np.exp(rng.uniform(np.log(0.5), np.log(5.0), size=n_objects))
+    return spheres(level, centers, radii), side
+
+
+def batched_workload(seed: int = 4, n_queries: int = 256, tri_range=(1e3, 5e5),
+                     point_range=(1e3, 1e7), overlap_every: int = 4):
+    """Independent queries with mesh sizes log-uniform in tri_range triangles (unions of
+    level-2 icospheres at the density of the many-spheres workload) and point counts
+    log-uniform in point_range. Every `overlap_every`-th query is an overlap query between
+    two such meshes (anisotropic, rotated icospheres of levels 2-4); the rest are
+    containment queries. Returns a list of dicts with numpy inputs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for q in range(n_queries):
+        tris = int(log_uniform_int(rng, *tri_range))
+        if overlap_every and q % overlap_every == overlap_every - 1:
+            n_obj = max(2, tris // 2240)
+            side = 1000.0 * (n_obj / 50_000) ** (1.0 / 3.0)
+            a = overlap_dataset(int(rng.integers(1 << 31)), n_objects=n_obj, world=side)
+            b = overlap_dataset(int(rng.integers(1 << 31)), n_objects=n_obj, world=side)
+            a.edges, b.edges = extract_edges(a), extract_edges(b)
+            out.append({"kind": "overlap", "a": a, "b": b})
+        else:
+            n_obj = max(1, tris // 320)
+            mesh, side = sphere_field(rng, n_obj)
+            npts = int(log_uniform_int(rng, *point_range))
+            pts = (rng.random(size=(npts, 3), dtype=np.float32) * np.float32(side)).astype(np.float32)
+            out.append({"kind": "contain", "mesh": mesh, "points": pts})
+    return out

@@ -38,4 +38,4 @@ def write_dataset(path: str, mesh: gen.Mesh, ctx: Context | None = None, grid_re
 def write_points_wkt(path: str, pts: np.ndarray) -> None:
     with open(path, "w") as f:
         for p in pts.astype(np.float64):
-            f.write(f"POINT Z ({p[0]!r} {p[1]!r} {p[2]!r})\n")
+            f.write(f"POINT Z ({float(p[0])!r} {float(p[1])!r} {float(p[2])!r})\n")

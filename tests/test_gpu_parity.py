@@ -316,3 +316,38 @@ def test_ray_reordering_is_invisible(ctx):
     bvh.free()
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[0][:, 0], np.arange(len(pts)))
+
+
+def test_batched_equals_per_query(ctx):
+    from rayspace3d_amd.device import batch_execute
+    rng = np.random.Generator(np.random.PCG64(91))
+    queries, expect, keep = [], [], []
+    for k in range(10):
+        if k % 3 == 2:
+            a, b = _overlap_case(100 + k, int(rng.integers(20, 120)), 60.0)
+            ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+            ea, eb = DevEdges(a.edges), DevEdges(b.edges)
+            queries.append({"kind": "overlap", "bvh1": ba, "edges1": ea, "bvh2": bb, "edges2": eb})
+            expect.append(("pairs", overlap_join(ctx, ba, ea, bb, eb, mode=1)))
+            keep += [ba, bb, ea, eb]
+        else:
+            n_obj = int(rng.integers(5, 400))
+            npts = int(rng.integers(1000, 150_000))
+            mesh, pts = gen.many_spheres_workload(seed=200 + k, n_objects=n_obj, n_points=npts, world=80.0)
+            bvh = Bvh.from_mesh(ctx, mesh)
+            dp = to_dev(pts)
+            out = torch.full((npts,), -7, dtype=torch.int32, device="cuda")
+            queries.append({"kind": "contain", "bvh": bvh, "points": dp, "out": out, "axis": k % 3})
+            expect.append(("ids", to_host(trace_points(ctx, bvh, dp, axis=k % 3))))
+            keep += [bvh]
+    got = batch_execute(ctx, queries)
+    for q, (kind, ref), g in zip(queries, expect, got):
+        if kind == "pairs":
+            assert np.array_equal(g, ref)
+        else:
+            assert len(g) == 0
+            assert np.array_equal(to_host(q["out"]), ref)
+    assert sum(len(r) for k, r in expect if k == "pairs") > 20
+    for b in keep:
+        if isinstance(b, Bvh):
+            b.free()
