@@ -267,6 +267,38 @@ int rs3d_batch_execute(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_q
                        uint64_t* pair_keys_out, uint64_t max_pairs, uint64_t* pair_offsets,
                        uint64_t* total_pairs);
 
+/* ---- serving: resident datasets, request queue, continuous batching ----------------
+ * No reference counterpart (the reference answers one query per process run). Datasets
+ * (device mesh arrays kept alive by the caller, their BVH, optional edges, optional grid
+ * histogram of the given resolution for the estimator) stay resident. Requests queue FIFO;
+ * each rs3d_server_step admits queued requests while slots are free and their estimated
+ * memory fits the budget, runs them all in one persistent launch, and completes them.
+ * Overlap / intersection pairs are (query-dataset object, resident-dataset object),
+ * identical to rs3d_overlap_join / rs3d_intersection_join of the two datasets. Calls on
+ * one server are serial (single host thread). */
+typedef struct rs3d_server rs3d_server;
+enum { RS3D_REQ_CONTAIN = 0, RS3D_REQ_OVERLAP = 1, RS3D_REQ_INTERSECT = 2 };
+typedef struct {
+    int32_t kind;
+    int32_t dataset;        /* resident dataset id */
+    int32_t query_dataset;  /* overlap / intersection: the request's own dataset id */
+    int32_t axis;           /* containment ray axis */
+    const float* points;    /* containment: float3[num_points] (device) */
+    uint64_t num_points;
+    int32_t* out_obj;       /* containment: int32[num_points] (device) */
+} rs3d_request;
+int rs3d_server_create(rs3d_ctx* ctx, int max_slots, uint64_t memory_budget, rs3d_server** out);
+int rs3d_server_destroy(rs3d_server* server);
+int rs3d_server_add_dataset(rs3d_server* server, const rs3d_mesh_view* mesh, const rs3d_bvh* bvh,
+                            const rs3d_edges* edges, int grid_resolution, int* id);
+int rs3d_server_submit(rs3d_server* server, const rs3d_request* request, int64_t* ticket);
+int rs3d_server_step(rs3d_server* server, int* admitted, int* completed);
+/* done: 0 queued, 1 complete, 2 failed; keys (host): the request's sorted pair keys. */
+int rs3d_server_result(rs3d_server* server, int64_t ticket, int* done, uint64_t* keys,
+                       uint64_t max_keys, uint64_t* n_keys);
+int rs3d_server_stats(rs3d_server* server, uint64_t* queued, uint64_t* peak_memory,
+                      uint64_t* steps);
+
 /* ---- selectivity estimation (replaces estimateOverlapSelectivity,
  *      query/src/cuda/estimated_overlap.cu:43-77) --------------------------------------
  * cells: GridCell[num_cells] (device). Deterministic reduction. result: host. */

@@ -22,6 +22,9 @@ OUT_RAY_RESULT = 0
 OUT_FLAG_U8 = 1
 OUT_OBJ_I32 = 2
 OPT_RAY_SORT_MIN = 1
+REQ_CONTAIN = 0
+REQ_OVERLAP = 1
+REQ_INTERSECT = 2
 
 
 class BvhOpts(C.Structure):
@@ -96,6 +99,18 @@ class BatchQuery(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("dataset", C.c_int32),
+        ("query_dataset", C.c_int32),
+        ("axis", C.c_int32),
+        ("points", C.c_void_p),
+        ("num_points", C.c_uint64),
+        ("out_obj", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 U64 = C.c_uint64
 I32 = C.c_int
@@ -139,6 +154,13 @@ SIGNATURES = {
     "rs3d_containment_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
                                     P, C.POINTER(Edges), U64, P, U64, PU64, P, U64, PU64, PU64]),
     "rs3d_batch_execute": (I32, [P, C.POINTER(BatchQuery), I32, P, U64, PU64, PU64]),
+    "rs3d_server_create": (I32, [P, I32, U64, C.POINTER(P)]),
+    "rs3d_server_destroy": (I32, [P]),
+    "rs3d_server_add_dataset": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), I32, C.POINTER(C.c_int)]),
+    "rs3d_server_submit": (I32, [P, C.POINTER(Request), PI64]),
+    "rs3d_server_step": (I32, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rs3d_server_result": (I32, [P, C.c_int64, C.POINTER(C.c_int), P, U64, PU64]),
+    "rs3d_server_stats": (I32, [P, PU64, PU64, PU64]),
     "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
     "rs3d_grid_stats": (I32, [P, P, U64, P, U64, P, U64, I32, C.c_float, P, C.POINTER(C.c_float)]),
     "rs3d_estimate_pairs": (I32, [P, P, P, U64, C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -1,0 +1,80 @@
+// batch_plan.h -- host-side batch engine shared by rs3d_batch_execute and the request
+// server: device descriptors of per-query ray work and the plan that runs them in one
+// persistent launch.
+#pragma once
+
+#include <vector>
+
+#include "rs3d_internal.h"
+#include "traverse.h"
+
+namespace rs3d {
+
+enum { BQ_CONTAIN = 0, BQ_EDGES = 1, BQ_OBJECTS = 2 };
+
+struct BatchDesc {
+    const BNode* nodes;
+    const TriRec* tris;
+    const float* pts;
+    int32_t* out;
+    EdgeArgs E;
+    // object rays (intersection fallback): source mesh, first triangle per object,
+    // per-object parity scratch of max_targets entries
+    const float* src_v;
+    const uint32_t* src_i;
+    const int* first_tri;
+    int* cand_ids;
+    uint8_t* cand_par;
+    int max_targets;
+    uint64_t n;         // rays
+    uint64_t ray_base;  // offset of this query's rays in the global ray order
+    int kind;
+    int axis;
+    int swap;
+    int qid;
+};
+
+struct Chunk {
+    uint32_t desc;
+    uint32_t start;
+};
+
+// Host-side descriptor plus the containment Morton frame (lo_u, lo_w, scale_u, scale_w).
+struct BatchItem : BatchDesc {
+    float lo_sc[4] = {0, 0, 0, 0};
+    BatchItem() : BatchDesc{} {}
+    BatchItem(const BatchDesc& d) : BatchDesc(d) {}
+};
+
+class BatchPlan {
+public:
+    explicit BatchPlan(rs3d_ctx* ctx) : ctx_(ctx) {}
+    void reset(rs3d_ctx* ctx) {
+        ctx_ = ctx;
+        items.clear();
+        depth = 24;
+        contain_rays = 0;
+        any_pairs = false;
+        num_qids = 0;
+        slot_hint = 1u << 20;
+    }
+    int add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int32_t* out, int axis, int qid);
+    int add_edges(const rs3d_bvh* target, const rs3d_edges* e, int swap, int qid);
+    int add_objects(const rs3d_bvh* target, const rs3d_mesh_view* src, const int* first_tri,
+                    int* cand_ids, uint8_t* cand_par, int max_targets, int swap, int qid);
+    // keys_out: sorted unique (qid << 48 | a << 24 | b); offsets[q]..offsets[q+1] per query
+    int run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets, uint64_t* overflow);
+    int num_qids = 0;
+    uint64_t slot_hint = 1u << 20;
+
+private:
+    int base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid);
+    rs3d_ctx* ctx_;
+    std::vector<BatchItem> items;
+    int depth = 24;
+    uint64_t contain_rays = 0;
+    bool any_pairs = false;
+    DevBuf meta, table;
+};
+
+}  // namespace rs3d

@@ -79,6 +79,10 @@ static int first_triangles(rs3d_ctx* ctx, const int32_t* tri_obj, uint64_t nT, u
     return RS3D_OK;
 }
 
+int first_triangles_dev(rs3d_ctx* ctx, const int32_t* tri_obj, uint64_t nT, uint64_t n_obj, int* first) {
+    return first_triangles(ctx, tri_obj, nT, n_obj, first);
+}
+
 static HashArgs hash_args(const rs3d_hash_table* ht) {
     HashArgs h{};
     if (ht) {

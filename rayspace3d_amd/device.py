@@ -312,3 +312,62 @@ def batch_execute(ctx, queries, max_pairs=1 << 22):
         check(rc, "rs3d_batch_execute")
         keys = to_host(out[:total.value], np.uint64)
         return [keys[offs[i]:offs[i + 1]].copy() for i in range(n)]
+
+
+class Server:
+    """Request server with continuous batching over resident datasets (rs3d_server_*)."""
+
+    def __init__(self, ctx, max_slots: int = 64, memory_budget: int = 8 << 30):
+        self.ctx = ctx
+        self.handle = C.c_void_p()
+        check(ctx.lib.rs3d_server_create(ctx.handle, max_slots, memory_budget, C.byref(self.handle)),
+              "rs3d_server_create")
+        self._keep = []
+
+    def add_dataset(self, bvh: Bvh, dedges: "DevEdges | None" = None, grid_res: int = 0) -> int:
+        view = bvh.view()
+        i = C.c_int()
+        torch.cuda.synchronize()
+        check(self.ctx.lib.rs3d_server_add_dataset(self.handle, C.byref(view), bvh.handle,
+                                                   C.byref(dedges.s) if dedges is not None else None,
+                                                   grid_res, C.byref(i)), "rs3d_server_add_dataset")
+        self._keep.append((bvh, dedges))
+        return i.value
+
+    def submit(self, kind: int, dataset: int, query_dataset: int = -1, points=None, out=None,
+               axis: int = 2) -> int:
+        r = _lib.Request(kind, dataset, query_dataset, axis,
+                         points.data_ptr() if points is not None else None,
+                         points.shape[0] if points is not None else 0,
+                         out.data_ptr() if out is not None else None)
+        t = C.c_int64()
+        check(self.ctx.lib.rs3d_server_submit(self.handle, C.byref(r), C.byref(t)), "rs3d_server_submit")
+        return t.value
+
+    def step(self):
+        a, c = C.c_int(), C.c_int()
+        check(self.ctx.lib.rs3d_server_step(self.handle, C.byref(a), C.byref(c)), "rs3d_server_step")
+        return a.value, c.value
+
+    def result(self, ticket: int):
+        done = C.c_int()
+        n = C.c_uint64()
+        check(self.ctx.lib.rs3d_server_result(self.handle, ticket, C.byref(done), None, 0, C.byref(n)),
+              "rs3d_server_result")
+        keys = np.empty(max(n.value, 1), np.uint64)
+        if done.value == 1 and n.value:
+            check(self.ctx.lib.rs3d_server_result(self.handle, ticket, C.byref(done),
+                                                  keys.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
+                  "rs3d_server_result")
+        return done.value, keys[:n.value]
+
+    def stats(self):
+        q, m, s = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(self.ctx.lib.rs3d_server_stats(self.handle, C.byref(q), C.byref(m), C.byref(s)),
+              "rs3d_server_stats")
+        return {"queued": q.value, "peak_memory": m.value, "steps": s.value}
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.rs3d_server_destroy(self.handle)
+            self.handle = C.c_void_p()

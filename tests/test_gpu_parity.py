@@ -351,3 +351,41 @@ def test_batched_equals_per_query(ctx):
     for b in keep:
         if isinstance(b, Bvh):
             b.free()
+
+
+def test_server_alone_equals_mixed(ctx):
+    from rayspace3d_amd.device import Server
+    rng = np.random.Generator(np.random.PCG64(17))
+    resident, _ = gen.many_spheres_workload(seed=401, n_objects=1500, n_points=10, world=80.0)
+    resident.edges = gen.extract_edges(resident)
+    qb = gen.overlap_dataset(402, n_objects=60, world=80.0, levels=(2, 3))
+    qc = gen.overlap_dataset(403, n_objects=40, world=80.0, levels=(2,))
+    qb.edges, qc.edges = gen.extract_edges(qb), gen.extract_edges(qc)
+    bA, bB, bC = Bvh.from_mesh(ctx, resident), Bvh.from_mesh(ctx, qb), Bvh.from_mesh(ctx, qc)
+    eA, eB, eC = DevEdges(resident.edges), DevEdges(qb.edges), DevEdges(qc.edges)
+    srv = Server(ctx, max_slots=3)
+    A = srv.add_dataset(bA, eA, grid_res=32)
+    B = srv.add_dataset(bB, eB, grid_res=32)
+    Cq = srv.add_dataset(bC, eC, grid_res=32)
+    pts = [to_dev(gen.uniform_points(rng, n, 0.0, 80.0)) for n in (5000, 120_000, 700)]
+    outs = [torch.full((p.shape[0],), -9, dtype=torch.int32, device="cuda") for p in pts]
+    for p, o in zip(pts, outs):
+        srv.submit(_lib.REQ_CONTAIN, A, points=p, out=o)
+    t_ov = srv.submit(_lib.REQ_OVERLAP, A, B)
+    t_in = srv.submit(_lib.REQ_INTERSECT, A, Cq)
+    t_ov2 = srv.submit(_lib.REQ_OVERLAP, A, Cq)
+    steps = 0
+    while srv.stats()["queued"]:
+        srv.step()
+        steps += 1
+    assert steps >= 2  # three slots: the six requests needed several iterations
+    for p, o in zip(pts, outs):
+        assert np.array_equal(to_host(o), to_host(trace_points(ctx, bA, p)))
+    for t, ref in ((t_ov, overlap_join(ctx, bB, eB, bA, eA, mode=1)),
+                   (t_ov2, overlap_join(ctx, bC, eC, bA, eA, mode=1)),
+                   (t_in, intersection_join(ctx, bC, eC, bA, eA)[0])):
+        done, keys = srv.result(t)
+        assert done == 1 and np.array_equal(keys, ref), (t, len(keys), len(ref))
+    srv.close()
+    for b in (bA, bB, bC):
+        b.free()
