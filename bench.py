@@ -33,10 +33,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
-    ap.add_argument("--workload", choices=["containment", "batched"], default="containment",
+    ap.add_argument("--workload", choices=["containment", "batched", "serving"], default="containment",
                     help="containment: the headline filter-refine workload; batched: many "
                          "independent queries, one persistent launch vs one launch per query")
     ap.add_argument("--queries", type=int, default=256)
+    ap.add_argument("--datasets", type=int, default=64, help="serving: resident datasets")
+    ap.add_argument("--requests", type=int, default=300, help="serving: requests per arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -46,6 +48,8 @@ def main():
 
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "serving":
+        return run_serving(args)
 
     import numpy as np
     import torch
@@ -264,6 +268,117 @@ def run_batched(args):
         for k in ("bvh", "bvh1", "bvh2"):
             if k in q:
                 q[k].free()
+    ctx.close()
+
+
+def run_serving(args):
+    """Poisson streams of mixed containment / overlap / intersection requests against
+    resident BVHs of the many-spheres generator, served with continuous batching; p50/p99
+    latency and throughput at three arrival rates (fractions of the measured capacity)."""
+    import numpy as np
+    import torch
+
+    from rayspace3d_amd import _lib, gen
+    from rayspace3d_amd.device import Bvh, Context, DevEdges, Server, to_dev
+
+    ctx = Context(0)
+    rng = np.random.Generator(np.random.PCG64(args.seed + 5))
+    t0 = time.time()
+    keep = []
+    srv_datasets = []
+    for i in range(args.datasets):
+        mesh, _ = gen.many_spheres_workload(seed=1000 + i, n_objects=args.objects, n_points=0)
+        edges = gen.sphere_field_edges(mesh, 2)
+        bvh = Bvh.from_mesh(ctx, mesh)
+        de = DevEdges(edges)
+        keep.append((bvh, de))
+        srv_datasets.append((bvh, de))
+        if i % 8 == 7:
+            print(f"[bench] {i + 1}/{args.datasets} resident datasets built ({time.time() - t0:.0f}s)",
+                  file=sys.stderr, flush=True)
+    queries = []
+    for i in range(32):  # request meshes: anisotropic rotated icospheres, levels 2-4
+        k = int(np.exp(rng.uniform(0.0, np.log(64.0))))
+        qm = gen.overlap_dataset(5000 + i, n_objects=max(k, 1), world=1000.0)
+        qm.edges = gen.extract_edges(qm)
+        queries.append((Bvh.from_mesh(ctx, qm), DevEdges(qm.edges)))
+    pool = to_dev(gen.uniform_points(rng, 20_000_000, 0.0, 1000.0))
+    out = torch.empty(pool.shape[0], dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"[bench] setup done in {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
+
+    # request mix: 1/2 containment (1e3..1e6 points), 1/4 overlap, 1/4 intersection
+    n = args.requests
+    kinds = rng.choice([0, 0, 1, 2], size=n)
+    targets = rng.integers(0, args.datasets, size=n)
+    qsel = rng.integers(0, len(queries), size=n)
+    npts = np.exp(rng.uniform(np.log(1e3), np.log(1e6), size=n)).astype(np.int64)
+    starts = rng.integers(0, pool.shape[0] - 1_000_000, size=n)
+
+    def serve(rate, max_slots):
+        srv = Server(ctx, max_slots=max_slots, memory_budget=32 << 30)
+        ids = [srv.add_dataset(b, e, grid_res=128) for b, e in srv_datasets]
+        qids = [srv.add_dataset(b, e, grid_res=128) for b, e in queries]
+        arrivals = np.zeros(n) if rate is None else np.cumsum(rng.exponential(1.0 / rate, size=n))
+        lat = np.zeros(n)
+        tickets = {}
+        nxt, done, rays, pairs = 0, 0, 0, 0
+        ts = time.perf_counter()
+        while done < n:
+            now = time.perf_counter() - ts
+            while nxt < n and arrivals[nxt] <= now:
+                if kinds[nxt] == 0:
+                    p = pool[starts[nxt]: starts[nxt] + npts[nxt]]
+                    o = out[starts[nxt]: starts[nxt] + npts[nxt]]
+                    t = srv.submit(_lib.REQ_CONTAIN, ids[targets[nxt]], points=p, out=o)
+                    rays += int(npts[nxt])
+                else:
+                    t = srv.submit(int(kinds[nxt]), ids[targets[nxt]], qids[qsel[nxt]])
+                    rays += srv_datasets[targets[nxt]][1].s.num_edges + queries[qsel[nxt]][1].s.num_edges
+                tickets[t] = nxt
+                nxt += 1
+            if srv.stats()["queued"]:
+                srv.step()
+                tnow = time.perf_counter() - ts
+                for t in list(tickets):
+                    st, keys = srv.result(t)
+                    if st == 1:
+                        lat[tickets[t]] = tnow - arrivals[tickets[t]]
+                        pairs += len(keys)
+                        del tickets[t]
+                        done += 1
+            elif nxt < n:
+                time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - ts), 0.001)))
+        wall = time.perf_counter() - ts
+        st = srv.stats()
+        srv.close()
+        return {"rate_req_s": rate, "max_slots": max_slots, "wall_s": round(wall, 3),
+                "throughput_req_s": round(n / wall, 1), "mrays_s": round(rays / wall / 1e6, 2),
+                "pairs_s": round(pairs / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"],
+                "peak_admitted_bytes": st["peak_memory"]}
+
+    serve(None, 64)  # warmup
+    closed = serve(None, 64)
+    single = serve(None, 1)
+    cap = closed["throughput_req_s"]
+    rates = [serve(f * cap, 64) for f in (0.3, 0.6, 0.9)]
+    line = {
+        "metric": "Mrays/s + join pairs/s, batched vs per-query on MI355X (exact); p50/p99 ms",
+        "value": closed["mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": closed["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving: {n} Poisson requests (1/2 containment 1e3-1e6 points, "
+                               "1/4 overlap, 1/4 intersection with 1-64-object request meshes) "
+                               f"against {args.datasets} resident many-spheres BVHs, grids 128^3",
+                   "datasets": args.datasets, "requests": n},
+        "closed_loop": closed, "one_at_a_time": single,
+        "continuous_batching_speedup": round(closed["throughput_req_s"] / single["throughput_req_s"], 3),
+        "poisson": rates,
+    }
+    print(json.dumps(line), flush=True)
+    for b, _ in keep + queries:
+        b.free()
     ctx.close()
 
 

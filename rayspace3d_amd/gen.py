@@ -256,3 +256,33 @@ def batched_workload(seed: int = 4, n_queries: int = 256, tri_range=(1e3, 5e5),
             pts = (rng.random(size=(npts, 3), dtype=np.float32) * np.float32(side)).astype(np.float32)
             out.append({"kind": "contain", "mesh": mesh, "points": pts})
     return out
+
+
+@lru_cache(maxsize=None)
+def _template_edges(level: int):
+    """Unique edges of the unit icosphere as vertex-index pairs in first-occurrence
+    direction (triangle order, (v0,v1),(v1,v2),(v2,v0))."""
+    _, f = icosphere(level)
+    a = f[:, [0, 1, 2]].reshape(-1)
+    b = f[:, [1, 2, 0]].reshape(-1)
+    key = np.minimum(a, b) * (len(f) * 3 + 7) + np.maximum(a, b)
+    _, first = np.unique(key, return_index=True)
+    first.sort()
+    return a[first], b[first]
+
+
+def sphere_field_edges(mesh: Mesh, level: int) -> EdgeSet:
+    """Edge payload of a mesh built by spheres() with one icosphere level per object:
+    the same (start, end, source) records as extract_edges (objects share no vertices),
+    grouped by object instead of sorted by quantised key. Join results do not depend on
+    edge order."""
+    v, f = icosphere(level)
+    nv, n_obj = len(v), mesh.num_objects
+    e0, e1 = _template_edges(level)
+    base = (np.arange(n_obj, dtype=np.int64) * nv)[:, None]
+    i0 = (base + e0[None, :]).reshape(-1)
+    i1 = (base + e1[None, :]).reshape(-1)
+    ne = len(i0)
+    src = np.repeat(np.arange(n_obj, dtype=np.int32), len(e0))
+    return EdgeSet(np.ascontiguousarray(mesh.vertices[i0]), np.ascontiguousarray(mesh.vertices[i1]),
+                   np.arange(ne, dtype=np.int32), np.ones(ne, dtype=np.int32), src)

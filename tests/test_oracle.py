@@ -290,3 +290,12 @@ def test_pre_v2_roundtrip(tmp_path):
         f.write(b"XXXX")
     with pytest.raises(ValueError):
         preio.read_pre(str(p))
+
+
+def test_sphere_field_edges_match_extractor():
+    m, _ = gen.many_spheres_workload(seed=5, n_objects=60, n_points=1)
+    e1, e2 = gen.extract_edges(m), gen.sphere_field_edges(m, 2)
+    rec = lambda e: set(zip(map(tuple, e.starts.tolist()), map(tuple, e.ends.tolist()),
+                            np.repeat(e.src_objects, 1).tolist()))
+    assert e1.num_edges == e2.num_edges == 60 * 480
+    assert rec(e1) == rec(e2)
