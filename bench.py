@@ -33,16 +33,21 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="points in the CPU-baseline sample (0 disables it)")
-    ap.add_argument("--workload", choices=["containment", "batched", "serving"], default="containment",
+    ap.add_argument("--workload", choices=["containment", "batched", "serving", "copy"],
+                    default="containment",
                     help="containment: the headline filter-refine workload; batched: many "
-                         "independent queries, one persistent launch vs one launch per query")
+                         "independent queries, one persistent launch vs one launch per query; "
+                         "serving: request server under load; copy: 1 GiB device copy x3, the "
+                         "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--rays-per-wave", type=int, default=128,
+                    help="lane-refill run length of the traversal (0: one ray per lane)")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--datasets", type=int, default=64, help="serving: resident datasets")
     ap.add_argument("--requests", type=int, default=300, help="serving: requests per arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the join shards by point or edge range across GPUs, see DESIGN.md)",
+              "(the join would shard by point or edge range across GPUs)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -50,6 +55,8 @@ def main():
         return run_batched(args)
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "copy":
+        return run_copy()
 
     import numpy as np
     import torch
@@ -65,6 +72,7 @@ def main():
           f"in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    ctx.set_option(_lib.OPT_RAYS_PER_WAVE, args.rays_per_wave)
     dV, dI, dO = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
     dP = to_dev(pts)
     torch.cuda.synchronize()
@@ -161,7 +169,8 @@ def main():
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved_gbs / 8000.0, 4), "traffic": traffic,
-                     "kernel": "k_points (refine traversal)",
+                     "kernel": ("k_points_refill" if args.rays_per_wave else "k_points")
+                               + " (refine traversal)",
                      "kernel_avg_ms": round(kern_avg_ms, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "alg_bytes_per_ray": round(bytes_per_ray, 1)},
@@ -175,6 +184,20 @@ def main():
     print(json.dumps(line), flush=True)
     bvh.free()
     ctx.close()
+
+
+def run_copy():
+    """Copies 1 GiB between two device buffers three times. Under rocprofv3 --pmc the
+    copy kernel's FETCH_SIZE and WRITE_SIZE per dispatch, against the 1 GiB it must read
+    and write, give the factors that turn the counters into HBM bytes."""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_per_copy": a.numel() * 4, "copies": 3}))
 
 
 def run_batched(args):

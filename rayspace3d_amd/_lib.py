@@ -22,6 +22,7 @@ OUT_RAY_RESULT = 0
 OUT_FLAG_U8 = 1
 OUT_OBJ_I32 = 2
 OPT_RAY_SORT_MIN = 1
+OPT_RAYS_PER_WAVE = 2
 REQ_CONTAIN = 0
 REQ_OVERLAP = 1
 REQ_INTERSECT = 2
@@ -125,6 +126,7 @@ SIGNATURES = {
     "rs3d_ctx_sync": (I32, [P]),
     "rs3d_ctx_stream": (P, [P]),
     "rs3d_ctx_set_option": (I32, [P, I32, C.c_int64]),
+    "rs3d_ctx_get_option": (I32, [P, I32, C.POINTER(C.c_int64)]),
     "rs3d_ctx_profile": (I32, [P, I32]),
     "rs3d_ctx_profile_read": (I32, [P, C.POINTER(C.c_double), PU64]),
     "rs3d_bvh_build": (I32, [P, P, U64, P, U64, P, C.POINTER(BvhOpts), C.POINTER(P)]),

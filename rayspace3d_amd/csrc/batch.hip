@@ -267,8 +267,10 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
                                                    reinterpret_cast<uint64_t*>(mb + o_base), contain_rays,
                                                    ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
+        int qbits = 0;  // the descriptor index occupies the bits above the 22 Morton bits
+        while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
         RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
-                                      contain_rays, 32));
+                                      contain_rays, 22 + qbits));
         perm = ctx->ray_vals.as<uint32_t>();
     }
     uint64_t slots = 1024;

@@ -1,11 +1,12 @@
 // lbvh.hip -- linear BVH builder (replaces OptiX's GAS build,
 // query/src/optix/OptixAccelerationStructure.cpp:22-68).
 //
-// 1. per-triangle padded AABB + centroid; centroid bounds by ordered-int atomics
+// 1. per-triangle padded AABB + centroid; centroid bounds by block reduction + ordered-int atomics
 // 2. 30-bit Morton codes of centroids, stable LSD radix sort (ties keep triangle order)
 // 3. triangles gathered into 48-byte records in Morton order
 // 4. Karras (HPG 2012) radix-tree emission over the unique 62-bit keys (code << 32 | rank)
 // 5. bottom-up refit: the second thread to reach a node merges its children's boxes
+//    (leaf windows per workgroup, then one workgroup for the nodes crossing windows)
 // 6. 64-byte node emission; subtrees of at most leaf_size triangles become leaves
 #include <cstring>
 
@@ -35,20 +36,55 @@ __device__ __forceinline__ float3 ld3(const float* p, uint64_t i) {
 __global__ void k_init_bounds(int* bounds) {
     int t = threadIdx.x;
     if (t < 12) bounds[t] = ((t / 3) % 2 == 0) ? 0x7fffffff : (int)0x80000000;
-    if (t == 12) bounds[t] = 0;
+    if (t == 12 || t == 13) bounds[t] = 0;
     if (t >= 16 && t < 22) bounds[t] = t < 19 ? 0x7fffffff : (int)0x80000000;
 }
 
+// Block-wide min/max of v[] (entries k with (k / 3) % 2 == 0 are minima, the others
+// maxima), merged into dst[] with one ordered-int atomic per entry per block. Every
+// thread of the block must call it.
+template <int NV>
+__device__ __forceinline__ void commit_bounds(float (&v)[NV], int* __restrict__ dst) {
+    __shared__ float red[BVH_BLOCK / 64][NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const bool is_min = (k / 3) % 2 == 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_xor(v[k], off, 64);
+            v[k] = is_min ? fminf(v[k], o) : fmaxf(v[k], o);
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        const bool is_min = (k / 3) % 2 == 0;
+        float r = red[0][k];
+        for (int q = 1; q < BVH_BLOCK / 64; ++q) r = is_min ? fminf(r, red[q][k]) : fmaxf(r, red[q][k]);
+        if (is_min)
+            atomicMin(&dst[k], f2ord(r));
+        else
+            atomicMax(&dst[k], f2ord(r));
+    }
+}
+
+// Grid-stride over triangles: padded box per triangle into pbox, and the centroid and
+// padded-box bounds of the whole set into bounds[0..11].
 __global__ __launch_bounds__(BVH_BLOCK) void k_prim_bounds(const float* __restrict__ V,
                                                            const uint32_t* __restrict__ I,
                                                            uint64_t n, uint64_t nV,
                                                            float4* __restrict__ pbox,
                                                            int* __restrict__ bounds,
                                                            int* __restrict__ bad) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float bmn[3] = {INFINITY, INFINITY, INFINITY}, bmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (i < n) {
+    // acc[0..2] centroid min, [3..5] centroid max, [6..8] box min, [9..11] box max
+    float acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = (k / 3) % 2 == 0 ? INFINITY : -INFINITY;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t a = I[3 * i], b = I[3 * i + 1], c = I[3 * i + 2];
         if (a >= nV || b >= nV || c >= nV) {
             atomicOr(bad, 1);
@@ -72,62 +108,32 @@ __global__ __launch_bounds__(BVH_BLOCK) void k_prim_bounds(const float* __restri
             plo[k] = lo[k] - pad;
             phi[k] = hi[k] + pad;
             const float cc = 0.5f * lo[k] + 0.5f * hi[k];
-            cmn[k] = cmx[k] = cc;
-            bmn[k] = plo[k];
-            bmx[k] = phi[k];
+            acc[k] = fminf(acc[k], cc);
+            acc[3 + k] = fmaxf(acc[3 + k], cc);
+            acc[6 + k] = fminf(acc[6 + k], plo[k]);
+            acc[9 + k] = fmaxf(acc[9 + k], phi[k]);
         }
         pbox[2 * i] = make_float4(plo[0], plo[1], plo[2], 0.0f);
         pbox[2 * i + 1] = make_float4(phi[0], phi[1], phi[2], 0.0f);
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        for (int off = 32; off > 0; off >>= 1) {
-            cmn[k] = fminf(cmn[k], __shfl_xor(cmn[k], off, 64));
-            cmx[k] = fmaxf(cmx[k], __shfl_xor(cmx[k], off, 64));
-            bmn[k] = fminf(bmn[k], __shfl_xor(bmn[k], off, 64));
-            bmx[k] = fmaxf(bmx[k], __shfl_xor(bmx[k], off, 64));
-        }
-    }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (cmn[k] <= cmx[k]) {
-                atomicMin(&bounds[k], f2ord(cmn[k]));
-                atomicMax(&bounds[3 + k], f2ord(cmx[k]));
-                atomicMin(&bounds[6 + k], f2ord(bmn[k]));
-                atomicMax(&bounds[9 + k], f2ord(bmx[k]));
-            }
-        }
-    }
+    commit_bounds<12>(acc, bounds);
 }
 
 // Exact min/max over all vertices (the dataset box of query/src/geometry/BoundingBox.cpp:4-10).
 __global__ __launch_bounds__(BVH_BLOCK) void k_vertex_bounds(const float* __restrict__ V, uint64_t nV,
                                                              int* __restrict__ vb) {
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nV;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const float3 p = ld3(V, i);
-        mn[0] = fminf(mn[0], p.x);
-        mn[1] = fminf(mn[1], p.y);
-        mn[2] = fminf(mn[2], p.z);
-        mx[0] = fmaxf(mx[0], p.x);
-        mx[1] = fmaxf(mx[1], p.y);
-        mx[2] = fmaxf(mx[2], p.z);
+        acc[0] = fminf(acc[0], p.x);
+        acc[1] = fminf(acc[1], p.y);
+        acc[2] = fminf(acc[2], p.z);
+        acc[3] = fmaxf(acc[3], p.x);
+        acc[4] = fmaxf(acc[4], p.y);
+        acc[5] = fmaxf(acc[5], p.z);
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        for (int off = 32; off > 0; off >>= 1) {
-            mn[k] = fminf(mn[k], __shfl_xor(mn[k], off, 64));
-            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off, 64));
-        }
-    if ((threadIdx.x & 63) == 0 && mn[0] <= mx[0]) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&vb[k], f2ord(mn[k]));
-            atomicMax(&vb[3 + k], f2ord(mx[k]));
-        }
-    }
+    commit_bounds<6>(acc, vb);
 }
 
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
@@ -246,44 +252,91 @@ __global__ __launch_bounds__(BVH_BLOCK) void k_karras(const uint32_t* __restrict
     if (i == 0) parent_node[0] = -1;
 }
 
-__global__ __launch_bounds__(BVH_BLOCK) void k_refit(const int2* __restrict__ child,
-                                                     const int* __restrict__ parent_node,
-                                                     const int* __restrict__ parent_leaf,
-                                                     const float4* __restrict__ sbox, int64_t n,
-                                                     float4* nbox, int* height, int* flags) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Bottom-up refit in two phases, so that threads only ever synchronise with threads of
+// their own workgroup (workgroup-scope fences; no device-wide fence per tree level).
+// A Karras node covers a contiguous leaf range [first, last] and its index is one end of
+// that range. Phase 1: a workgroup owns a window of REFIT_WIN consecutive leaves and refits
+// every node whose range lies inside the window; a thread that reaches a node crossing
+// the window's edge records it and stops. Phase 2: one workgroup takes those records and
+// refits the crossing nodes, which form the top of the tree.
+constexpr int REFIT_WIN = 1024;
+
+__device__ __forceinline__ void refit_node(const int2* __restrict__ child,
+                                           const float4* __restrict__ sbox, float4* nbox,
+                                           int* height, int p) {
+    const int2 c = child[p];
+    float4 lo0, hi0, lo1, hi1;
+    int h0 = 0, h1 = 0;
+    if (c.x < 0) {
+        lo0 = sbox[2 * (int64_t)(~c.x)];
+        hi0 = sbox[2 * (int64_t)(~c.x) + 1];
+    } else {
+        lo0 = nbox[2 * (int64_t)c.x];
+        hi0 = nbox[2 * (int64_t)c.x + 1];
+        h0 = height[c.x];
+    }
+    if (c.y < 0) {
+        lo1 = sbox[2 * (int64_t)(~c.y)];
+        hi1 = sbox[2 * (int64_t)(~c.y) + 1];
+    } else {
+        lo1 = nbox[2 * (int64_t)c.y];
+        hi1 = nbox[2 * (int64_t)c.y + 1];
+        h1 = height[c.y];
+    }
+    nbox[2 * (int64_t)p] = make_float4(fminf(lo0.x, lo1.x), fminf(lo0.y, lo1.y),
+                                       fminf(lo0.z, lo1.z), 0.0f);
+    nbox[2 * (int64_t)p + 1] = make_float4(fmaxf(hi0.x, hi1.x), fmaxf(hi0.y, hi1.y),
+                                           fmaxf(hi0.z, hi1.z), 0.0f);
+    height[p] = (h0 > h1 ? h0 : h1) + 1;
+}
+
+__global__ __launch_bounds__(REFIT_WIN) void k_refit_local(const int2* __restrict__ child,
+                                                           const int2* __restrict__ range,
+                                                           const int* __restrict__ parent_node,
+                                                           const int* __restrict__ parent_leaf,
+                                                           const float4* __restrict__ sbox,
+                                                           int64_t n, float4* nbox, int* height,
+                                                           int* __restrict__ pending,
+                                                           int* __restrict__ num_pending) {
+    __shared__ int arrived[REFIT_WIN];
+    const int64_t base = (int64_t)blockIdx.x * REFIT_WIN;
+    arrived[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t j = base + threadIdx.x;
     if (j >= n) return;
     int p = parent_leaf[j];
     while (p >= 0) {
-        __threadfence();
-        const int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;  // sibling subtree not finished; its thread continues
-        __threadfence();
-        const int2 c = child[p];
-        float4 lo0, hi0, lo1, hi1;
-        int h0 = 0, h1 = 0;
-        if (c.x < 0) {
-            lo0 = sbox[2 * (int64_t)(~c.x)];
-            hi0 = sbox[2 * (int64_t)(~c.x) + 1];
-        } else {
-            lo0 = nbox[2 * (int64_t)c.x];
-            hi0 = nbox[2 * (int64_t)c.x + 1];
-            h0 = height[c.x];
+        const int2 r = range[p];
+        if (r.x < base || r.y >= base + REFIT_WIN) {
+            pending[atomicAdd(num_pending, 1)] = p;
+            return;
         }
-        if (c.y < 0) {
-            lo1 = sbox[2 * (int64_t)(~c.y)];
-            hi1 = sbox[2 * (int64_t)(~c.y) + 1];
-        } else {
-            lo1 = nbox[2 * (int64_t)c.y];
-            hi1 = nbox[2 * (int64_t)c.y + 1];
-            h1 = height[c.y];
-        }
-        nbox[2 * (int64_t)p] = make_float4(fminf(lo0.x, lo1.x), fminf(lo0.y, lo1.y),
-                                           fminf(lo0.z, lo1.z), 0.0f);
-        nbox[2 * (int64_t)p + 1] = make_float4(fmaxf(hi0.x, hi1.x), fmaxf(hi0.y, hi1.y),
-                                               fmaxf(hi0.z, hi1.z), 0.0f);
-        height[p] = (h0 > h1 ? h0 : h1) + 1;
+        __threadfence_block();
+        if (atomicAdd(&arrived[p - base], 1) == 0) return;  // sibling still in progress
+        __threadfence_block();
+        refit_node(child, sbox, nbox, height, p);
         p = parent_node[p];
+    }
+}
+
+__global__ __launch_bounds__(REFIT_WIN) void k_refit_top(const int2* __restrict__ child,
+                                                         const int* __restrict__ parent_node,
+                                                         const float4* __restrict__ sbox,
+                                                         float4* nbox, int* height, int* flags,
+                                                         const int* __restrict__ pending,
+                                                         const int* __restrict__ num_pending) {
+    const int m = *num_pending;
+    for (int it = threadIdx.x; it < m; it += REFIT_WIN) {
+        int p = pending[it];
+        while (p >= 0) {
+            __threadfence_block();
+            if (__hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+            __threadfence_block();
+            refit_node(child, sbox, nbox, height, p);
+            p = parent_node[p];
+        }
     }
 }
 
@@ -434,10 +487,11 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
     int* bad = bounds + 12;
     k_init_bounds<<<1, 64, 0, s>>>(bounds);
     {
-        unsigned nbv = (unsigned)std::min<uint64_t>((num_vertices + BVH_BLOCK - 1) / BVH_BLOCK, 4096);
+        unsigned nbv = (unsigned)std::min<uint64_t>((num_vertices + BVH_BLOCK - 1) / BVH_BLOCK, 1024);
         k_vertex_bounds<<<nbv ? nbv : 1, BVH_BLOCK, 0, s>>>(vertices, num_vertices, bounds + 16);
     }
-    k_prim_bounds<<<nb, BVH_BLOCK, 0, s>>>(vertices, indices, n, num_vertices, pbox, bounds, bad);
+    k_prim_bounds<<<std::min(nb, 1024u), BVH_BLOCK, 0, s>>>(vertices, indices, n, num_vertices, pbox,
+                                                            bounds, bad);
     k_morton<<<nb, BVH_BLOCK, 0, s>>>(pbox, n, bounds, keys, vals);
     if ((e = hipGetLastError()) != hipSuccess) {
         set_error("BVH kernels: %s", hipGetErrorString(e));
@@ -454,7 +508,13 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
             set_error("hipMemsetAsync: %s", hipGetErrorString(e));
             return fail(RS3D_E_HIP);
         }
-        k_refit<<<nb, BVH_BLOCK, 0, s>>>(child, parent_node, parent_leaf, sbox, n, nbox, height, flags);
+        int* num_pending = bounds + 13;
+        int* pending = reinterpret_cast<int*>(vals);  // the sort permutation is dead here
+        const unsigned nw = (unsigned)((n + REFIT_WIN - 1) / REFIT_WIN);
+        k_refit_local<<<nw, REFIT_WIN, 0, s>>>(child, range, parent_node, parent_leaf, sbox, n,
+                                               nbox, height, pending, num_pending);
+        k_refit_top<<<1, REFIT_WIN, 0, s>>>(child, parent_node, sbox, nbox, height, flags,
+                                            pending, num_pending);
         k_emit_nodes<<<nbi, BVH_BLOCK, 0, s>>>(child, range, sbox, nbox, n, leaf, bvh->nodes);
     } else {
         k_emit_nodes<<<1, 64, 0, s>>>(child, range, sbox, nbox, n, leaf, bvh->nodes);

@@ -249,6 +249,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys
     hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one LSD pass. The tile is first reordered by digit inside LDS (stable:
+// groups of 64 keys in input order, ballot rank inside a group), then written out so that
+// consecutive threads store consecutive addresses of one digit's run.
 template <typename K, bool HAS_VAL>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
@@ -257,22 +260,29 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
                                                          int shift,
                                                          const uint32_t* __restrict__ hist,
                                                          uint32_t nblocks) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[RS_WAVES][256];
+    constexpr int G = RS_ITEMS * RS_WAVES;  // groups of 64 keys per tile, in input order
+    __shared__ uint16_t gcnt[G][256];
+    __shared__ uint32_t dstart[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t wsum[RS_WAVES];
+    __shared__ K sk[RS_TILE];
+    __shared__ uint32_t sv[HAS_VAL ? RS_TILE : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    run[tid] = hist[(uint64_t)tid * nblocks + blockIdx.x];
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint32_t count = (uint32_t)((n - base) < (uint64_t)RS_TILE ? (n - base) : (uint64_t)RS_TILE);
+    for (int i = tid; i < G * 256; i += RS_BLOCK) (&gcnt[0][0])[i] = 0;
+    gbase[tid] = hist[(uint64_t)tid * nblocks + blockIdx.x];
+    __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int j = 0; j < RS_ITEMS; ++j) {
+    K key[RS_ITEMS];
+    uint32_t val[RS_ITEMS], rank[RS_ITEMS];
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
-        const uint64_t i = base + (uint64_t)j * RS_BLOCK + tid;
-        const bool valid = i < n;
-        K key = valid ? kin[i] : (K)0;
-        uint32_t val = 0;
-        if (HAS_VAL && valid) val = vin[i];
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const uint32_t li = (uint32_t)(j * RS_BLOCK + tid);
+        const bool valid = li < count;
+        key[j] = valid ? kin[base + li] : (K)0;
+        val[j] = (HAS_VAL && valid) ? vin[base + li] : 0u;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -280,26 +290,45 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
             const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        const uint32_t rank = (uint32_t)__popcll(m & lt_mask);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        {
-            uint32_t r = run[tid];
+        rank[j] = (uint32_t)__popcll(m & lt_mask);
+        if (valid && rank[j] == 0) gcnt[j * RS_WAVES + wave][d] = (uint16_t)__popcll(m);
+    }
+    __syncthreads();
+    // thread tid owns digit tid: exclusive prefix over the groups, then over the digits
+    uint32_t tot = 0;
+    for (int g = 0; g < G; ++g) {
+        const uint32_t c = gcnt[g][tid];
+        gcnt[g][tid] = (uint16_t)tot;
+        tot += c;
+    }
+    uint32_t inc = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int w = 0; w < wave; ++w) woff += wsum[w];
+    dstart[tid] = woff + inc - tot;
+    __syncthreads();
 #pragma unroll
-            for (int w = 0; w < RS_WAVES; ++w) {
-                uint32_t c = wcnt[w][tid];
-                wcnt[w][tid] = r;
-                r += c;
-            }
-            run[tid] = r;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const uint32_t li = (uint32_t)(j * RS_BLOCK + tid);
+        if (li < count) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+            const uint32_t pos = dstart[d] + gcnt[j * RS_WAVES + wave][d] + rank[j];
+            sk[pos] = key[j];
+            if (HAS_VAL) sv[pos] = val[j];
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t pos = wcnt[wave][d] + rank;
-            kout[pos] = key;
-            if (HAS_VAL) vout[pos] = val;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < count; i += RS_BLOCK) {
+        const K k = sk[i];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = gbase[d] + i - dstart[d];
+        kout[pos] = k;
+        if (HAS_VAL) vout[pos] = sv[i];
     }
 }
 
@@ -557,7 +586,32 @@ int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value) {
         ctx->opt_ray_sort_min = value;
         return RS3D_OK;
     }
+    if (option == RS3D_OPT_RAYS_PER_WAVE) {
+        if (value != 0 && (value < 64 || value > (1 << 20))) {
+            set_error("RS3D_OPT_RAYS_PER_WAVE must be 0 or in [64, 2^20], got %lld", (long long)value);
+            return RS3D_E_ARG;
+        }
+        ctx->opt_rays_per_wave = value;
+        return RS3D_OK;
+    }
     set_error("rs3d_ctx_set_option: unknown option %d", option);
+    return RS3D_E_ARG;
+}
+
+int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value) {
+    if (!ctx || !value) {
+        set_error("rs3d_ctx_get_option: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (option == RS3D_OPT_RAY_SORT_MIN) {
+        *value = ctx->opt_ray_sort_min;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_RAYS_PER_WAVE) {
+        *value = ctx->opt_rays_per_wave;
+        return RS3D_OK;
+    }
+    set_error("rs3d_ctx_get_option: unknown option %d", option);
     return RS3D_E_ARG;
 }
 

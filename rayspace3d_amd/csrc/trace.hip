@@ -43,6 +43,140 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points(const BNode* __restrict__ n
     if (STATS) flush_counters(ctr, stats);
 }
 
+// Lane-refill form of k_points: a wave owns a run of `per_wave` consecutive rays of the
+// (Morton-ordered) ray list, and a lane whose ray has finished takes the next ray of the
+// run in the same iteration, so the wave keeps all lanes busy until its run is spent
+// instead of idling behind its longest ray. Each ray runs exactly the program of
+// point_ray (same traversal order, hit buffer, restarts and chain), one traversal step
+// per loop iteration, so the results are those of k_points.
+template <int AX, int DEPTH>
+__global__ __launch_bounds__(TR_BLOCK) void k_points_refill(const BNode* __restrict__ nodes,
+                                                            const TriRec* __restrict__ tris,
+                                                            const float* __restrict__ pts,
+                                                            uint64_t n,
+                                                            const uint32_t* __restrict__ imap,
+                                                            int out_kind, void* __restrict__ out,
+                                                            uint32_t per_wave) {
+    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    int* stack = stack_mem + threadIdx.x;
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float eps = 1e-6f, tmax = 1e16f;
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    const int lane = threadIdx.x;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t run_lo = (uint64_t)blockIdx.x * per_wave;
+    const uint64_t run_hi = run_lo + per_wave < n ? run_lo + per_wave : n;
+    uint64_t next = run_lo;  // wave-uniform: first ray of the run not handed out yet
+    bool active = false;
+    uint32_t pi = 0;
+    float3 o = make_float3(0.0f, 0.0f, 0.0f);
+    float thr = eps;
+    int count = 0, first_obj = -1, node = 0, sp = 0;
+    uint32_t total = 0;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    kbuf_clear(bt, bi, bo);
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (idle && next < run_hi) {
+            const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
+            if (!active && r < run_hi) {
+                pi = imap ? imap[r] : (uint32_t)r;
+                o = ld3f(pts, pi);
+                thr = eps;
+                count = 0;
+                first_obj = -1;
+                total = 0;
+                node = 0;
+                sp = 0;
+                kbuf_clear(bt, bi, bo);
+                active = true;
+            }
+            next += (uint64_t)__popcll(idle);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+        const NodeBoxes b = load_node(nodes, node);
+        const bool h0 = ou >= b.lo0[U] && ou <= b.hi0[U] && ow >= b.lo0[W] && ow <= b.hi0[W] &&
+                        b.hi0[AX] >= oa;
+        const bool h1 = ou >= b.lo1[U] && ou <= b.hi1[U] && ow >= b.lo1[W] && ow <= b.hi1[W] &&
+                        b.hi1[AX] >= oa;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bool hit = c == 0 ? h0 : h1;
+            const int cnt = c == 0 ? b.n0 : b.n1;
+            const int first = c == 0 ? b.l0 : b.l1;
+            if (hit && cnt > 0) {
+                for (int j = first; j < first + cnt; ++j) {
+                    float3 v0, v1, v2;
+                    uint32_t orig;
+                    int obj;
+                    load_tri(tris, j, v0, v1, v2, orig, obj);
+                    float t;
+                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t < tmax) {
+                        ++total;
+                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, obj);
+                    }
+                }
+            }
+        }
+        const bool in0 = h0 && b.n0 == 0;
+        const bool in1 = h1 && b.n1 == 0;
+        if (in0 && in1) {
+            const bool first0 = b.lo0[AX] <= b.lo1[AX];
+            stack[sp * TR_BLOCK] = first0 ? b.l1 : b.l0;
+            ++sp;
+            node = first0 ? b.l0 : b.l1;
+            continue;
+        }
+        if (in0 || in1) {
+            node = in0 ? b.l0 : b.l1;
+            continue;
+        }
+        if (sp > 0) {
+            --sp;
+            node = stack[sp * TR_BLOCK];
+            continue;
+        }
+        // traversal pass complete: walk the chain inside the buffer (as point_ray)
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                if (count == 0) first_obj = bo[k];
+                ++count;
+                float nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                thr = nthr;
+            }
+        }
+        if (total > (uint32_t)KBUF && count < 10000) {
+            kbuf_clear(bt, bi, bo);
+            total = 0;
+            node = 0;
+            continue;
+        }
+        const int res = (count & 1) ? first_obj : -1;
+        if (out_kind == RS3D_OUT_RAY_RESULT) {
+            rs3d_ray_result rr;
+            rr.ray_id = (int32_t)pi;
+            rr.polygon_index = res;
+            static_cast<rs3d_ray_result*>(out)[pi] = rr;
+        } else if (out_kind == RS3D_OUT_FLAG_U8) {
+            static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
+        } else {
+            static_cast<int32_t*>(out)[pi] = res;
+        }
+        active = false;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Filter: parity of the +AX ray against the 12-triangle box mesh of
 // query/src/geometry/BoundingBox.cpp:26-49 (same vertex order and triangulation).
@@ -168,14 +302,17 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
 // ------------------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------------------
+// The traversal pushes one child only where both children are inner nodes, and such a
+// node has binary height >= 2; heights fall strictly along a path, so at most
+// max_depth - 1 entries are ever pending on the stack.
 int pick_depth(const rs3d_bvh* bvh, int* depth) {
-    const int need = bvh->max_depth + 1;
+    const int need = bvh->max_depth - 1;
     if (need <= 24) *depth = 24;
     else if (need <= 32) *depth = 32;
     else if (need <= 48) *depth = 48;
     else if (need <= 64) *depth = 64;
     else {
-        set_error("BVH depth %d exceeds the traversal stack (64)", bvh->max_depth);
+        set_error("BVH height %d needs more than the traversal stack's 64 entries", bvh->max_depth);
         return RS3D_E_DEPTH;
     }
     return RS3D_OK;
@@ -186,6 +323,19 @@ static int launch_points_ax(rs3d_ctx* ctx, const rs3d_bvh* bvh, int depth, const
                             uint64_t n, const uint32_t* imap, int out_kind, void* out,
                             unsigned long long* stats) {
     unsigned nb;
+    const int64_t pw = ctx->opt_rays_per_wave;
+    if (!STATS && pw >= TR_BLOCK && n <= 0xffffffffull) {
+        RS3D_TRY(grid_blocks(n, (uint64_t)pw, &nb));
+        const uint32_t w = (uint32_t)pw;
+        switch (depth) {
+            case 24: k_points_refill<AX, 24><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, w); break;
+            case 32: k_points_refill<AX, 32><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, w); break;
+            case 48: k_points_refill<AX, 48><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, w); break;
+            default: k_points_refill<AX, 64><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, w); break;
+        }
+        RS3D_HIP(hipGetLastError());
+        return RS3D_OK;
+    }
     RS3D_TRY(grid_blocks(n, TR_BLOCK, &nb));
     switch (depth) {
         case 24: k_points<AX, 24, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, pts, n, imap, out_kind, out, stats); break;
@@ -235,11 +385,12 @@ __device__ __forceinline__ uint32_t spread16(uint32_t v) {
     return v;
 }
 
-// 2-D Morton key of each ray origin in the plane perpendicular to the ray axis.
+// 2-D Morton key of each ray origin in the plane perpendicular to the ray axis, on a grid
+// of (qmax + 1)^2 cells.
 __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ pts, uint64_t n,
                                                   const uint32_t* __restrict__ imap, int axis,
                                                   float lo_u, float lo_w, float sc_u, float sc_w,
-                                                  uint32_t* __restrict__ keys,
+                                                  float qmax, uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -247,8 +398,8 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float* __restrict__ pts,
     const int U = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
     const int W = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
     const float u = pts[3 * (uint64_t)pi + U], w = pts[3 * (uint64_t)pi + W];
-    const float fu = fminf(fmaxf((u - lo_u) * sc_u, 0.0f), 65535.0f);
-    const float fw = fminf(fmaxf((w - lo_w) * sc_w, 0.0f), 65535.0f);
+    const float fu = fminf(fmaxf((u - lo_u) * sc_u, 0.0f), qmax);
+    const float fw = fminf(fmaxf((w - lo_w) * sc_w, 0.0f), qmax);
     keys[i] = (spread16((uint32_t)fu) << 1) | spread16((uint32_t)fw);
     vals[i] = pi;
 }
@@ -273,13 +424,19 @@ int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, 
         const int W = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
         const float lo_u = bvh->bmin[U], lo_w = bvh->bmin[W];
         const float eu = bvh->bmax[U] - lo_u, ew = bvh->bmax[W] - lo_w;
-        const float sc_u = eu > 0.0f ? 65536.0f / eu : 0.0f, sc_w = ew > 0.0f ? 65536.0f / ew : 0.0f;
+        // cells of about 64..256 rays: q bits per axis with 4^q ~ n / 64
+        int q = 4;
+        while (q < 16 && ((uint64_t)64 << (2 * (q + 1))) <= n) ++q;
+        const float cells = (float)(1u << q);
+        const float sc_u = eu > 0.0f ? cells / eu : 0.0f, sc_w = ew > 0.0f ? cells / ew : 0.0f;
         unsigned nb;
         RS3D_TRY(grid_blocks(n, 256, &nb));
         k_ray_keys<<<nb, 256, 0, ctx->stream>>>(pts, n, imap, axis, lo_u, lo_w, sc_u, sc_w,
-                                                 ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
+                                                 cells - 1.0f, ctx->ray_keys.as<uint32_t>(),
+                                                 ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(), n, 32));
+        RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
+                                      n, 2 * q));
         imap = ctx->ray_vals.as<uint32_t>();
     }
     if (ctx->profile) {

@@ -65,6 +65,11 @@ class Context:
     def set_option(self, option: int, value: int):
         check(self.lib.rs3d_ctx_set_option(self.handle, option, value), "rs3d_ctx_set_option")
 
+    def get_option(self, option: int) -> int:
+        v = C.c_int64(0)
+        check(self.lib.rs3d_ctx_get_option(self.handle, option, C.byref(v)), "rs3d_ctx_get_option")
+        return v.value
+
     def profile(self, enable: bool):
         check(self.lib.rs3d_ctx_profile(self.handle, int(enable)), "rs3d_ctx_profile")
 

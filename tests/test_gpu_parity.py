@@ -318,6 +318,46 @@ def test_ray_reordering_is_invisible(ctx):
     assert np.array_equal(outs[0][:, 0], np.arange(len(pts)))
 
 
+@pytest.mark.parametrize("per_wave", [0, 64, 100, 512, 5000])
+def test_lane_refill_matches_oracle(ctx, per_wave):
+    # lane refill on and off, run lengths that do and do not divide the ray count; the
+    # nested-sphere case forces hit-buffer restarts inside refilled lanes
+    print(f"lane refill, rays per wave {per_wave}", flush=True)
+    ctx.set_option(_lib.OPT_RAYS_PER_WAVE, per_wave)
+    try:
+        mesh, pts = gen.many_spheres_workload(seed=29, n_objects=800, n_points=150_001, world=100.0)
+        out, _ = _contain(ctx, mesh, pts, 2, kind=_lib.OUT_RAY_RESULT)
+        ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+        assert np.array_equal(out[:, 1], ref) and np.array_equal(out[:, 0], np.arange(len(pts)))
+        n = 30
+        nested = gen.spheres(2, np.zeros((n, 3)), np.linspace(1.0, 3.0, n))
+        p2 = gen.uniform_points(np.random.Generator(np.random.PCG64(6)), 70_000, -3.2, 3.2)
+        out2, _ = _contain(ctx, nested, p2, 0)
+        ref2 = orc.points_contain(nested.vertices, nested.indices, nested.tri_to_obj, p2, axis=0)
+        assert np.array_equal(out2, ref2)
+    finally:
+        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, 128)
+
+
+def test_autotune_restores_options(ctx):
+    from rayspace3d_amd import autotune
+    mesh, pts = gen.many_spheres_workload(seed=41, n_objects=200, n_points=40_000, world=60.0)
+    dv, di, do = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
+    before = (ctx.get_option(_lib.OPT_RAY_SORT_MIN), ctx.get_option(_lib.OPT_RAYS_PER_WAVE))
+    best, trials = autotune.tune_traversal(ctx, dv, di, do, to_dev(pts), leaf_sizes=(2, 4),
+                                           rays_per_wave=(0, 128), reps=1)
+    assert len(trials) == 8 and best in trials and all(t["ms"] > 0 for t in trials)
+    assert (ctx.get_option(_lib.OPT_RAY_SORT_MIN), ctx.get_option(_lib.OPT_RAYS_PER_WAVE)) == before
+    autotune.apply(ctx, best)
+    try:
+        out, _ = _contain(ctx, mesh, pts, 2, leaf=best["leaf_size"])
+    finally:
+        ctx.set_option(_lib.OPT_RAY_SORT_MIN, before[0])
+        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, before[1])
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    assert np.array_equal(out, ref)
+
+
 def test_batched_equals_per_query(ctx):
     from rayspace3d_amd.device import batch_execute
     rng = np.random.Generator(np.random.PCG64(91))
