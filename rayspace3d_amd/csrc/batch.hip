@@ -1,14 +1,15 @@
 // batch.hip -- many independent containment and overlap queries in one persistent launch.
 //
-// Every query is cut into 64-ray chunks (points of a containment query, edges of one
-// direction of an overlap query); a device chunk table lists (query, first ray) and one
-// persistent kernel of wave-sized workgroups pulls chunks from an atomic queue head until
-// the table is drained, so small and large queries share the GPU without a launch each.
-// Containment rays of all queries are first reordered by one radix sort on
-// (query, 2-D Morton of the origin) so waves stay coherent; overlap pairs of all queries go
-// into one hash table keyed (query << 48 | a << 24 | b), are compacted, radix-sorted once
-// and split per query. Per ray the same programs as the single-query kernels run, so every
-// result is identical to running the query alone.
+// Every query is cut into tasks of TASK_RAYS rays (points of a containment query, edges
+// of one direction of an overlap query, objects of an intersection fallback); one
+// persistent kernel of wave-sized workgroups takes tasks from striped atomic queue heads
+// until all are done, so small and large queries share the GPU without a launch each.
+// A containment task runs the lane-refill point program; edge and object tasks run 64
+// rays at a time. Containment rays of all queries are first reordered by one radix sort
+// on (query, 2-D Morton of the origin) so waves stay coherent; overlap pairs of all
+// queries go into one hash table keyed (query << 48 | a << 24 | b), are compacted,
+// radix-sorted once and split per query. Per ray the same programs as the single-query
+// kernels run, so every result is identical to running the query alone.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -26,96 +27,114 @@ __device__ __forceinline__ uint32_t spread2_11(uint32_t v) {  // 11 bits -> ever
     return v;
 }
 
-// Sort keys of containment rays: (descriptor index << 22) | 22-bit Morton of (u, w).
-__global__ __launch_bounds__(256) void k_batch_keys(const BatchDesc* __restrict__ qs,
-                                                    const float* __restrict__ lo_sc, uint32_t nq,
-                                                    const uint64_t* __restrict__ ray_base,
-                                                    uint64_t total, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= total) return;
-    // query of ray r: binary search over the ray bases
-    uint32_t lo = 0, hi = nq;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ray_base[mid] <= r) lo = mid;
-        else hi = mid;
+// Sort keys of containment rays: (descriptor index << 16) | 16-bit Morton of (u, w) on the
+// query's own grid. One workgroup per containment task (the first tasks of the table).
+__global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __restrict__ qs,
+                                                          const uint64_t* __restrict__ task_off,
+                                                          const float* __restrict__ lo_sc,
+                                                          uint32_t n_contain,
+                                                          uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+    const uint64_t t = blockIdx.x;
+    uint32_t qa = 0, qb = n_contain;  // task_off[qa] <= t < task_off[qb]
+    while (qb - qa > 1) {
+        const uint32_t m = (qa + qb) >> 1;
+        if (task_off[m] <= t) qa = m;
+        else qb = m;
     }
-    const BatchDesc& q = qs[lo];
-    const uint64_t i = r - q.ray_base;
+    const BatchDesc& q = qs[qa];
+    const uint64_t i = (t - task_off[qa]) * TASK_RAYS + threadIdx.x;
+    if (i >= q.n) return;
     const int U = q.axis == 0 ? 1 : (q.axis == 1 ? 2 : 0);
     const int W = q.axis == 0 ? 2 : (q.axis == 1 ? 0 : 1);
     const float u = q.pts[3 * i + U], w = q.pts[3 * i + W];
-    const float fu = fminf(fmaxf((u - lo_sc[4 * lo]) * lo_sc[4 * lo + 2], 0.0f), 2047.0f);
-    const float fw = fminf(fmaxf((w - lo_sc[4 * lo + 1]) * lo_sc[4 * lo + 3], 0.0f), 2047.0f);
-    keys[r] = (lo << 22) | (spread2_11((uint32_t)fu) << 1) | spread2_11((uint32_t)fw);
-    vals[r] = (uint32_t)i;
+    const float fu = fminf(fmaxf((u - lo_sc[4 * qa]) * lo_sc[4 * qa + 2], 0.0f), 255.0f);
+    const float fw = fminf(fmaxf((w - lo_sc[4 * qa + 1]) * lo_sc[4 * qa + 3], 0.0f), 255.0f);
+    keys[q.ray_base + i] = (qa << 16) | (spread2_11((uint32_t)fu) << 1) | spread2_11((uint32_t)fw);
+    vals[q.ray_base + i] = (uint32_t)i;
 }
 
 template <int DEPTH>
 __global__ __launch_bounds__(TR_BLOCK) void k_batch(const BatchDesc* __restrict__ qs,
-                                                    const Chunk* __restrict__ chunks, uint32_t nchunks,
-                                                    unsigned int* __restrict__ head,
+                                                    const uint64_t* __restrict__ task_off,
+                                                    uint32_t nq, uint64_t ntasks,
+                                                    unsigned int* __restrict__ heads,
                                                     const uint32_t* __restrict__ perm, HashArgs H,
                                                     unsigned long long* __restrict__ bad_ids) {
     __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    int* stack = stack_mem + threadIdx.x;
     const int lane = threadIdx.x;
     TraceCounters ctr;
-    while (true) {
-        unsigned int c = 0;
-        if (lane == 0) c = atomicAdd(head, 1u);
-        c = __shfl(c, 0, 64);
-        if (c >= nchunks) break;
-        const Chunk ch = chunks[c];
-        const BatchDesc q = qs[ch.desc];
-        const uint64_t i = (uint64_t)ch.start + lane;
-        if (i >= q.n) continue;
+    uint32_t stripe = blockIdx.x % NSTRIPE;
+    for (int dry = 0; dry < NSTRIPE;) {
+        unsigned int k = 0;
+        if (lane == 0) k = atomicAdd(&heads[stripe * STRIPE_PITCH], 1u);
+        k = __shfl(k, 0, 64);
+        const uint64_t t = (uint64_t)k * NSTRIPE + stripe;
+        if (t >= ntasks) {
+            stripe = stripe + 1 == NSTRIPE ? 0 : stripe + 1;
+            ++dry;
+            continue;
+        }
+        uint32_t qa = 0, qb = nq;  // task_off[qa] <= t < task_off[qb]
+        while (qb - qa > 1) {
+            const uint32_t m = (qa + qb) >> 1;
+            if (task_off[m] <= t) qa = m;
+            else qb = m;
+        }
+        const BatchDesc& q = qs[qa];
+        const uint64_t lo = (t - task_off[qa]) * TASK_RAYS;
+        const uint64_t hi = lo + TASK_RAYS < q.n ? lo + TASK_RAYS : q.n;
         if (q.kind == BQ_CONTAIN) {
-            const uint64_t pi = perm ? perm[q.ray_base + i] : i;
-            const float3 o = ld3f(q.pts, pi);
-            int obj;
-            if (q.axis == 0) obj = point_ray<0, DEPTH, false>(q.nodes, q.tris, o, stack_mem + lane, ctr);
-            else if (q.axis == 1) obj = point_ray<1, DEPTH, false>(q.nodes, q.tris, o, stack_mem + lane, ctr);
-            else obj = point_ray<2, DEPTH, false>(q.nodes, q.tris, o, stack_mem + lane, ctr);
-            q.out[pi] = obj;
-        } else if (q.kind == BQ_OBJECTS) {
-            const int ft = q.first_tri[i];
-            if (ft == 0x7fffffff) continue;
-            const float3 o = ld3f(q.src_v, q.src_i[3 * (uint64_t)ft]);
-            int* ids = q.cand_ids + i * (uint64_t)q.max_targets;
-            uint8_t* par = q.cand_par + i * (uint64_t)q.max_targets;
-            unsigned long long over = 0;
-            const int cnt = object_ray_parity<DEPTH>(q.nodes, q.tris, o, stack_mem + lane, ids, par,
-                                                     q.max_targets, over);
-            if (over) atomicAdd(bad_ids + 1, over);
-            const uint64_t qk = (uint64_t)q.qid << 48;
-            for (int j = 0; j < cnt; ++j) {
-                if (!(par[j] & 1)) continue;
-                const uint32_t a = (uint32_t)(q.swap ? ids[j] : (int)i);
-                const uint32_t b = (uint32_t)(q.swap ? (int)i : ids[j]);
-                if ((a | b) >= (1u << 24)) {
-                    atomicAdd(bad_ids, 1ull);
-                    continue;
+            const uint32_t* im = perm ? perm + q.ray_base : nullptr;
+            if (q.axis == 0)
+                points_refill<0, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
+            else if (q.axis == 1)
+                points_refill<1, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
+            else
+                points_refill<2, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
+            continue;
+        }
+        for (uint64_t i = lo + lane; i < hi; i += TR_BLOCK) {
+            if (q.kind == BQ_OBJECTS) {
+                const int ft = q.first_tri[i];
+                if (ft == 0x7fffffff) continue;
+                const float3 o = ld3f(q.src_v, q.src_i[3 * (uint64_t)ft]);
+                int* ids = q.cand_ids + i * (uint64_t)q.max_targets;
+                uint8_t* par = q.cand_par + i * (uint64_t)q.max_targets;
+                unsigned long long over = 0;
+                const int cnt = object_ray_parity<DEPTH>(q.nodes, q.tris, o, stack, ids, par,
+                                                         q.max_targets, over);
+                if (over) atomicAdd(bad_ids + 1, over);
+                const uint64_t qk = (uint64_t)q.qid << 48;
+                for (int j = 0; j < cnt; ++j) {
+                    if (!(par[j] & 1)) continue;
+                    const uint32_t a = (uint32_t)(q.swap ? ids[j] : (int)i);
+                    const uint32_t b = (uint32_t)(q.swap ? (int)i : ids[j]);
+                    if ((a | b) >= (1u << 24)) {
+                        atomicAdd(bad_ids, 1ull);
+                        continue;
+                    }
+                    ht_insert(H, qk | ((uint64_t)a << 24) | b);
                 }
-                ht_insert(H, qk | ((uint64_t)a << 24) | b);
-            }
-        } else {
-            const int nsrc = q.E.src_cnt[i];
-            const int soff = q.E.src_off[i];
-            const uint64_t qk = (uint64_t)q.qid << 48;
-            edge_ray<DEPTH, false>(q.nodes, q.tris, ld3f(q.E.starts, i), ld3f(q.E.ends, i), CHAIN_EPS, 100,
-                                   stack_mem + lane, ctr, [&](int tgt) {
-                                       for (int s = 0; s < nsrc; ++s) {
-                                           const int src = q.E.src_obj[soff + s];
-                                           const uint32_t a = (uint32_t)(q.swap ? tgt : src);
-                                           const uint32_t b = (uint32_t)(q.swap ? src : tgt);
-                                           if ((a | b) >= (1u << 24)) {
-                                               atomicAdd(bad_ids, 1ull);
-                                               continue;
+            } else {
+                const int nsrc = q.E.src_cnt[i];
+                const int soff = q.E.src_off[i];
+                const uint64_t qk = (uint64_t)q.qid << 48;
+                edge_ray<DEPTH, false>(q.nodes, q.tris, ld3f(q.E.starts, i), ld3f(q.E.ends, i), CHAIN_EPS,
+                                       100, stack, ctr, [&](int tgt) {
+                                           for (int s = 0; s < nsrc; ++s) {
+                                               const int src = q.E.src_obj[soff + s];
+                                               const uint32_t a = (uint32_t)(q.swap ? tgt : src);
+                                               const uint32_t b = (uint32_t)(q.swap ? src : tgt);
+                                               if ((a | b) >= (1u << 24)) {
+                                                   atomicAdd(bad_ids, 1ull);
+                                                   continue;
+                                               }
+                                               ht_insert(H, qk | ((uint64_t)a << 24) | b);
                                            }
-                                           ht_insert(H, qk | ((uint64_t)a << 24) | b);
-                                       }
-                                   });
+                                       });
+            }
         }
     }
 }
@@ -150,8 +169,11 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
     const float eu = b->bmax[U] - b->bmin[U], ew = b->bmax[W] - b->bmin[W];
     d.lo_sc[0] = b->bmin[U];
     d.lo_sc[1] = b->bmin[W];
-    d.lo_sc[2] = eu > 0 ? 2048.0f / eu : 0.0f;
-    d.lo_sc[3] = ew > 0 ? 2048.0f / ew : 0.0f;
+    int qb = 1;  // 4^qb cells of about 64..256 rays, at most 256 x 256
+    while (qb < 8 && ((uint64_t)64 << (2 * (qb + 1))) <= n) ++qb;
+    const float cells = (float)(1u << qb);
+    d.lo_sc[2] = eu > 0 ? cells / eu : 0.0f;
+    d.lo_sc[3] = ew > 0 ? cells / ew : 0.0f;
     items.push_back(d);
     return RS3D_OK;
 }
@@ -215,43 +237,33 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
         set_error("batch: at most 1023 containment queries per launch");
         return RS3D_E_ARG;
     }
-    std::vector<BatchDesc> descs(items.begin(), items.end());
-    std::vector<float> lo_sc;
-    std::vector<uint64_t> ray_base(n_contain + 1, contain_rays);
-    for (uint32_t k = 0; k < n_contain; ++k) {
-        ray_base[k] = items[k].ray_base;
-        lo_sc.insert(lo_sc.end(), items[k].lo_sc, items[k].lo_sc + 4);
-    }
-    std::vector<Chunk> chunks;
-    for (uint32_t k = 0; k < descs.size(); ++k)
-        for (uint64_t s = 0; s < descs[k].n; s += TR_BLOCK) {
-            if (s > 0xffffffffull) {
-                set_error("batch: query too large");
-                return RS3D_E_ARG;
-            }
-            chunks.push_back({k, (uint32_t)s});
-        }
-    // big queries first: their chunks are pulled first, the small ones fill the tail
-    std::stable_sort(chunks.begin(), chunks.end(),
-                     [&](const Chunk& a, const Chunk& b) { return descs[a.desc].n > descs[b.desc].n; });
+    const uint32_t nq = (uint32_t)items.size();
+    std::vector<uint64_t> task_off(nq + 1, 0);
+    for (uint32_t k = 0; k < nq; ++k) task_off[k + 1] = task_off[k] + (items[k].n + TASK_RAYS - 1) / TASK_RAYS;
+    const uint64_t ntasks = task_off[nq];
+    // one staging block: descriptors | Morton frames | task offsets
+    const auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t o_desc = 0;
-    const size_t o_chunk = (descs.size() * sizeof(BatchDesc) + 255) & ~size_t(255);
-    const size_t o_losc = o_chunk + ((chunks.size() * sizeof(Chunk) + 255) & ~size_t(255));
-    const size_t o_base = o_losc + ((lo_sc.size() * sizeof(float) + 255) & ~size_t(255));
-    const size_t o_head = o_base + (((n_contain + 1) * sizeof(uint64_t) + 255) & ~size_t(255));
-    RS3D_TRY(meta.ensure(o_head + 256));
+    const size_t o_losc = up(nq * sizeof(BatchDesc));
+    const size_t o_task = o_losc + up(4 * (size_t)n_contain * sizeof(float));
+    const size_t o_head = o_task + up((nq + 1) * sizeof(uint64_t));
+    const size_t o_bad = o_head + NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);
+    const size_t bytes = o_bad + 256;
+    host_meta.assign(o_head, 0);
+    for (uint32_t k = 0; k < nq; ++k) {
+        const BatchDesc d = items[k];
+        std::memcpy(host_meta.data() + o_desc + k * sizeof(BatchDesc), &d, sizeof d);
+    }
+    for (uint32_t k = 0; k < n_contain; ++k) {
+        std::memcpy(host_meta.data() + o_losc + 4 * k * sizeof(float), items[k].lo_sc, 4 * sizeof(float));
+    }
+    std::memcpy(host_meta.data() + o_task, task_off.data(), (nq + 1) * sizeof(uint64_t));
+    RS3D_TRY(meta.ensure(bytes));
     uint8_t* mb = meta.as<uint8_t>();
-    if (!descs.empty())
-        RS3D_HIP(hipMemcpyAsync(mb + o_desc, descs.data(), descs.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, ctx->stream));
-    if (!chunks.empty())
-        RS3D_HIP(hipMemcpyAsync(mb + o_chunk, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, ctx->stream));
-    if (!lo_sc.empty())
-        RS3D_HIP(hipMemcpyAsync(mb + o_losc, lo_sc.data(), lo_sc.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    RS3D_HIP(hipMemcpyAsync(mb + o_base, ray_base.data(), ray_base.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
-    unsigned int* head = reinterpret_cast<unsigned int*>(mb + o_head);
-    unsigned long long* bad = reinterpret_cast<unsigned long long*>(mb + o_head + 64);
-    // host copies of the descriptors only stay valid until the stream drains
-    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    RS3D_HIP(hipMemcpyAsync(mb, host_meta.data(), o_head, hipMemcpyHostToDevice, ctx->stream));
+    const BatchDesc* dp = reinterpret_cast<const BatchDesc*>(mb + o_desc);
+    unsigned int* heads = reinterpret_cast<unsigned int*>(mb + o_head);
+    unsigned long long* bad = reinterpret_cast<unsigned long long*>(mb + o_bad);
     const uint32_t* perm = nullptr;
     if (contain_rays > 0 && ctx->opt_ray_sort_min > 0) {
         if (contain_rays > 0xffffffffull) {
@@ -260,22 +272,24 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
         }
         RS3D_TRY(ctx->ray_keys.ensure(contain_rays * sizeof(uint32_t)));
         RS3D_TRY(ctx->ray_vals.ensure(contain_rays * sizeof(uint32_t)));
-        unsigned nb;
-        RS3D_TRY(grid_blocks(contain_rays, 256, &nb));
-        k_batch_keys<<<nb, 256, 0, ctx->stream>>>(reinterpret_cast<BatchDesc*>(mb + o_desc),
-                                                   reinterpret_cast<float*>(mb + o_losc), n_contain,
-                                                   reinterpret_cast<uint64_t*>(mb + o_base), contain_rays,
-                                                   ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
+        const uint64_t ctasks = task_off[n_contain];
+        if (ctasks > 0x7fffffffull) {
+            set_error("batch: too many containment tasks");
+            return RS3D_E_ARG;
+        }
+        k_batch_keys<<<(unsigned)ctasks, TASK_RAYS, 0, ctx->stream>>>(
+            dp, reinterpret_cast<const uint64_t*>(mb + o_task), reinterpret_cast<const float*>(mb + o_losc),
+            n_contain, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
-        int qbits = 0;  // the descriptor index occupies the bits above the 22 Morton bits
+        int qbits = 0;  // the descriptor index occupies the bits above the 16 Morton bits
         while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
         RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
-                                      contain_rays, 22 + qbits));
+                                      contain_rays, 16 + qbits));
         perm = ctx->ray_vals.as<uint32_t>();
     }
     uint64_t slots = 1024;
     while (slots < slot_hint) slots <<= 1;
-    const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 16u;
+    const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 20u;
     while (true) {
         HashArgs H{};
         if (any_pairs) {
@@ -287,17 +301,15 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
             H.failed = reinterpret_cast<unsigned long long*>(table.as<uint8_t>() + slots * sizeof(uint64_t));
             RS3D_TRY(ht_clear_internal(ctx, table.as<uint64_t>(), slots, reinterpret_cast<uint64_t*>(H.failed)));
         }
-        RS3D_HIP(hipMemsetAsync(head, 0, 128, ctx->stream));
-        if (!chunks.empty()) {
-            const Chunk* cp = reinterpret_cast<Chunk*>(mb + o_chunk);
-            const BatchDesc* dp = reinterpret_cast<BatchDesc*>(mb + o_desc);
-            const unsigned grid = (unsigned)std::min<uint64_t>(waves, chunks.size());
-            const uint32_t nc = (uint32_t)chunks.size();
+        RS3D_TRY(fill_u32(ctx, heads, (bytes - o_head) / sizeof(unsigned int), 0u));
+        if (ntasks > 0) {
+            const uint64_t* tp = reinterpret_cast<const uint64_t*>(mb + o_task);
+            const unsigned grid = (unsigned)std::min<uint64_t>(waves, ntasks);
             switch (depth) {
-                case 24: k_batch<24><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, cp, nc, head, perm, H, bad); break;
-                case 32: k_batch<32><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, cp, nc, head, perm, H, bad); break;
-                case 48: k_batch<48><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, cp, nc, head, perm, H, bad); break;
-                default: k_batch<64><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, cp, nc, head, perm, H, bad); break;
+                case 24: k_batch<24><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+                case 32: k_batch<32><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+                case 48: k_batch<48><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+                default: k_batch<64><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
             }
             RS3D_HIP(hipGetLastError());
         }

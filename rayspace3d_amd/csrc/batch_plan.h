@@ -34,10 +34,14 @@ struct BatchDesc {
     int qid;
 };
 
-struct Chunk {
-    uint32_t desc;
-    uint32_t start;
-};
+// Work distribution: every query is cut into tasks of TASK_RAYS consecutive rays; task
+// t belongs to the query q with task_off[q] <= t < task_off[q + 1]. Tasks are dealt from
+// NSTRIPE queue heads (task t sits in stripe t % NSTRIPE), each on its own 256-byte line,
+// so the atomics of the persistent waves spread over many addresses; a wave starts on the
+// stripe of its workgroup index and moves on when a stripe runs dry.
+constexpr int TASK_RAYS = 128;
+constexpr int NSTRIPE = 32;
+constexpr int STRIPE_PITCH = 64;  // 32-bit words between heads
 
 // Host-side descriptor plus the containment Morton frame (lo_u, lo_w, scale_u, scale_w).
 struct BatchItem : BatchDesc {
@@ -75,6 +79,7 @@ private:
     uint64_t contain_rays = 0;
     bool any_pairs = false;
     DevBuf meta, table;
+    std::vector<uint8_t> host_meta;  // staging of the uploaded tables (outlives the copy)
 };
 
 }  // namespace rs3d

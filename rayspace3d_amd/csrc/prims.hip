@@ -369,6 +369,21 @@ static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, i
     return RS3D_OK;
 }
 
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Fill with a kernel rather than hipMemsetAsync, so the call can sit in a captured graph.
+int fill_u32(rs3d_ctx* ctx, void* p, uint64_t n, uint32_t v) {
+    if (n == 0) return RS3D_OK;
+    const unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<nb, 256, 0, ctx->stream>>>(static_cast<uint32_t*>(p), n, v);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
 int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit) {
     if (n <= 1) return RS3D_OK;
     RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint32_t)));

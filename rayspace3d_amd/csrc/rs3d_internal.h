@@ -199,6 +199,7 @@ int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
 int grid_blocks(uint64_t n, int block, unsigned* out);
+int fill_u32(rs3d_ctx* ctx, void* p, uint64_t n, uint32_t v);  // kernel fill (graph-safe)
 int prof_collect(rs3d_ctx* ctx);  // folds a finished timed launch into the totals
 int pick_depth(const rs3d_bvh* bvh, int* depth);  // LDS stack size for a BVH
 int ht_clear_internal(rs3d_ctx* ctx, uint64_t* slots, uint64_t size, uint64_t* failed);

@@ -43,12 +43,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points(const BNode* __restrict__ n
     if (STATS) flush_counters(ctr, stats);
 }
 
-// Lane-refill form of k_points: a wave owns a run of `per_wave` consecutive rays of the
-// (Morton-ordered) ray list, and a lane whose ray has finished takes the next ray of the
-// run in the same iteration, so the wave keeps all lanes busy until its run is spent
-// instead of idling behind its longest ray. Each ray runs exactly the program of
-// point_ray (same traversal order, hit buffer, restarts and chain), one traversal step
-// per loop iteration, so the results are those of k_points.
+// Lane-refill form of k_points (see points_refill): wave b owns rays
+// [b * per_wave, (b + 1) * per_wave) of the ray list.
 template <int AX, int DEPTH>
 __global__ __launch_bounds__(TR_BLOCK) void k_points_refill(const BNode* __restrict__ nodes,
                                                             const TriRec* __restrict__ tris,
@@ -58,123 +54,9 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_refill(const BNode* __restr
                                                             int out_kind, void* __restrict__ out,
                                                             uint32_t per_wave) {
     __shared__ int stack_mem[DEPTH * TR_BLOCK];
-    int* stack = stack_mem + threadIdx.x;
-    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    const float eps = 1e-6f, tmax = 1e16f;
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
-    const int lane = threadIdx.x;
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t run_lo = (uint64_t)blockIdx.x * per_wave;
-    const uint64_t run_hi = run_lo + per_wave < n ? run_lo + per_wave : n;
-    uint64_t next = run_lo;  // wave-uniform: first ray of the run not handed out yet
-    bool active = false;
-    uint32_t pi = 0;
-    float3 o = make_float3(0.0f, 0.0f, 0.0f);
-    float thr = eps;
-    int count = 0, first_obj = -1, node = 0, sp = 0;
-    uint32_t total = 0;
-    float bt[KBUF];
-    uint32_t bi[KBUF];
-    int bo[KBUF];
-    kbuf_clear(bt, bi, bo);
-    while (true) {
-        const uint64_t idle = __ballot(!active);
-        if (idle && next < run_hi) {
-            const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
-            if (!active && r < run_hi) {
-                pi = imap ? imap[r] : (uint32_t)r;
-                o = ld3f(pts, pi);
-                thr = eps;
-                count = 0;
-                first_obj = -1;
-                total = 0;
-                node = 0;
-                sp = 0;
-                kbuf_clear(bt, bi, bo);
-                active = true;
-            }
-            next += (uint64_t)__popcll(idle);
-        }
-        if (__ballot(active) == 0) break;
-        if (!active) continue;
-        const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
-        const NodeBoxes b = load_node(nodes, node);
-        const bool h0 = ou >= b.lo0[U] && ou <= b.hi0[U] && ow >= b.lo0[W] && ow <= b.hi0[W] &&
-                        b.hi0[AX] >= oa;
-        const bool h1 = ou >= b.lo1[U] && ou <= b.hi1[U] && ow >= b.lo1[W] && ow <= b.hi1[W] &&
-                        b.hi1[AX] >= oa;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const bool hit = c == 0 ? h0 : h1;
-            const int cnt = c == 0 ? b.n0 : b.n1;
-            const int first = c == 0 ? b.l0 : b.l1;
-            if (hit && cnt > 0) {
-                for (int j = first; j < first + cnt; ++j) {
-                    float3 v0, v1, v2;
-                    uint32_t orig;
-                    int obj;
-                    load_tri(tris, j, v0, v1, v2, orig, obj);
-                    float t;
-                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t < tmax) {
-                        ++total;
-                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
-                            kbuf_insert(bt, bi, bo, t, orig, obj);
-                    }
-                }
-            }
-        }
-        const bool in0 = h0 && b.n0 == 0;
-        const bool in1 = h1 && b.n1 == 0;
-        if (in0 && in1) {
-            const bool first0 = b.lo0[AX] <= b.lo1[AX];
-            stack[sp * TR_BLOCK] = first0 ? b.l1 : b.l0;
-            ++sp;
-            node = first0 ? b.l0 : b.l1;
-            continue;
-        }
-        if (in0 || in1) {
-            node = in0 ? b.l0 : b.l1;
-            continue;
-        }
-        if (sp > 0) {
-            --sp;
-            node = stack[sp * TR_BLOCK];
-            continue;
-        }
-        // traversal pass complete: walk the chain inside the buffer (as point_ray)
-#pragma unroll
-        for (int k = 0; k < KBUF; ++k) {
-            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
-                if (count == 0) first_obj = bo[k];
-                ++count;
-                float nthr = bt[k] + eps;
-                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
-                thr = nthr;
-            }
-        }
-        if (total > (uint32_t)KBUF && count < 10000) {
-            kbuf_clear(bt, bi, bo);
-            total = 0;
-            node = 0;
-            continue;
-        }
-        const int res = (count & 1) ? first_obj : -1;
-        if (out_kind == RS3D_OUT_RAY_RESULT) {
-            rs3d_ray_result rr;
-            rr.ray_id = (int32_t)pi;
-            rr.polygon_index = res;
-            static_cast<rs3d_ray_result*>(out)[pi] = rr;
-        } else if (out_kind == RS3D_OUT_FLAG_U8) {
-            static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
-        } else {
-            static_cast<int32_t*>(out)[pi] = res;
-        }
-        active = false;
-    }
+    const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
+    const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
+    points_refill<AX, DEPTH>(nodes, tris, pts, imap, lo, hi, out_kind, out, stack_mem + threadIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
