@@ -119,7 +119,6 @@ typedef struct {
 } rs3d_trace_stats;
 int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
                             int axis, rs3d_trace_stats* stats /* host */);
-
 /* ---- filter-refine (replaces FilterRefinePipeline::execute,
  *      query/src/raytracing/FilterRefine.cpp:81-122) --------------------------------------
  * Filter: +axis ray parity against the 12-triangle box mesh of the dataset AABB
@@ -163,6 +162,15 @@ typedef struct {
 
 /* Pass 1: counts[e] = chain hits * source objects of edge e. */
 int rs3d_edge_count(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int32_t* counts);
+/* Work counters of the edge rays of `edges` against `target` (one counting pass of the
+ * A.2 chain, as rs3d_edge_count): pair_emits = sum over edges of chain hits x sources
+ * (the hash inserts of a hash-mode join), source_refs = sum of the edges' source counts.
+ * No reference counterpart (measurement of the algorithmic bytes). */
+typedef struct {
+    uint64_t rays, nodes_visited, tris_tested, hits, restarts, pair_emits, source_refs;
+} rs3d_edge_stats_t;
+int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
+                    rs3d_edge_stats_t* stats /* host */);
 /* Pass 2: writes raw pairs at offsets[e] (exclusive scan of counts). swap: emit (tgt, src). */
 int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
                     const int64_t* offsets, int swap, rs3d_pair* out);

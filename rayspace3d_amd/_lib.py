@@ -54,6 +54,11 @@ class TraceStats(C.Structure):
     ]
 
 
+class EdgeStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("rays", "nodes_visited", "tris_tested", "hits",
+                                           "restarts", "pair_emits", "source_refs")]
+
+
 class Edges(C.Structure):
     _fields_ = [
         ("starts", C.c_void_p),
@@ -134,6 +139,7 @@ SIGNATURES = {
     "rs3d_bvh_info_get": (I32, [P, C.POINTER(BvhInfo)]),
     "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
+    "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),
     "rs3d_filter_refine": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
     "rs3d_filter_box": (I32, [P, C.POINTER(C.c_float), P, U64, I32, P]),
     "rs3d_scan_i32_i64": (I32, [P, P, P, U64, PI64]),

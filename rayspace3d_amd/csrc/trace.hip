@@ -181,6 +181,16 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
     if (STATS) flush_counters(ctr, stats);
 }
 
+__global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ v, uint64_t n,
+                                                 unsigned long long* __restrict__ acc) {
+    unsigned long long s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        s += (unsigned long long)(v[i] > 0 ? v[i] : 0);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, s);
+}
+
 // ------------------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------------------
@@ -336,7 +346,8 @@ int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, 
 template <int MODE>
 static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* edges, int swap,
                         int32_t* counts, const int64_t* offsets, int64_t bias, void* out,
-                        const HashArgs& H, int chain_rule = CHAIN_EPS, int max_iter = 100) {
+                        const HashArgs& H, int chain_rule = CHAIN_EPS, int max_iter = 100,
+                        unsigned long long* stats = nullptr) {
     if (edges->num_edges == 0) return RS3D_OK;
     EdgeArgs E{edges->starts, edges->ends, edges->src_offsets, edges->src_counts, edges->src_objects,
                edges->num_edges};
@@ -349,6 +360,20 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     }
     int depth;
     RS3D_TRY(pick_depth(bvh, &depth));
+    if (stats) {
+        switch (depth) {
+            case 24: k_edges<MODE, 24, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            case 32: k_edges<MODE, 32, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            case 48: k_edges<MODE, 48, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            default: k_edges<MODE, 64, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+        }
+        RS3D_HIP(hipGetLastError());
+        return RS3D_OK;
+    }
+    if (ctx->profile) {
+        RS3D_TRY(prof_collect(ctx));
+        RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    }
     switch (depth) {
         case 24: k_edges<MODE, 24, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
         case 32: k_edges<MODE, 32, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
@@ -356,6 +381,10 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
         default: k_edges<MODE, 64, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
     }
     RS3D_HIP(hipGetLastError());
+    if (ctx->profile) {
+        RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        ctx->prof_pending = true;
+    }
     return RS3D_OK;
 }
 
@@ -419,6 +448,41 @@ int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* poi
     stats->tris_tested = h[1];
     stats->hits = h[2];
     stats->restarts = h[3];
+    return RS3D_OK;
+}
+
+int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
+                    rs3d_edge_stats_t* stats) {
+    if (!ctx || !target || !stats || !edges_ok(edges)) {
+        set_error("rs3d_edge_stats: bad arguments");
+        return RS3D_E_ARG;
+    }
+    memset(stats, 0, sizeof *stats);
+    const uint64_t n = edges->num_edges;
+    stats->rays = n;
+    if (n == 0 || target->num_tris == 0) return RS3D_OK;
+    RS3D_TRY(ctx->small.ensure(256));
+    RS3D_TRY(ctx->tmp_a.ensure(n * sizeof(int32_t)));
+    unsigned long long* dstats = ctx->small.as<unsigned long long>() + 16;
+    unsigned long long* dsum = dstats + 4;
+    RS3D_HIP(hipMemsetAsync(dstats, 0, 6 * sizeof(unsigned long long), ctx->stream));
+    HashArgs H{};
+    int32_t* counts = ctx->tmp_a.as<int32_t>();
+    RS3D_TRY(launch_edges<EMODE_COUNT>(ctx, target, edges, 0, counts, nullptr, 0, nullptr, H,
+                                       CHAIN_EPS, 100, dstats));
+    unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
+    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(counts, n, dsum);
+    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(edges->src_counts, n, dsum + 1);
+    RS3D_HIP(hipGetLastError());
+    unsigned long long h[6];
+    RS3D_HIP(hipMemcpyAsync(h, dstats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    stats->nodes_visited = h[0];
+    stats->tris_tested = h[1];
+    stats->hits = h[2];
+    stats->restarts = h[3];
+    stats->pair_emits = h[4];
+    stats->source_refs = h[5];
     return RS3D_OK;
 }
 

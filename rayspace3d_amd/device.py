@@ -212,6 +212,15 @@ def edge_count(ctx, bvh, dedges: DevEdges) -> torch.Tensor:
     return counts[:dedges.s.num_edges]
 
 
+def edge_stats(ctx, bvh, dedges: DevEdges) -> dict:
+    """Work counters of the edge rays of `dedges` against `bvh` (rs3d_edge_stats)."""
+    st = _lib.EdgeStats()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_edge_stats(ctx.handle, bvh.handle, C.byref(dedges.s), C.byref(st)),
+          "rs3d_edge_stats")
+    return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+
 def overlap_join(ctx, bvh1, dedges1, bvh2, dedges2, mode: int = 0, max_out: int = 1 << 22) -> np.ndarray:
     """Sorted unique (mesh1, mesh2) keys of both edge directions (uint64 a << 32 | b)."""
     while True:

@@ -152,15 +152,17 @@ def many_spheres_workload(seed: int = 2, n_objects: int = 10_000, n_points: int 
 
 
 def overlap_dataset(seed: int, n_objects: int = 50_000, world: float = 1000.0,
-                    levels=(2, 3, 4), r0: float = 5.0) -> Mesh:
+                    levels=(2, 3, 4), r0: float = 5.0, return_levels: bool = False):
     """Anisotropically scaled (0.5-2x per axis), randomly rotated icospheres of levels
-    drawn uniformly from `levels`, centres uniform in [0, world]^3."""
+    drawn uniformly from `levels`, centres uniform in [0, world]^3. With return_levels,
+    also returns the per-object level array (for sphere_field_edges)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lv = rng.choice(np.asarray(levels), size=n_objects)
     centers = rng.uniform(0.0, world, size=(n_objects, 3))
     scales = rng.uniform(0.5, 2.0, size=(n_objects, 3))
     rots = random_rotations(rng, n_objects)
-    return spheres(lv, centers, np.full(n_objects, r0), scales=scales, rotations=rots)
+    mesh = spheres(lv, centers, np.full(n_objects, r0), scales=scales, rotations=rots)
+    return (mesh, lv) if return_levels else mesh
 
 
 # ---- unique-edge extraction (preprocess/src/EdgePreprocessor.cpp:41-119) --------------
@@ -271,18 +273,31 @@ def _template_edges(level: int):
     return a[first], b[first]
 
 
-def sphere_field_edges(mesh: Mesh, level: int) -> EdgeSet:
-    """Edge payload of a mesh built by spheres() with one icosphere level per object:
-    the same (start, end, source) records as extract_edges (objects share no vertices),
-    grouped by object instead of sorted by quantised key. Join results do not depend on
-    edge order."""
-    v, f = icosphere(level)
-    nv, n_obj = len(v), mesh.num_objects
-    e0, e1 = _template_edges(level)
-    base = (np.arange(n_obj, dtype=np.int64) * nv)[:, None]
-    i0 = (base + e0[None, :]).reshape(-1)
-    i1 = (base + e1[None, :]).reshape(-1)
-    ne = len(i0)
-    src = np.repeat(np.arange(n_obj, dtype=np.int32), len(e0))
-    return EdgeSet(np.ascontiguousarray(mesh.vertices[i0]), np.ascontiguousarray(mesh.vertices[i1]),
-                   np.arange(ne, dtype=np.int32), np.ones(ne, dtype=np.int32), src)
+def sphere_field_edges(mesh: Mesh, level) -> EdgeSet:
+    """Edge payload of a mesh built by spheres(): the same (start, end, source) records as
+    extract_edges (objects share no vertices), grouped by icosphere level and object
+    instead of sorted by quantised key; join results do not depend on edge order. `level`
+    is one level for every object or the per-object level array."""
+    n_obj = mesh.num_objects
+    lv = np.broadcast_to(np.asarray(level), (n_obj,))
+    nv_of = {int(l): len(icosphere(int(l))[0]) for l in np.unique(lv)}
+    nv = np.array([nv_of[int(l)] for l in lv], dtype=np.int64) if len(nv_of) > 1 else \
+        np.full(n_obj, next(iter(nv_of.values())) if nv_of else 0, dtype=np.int64)
+    voff = np.concatenate([[0], np.cumsum(nv)[:-1]]) if n_obj else np.zeros(0, np.int64)
+    s_parts, e_parts, src_parts = [], [], []
+    for l in sorted(nv_of):
+        sel = np.nonzero(lv == l)[0]
+        e0, e1 = _template_edges(l)
+        base = voff[sel][:, None]
+        s_parts.append(mesh.vertices[(base + e0[None, :]).reshape(-1)])
+        e_parts.append(mesh.vertices[(base + e1[None, :]).reshape(-1)])
+        src_parts.append(np.repeat(sel.astype(np.int32), len(e0)))
+    if not s_parts:
+        z3 = np.zeros((0, 3), np.float32)
+        zi = np.zeros(0, np.int32)
+        return EdgeSet(z3, z3.copy(), zi, zi.copy(), zi.copy())
+    starts = np.ascontiguousarray(np.concatenate(s_parts))
+    ends = np.ascontiguousarray(np.concatenate(e_parts))
+    src = np.concatenate(src_parts)
+    ne = len(src)
+    return EdgeSet(starts, ends, np.arange(ne, dtype=np.int32), np.ones(ne, dtype=np.int32), src)

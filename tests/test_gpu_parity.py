@@ -358,6 +358,21 @@ def test_autotune_restores_options(ctx):
     assert np.array_equal(out, ref)
 
 
+def test_edge_stats_counters(ctx):
+    from rayspace3d_amd.device import edge_stats
+    a, b = _overlap_case(71, 60, 40.0)
+    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    ea = DevEdges(a.edges)
+    st = edge_stats(ctx, bb, ea)
+    counts = to_host(edge_count(ctx, bb, ea))
+    assert st["rays"] == a.edges.num_edges
+    assert st["pair_emits"] == int(counts.sum()) > 0
+    assert st["source_refs"] == int(a.edges.src_counts.sum())
+    assert st["nodes_visited"] >= st["rays"] and st["tris_tested"] >= st["hits"] > 0
+    ba.free()
+    bb.free()
+
+
 def test_batched_equals_per_query(ctx):
     from rayspace3d_amd.device import batch_execute
     rng = np.random.Generator(np.random.PCG64(91))

@@ -299,3 +299,8 @@ def test_sphere_field_edges_match_extractor():
                             np.repeat(e.src_objects, 1).tolist()))
     assert e1.num_edges == e2.num_edges == 60 * 480
     assert rec(e1) == rec(e2)
+    # mixed levels (the overlap generator): same records as the extractor
+    m3, lv = gen.overlap_dataset(17, n_objects=40, world=60.0, levels=(2, 3, 4), return_levels=True)
+    f1, f2 = gen.extract_edges(m3), gen.sphere_field_edges(m3, lv)
+    assert f1.num_edges == f2.num_edges == int(sum({2: 480, 3: 1920, 4: 7680}[int(l)] for l in lv))
+    assert rec(f1) == rec(f2)
