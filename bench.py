@@ -40,8 +40,11 @@ def main():
                          "serving: request server under load; overlap: mesh overlap join of two "
                          "50,000-object datasets; copy: 1 GiB device copy x3, the "
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
-    ap.add_argument("--rays-per-wave", type=int, default=128,
-                    help="lane-refill run length of the traversal (0: one ray per lane)")
+    ap.add_argument("--rays-per-wave", type=int, default=-1,
+                    help="lane-refill run length of the traversal (0: one ray per lane; "
+                         "-1: leaf size, reordering and run length from the autotuner)")
+    ap.add_argument("--leaf-size", type=int, default=1,
+                    help="BVH leaf size when the autotuner is off")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--datasets", type=int, default=64, help="serving: resident datasets")
     ap.add_argument("--overlap-objects", type=int, default=50_000,
@@ -77,10 +80,21 @@ def main():
           f"in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     ctx = Context(0)
-    ctx.set_option(_lib.OPT_RAYS_PER_WAVE, args.rays_per_wave)
     dV, dI, dO = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
     dP = to_dev(pts)
     torch.cuda.synchronize()
+    # autotuner (untimed): leaf size, ray reordering and refill run length on these rays
+    tuned = None
+    if args.rays_per_wave < 0:
+        from rayspace3d_amd import autotune
+        tuned, trials = autotune.tune_traversal(ctx, dV, dI, dO, dP, leaf_sizes=(1, 2, 4),
+                                                rays_per_wave=(64, 128, 256), reps=2)
+        autotune.apply(ctx, tuned)
+        leaf = tuned["leaf_size"]
+        print(f"[bench] autotuner: {tuned}", file=sys.stderr, flush=True)
+    else:
+        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, args.rays_per_wave)
+        leaf = args.leaf_size
     # BVH build (outside the timed query, like the reference's "Build Index" phase)
     builds = []
     bvh = None
@@ -88,7 +102,7 @@ def main():
         if bvh is not None:
             bvh.free()
         tb = time.perf_counter()
-        bvh = Bvh(ctx, dV, dI, dO, leaf_size=4)
+        bvh = Bvh(ctx, dV, dI, dO, leaf_size=leaf)
         ctx.sync()
         builds.append((time.perf_counter() - tb) * 1e3)
     info = bvh.info()
@@ -119,11 +133,14 @@ def main():
     result = to_host(out)
     inside = int((result >= 0).sum())
 
-    # Canonical algorithmic bytes of the traversal (counted on the refine candidates):
+    # Canonical algorithmic bytes of the traversal (counted on the refine candidates over
+    # the canonical leaf-4 LBVH, whatever tree the timed run used):
     # 12 B origin + 4 B id + 64 B per node fetch + 64 B per triangle fetch.
     vmin, vmax = mesh.vertices.min(0), mesh.vertices.max(0)
     cand = np.all((pts > vmin) & (pts < vmax), axis=1)
-    st = trace_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
+    canon = Bvh(ctx, dV, dI, dO, leaf_size=4)
+    st = trace_stats(ctx, canon, to_dev(pts[cand]), axis=2)
+    canon.free()
     ncand = int(res.candidate_count)
     alg_bytes = 16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"]
     bytes_per_ray = alg_bytes / max(st["rays"], 1)
@@ -137,22 +154,34 @@ def main():
     if args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
+        threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                             or (os.cpu_count() or 1)))
         sample = pts[: args.cpu_sample]
         small = pts[:1000]
-        tc0 = time.perf_counter()
-        orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, small, axis=2)
-        tc1 = time.perf_counter()
-        ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, sample, axis=2)
-        tc2 = time.perf_counter()
-        dt = (tc2 - tc1) - (tc1 - tc0)
-        cpu_rate = (len(sample) - len(small)) / max(dt, 1e-9)
+
+        def cpu_rate_of(pts_s, nthreads):
+            # the small call pays the (serial) grid build alone; subtract it
+            tc0 = time.perf_counter()
+            orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, small, axis=2,
+                               threads=nthreads)
+            tc1 = time.perf_counter()
+            r = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts_s, axis=2,
+                                   threads=nthreads)
+            tc2 = time.perf_counter()
+            return r, (len(pts_s) - len(small)) / max((tc2 - tc1) - (tc1 - tc0), 1e-9)
+
+        ref, cpu_rate = cpu_rate_of(sample, threads)
+        _, cpu_rate_1 = cpu_rate_of(sample[: max(len(sample) // 8, 2000)], 1)
         parity_ok = bool(np.array_equal(ref, result[: len(sample)]))
-        cpu = {"value": round(cpu_rate / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+        cpu = {"value": round(cpu_rate / 1e6, 4), "unit": "Mrays/s", "cores": threads,
+               "kind": "port",
                "sample": f"first {len(sample)} of the {len(pts)} points against the full mesh, "
-                         "single-threaded CPU oracle (binned candidates, same chain rules), "
-                         "grid build excluded",
+                         f"CPU oracle on {threads} OpenMP threads (binned candidates, same chain "
+                         "rules), grid build excluded",
+               "value_1core": round(cpu_rate_1 / 1e6, 4),
                "parity_on_sample": parity_ok}
-        print(f"[bench] cpu oracle {cpu_rate / 1e6:.3f} Mrays/s, parity on sample: {parity_ok}",
+        print(f"[bench] cpu oracle {cpu_rate / 1e6:.3f} Mrays/s on {threads} threads "
+              f"({cpu_rate_1 / 1e6:.3f} on 1), parity on sample: {parity_ok}",
               file=sys.stderr, flush=True)
 
     line = {
@@ -174,8 +203,8 @@ def main():
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved_gbs / 8000.0, 4), "traffic": traffic,
-                     "kernel": ("k_points_refill" if args.rays_per_wave else "k_points")
-                               + " (refine traversal)",
+                     "kernel": ("k_points_refill" if ctx.get_option(_lib.OPT_RAYS_PER_WAVE)
+                                else "k_points") + " (refine traversal)",
                      "kernel_avg_ms": round(kern_avg_ms, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "alg_bytes_per_ray": round(bytes_per_ray, 1)},
@@ -183,7 +212,10 @@ def main():
         "build_ms": round(min(builds), 3),
         "candidates": ncand,
         "inside": inside,
-        "bvh": {"nodes": int(info.num_nodes), "depth": int(info.max_depth), "leaf_size": 4},
+        "bvh": {"nodes": int(info.num_nodes), "depth": int(info.max_depth), "leaf_size": leaf},
+        "traversal_options": {"ray_sort_min": ctx.get_option(_lib.OPT_RAY_SORT_MIN),
+                              "rays_per_wave": ctx.get_option(_lib.OPT_RAYS_PER_WAVE),
+                              "autotuned": tuned is not None},
         "trace_stats": st,
     }
     print(json.dumps(line), flush=True)
@@ -212,25 +244,34 @@ def run_batched(args):
     import numpy as np
     import torch
 
-    from rayspace3d_amd import gen
-    from rayspace3d_amd.device import (Bvh, Context, DevEdges, batch_execute, overlap_join,
-                                       to_dev, to_host, trace_points)
+    from rayspace3d_amd import autotune, gen
+    from rayspace3d_amd.device import (Bvh, CapturedBatch, Context, DevEdges, batch_execute,
+                                       overlap_join, to_dev, to_host, trace_points)
 
     t0 = time.time()
     work = gen.batched_workload(seed=args.seed + 2, n_queries=args.queries)
     print(f"[bench] generated {len(work)} queries in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ctx = Context(0)
+    # autotuner (untimed): leaf size, reordering and refill run on the largest containment
+    # query; every BVH of the batch uses the chosen leaf size
+    big = max((w for w in work if w["kind"] == "contain"), key=lambda w: len(w["points"]))
+    best, trials = autotune.tune_traversal(ctx, to_dev(big["mesh"].vertices), to_dev(big["mesh"].indices),
+                                           to_dev(big["mesh"].tri_to_obj), to_dev(big["points"]),
+                                           leaf_sizes=(1, 2, 4), rays_per_wave=(64, 128, 256), reps=2)
+    autotune.apply(ctx, best)
+    leaf = best["leaf_size"]
+    print(f"[bench] autotuner: {best}", file=sys.stderr, flush=True)
     queries, rays, keep = [], 0, []
     for w in work:
         if w["kind"] == "contain":
-            bvh = Bvh.from_mesh(ctx, w["mesh"])
+            bvh = Bvh.from_mesh(ctx, w["mesh"], leaf_size=leaf)
             dp = to_dev(w["points"])
             q = {"kind": "contain", "bvh": bvh, "points": dp, "axis": 2,
                  "out": torch.empty(len(w["points"]), dtype=torch.int32, device="cuda"),
                  "out_single": torch.empty(len(w["points"]), dtype=torch.int32, device="cuda")}
             rays += len(w["points"])
         else:
-            ba, bb = Bvh.from_mesh(ctx, w["a"]), Bvh.from_mesh(ctx, w["b"])
+            ba, bb = Bvh.from_mesh(ctx, w["a"], leaf_size=leaf), Bvh.from_mesh(ctx, w["b"], leaf_size=leaf)
             q = {"kind": "overlap", "bvh1": ba, "edges1": DevEdges(w["a"].edges), "bvh2": bb,
                  "edges2": DevEdges(w["b"].edges)}
             rays += w["a"].edges.num_edges + w["b"].edges.num_edges
@@ -256,9 +297,17 @@ def run_batched(args):
         torch.cuda.synchronize()
         return r
 
+    cb = CapturedBatch(ctx, queries)
+
+    def captured():
+        r = cb.run()
+        torch.cuda.synchronize()
+        return r
+
     for _ in range(max(args.warmup, 1)):
         single = per_query()
         multi = batched()
+        graphed = captured()
     steps = max(1, min(args.steps, 5))
     ts = time.perf_counter()
     for _ in range(steps):
@@ -267,15 +316,23 @@ def run_batched(args):
     ts = time.perf_counter()
     for _ in range(steps):
         multi = batched()
+    t_eager = (time.perf_counter() - ts) / steps
+    outs_eager = [q["out"].clone() for q in queries if q["kind"] == "contain"]
+    ts = time.perf_counter()
+    for _ in range(steps):
+        graphed = captured()
     t_batch = (time.perf_counter() - ts) / steps
+    cb.close()
     same = True
     pairs = 0
-    for q, a, b in zip(queries, single, multi):
+    for q, a, b, g in zip(queries, single, multi, graphed):
         if q["kind"] == "contain":
             same &= bool(torch.equal(q["out"], q["out_single"]))
         else:
-            same &= bool(np.array_equal(a, b))
+            same &= bool(np.array_equal(a, b)) and bool(np.array_equal(a, g))
             pairs += len(b)
+    same &= all(bool(torch.equal(o, q["out"])) for o, q in
+                zip(outs_eager, [q for q in queries if q["kind"] == "contain"]))
     line = {
         "metric": "Mrays/s + join pairs/s, batched vs per-query on MI355X (exact); p50/p99 ms",
         "value": round(rays / t_batch / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
@@ -283,11 +340,13 @@ def run_batched(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batched: {len(queries)} independent queries ({n_contain} containment, "
                                f"{len(queries) - n_contain} overlap), mesh tris log-uniform [1e3,5e5], "
-                               "points log-uniform [1e3,1e7], one persistent launch",
+                               "points log-uniform [1e3,1e7], one persistent launch, hipGraph replay",
                    "rays": rays, "seed": args.seed + 2},
         "per_query_ms": round(t_single * 1e3, 3),
         "per_query_mrays_s": round(rays / t_single / 1e6, 3),
+        "eager_batch_ms": round(t_eager * 1e3, 3),
         "batched_speedup": round(t_single / t_batch, 3),
+        "autotuner": {"best": best, "trials": len(trials)},
         "pairs": pairs, "pairs_per_s": round(pairs / t_batch, 1),
         "identical_results": same,
     }

@@ -75,7 +75,7 @@ int rs3d_ctx_profile_read(rs3d_ctx* ctx, double* total_ms, uint64_t* launches);
  * Karras hierarchy, bottom-up refit. vertices: float3[nV]; indices: uint3[nT];
  * tri_to_obj: int32[nT] (may be NULL: object id = triangle index). */
 typedef struct {
-    int32_t leaf_size; /* 1..8, 0 = default (4) */
+    int32_t leaf_size; /* 1..8, 0 = default (1: one triangle per leaf) */
     int32_t reserved[7];
 } rs3d_bvh_opts;
 
@@ -278,6 +278,24 @@ typedef struct {
 int rs3d_batch_execute(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_queries,
                        uint64_t* pair_keys_out, uint64_t max_pairs, uint64_t* pair_offsets,
                        uint64_t* total_pairs);
+
+/* Captured batch (no reference counterpart; the batched form of the hot path).
+ * rs3d_batch_create plans the queries once, executes them once (which sizes every
+ * buffer; that execution's results are discarded) and captures the device work of an
+ * execution -- ray reorder keys and sort, pair-table clear, queue reset and the persistent
+ * traversal launch -- into a hipGraph on the context's stream. rs3d_batch_run replays the
+ * graph and then compacts, sorts and splits the pairs exactly as rs3d_batch_execute does
+ * (same results). Each run reads the current contents of the queries' inputs (points,
+ * meshes, edges, BVHs), so callers may rewrite them in place between runs, keeping their
+ * sizes and addresses; containment ids land in the queries' out_obj arrays. If a run finds
+ * more pairs than the captured table holds it reruns uncaptured and captures again. The
+ * batch keeps its own scratch memory; it must be destroyed before the context. */
+typedef struct rs3d_batch rs3d_batch;
+int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_queries,
+                      rs3d_batch** out);
+int rs3d_batch_run(rs3d_batch* batch, uint64_t* pair_keys_out, uint64_t max_pairs,
+                   uint64_t* pair_offsets, uint64_t* total_pairs);
+int rs3d_batch_destroy(rs3d_batch* batch);
 
 /* ---- serving: resident datasets, request queue, continuous batching ----------------
  * No reference counterpart (the reference answers one query per process run). Datasets

@@ -251,60 +251,74 @@ static void test_tri_collect(const float* V, const uint32_t* I, int64_t t, const
     hv_push(hv, th, (uint32_t)t);
 }
 
-int64_t orc_points_contain(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
-                           const int32_t* tri_obj, const float* P, int64_t n, int axis,
-                           int use_grid, int32_t* out_obj) {
+/* One point of orc_points_contain: the canonical chain over the collected hits. */
+static int32_t contain_one(const float* V, const uint32_t* I, int64_t nT, const int32_t* tri_obj,
+                           const float* o, const orc_raypre* rp, const grid_t* G,
+                           const int* axes, int use_grid, hitvec* hv) {
+    const float eps = 1e-6f, tmax = 1e16f;
+    hv->n = 0;
+    if (use_grid && nT > 0) {
+        int cu = cell_coord(G, 0, o[axes[0]]);
+        int cv = cell_coord(G, 1, o[axes[1]]);
+        if (cu >= 0 && cu < G->g[0] && cv >= 0 && cv < G->g[1]) {
+            int64_t c = cu + (int64_t)G->g[0] * cv;
+            for (int64_t k = G->start[c]; k < G->start[c + 1]; ++k)
+                test_tri_collect(V, I, G->items[k], o, rp, eps, tmax, 0, hv);
+        }
+    } else {
+        for (int64_t t = 0; t < nT; ++t) test_tri_collect(V, I, t, o, rp, eps, tmax, 0, hv);
+    }
+    qsort(hv->h, (size_t)hv->n, sizeof(hit_t), hit_cmp);
+    float thr = eps;
+    int count = 0;
+    uint32_t first = 0;
+    int64_t j = 0;
+    while (count < 10000) {
+        while (j < hv->n && hv->h[j].t < thr) ++j;
+        if (j == hv->n) break;
+        if (count == 0) first = hv->h[j].tri;
+        ++count;
+        const float t = hv->h[j].t;
+        float nthr = t + eps;
+        if (nthr <= t) nthr = nextafterf(t, INFINITY);
+        thr = nthr;
+        ++j;
+    }
+    return (count & 1) ? tri_obj[first] : -1;
+}
+
+int64_t orc_points_contain_mt(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                              const int32_t* tri_obj, const float* P, int64_t n, int axis,
+                              int use_grid, int32_t* out_obj, int threads) {
     (void)nV;
     float d[3] = {0.0f, 0.0f, 0.0f};
     d[axis] = 1.0f;
     orc_raypre rp;
     orc_ray_pre(d, &rp);
-    const float eps = 1e-6f, tmax = 1e16f;
     grid_t G;
+    memset(&G, 0, sizeof G);
     int axes[2] = {(axis + 1) % 3, (axis + 2) % 3};
     if (use_grid && nT > 0) grid_build(&G, 2, axes, V, I, nT, 1.0);
-    hitvec hv = {0, 0, 0};
     int64_t inside = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        const float o[3] = {P[3 * i], P[3 * i + 1], P[3 * i + 2]};
-        hv.n = 0;
-        if (use_grid && nT > 0) {
-            int cu = cell_coord(&G, 0, o[axes[0]]);
-            int cv = cell_coord(&G, 1, o[axes[1]]);
-            if (cu >= 0 && cu < G.g[0] && cv >= 0 && cv < G.g[1]) {
-                int64_t c = cu + (int64_t)G.g[0] * cv;
-                for (int64_t k = G.start[c]; k < G.start[c + 1]; ++k)
-                    test_tri_collect(V, I, G.items[k], o, &rp, eps, tmax, 0, &hv);
-            }
-        } else {
-            for (int64_t t = 0; t < nT; ++t) test_tri_collect(V, I, t, o, &rp, eps, tmax, 0, &hv);
+#pragma omp parallel num_threads(threads > 0 ? threads : 1) reduction(+ : inside)
+    {
+        hitvec hv = {0, 0, 0};
+#pragma omp for schedule(dynamic, 4096)
+        for (int64_t i = 0; i < n; ++i) {
+            const float o[3] = {P[3 * i], P[3 * i + 1], P[3 * i + 2]};
+            out_obj[i] = contain_one(V, I, nT, tri_obj, o, &rp, &G, axes, use_grid, &hv);
+            inside += out_obj[i] >= 0;
         }
-        qsort(hv.h, (size_t)hv.n, sizeof(hit_t), hit_cmp);
-        float thr = eps;
-        int count = 0;
-        uint32_t first = 0;
-        int64_t j = 0;
-        while (count < 10000) {
-            while (j < hv.n && hv.h[j].t < thr) ++j;
-            if (j == hv.n) break;
-            if (count == 0) first = hv.h[j].tri;
-            ++count;
-            const float t = hv.h[j].t;
-            float nthr = t + eps;
-            if (nthr <= t) nthr = nextafterf(t, INFINITY);
-            thr = nthr;
-            ++j;
-        }
-        if (count & 1) {
-            out_obj[i] = tri_obj[first];
-            ++inside;
-        } else {
-            out_obj[i] = -1;
-        }
+        free(hv.h);
     }
-    free(hv.h);
     if (use_grid && nT > 0) grid_free(&G);
     return inside;
+}
+
+int64_t orc_points_contain(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                           const int32_t* tri_obj, const float* P, int64_t n, int axis,
+                           int use_grid, int32_t* out_obj) {
+    return orc_points_contain_mt(V, nV, I, nT, tri_obj, P, n, axis, use_grid, out_obj, 1);
 }
 
 /* ------------------------------------------------------------------------------------ */

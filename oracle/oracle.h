@@ -41,6 +41,10 @@ int orc_tri_hit(const float o[3], const orc_raypre* r, const float v0[3], const 
 int64_t orc_points_contain(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
                            const int32_t* tri_obj, const float* P, int64_t n, int axis,
                            int use_grid, int32_t* out_obj);
+/* Same, over `threads` OpenMP threads (results identical for any thread count). */
+int64_t orc_points_contain_mt(const float* V, int64_t nV, const uint32_t* I, int64_t nT,
+                              const int32_t* tri_obj, const float* P, int64_t n, int axis,
+                              int use_grid, int32_t* out_obj, int threads);
 
 /* A.2 edge rays against a target mesh. Writes up to cap raw pair keys
  * ((u64)a << 32 | b, a = mesh1 object) into out_keys in edge order, and the per-edge

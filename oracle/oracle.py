@@ -17,6 +17,7 @@ _sig = {
     "orc_ray_pre": (None, [_P, _P]),
     "orc_tri_hit": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "orc_points_contain": (_I64, [_P, _I64, _P, _I64, _P, _P, _I64, C.c_int, C.c_int, _P]),
+    "orc_points_contain_mt": (_I64, [_P, _I64, _P, _I64, _P, _P, _I64, C.c_int, C.c_int, _P, C.c_int]),
     "orc_edge_pairs": (_I64, [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
                               _P, _I64, _P]),
     "orc_edge_pairs_ex": (_I64, [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
@@ -71,11 +72,11 @@ def tri_hit(o, d, v0, v1, v2):
     return bool(h), float(t[0])
 
 
-def points_contain(V, I, tri_obj, P, axis=2, use_grid=True):
+def points_contain(V, I, tri_obj, P, axis=2, use_grid=True, threads=1):
     V = _c(V, np.float32); I = _c(I, np.uint32); O = _c(tri_obj, np.int32); P = _c(P, np.float32)
     out = np.empty(len(P), np.int32)
-    lib().orc_points_contain(_p(V), len(V), _p(I), len(I), _p(O), _p(P), len(P), axis,
-                             1 if use_grid else 0, _p(out))
+    lib().orc_points_contain_mt(_p(V), len(V), _p(I), len(I), _p(O), _p(P), len(P), axis,
+                                1 if use_grid else 0, _p(out), int(threads))
     return out
 
 

@@ -162,6 +162,9 @@ SIGNATURES = {
     "rs3d_containment_join": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), C.POINTER(MeshView),
                                     P, C.POINTER(Edges), U64, P, U64, PU64, P, U64, PU64, PU64]),
     "rs3d_batch_execute": (I32, [P, C.POINTER(BatchQuery), I32, P, U64, PU64, PU64]),
+    "rs3d_batch_create": (I32, [P, C.POINTER(BatchQuery), I32, C.POINTER(P)]),
+    "rs3d_batch_run": (I32, [P, P, U64, PU64, PU64]),
+    "rs3d_batch_destroy": (I32, [P]),
     "rs3d_server_create": (I32, [P, I32, U64, C.POINTER(P)]),
     "rs3d_server_destroy": (I32, [P]),
     "rs3d_server_add_dataset": (I32, [P, C.POINTER(MeshView), P, C.POINTER(Edges), I32, C.POINTER(C.c_int)]),

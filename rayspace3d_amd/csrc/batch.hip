@@ -224,62 +224,67 @@ int BatchPlan::base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid) {
     return RS3D_OK;
 }
 
-int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets,
-                   uint64_t* overflow) {
+// Host tables (descriptors, Morton frames, task offsets) and their upload; the device
+// layout stays fixed until the next prepare, so a captured device pass stays valid.
+int BatchPlan::prepare() {
     rs3d_ctx* ctx = ctx_;
-    *n_keys = 0;
-    if (overflow) *overflow = 0;
-    offsets.assign(num_qids + 1, 0);
     std::stable_partition(items.begin(), items.end(), [](const BatchItem& d) { return d.kind == BQ_CONTAIN; });
-    uint32_t n_contain = 0;
+    n_contain = 0;
     for (const auto& d : items) n_contain += d.kind == BQ_CONTAIN;
     if (n_contain >= (1u << 10)) {
         set_error("batch: at most 1023 containment queries per launch");
         return RS3D_E_ARG;
     }
-    const uint32_t nq = (uint32_t)items.size();
-    std::vector<uint64_t> task_off(nq + 1, 0);
+    if (contain_rays > 0xffffffffull) {
+        set_error("batch: more than 2^32-1 containment rays");
+        return RS3D_E_ARG;
+    }
+    nq = (uint32_t)items.size();
+    task_off.assign(nq + 1, 0);
     for (uint32_t k = 0; k < nq; ++k) task_off[k + 1] = task_off[k] + (items[k].n + TASK_RAYS - 1) / TASK_RAYS;
-    const uint64_t ntasks = task_off[nq];
-    // one staging block: descriptors | Morton frames | task offsets
+    ntasks = task_off[nq];
+    if (task_off[n_contain] > 0x7fffffffull) {
+        set_error("batch: too many containment tasks");
+        return RS3D_E_ARG;
+    }
+    // one staging block: descriptors | Morton frames | task offsets | queue heads | flags
     const auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t o_desc = 0;
-    const size_t o_losc = up(nq * sizeof(BatchDesc));
-    const size_t o_task = o_losc + up(4 * (size_t)n_contain * sizeof(float));
-    const size_t o_head = o_task + up((nq + 1) * sizeof(uint64_t));
-    const size_t o_bad = o_head + NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);
-    const size_t bytes = o_bad + 256;
+    o_losc = up(nq * sizeof(BatchDesc));
+    o_task = o_losc + up(4 * (size_t)n_contain * sizeof(float));
+    o_head = o_task + up((nq + 1) * sizeof(uint64_t));
+    o_bad = o_head + NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);
+    meta_bytes = o_bad + 256;
     host_meta.assign(o_head, 0);
     for (uint32_t k = 0; k < nq; ++k) {
         const BatchDesc d = items[k];
-        std::memcpy(host_meta.data() + o_desc + k * sizeof(BatchDesc), &d, sizeof d);
+        std::memcpy(host_meta.data() + k * sizeof(BatchDesc), &d, sizeof d);
     }
-    for (uint32_t k = 0; k < n_contain; ++k) {
+    for (uint32_t k = 0; k < n_contain; ++k)
         std::memcpy(host_meta.data() + o_losc + 4 * k * sizeof(float), items[k].lo_sc, 4 * sizeof(float));
-    }
     std::memcpy(host_meta.data() + o_task, task_off.data(), (nq + 1) * sizeof(uint64_t));
-    RS3D_TRY(meta.ensure(bytes));
+    RS3D_TRY(meta.ensure(meta_bytes));
+    RS3D_HIP(hipMemcpyAsync(meta.ptr, host_meta.data(), o_head, hipMemcpyHostToDevice, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    return RS3D_OK;
+}
+
+// The device work of one execution: ray keys and reorder sort, pair-table clear, queue
+// reset, persistent launch. Kernel launches, kernel fills and device copies only (no host
+// synchronisation, no allocation once the buffers have their size), so it can be captured.
+int BatchPlan::device_pass(uint64_t slots) {
+    rs3d_ctx* ctx = ctx_;
     uint8_t* mb = meta.as<uint8_t>();
-    RS3D_HIP(hipMemcpyAsync(mb, host_meta.data(), o_head, hipMemcpyHostToDevice, ctx->stream));
-    const BatchDesc* dp = reinterpret_cast<const BatchDesc*>(mb + o_desc);
+    const BatchDesc* dp = reinterpret_cast<const BatchDesc*>(mb);
     unsigned int* heads = reinterpret_cast<unsigned int*>(mb + o_head);
     unsigned long long* bad = reinterpret_cast<unsigned long long*>(mb + o_bad);
+    const uint64_t* tp = reinterpret_cast<const uint64_t*>(mb + o_task);
     const uint32_t* perm = nullptr;
     if (contain_rays > 0 && ctx->opt_ray_sort_min > 0) {
-        if (contain_rays > 0xffffffffull) {
-            set_error("batch: more than 2^32-1 containment rays");
-            return RS3D_E_ARG;
-        }
         RS3D_TRY(ctx->ray_keys.ensure(contain_rays * sizeof(uint32_t)));
         RS3D_TRY(ctx->ray_vals.ensure(contain_rays * sizeof(uint32_t)));
-        const uint64_t ctasks = task_off[n_contain];
-        if (ctasks > 0x7fffffffull) {
-            set_error("batch: too many containment tasks");
-            return RS3D_E_ARG;
-        }
-        k_batch_keys<<<(unsigned)ctasks, TASK_RAYS, 0, ctx->stream>>>(
-            dp, reinterpret_cast<const uint64_t*>(mb + o_task), reinterpret_cast<const float*>(mb + o_losc),
-            n_contain, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
+        k_batch_keys<<<(unsigned)task_off[n_contain], TASK_RAYS, 0, ctx->stream>>>(
+            dp, tp, reinterpret_cast<const float*>(mb + o_losc), n_contain,
+            ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
         int qbits = 0;  // the descriptor index occupies the bits above the 16 Morton bits
         while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
@@ -287,59 +292,85 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
                                       contain_rays, 16 + qbits));
         perm = ctx->ray_vals.as<uint32_t>();
     }
-    uint64_t slots = 1024;
-    while (slots < slot_hint) slots <<= 1;
-    const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 20u;
-    while (true) {
-        HashArgs H{};
-        if (any_pairs) {
-            RS3D_TRY(table.ensure(slots * sizeof(uint64_t) + 64));
-            H.slots = table.as<unsigned long long>();
-            H.size = slots;
-            H.bitwise = 1;
-            H.max_probes = 0;
-            H.failed = reinterpret_cast<unsigned long long*>(table.as<uint8_t>() + slots * sizeof(uint64_t));
-            RS3D_TRY(ht_clear_internal(ctx, table.as<uint64_t>(), slots, reinterpret_cast<uint64_t*>(H.failed)));
+    HashArgs H{};
+    if (any_pairs) {
+        RS3D_TRY(table.ensure(slots * sizeof(uint64_t) + 64));
+        H.slots = table.as<unsigned long long>();
+        H.size = slots;
+        H.bitwise = 1;
+        H.max_probes = 0;
+        H.failed = reinterpret_cast<unsigned long long*>(table.as<uint8_t>() + slots * sizeof(uint64_t));
+        RS3D_TRY(ht_clear_internal(ctx, table.as<uint64_t>(), slots, reinterpret_cast<uint64_t*>(H.failed)));
+    }
+    RS3D_TRY(fill_u32(ctx, heads, (meta_bytes - o_head) / sizeof(unsigned int), 0u));
+    if (ntasks > 0) {
+        const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 20u;
+        const unsigned grid = (unsigned)std::min<uint64_t>(waves, ntasks);
+        switch (depth) {
+            case 24: k_batch<24><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+            case 32: k_batch<32><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+            case 48: k_batch<48><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+            default: k_batch<64><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
         }
-        RS3D_TRY(fill_u32(ctx, heads, (bytes - o_head) / sizeof(unsigned int), 0u));
-        if (ntasks > 0) {
-            const uint64_t* tp = reinterpret_cast<const uint64_t*>(mb + o_task);
-            const unsigned grid = (unsigned)std::min<uint64_t>(waves, ntasks);
-            switch (depth) {
-                case 24: k_batch<24><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-                case 32: k_batch<32><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-                case 48: k_batch<48><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-                default: k_batch<64><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-            }
-            RS3D_HIP(hipGetLastError());
-        }
-        if (!any_pairs) {
-            RS3D_HIP(hipStreamSynchronize(ctx->stream));
+        RS3D_HIP(hipGetLastError());
+    }
+    return RS3D_OK;
+}
+
+// After a device pass: checks, pair-table compaction, sort/unique and per-query split.
+// *grow is set when the table overflowed or is more than half full (the pass must be
+// rerun with a larger table).
+int BatchPlan::finish(uint64_t slots, DevBuf& keys_out, uint64_t* n_keys,
+                      std::vector<uint64_t>& offsets, uint64_t* overflow, bool* grow) {
+    rs3d_ctx* ctx = ctx_;
+    *grow = false;
+    *n_keys = 0;
+    if (overflow) *overflow = 0;
+    offsets.assign(num_qids + 1, 0);
+    if (!any_pairs) {
+        RS3D_HIP(hipStreamSynchronize(ctx->stream));
+        return RS3D_OK;
+    }
+    unsigned long long hb[2] = {0, 0};
+    uint64_t failed = 0;
+    RS3D_TRY(read_u64(ctx, table.as<uint8_t>() + slots * sizeof(uint64_t), &failed));
+    RS3D_HIP(hipMemcpy(hb, meta.as<uint8_t>() + o_bad, sizeof hb, hipMemcpyDeviceToHost));
+    if (hb[0]) {
+        set_error("batch: object ids of pair queries must be < 2^24");
+        return RS3D_E_ARG;
+    }
+    if (overflow) *overflow = hb[1];
+    if (failed == 0) {
+        uint64_t occ = 0;
+        RS3D_TRY(keys_out.ensure(slots * sizeof(uint64_t)));
+        RS3D_TRY(ht_compact_internal(ctx, table.as<uint64_t>(), slots, keys_out.as<uint64_t>(), slots, &occ));
+        if (occ * 2 <= slots) {
+            uint64_t u = 0;
+            RS3D_TRY(rs3d_sort_unique_u64(ctx, keys_out.as<uint64_t>(), occ, &u));
+            std::vector<uint64_t> hk(u);
+            if (u) RS3D_HIP(hipMemcpy(hk.data(), keys_out.ptr, u * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            for (int q = 0; q <= num_qids; ++q)
+                offsets[q] = (uint64_t)(std::lower_bound(hk.begin(), hk.end(), (uint64_t)q << 48) - hk.begin());
+            *n_keys = u;
             return RS3D_OK;
         }
-        unsigned long long hb[2] = {0, 0};
-        uint64_t failed = 0;
-        RS3D_TRY(read_u64(ctx, H.failed, &failed));
-        RS3D_HIP(hipMemcpy(hb, bad, sizeof hb, hipMemcpyDeviceToHost));
-        if (hb[0]) {
-            set_error("batch: object ids of pair queries must be < 2^24");
-            return RS3D_E_ARG;
-        }
-        if (overflow) *overflow = hb[1];
-        if (failed == 0) {
-            uint64_t occ = 0;
-            RS3D_TRY(keys_out.ensure(slots * sizeof(uint64_t)));
-            RS3D_TRY(ht_compact_internal(ctx, table.as<uint64_t>(), slots, keys_out.as<uint64_t>(), slots, &occ));
-            if (occ * 2 <= slots) {
-                uint64_t u = 0;
-                RS3D_TRY(rs3d_sort_unique_u64(ctx, keys_out.as<uint64_t>(), occ, &u));
-                std::vector<uint64_t> hk(u);
-                if (u) RS3D_HIP(hipMemcpy(hk.data(), keys_out.ptr, u * sizeof(uint64_t), hipMemcpyDeviceToHost));
-                for (int q = 0; q <= num_qids; ++q)
-                    offsets[q] = (uint64_t)(std::lower_bound(hk.begin(), hk.end(), (uint64_t)q << 48) - hk.begin());
-                *n_keys = u;
-                return RS3D_OK;
-            }
+    }
+    *grow = true;
+    return RS3D_OK;
+}
+
+int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets,
+                   uint64_t* overflow) {
+    RS3D_TRY(prepare());
+    uint64_t slots = 1024;
+    while (slots < slot_hint) slots <<= 1;
+    while (true) {
+        RS3D_TRY(device_pass(slots));
+        bool grow = false;
+        RS3D_TRY(finish(slots, keys_out, n_keys, offsets, overflow, &grow));
+        if (!grow) {
+            slots_used = slots;
+            return RS3D_OK;
         }
         slots <<= 2;
         if (slots > (1ull << 31)) {
@@ -347,6 +378,35 @@ int BatchPlan::run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& of
             return RS3D_E_CAPACITY;
         }
     }
+}
+
+int BatchPlan::capture(hipGraphExec_t* exec) {
+    rs3d_ctx* ctx = ctx_;
+    hipGraph_t graph = nullptr;
+    RS3D_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = device_pass(slots_used);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != RS3D_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error("batch capture: %s", hipGetErrorString(e));
+        return RS3D_E_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        set_error("batch graph instantiate: %s", hipGetErrorString(ei));
+        return RS3D_E_HIP;
+    }
+    return RS3D_OK;
+}
+
+int BatchPlan::replay(hipGraphExec_t exec, DevBuf& keys_out, uint64_t* n_keys,
+                      std::vector<uint64_t>& offsets, uint64_t* overflow, bool* grow) {
+    RS3D_HIP(hipGraphLaunch(exec, ctx_->stream));
+    return finish(slots_used, keys_out, n_keys, offsets, overflow, grow);
 }
 
 int batch_keys_to_pairs(rs3d_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out) {
@@ -364,6 +424,50 @@ using namespace rs3d;
 
 extern "C" {
 
+// Adds the queries to a plan (shared by rs3d_batch_execute and rs3d_batch_create).
+static int add_queries(BatchPlan& plan, const rs3d_batch_query* queries, int num_queries,
+                       const char* who) {
+    for (int q = 0; q < num_queries; ++q) {
+        const rs3d_batch_query& Q = queries[q];
+        if (Q.kind == RS3D_BATCH_CONTAIN) {
+            if (!Q.bvh1 || (Q.num_points && (!Q.points || !Q.out_obj)) || Q.axis < 0 || Q.axis > 2) {
+                set_error("%s: bad containment query %d", who, q);
+                return RS3D_E_ARG;
+            }
+            RS3D_TRY(plan.add_contain(Q.bvh1, Q.points, Q.num_points, Q.out_obj, Q.axis, q));
+        } else if (Q.kind == RS3D_BATCH_OVERLAP) {
+            if (!Q.bvh1 || !Q.bvh2 || !Q.edges1 || !Q.edges2) {
+                set_error("%s: bad overlap query %d", who, q);
+                return RS3D_E_ARG;
+            }
+            RS3D_TRY(plan.add_edges(Q.bvh2, Q.edges1, 0, q));
+            RS3D_TRY(plan.add_edges(Q.bvh1, Q.edges2, 1, q));
+        } else {
+            set_error("%s: unknown query kind %d", who, Q.kind);
+            return RS3D_E_ARG;
+        }
+    }
+    plan.num_qids = num_queries;
+    return RS3D_OK;
+}
+
+// Per-query offsets, capacity check and (query, a, b) -> (a, b) conversion of a finished
+// execution into the caller's buffers.
+static int deliver_pairs(rs3d_ctx* ctx, const DevBuf& keys, uint64_t n, const std::vector<uint64_t>& offs,
+                         int num_queries, uint64_t* pair_keys_out, uint64_t max_pairs,
+                         uint64_t* pair_offsets, uint64_t* total_pairs, const char* who) {
+    for (int q = 0; q <= num_queries; ++q) pair_offsets[q] = q < (int)offs.size() ? offs[q] : n;
+    if (total_pairs) *total_pairs = n;
+    if (n > max_pairs) {
+        set_error("%s: %llu pairs exceed the output capacity %llu", who, (unsigned long long)n,
+                  (unsigned long long)max_pairs);
+        return RS3D_E_CAPACITY;
+    }
+    RS3D_TRY(batch_keys_to_pairs(ctx, keys.as<uint64_t>(), n, pair_keys_out));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    return RS3D_OK;
+}
+
 int rs3d_batch_execute(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_queries,
                        uint64_t* pair_keys_out, uint64_t max_pairs, uint64_t* pair_offsets,
                        uint64_t* total_pairs) {
@@ -375,40 +479,112 @@ int rs3d_batch_execute(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_q
     if (total_pairs) *total_pairs = 0;
     static thread_local BatchPlan plan(nullptr);
     plan.reset(ctx);
-    for (int q = 0; q < num_queries; ++q) {
-        const rs3d_batch_query& Q = queries[q];
-        if (Q.kind == RS3D_BATCH_CONTAIN) {
-            if (!Q.bvh1 || (Q.num_points && (!Q.points || !Q.out_obj)) || Q.axis < 0 || Q.axis > 2) {
-                set_error("rs3d_batch_execute: bad containment query %d", q);
-                return RS3D_E_ARG;
-            }
-            RS3D_TRY(plan.add_contain(Q.bvh1, Q.points, Q.num_points, Q.out_obj, Q.axis, q));
-        } else if (Q.kind == RS3D_BATCH_OVERLAP) {
-            if (!Q.bvh1 || !Q.bvh2 || !Q.edges1 || !Q.edges2) {
-                set_error("rs3d_batch_execute: bad overlap query %d", q);
-                return RS3D_E_ARG;
-            }
-            RS3D_TRY(plan.add_edges(Q.bvh2, Q.edges1, 0, q));
-            RS3D_TRY(plan.add_edges(Q.bvh1, Q.edges2, 1, q));
-        } else {
-            set_error("rs3d_batch_execute: unknown query kind %d", Q.kind);
-            return RS3D_E_ARG;
-        }
-    }
-    plan.num_qids = num_queries;
+    RS3D_TRY(add_queries(plan, queries, num_queries, "rs3d_batch_execute"));
     static thread_local DevBuf keys;
     uint64_t n = 0;
     std::vector<uint64_t> offs;
     RS3D_TRY(plan.run(keys, &n, offs, nullptr));
-    for (int q = 0; q <= num_queries; ++q) pair_offsets[q] = q < (int)offs.size() ? offs[q] : n;
-    if (total_pairs) *total_pairs = n;
-    if (n > max_pairs) {
-        set_error("rs3d_batch_execute: %llu pairs exceed the output capacity %llu",
-                  (unsigned long long)n, (unsigned long long)max_pairs);
-        return RS3D_E_CAPACITY;
+    return deliver_pairs(ctx, keys, n, offs, num_queries, pair_keys_out, max_pairs, pair_offsets,
+                         total_pairs, "rs3d_batch_execute");
+}
+
+}  // extern "C"
+
+// A prepared batch: its own scratch (a private context on the caller's stream, so other
+// calls on the caller's context cannot move buffers the graph points into), the plan, and
+// the captured device pass.
+struct rs3d_batch {
+    rs3d_ctx* user = nullptr;
+    rs3d_ctx priv;
+    BatchPlan plan{nullptr};
+    DevBuf keys;
+    hipGraphExec_t exec = nullptr;
+    int num_queries = 0;
+};
+
+static void batch_release(rs3d_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->plan.release();
+    b->keys.release();
+    rs3d_ctx& p = b->priv;
+    for (DevBuf* d : {&p.sort_keys, &p.sort_vals, &p.sort_hist, &p.scan_tmp, &p.tmp_a, &p.tmp_b,
+                      &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals})
+        d->release();
+    if (p.pinned) (void)hipHostFree(p.pinned);
+    p.pinned = nullptr;
+}
+
+extern "C" {
+
+int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_queries,
+                      rs3d_batch** out) {
+    if (!ctx || !out || num_queries < 0 || (num_queries && !queries) || num_queries >= (1 << 15)) {
+        set_error("rs3d_batch_create: bad arguments");
+        return RS3D_E_ARG;
     }
-    RS3D_TRY(batch_keys_to_pairs(ctx, keys.as<uint64_t>(), n, pair_keys_out));
-    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    *out = nullptr;
+    RS3D_HIP(hipSetDevice(ctx->device));
+    rs3d_batch* b = new rs3d_batch();
+    b->user = ctx;
+    b->num_queries = num_queries;
+    rs3d_ctx& p = b->priv;
+    p.device = ctx->device;
+    p.stream = ctx->stream;
+    p.num_cus = ctx->num_cus;
+    p.opt_ray_sort_min = ctx->opt_ray_sort_min;
+    p.opt_rays_per_wave = ctx->opt_rays_per_wave;
+    if (hipHostMalloc((void**)&p.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+        delete b;
+        set_error("rs3d_batch_create: hipHostMalloc failed");
+        return RS3D_E_NOMEM;
+    }
+    b->plan.reset(&p);
+    int rc = add_queries(b->plan, queries, num_queries, "rs3d_batch_create");
+    uint64_t n = 0;
+    std::vector<uint64_t> offs;
+    if (rc == RS3D_OK) rc = b->plan.run(b->keys, &n, offs, nullptr);  // sizes every buffer
+    if (rc == RS3D_OK) rc = b->plan.capture(&b->exec);
+    if (rc != RS3D_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return RS3D_OK;
+}
+
+int rs3d_batch_run(rs3d_batch* b, uint64_t* pair_keys_out, uint64_t max_pairs,
+                   uint64_t* pair_offsets, uint64_t* total_pairs) {
+    if (!b || !pair_offsets || (max_pairs && !pair_keys_out)) {
+        set_error("rs3d_batch_run: bad arguments");
+        return RS3D_E_ARG;
+    }
+    if (total_pairs) *total_pairs = 0;
+    RS3D_HIP(hipSetDevice(b->priv.device));
+    uint64_t n = 0;
+    std::vector<uint64_t> offs;
+    bool grow = false;
+    RS3D_TRY(b->plan.replay(b->exec, b->keys, &n, offs, nullptr, &grow));
+    if (grow) {
+        // more pairs than the captured table holds: run uncaptured with a growing table,
+        // then capture again at the new size
+        RS3D_TRY(b->plan.run(b->keys, &n, offs, nullptr));
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        RS3D_TRY(b->plan.capture(&b->exec));
+    }
+    return deliver_pairs(&b->priv, b->keys, n, offs, b->num_queries, pair_keys_out, max_pairs,
+                         pair_offsets, total_pairs, "rs3d_batch_run");
+}
+
+int rs3d_batch_destroy(rs3d_batch* b) {
+    if (!b) return RS3D_OK;
+    (void)hipSetDevice(b->priv.device);
+    (void)hipStreamSynchronize(b->priv.stream);
+    batch_release(b);
+    delete b;
     return RS3D_OK;
 }
 

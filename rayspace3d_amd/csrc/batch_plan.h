@@ -68,11 +68,25 @@ public:
                     int* cand_ids, uint8_t* cand_par, int max_targets, int swap, int qid);
     // keys_out: sorted unique (qid << 48 | a << 24 | b); offsets[q]..offsets[q+1] per query
     int run(DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets, uint64_t* overflow);
+    // Graph form (after a successful run): capture the device pass with the pair-table
+    // size that run settled on, then replay it; *grow on replay means the table was too
+    // small for this execution's pairs (run again, then capture again).
+    int capture(hipGraphExec_t* exec);
+    int replay(hipGraphExec_t exec, DevBuf& keys_out, uint64_t* n_keys,
+               std::vector<uint64_t>& offsets, uint64_t* overflow, bool* grow);
+    void release() {
+        meta.release();
+        table.release();
+    }
     int num_qids = 0;
     uint64_t slot_hint = 1u << 20;
 
 private:
     int base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid);
+    int prepare();
+    int device_pass(uint64_t slots);
+    int finish(uint64_t slots, DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets,
+               uint64_t* overflow, bool* grow);
     rs3d_ctx* ctx_;
     std::vector<BatchItem> items;
     int depth = 24;
@@ -80,6 +94,10 @@ private:
     bool any_pairs = false;
     DevBuf meta, table;
     std::vector<uint8_t> host_meta;  // staging of the uploaded tables (outlives the copy)
+    std::vector<uint64_t> task_off;
+    uint32_t nq = 0, n_contain = 0;
+    uint64_t ntasks = 0, slots_used = 0;
+    size_t o_losc = 0, o_task = 0, o_head = 0, o_bad = 0, meta_bytes = 0;
 };
 
 }  // namespace rs3d

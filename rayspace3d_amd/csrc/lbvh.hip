@@ -439,7 +439,7 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
                   (unsigned long long)num_triangles);
         return RS3D_E_ARG;
     }
-    int leaf = opts && opts->leaf_size > 0 ? opts->leaf_size : 4;
+    int leaf = opts && opts->leaf_size > 0 ? opts->leaf_size : 1;
     if (leaf > 8) {
         set_error("rs3d_bvh_build: leaf_size %d > 8", leaf);
         return RS3D_E_ARG;

@@ -96,7 +96,7 @@ class Bvh:
     """Linear BVH over a device-resident indexed mesh."""
 
     def __init__(self, ctx: Context, vertices: torch.Tensor, indices: torch.Tensor,
-                 tri_to_obj: torch.Tensor | None, leaf_size: int = 4):
+                 tri_to_obj: torch.Tensor | None, leaf_size: int = 1):
         self.ctx = ctx
         self.handle = C.c_void_p()
         opts = _lib.BvhOpts()
@@ -109,7 +109,7 @@ class Bvh:
         self._src = (vertices, indices, tri_to_obj)
 
     @classmethod
-    def from_mesh(cls, ctx: Context, mesh, leaf_size: int = 4) -> "Bvh":
+    def from_mesh(cls, ctx: Context, mesh, leaf_size: int = 1) -> "Bvh":
         b = cls(ctx, to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj),
                 leaf_size)
         b.num_objects = int(mesh.tri_to_obj.max()) + 1 if len(mesh.tri_to_obj) else 0
@@ -291,12 +291,7 @@ def containment_join(ctx, bvha, dedgesa, bvhb, dedgesb, max_out=1 << 20):
         return (to_host(cont[:nc.value], np.uint64), to_host(ov[:no.value], np.uint64), over.value)
 
 
-def batch_execute(ctx, queries, max_pairs=1 << 22):
-    """Runs many queries in one persistent launch. Each query is a dict:
-    {"kind": "contain", "bvh": Bvh, "points": tensor (n,3), "out": int32 tensor (n,), "axis": 2}
-    {"kind": "overlap", "bvh1": Bvh, "edges1": DevEdges, "bvh2": Bvh, "edges2": DevEdges}
-    Returns a list with the sorted pair keys (uint64 numpy) of each query (empty for
-    containment queries, whose ids land in their "out" tensors)."""
+def _batch_array(queries):
     n = len(queries)
     arr = (_lib.BatchQuery * max(n, 1))()
     for i, q in enumerate(queries):
@@ -314,6 +309,17 @@ def batch_execute(ctx, queries, max_pairs=1 << 22):
             b.bvh2 = q["bvh2"].handle.value
             b.edges1 = C.cast(C.pointer(q["edges1"].s), C.c_void_p).value
             b.edges2 = C.cast(C.pointer(q["edges2"].s), C.c_void_p).value
+    return arr
+
+
+def batch_execute(ctx, queries, max_pairs=1 << 22):
+    """Runs many queries in one persistent launch. Each query is a dict:
+    {"kind": "contain", "bvh": Bvh, "points": tensor (n,3), "out": int32 tensor (n,), "axis": 2}
+    {"kind": "overlap", "bvh1": Bvh, "edges1": DevEdges, "bvh2": Bvh, "edges2": DevEdges}
+    Returns a list with the sorted pair keys (uint64 numpy) of each query (empty for
+    containment queries, whose ids land in their "out" tensors)."""
+    n = len(queries)
+    arr = _batch_array(queries)
     offs = (C.c_uint64 * (n + 1))()
     torch.cuda.synchronize()
     while True:
@@ -326,6 +332,39 @@ def batch_execute(ctx, queries, max_pairs=1 << 22):
         check(rc, "rs3d_batch_execute")
         keys = to_host(out[:total.value], np.uint64)
         return [keys[offs[i]:offs[i + 1]].copy() for i in range(n)]
+
+
+class CapturedBatch:
+    """A batch planned once and captured as a hipGraph (rs3d_batch_*): run() replays it on
+    the current contents of the queries' inputs and returns what batch_execute returns."""
+
+    def __init__(self, ctx, queries):
+        self.ctx, self.n = ctx, len(queries)
+        self._arr = _batch_array(queries)
+        self._keep = queries
+        self.handle = C.c_void_p()
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_batch_create(ctx.handle, self._arr, self.n, C.byref(self.handle)),
+              "rs3d_batch_create")
+
+    def run(self, max_pairs=1 << 22):
+        offs = (C.c_uint64 * (self.n + 1))()
+        torch.cuda.synchronize()
+        while True:
+            out = torch.empty(max(max_pairs, 1), dtype=torch.int64, device="cuda")
+            total = C.c_uint64()
+            rc = self.ctx.lib.rs3d_batch_run(self.handle, _ptr(out), max_pairs, offs, C.byref(total))
+            if rc == _lib.RS3D_E_CAPACITY and total.value > max_pairs:
+                max_pairs = int(total.value)
+                continue
+            check(rc, "rs3d_batch_run")
+            keys = to_host(out[:total.value], np.uint64)
+            return [keys[offs[i]:offs[i + 1]].copy() for i in range(self.n)]
+
+    def close(self):
+        if self.handle:
+            check(self.ctx.lib.rs3d_batch_destroy(self.handle), "rs3d_batch_destroy")
+            self.handle = C.c_void_p()
 
 
 class Server:

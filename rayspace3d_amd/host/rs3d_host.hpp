@@ -176,7 +176,7 @@ private:
 
 class AccelerationStructure {
 public:
-    AccelerationStructure(DeviceContext& ctx, GeometryUploader& geometry, int leafSize = 4);
+    AccelerationStructure(DeviceContext& ctx, GeometryUploader& geometry, int leafSize = 1);
     ~AccelerationStructure();
     AccelerationStructure(const AccelerationStructure&) = delete;
     AccelerationStructure& operator=(const AccelerationStructure&) = delete;

@@ -15,7 +15,7 @@ from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, containm
 pytestmark = pytest.mark.gpu
 
 
-def _contain(ctx, mesh, pts, axis, leaf=4, kind=_lib.OUT_OBJ_I32):
+def _contain(ctx, mesh, pts, axis, leaf=1, kind=_lib.OUT_OBJ_I32):
     bvh = Bvh.from_mesh(ctx, mesh, leaf_size=leaf)
     out = to_host(trace_points(ctx, bvh, to_dev(pts), axis=axis, out_kind=kind))
     info = bvh.info()
@@ -403,6 +403,59 @@ def test_batched_equals_per_query(ctx):
             assert len(g) == 0
             assert np.array_equal(to_host(q["out"]), ref)
     assert sum(len(r) for k, r in expect if k == "pairs") > 20
+    for b in keep:
+        if isinstance(b, Bvh):
+            b.free()
+
+
+def test_captured_batch_replays(ctx):
+    # graph replays must equal eager batch execution, replay after replay, with eager work
+    # and synchronisations in between, and must read inputs rewritten in place
+    from rayspace3d_amd.device import CapturedBatch, batch_execute
+    queries, keep = [], []
+    for k in range(6):
+        if k % 3 == 1:
+            a, b = _overlap_case(300 + k, 80, 50.0)
+            ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+            ea, eb = DevEdges(a.edges), DevEdges(b.edges)
+            queries.append({"kind": "overlap", "bvh1": ba, "edges1": ea, "bvh2": bb, "edges2": eb})
+            keep += [ba, bb, ea, eb]
+        else:
+            mesh, pts = gen.many_spheres_workload(seed=400 + k, n_objects=300, n_points=90_000 + k,
+                                                  world=70.0)
+            bvh = Bvh.from_mesh(ctx, mesh)
+            queries.append({"kind": "contain", "bvh": bvh, "points": to_dev(pts), "axis": k % 3,
+                            "out": torch.full((len(pts),), -7, dtype=torch.int32, device="cuda")})
+            keep += [bvh]
+    cb = CapturedBatch(ctx, queries)
+    try:
+        for rep in range(4):
+            print(f"captured batch replay {rep}", flush=True)
+            if rep == 2:  # rewrite every point set in place (same size and address)
+                for q in queries:
+                    if q["kind"] == "contain":
+                        q["points"].mul_(0.9).add_(3.0)
+            got = cb.run()
+            outs = [to_host(q["out"]).copy() for q in queries if q["kind"] == "contain"]
+            # eager reference of the same inputs
+            for q in queries:
+                if q["kind"] == "contain":
+                    q["out"].fill_(-7)
+            ref = batch_execute(ctx, queries)
+            refs = [to_host(q["out"]) for q in queries if q["kind"] == "contain"]
+            torch.cuda.synchronize()
+            assert all(np.array_equal(g, r) for g, r in zip(got, ref))
+            assert all(np.array_equal(o, r) for o, r in zip(outs, refs))
+            assert sum(len(r) for r in ref) > 0
+            # the eager path per query as well
+            ci = 0
+            for q in queries:
+                if q["kind"] == "contain":
+                    alone = to_host(trace_points(ctx, q["bvh"], q["points"], axis=q["axis"]))
+                    assert np.array_equal(alone, outs[ci])
+                    ci += 1
+    finally:
+        cb.close()
     for b in keep:
         if isinstance(b, Bvh):
             b.free()

@@ -304,3 +304,10 @@ def test_sphere_field_edges_match_extractor():
     f1, f2 = gen.extract_edges(m3), gen.sphere_field_edges(m3, lv)
     assert f1.num_edges == f2.num_edges == int(sum({2: 480, 3: 1920, 4: 7680}[int(l)] for l in lv))
     assert rec(f1) == rec(f2)
+
+
+def test_threaded_oracle_matches_single_thread():
+    m, p = gen.many_spheres_workload(seed=8, n_objects=300, n_points=60_000, world=80.0)
+    a = orc.points_contain(m.vertices, m.indices, m.tri_to_obj, p, 2)
+    b = orc.points_contain(m.vertices, m.indices, m.tri_to_obj, p, 2, threads=4)
+    assert np.array_equal(a, b) and (a >= 0).any()
