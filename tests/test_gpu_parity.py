@@ -146,10 +146,12 @@ def _overlap_case(seed, n, world):
     return a, b
 
 
-def test_edge_counts_and_overlap_join(ctx):
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_edge_counts_and_overlap_join(ctx, leaf):
     a, b = _overlap_case(41, 300, 120.0)
-    print(f"overlap 300 x 300 objects, {a.edges.num_edges} + {b.edges.num_edges} edges", flush=True)
-    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    print(f"overlap 300 x 300 objects, {a.edges.num_edges} + {b.edges.num_edges} edges, "
+          f"leaf {leaf}", flush=True)
+    ba, bb = Bvh.from_mesh(ctx, a, leaf_size=leaf), Bvh.from_mesh(ctx, b, leaf_size=leaf)
     ea, eb = DevEdges(a.edges), DevEdges(b.edges)
     keys, counts = orc.edge_pairs(a.edges, b.vertices, b.indices, b.tri_to_obj, with_counts=True)
     got_counts = to_host(edge_count(ctx, bb, ea))
