@@ -40,11 +40,13 @@ def main():
                          "serving: request server under load; overlap: mesh overlap join of two "
                          "50,000-object datasets; copy: 1 GiB device copy x3, the "
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
-    ap.add_argument("--rays-per-wave", type=int, default=-1,
-                    help="lane-refill run length of the traversal (0: one ray per lane; "
-                         "-1: leaf size, reordering and run length from the autotuner)")
-    ap.add_argument("--leaf-size", type=int, default=1,
-                    help="BVH leaf size when the autotuner is off")
+    ap.add_argument("--rays-per-wave", type=int, default=64,
+                    help="lane-refill run length of the traversal (0: one ray per lane)")
+    ap.add_argument("--leaf-size", type=int, default=1, help="BVH leaf size")
+    ap.add_argument("--autotune", action="store_true",
+                    help="containment: pick leaf size, reordering and run length with the "
+                         "autotuner on the workload's own rays first (the defaults are its "
+                         "choice on this workload)")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--datasets", type=int, default=64, help="serving: resident datasets")
     ap.add_argument("--overlap-objects", type=int, default=50_000,
@@ -85,7 +87,7 @@ def main():
     torch.cuda.synchronize()
     # autotuner (untimed): leaf size, ray reordering and refill run length on these rays
     tuned = None
-    if args.rays_per_wave < 0:
+    if args.autotune:
         from rayspace3d_amd import autotune
         tuned, trials = autotune.tune_traversal(ctx, dV, dI, dO, dP, leaf_sizes=(1, 2, 4),
                                                 rays_per_wave=(64, 128, 256), reps=2)

@@ -57,7 +57,7 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * ray still writes its own slot). 0 disables; default 65536.
  * RS3D_OPT_RAYS_PER_WAVE: point launches give each 64-lane wave a run of this many
  * consecutive rays and refill a lane from the run as soon as its ray finishes; 0 uses one
- * ray per lane. Results are unchanged. Default 128; 0 or 64..2^20. */
+ * ray per lane. Results are unchanged. Default 64; 0 or 64..2^20. */
 enum { RS3D_OPT_RAY_SORT_MIN = 1, RS3D_OPT_RAYS_PER_WAVE = 2 };
 int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value);
 int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value);

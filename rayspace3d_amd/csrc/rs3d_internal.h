@@ -163,7 +163,7 @@ struct rs3d_ctx {
     rs3d::DevBuf sort_keys, sort_vals, sort_hist, scan_tmp, tmp_a, tmp_b, tmp_c, small;
     rs3d::DevBuf ray_keys, ray_vals;  // ray reordering (coherence) keys and permutation
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
-    int64_t opt_rays_per_wave = 128;     // lane-refill run length of point launches (0: off)
+    int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
     uint64_t* pinned = nullptr;  // small pinned host buffer for counts
     // traversal-kernel timing (rs3d_ctx_profile)
     int profile = 0;

@@ -338,7 +338,7 @@ def test_lane_refill_matches_oracle(ctx, per_wave):
         ref2 = orc.points_contain(nested.vertices, nested.indices, nested.tri_to_obj, p2, axis=0)
         assert np.array_equal(out2, ref2)
     finally:
-        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, 128)
+        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, 64)
 
 
 def test_autotune_restores_options(ctx):
