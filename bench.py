@@ -342,7 +342,7 @@ def run_batched(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batched: {len(queries)} independent queries ({n_contain} containment, "
                                f"{len(queries) - n_contain} overlap), mesh tris log-uniform [1e3,5e5], "
-                               "points log-uniform [1e3,1e7], one persistent launch, hipGraph replay",
+                               "points log-uniform [1e3,1e7], persistent launches (one per ray program), hipGraph replay",
                    "rays": rays, "seed": args.seed + 2},
         "per_query_ms": round(t_single * 1e3, 3),
         "per_query_mrays_s": round(rays / t_single / 1e6, 3),

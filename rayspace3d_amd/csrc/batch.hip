@@ -31,17 +31,12 @@ __device__ __forceinline__ uint32_t spread2_11(uint32_t v) {  // 11 bits -> ever
 // query's own grid. One workgroup per containment task (the first tasks of the table).
 __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __restrict__ qs,
                                                           const uint64_t* __restrict__ task_off,
+                                                          const uint32_t* __restrict__ task_q,
                                                           const float* __restrict__ lo_sc,
-                                                          uint32_t n_contain,
                                                           uint32_t* __restrict__ keys,
                                                           uint32_t* __restrict__ vals) {
     const uint64_t t = blockIdx.x;
-    uint32_t qa = 0, qb = n_contain;  // task_off[qa] <= t < task_off[qb]
-    while (qb - qa > 1) {
-        const uint32_t m = (qa + qb) >> 1;
-        if (task_off[m] <= t) qa = m;
-        else qb = m;
-    }
+    const uint32_t qa = task_q[t];
     const BatchDesc& q = qs[qa];
     const uint64_t i = (t - task_off[qa]) * TASK_RAYS + threadIdx.x;
     if (i >= q.n) return;
@@ -54,14 +49,25 @@ __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __res
     vals[q.ray_base + i] = (uint32_t)i;
 }
 
-template <int DEPTH>
-__global__ __launch_bounds__(TR_BLOCK) void k_batch(const BatchDesc* __restrict__ qs,
+// task_q[t] = the query owning task t (task_off[q] <= t < task_off[q + 1]).
+__global__ __launch_bounds__(256) void k_task_owner(const uint64_t* __restrict__ task_off,
+                                                    uint32_t* __restrict__ task_q) {
+    const uint32_t q = blockIdx.x;
+    for (uint64_t t = task_off[q] + threadIdx.x; t < task_off[q + 1]; t += blockDim.x) task_q[t] = q;
+}
+
+// Persistent batch kernel over tasks [t0, t1). KINDS selects the ray programs compiled in
+// (1: containment, 2: edges and object rays, 3: both) and AX the containment ray axis (-1:
+// each query's own): a kernel holding only one point program needs fewer registers and
+// keeps more waves resident.
+template <int DEPTH, int KINDS, int AX>
+__device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
                                                     const uint64_t* __restrict__ task_off,
-                                                    uint32_t nq, uint64_t ntasks,
+                                                    const uint32_t* __restrict__ task_q,
+                                                    uint64_t t0, uint64_t t1,
                                                     unsigned int* __restrict__ heads,
                                                     const uint32_t* __restrict__ perm, HashArgs H,
-                                                    unsigned long long* __restrict__ bad_ids) {
-    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+                                                    unsigned long long* __restrict__ bad_ids, int* stack_mem) {
     int* stack = stack_mem + threadIdx.x;
     const int lane = threadIdx.x;
     TraceCounters ctr;
@@ -69,32 +75,29 @@ __global__ __launch_bounds__(TR_BLOCK) void k_batch(const BatchDesc* __restrict_
     for (int dry = 0; dry < NSTRIPE;) {
         unsigned int k = 0;
         if (lane == 0) k = atomicAdd(&heads[stripe * STRIPE_PITCH], 1u);
-        k = __shfl(k, 0, 64);
-        const uint64_t t = (uint64_t)k * NSTRIPE + stripe;
-        if (t >= ntasks) {
+        k = __builtin_amdgcn_readfirstlane(k);  // wave-uniform: descriptor loads stay scalar
+        const uint64_t t = t0 + (uint64_t)k * NSTRIPE + stripe;
+        if (t >= t1) {
             stripe = stripe + 1 == NSTRIPE ? 0 : stripe + 1;
             ++dry;
             continue;
         }
-        uint32_t qa = 0, qb = nq;  // task_off[qa] <= t < task_off[qb]
-        while (qb - qa > 1) {
-            const uint32_t m = (qa + qb) >> 1;
-            if (task_off[m] <= t) qa = m;
-            else qb = m;
-        }
+        const uint32_t qa = task_q[t];
         const BatchDesc& q = qs[qa];
         const uint64_t lo = (t - task_off[qa]) * TASK_RAYS;
         const uint64_t hi = lo + TASK_RAYS < q.n ? lo + TASK_RAYS : q.n;
-        if (q.kind == BQ_CONTAIN) {
+        if ((KINDS & 1) && q.kind == BQ_CONTAIN) {
             const uint32_t* im = perm ? perm + q.ray_base : nullptr;
-            if (q.axis == 0)
+            const int ax = AX >= 0 ? AX : q.axis;
+            if (ax == 0)
                 points_refill<0, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
-            else if (q.axis == 1)
+            else if (ax == 1)
                 points_refill<1, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
             else
                 points_refill<2, DEPTH>(q.nodes, q.tris, q.pts, im, lo, hi, RS3D_OUT_OBJ_I32, q.out, stack);
             continue;
         }
+        if (!(KINDS & 2)) continue;
         for (uint64_t i = lo + lane; i < hi; i += TR_BLOCK) {
             if (q.kind == BQ_OBJECTS) {
                 const int ft = q.first_tri[i];
@@ -137,6 +140,26 @@ __global__ __launch_bounds__(TR_BLOCK) void k_batch(const BatchDesc* __restrict_
             }
         }
     }
+}
+
+// Containment-only launches fit 5 waves per SIMD (91 VGPRs); the compiler is asked for
+// that occupancy explicitly. Pair launches keep the default register budget.
+template <int DEPTH, int AX>
+__global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_points(
+    const BatchDesc* __restrict__ qs, const uint64_t* __restrict__ task_off,
+    const uint32_t* __restrict__ task_q, uint64_t t0, uint64_t t1, unsigned int* __restrict__ heads,
+    const uint32_t* __restrict__ perm, HashArgs H, unsigned long long* __restrict__ bad_ids) {
+    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    batch_loop<DEPTH, 1, AX>(qs, task_off, task_q, t0, t1, heads, perm, H, bad_ids, stack_mem);
+}
+
+template <int DEPTH, int KINDS, int AX>
+__global__ __launch_bounds__(TR_BLOCK) void k_batch(
+    const BatchDesc* __restrict__ qs, const uint64_t* __restrict__ task_off,
+    const uint32_t* __restrict__ task_q, uint64_t t0, uint64_t t1, unsigned int* __restrict__ heads,
+    const uint32_t* __restrict__ perm, HashArgs H, unsigned long long* __restrict__ bad_ids) {
+    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    batch_loop<DEPTH, KINDS, AX>(qs, task_off, task_q, t0, t1, heads, perm, H, bad_ids, stack_mem);
 }
 
 __global__ __launch_bounds__(256) void k_batch_split(const uint64_t* __restrict__ keys, uint64_t n,
@@ -228,9 +251,29 @@ int BatchPlan::base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid) {
 // layout stays fixed until the next prepare, so a captured device pass stays valid.
 int BatchPlan::prepare() {
     rs3d_ctx* ctx = ctx_;
-    std::stable_partition(items.begin(), items.end(), [](const BatchItem& d) { return d.kind == BQ_CONTAIN; });
+    // containment first, grouped by ray axis; their rays take consecutive ranges of the
+    // reordered ray list in this order
+    std::stable_sort(items.begin(), items.end(), [](const BatchItem& a, const BatchItem& b) {
+        const int ka = a.kind == BQ_CONTAIN ? a.axis : 3, kb = b.kind == BQ_CONTAIN ? b.axis : 3;
+        return ka < kb;
+    });
     n_contain = 0;
-    for (const auto& d : items) n_contain += d.kind == BQ_CONTAIN;
+    uint64_t rb = 0;
+    for (auto& d : items)
+        if (d.kind == BQ_CONTAIN) {
+            ++n_contain;
+            d.ray_base = rb;
+            rb += d.n;
+        }
+    for (int a = 0; a <= 3; ++a) axis_first[a] = 0;
+    {
+        uint32_t k = 0;
+        for (int a = 0; a < 3; ++a) {
+            axis_first[a] = k;
+            while (k < n_contain && items[k].axis == a) ++k;
+        }
+        axis_first[3] = n_contain;
+    }
     if (n_contain >= (1u << 10)) {
         set_error("batch: at most 1023 containment queries per launch");
         return RS3D_E_ARG;
@@ -252,7 +295,7 @@ int BatchPlan::prepare() {
     o_losc = up(nq * sizeof(BatchDesc));
     o_task = o_losc + up(4 * (size_t)n_contain * sizeof(float));
     o_head = o_task + up((nq + 1) * sizeof(uint64_t));
-    o_bad = o_head + NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);
+    o_bad = o_head + 4 * NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);  // one queue per launch
     meta_bytes = o_bad + 256;
     host_meta.assign(o_head, 0);
     for (uint32_t k = 0; k < nq; ++k) {
@@ -264,7 +307,44 @@ int BatchPlan::prepare() {
     std::memcpy(host_meta.data() + o_task, task_off.data(), (nq + 1) * sizeof(uint64_t));
     RS3D_TRY(meta.ensure(meta_bytes));
     RS3D_HIP(hipMemcpyAsync(meta.ptr, host_meta.data(), o_head, hipMemcpyHostToDevice, ctx->stream));
+    RS3D_TRY(task_q.ensure((ntasks + 1) * sizeof(uint32_t)));
+    if (nq > 0) {
+        k_task_owner<<<nq, 256, 0, ctx->stream>>>(reinterpret_cast<const uint64_t*>(meta.as<uint8_t>() + o_task),
+                                                  task_q.as<uint32_t>());
+        RS3D_HIP(hipGetLastError());
+    }
     RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    return RS3D_OK;
+}
+
+template <int KINDS, int AX>
+int BatchPlan::launch_batch(const BatchDesc* dp, const uint64_t* tp, uint64_t t0, uint64_t t1,
+                            unsigned int* heads, const uint32_t* perm, const HashArgs& H,
+                            unsigned long long* bad) {
+    if (t1 <= t0) return RS3D_OK;
+    rs3d_ctx* ctx = ctx_;
+    const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 20u;
+    const unsigned grid = (unsigned)std::min<uint64_t>(waves, t1 - t0);
+    const uint32_t* tq = task_q.as<uint32_t>();
+    switch (depth) {
+        case 24:
+            if constexpr (KINDS == 1) k_batch_points<24, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            else k_batch<24, KINDS, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            break;
+        case 32:
+            if constexpr (KINDS == 1) k_batch_points<32, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            else k_batch<32, KINDS, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            break;
+        case 48:
+            if constexpr (KINDS == 1) k_batch_points<48, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            else k_batch<48, KINDS, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            break;
+        default:
+            if constexpr (KINDS == 1) k_batch_points<64, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            else k_batch<64, KINDS, AX><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, tq, t0, t1, heads, perm, H, bad);
+            break;
+    }
+    RS3D_HIP(hipGetLastError());
     return RS3D_OK;
 }
 
@@ -283,14 +363,15 @@ int BatchPlan::device_pass(uint64_t slots) {
         RS3D_TRY(ctx->ray_keys.ensure(contain_rays * sizeof(uint32_t)));
         RS3D_TRY(ctx->ray_vals.ensure(contain_rays * sizeof(uint32_t)));
         k_batch_keys<<<(unsigned)task_off[n_contain], TASK_RAYS, 0, ctx->stream>>>(
-            dp, tp, reinterpret_cast<const float*>(mb + o_losc), n_contain,
+            dp, tp, task_q.as<uint32_t>(), reinterpret_cast<const float*>(mb + o_losc),
             ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
         int qbits = 0;  // the descriptor index occupies the bits above the 16 Morton bits
         while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
-        RS3D_TRY(radix_sort_u32_pairs(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
-                                      contain_rays, 16 + qbits));
-        perm = ctx->ray_vals.as<uint32_t>();
+        uint32_t *sk = nullptr, *sv = nullptr;
+        RS3D_TRY(radix_sort_u32_pairs_to(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
+                                         contain_rays, 16 + qbits, &sk, &sv));
+        perm = sv;
     }
     HashArgs H{};
     if (any_pairs) {
@@ -303,16 +384,24 @@ int BatchPlan::device_pass(uint64_t slots) {
         RS3D_TRY(ht_clear_internal(ctx, table.as<uint64_t>(), slots, reinterpret_cast<uint64_t*>(H.failed)));
     }
     RS3D_TRY(fill_u32(ctx, heads, (meta_bytes - o_head) / sizeof(unsigned int), 0u));
-    if (ntasks > 0) {
-        const unsigned waves = (unsigned)std::max(ctx->num_cus, 1) * 20u;
-        const unsigned grid = (unsigned)std::min<uint64_t>(waves, ntasks);
-        switch (depth) {
-            case 24: k_batch<24><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-            case 32: k_batch<32><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-            case 48: k_batch<48><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
-            default: k_batch<64><<<grid, TR_BLOCK, 0, ctx->stream>>>(dp, tp, nq, ntasks, heads, perm, H, bad); break;
+    // Containment tasks come first in the task table ([0, ct)), pair tasks after. By default
+    // each kind runs in its own persistent launch (the point kernel without the edge and
+    // object programs keeps 5 waves per SIMD instead of 4); split_kinds = false runs every
+    // task in one launch.
+    // Containment queries are ordered by axis, so each axis's tasks form one range and get
+    // a kernel specialised to that axis.
+    const uint64_t ct = task_off[n_contain];
+    if (split_kinds) {
+        for (int a = 0; a < 3; ++a) {
+            unsigned int* hq = heads + (size_t)a * NSTRIPE * STRIPE_PITCH;
+            const uint64_t a0 = task_off[axis_first[a]], a1 = task_off[axis_first[a + 1]];
+            if (a == 0) RS3D_TRY((launch_batch<1, 0>(dp, tp, a0, a1, hq, perm, H, bad)));
+            if (a == 1) RS3D_TRY((launch_batch<1, 1>(dp, tp, a0, a1, hq, perm, H, bad)));
+            if (a == 2) RS3D_TRY((launch_batch<1, 2>(dp, tp, a0, a1, hq, perm, H, bad)));
         }
-        RS3D_HIP(hipGetLastError());
+        RS3D_TRY((launch_batch<2, 2>(dp, tp, ct, ntasks, heads + 3 * NSTRIPE * STRIPE_PITCH, perm, H, bad)));
+    } else {
+        RS3D_TRY((launch_batch<3, -1>(dp, tp, 0, ntasks, heads, perm, H, bad)));
     }
     return RS3D_OK;
 }
@@ -508,7 +597,7 @@ static void batch_release(rs3d_batch* b) {
     b->plan.release();
     b->keys.release();
     rs3d_ctx& p = b->priv;
-    for (DevBuf* d : {&p.sort_keys, &p.sort_vals, &p.sort_hist, &p.scan_tmp, &p.tmp_a, &p.tmp_b,
+    for (DevBuf* d : {&p.sort_keys, &p.sort_vals, &p.sort_hist, &p.sort_csum, &p.scan_tmp, &p.tmp_a, &p.tmp_b,
                       &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals})
         d->release();
     if (p.pinned) (void)hipHostFree(p.pinned);

@@ -77,14 +77,20 @@ public:
     void release() {
         meta.release();
         table.release();
+        task_q.release();
     }
     int num_qids = 0;
     uint64_t slot_hint = 1u << 20;
+    bool split_kinds = true;  // one persistent launch per ray-program kind
 
 private:
     int base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid);
     int prepare();
     int device_pass(uint64_t slots);
+    template <int KINDS, int AX>
+    int launch_batch(const BatchDesc* dp, const uint64_t* tp, uint64_t t0, uint64_t t1,
+                     unsigned int* heads, const uint32_t* perm, const HashArgs& H,
+                     unsigned long long* bad);
     int finish(uint64_t slots, DevBuf& keys_out, uint64_t* n_keys, std::vector<uint64_t>& offsets,
                uint64_t* overflow, bool* grow);
     rs3d_ctx* ctx_;
@@ -92,10 +98,11 @@ private:
     int depth = 24;
     uint64_t contain_rays = 0;
     bool any_pairs = false;
-    DevBuf meta, table;
+    DevBuf meta, table, task_q;
     std::vector<uint8_t> host_meta;  // staging of the uploaded tables (outlives the copy)
     std::vector<uint64_t> task_off;
     uint32_t nq = 0, n_contain = 0;
+    uint32_t axis_first[4] = {0, 0, 0, 0};  // containment items of axis a: [axis_first[a], axis_first[a+1])
     uint64_t ntasks = 0, slots_used = 0;
     size_t o_losc = 0, o_task = 0, o_head = 0, o_bad = 0, meta_bytes = 0;
 };

@@ -223,9 +223,13 @@ int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, ui
 }
 
 // ------------------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits. Per pass: (1) per-tile digit histograms,
-// (2) exclusive scan of the digit-major histogram table, (3) stable scatter: each tile
-// ranks its keys with wave64 ballot matching and per-wave digit counters in LDS.
+// Stable LSD radix sort, 8-bit digits. Per pass: (1) per-tile digit histograms, counted
+// with wave64 ballot matching (one LDS add per distinct digit per wave, so runs of equal
+// digits do not serialise on one counter) and stored tile-major ([tile][digit], coalesced);
+// (2) the tile-major table becomes each tile's global start per digit (digit-major order:
+// all of digit d before digit d + 1, tiles in order inside a digit) by a column scan;
+// (3) stable scatter: each tile ranks its keys with ballot matching, reorders them in LDS
+// and stores each digit's run with consecutive lanes.
 // ------------------------------------------------------------------------------------
 constexpr int RS_BLOCK = 256;
 constexpr int RS_ITEMS = 8;
@@ -234,19 +238,78 @@ constexpr int RS_WAVES = RS_BLOCK / 64;
 
 template <typename K>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys, uint64_t n,
-                                                      int shift, uint32_t* __restrict__ hist,
-                                                      uint32_t nblocks) {
+                                                      int shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+        const uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & 255u : 0u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        if (valid && (m & lt_mask) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
     }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[(uint64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// Column scan of the tile-major histogram (nb rows of 256 digit counts) into global
+// starts: per chunk of RS_CHUNK tiles the digit sums are written digit-major
+// (csum[d * nchunks + c]); one flat exclusive scan of that array gives, for every (digit,
+// chunk), all counts of smaller digits plus this digit's counts in earlier chunks; each
+// chunk then rewrites its rows as running starts. Rows are read with independent loads.
+constexpr int RS_CHUNK = 16;
+
+__global__ __launch_bounds__(256) void k_rs_colsum(const uint32_t* __restrict__ hist, uint32_t nb,
+                                                   uint32_t nchunks, uint32_t* __restrict__ csum) {
+    const uint32_t r0 = blockIdx.x * RS_CHUNK;
+    uint32_t v[RS_CHUNK];
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k)
+        v[k] = r0 + k < nb ? hist[(uint64_t)(r0 + k) * 256 + threadIdx.x] : 0u;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k) s += v[k];
+    csum[(uint64_t)threadIdx.x * nchunks + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_rs_colapply(uint32_t* __restrict__ hist, uint32_t nb,
+                                                     uint32_t nchunks,
+                                                     const uint32_t* __restrict__ csum) {
+    const uint32_t r0 = blockIdx.x * RS_CHUNK;
+    uint32_t v[RS_CHUNK];
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k)
+        v[k] = r0 + k < nb ? hist[(uint64_t)(r0 + k) * 256 + threadIdx.x] : 0u;
+    uint32_t run = csum[(uint64_t)threadIdx.x * nchunks + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k) {
+        if (r0 + k < nb) hist[(uint64_t)(r0 + k) * 256 + threadIdx.x] = run;
+        run += v[k];
+    }
+}
+
+// Tile-major counts -> tile-major global starts (in place).
+static int rs_offsets(rs3d_ctx* ctx, uint32_t* hist, uint32_t nb) {
+    const uint32_t nchunks = (nb + RS_CHUNK - 1) / RS_CHUNK;
+    RS3D_TRY(ctx->sort_csum.ensure((size_t)nchunks * 256 * sizeof(uint32_t)));
+    uint32_t* csum = ctx->sort_csum.as<uint32_t>();
+    k_rs_colsum<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, csum);
+    RS3D_HIP(hipGetLastError());
+    RS3D_TRY(scan_u32_exclusive(ctx, csum, csum, (uint64_t)nchunks * 256));
+    k_rs_colapply<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, csum);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
 }
 
 // Scatter of one LSD pass. The tile is first reordered by digit inside LDS (stable:
@@ -258,8 +321,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
                                                          K* __restrict__ kout,
                                                          uint32_t* __restrict__ vout, uint64_t n,
                                                          int shift,
-                                                         const uint32_t* __restrict__ hist,
-                                                         uint32_t nblocks) {
+                                                         const uint32_t* __restrict__ hist) {
     constexpr int G = RS_ITEMS * RS_WAVES;  // groups of 64 keys per tile, in input order
     __shared__ uint16_t gcnt[G][256];
     __shared__ uint32_t dstart[256];
@@ -271,7 +333,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     const uint32_t count = (uint32_t)((n - base) < (uint64_t)RS_TILE ? (n - base) : (uint64_t)RS_TILE);
     for (int i = tid; i < G * 256; i += RS_BLOCK) (&gcnt[0][0])[i] = 0;
-    gbase[tid] = hist[(uint64_t)tid * nblocks + blockIdx.x];
+    gbase[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     K key[RS_ITEMS];
@@ -332,9 +394,15 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
     }
 }
 
+// Sorts (keys, vals) over bits [begin_bit, end_bit). With out_k/out_v the result is left in
+// whichever of the two buffer pairs the last pass wrote and returned there; without them
+// it is copied back into keys/vals when that is the alternate pair.
 template <typename K, bool HAS_VAL>
 static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, int begin_bit,
-                           int end_bit, K* alt_keys, uint32_t* alt_vals) {
+                           int end_bit, K* alt_keys, uint32_t* alt_vals, K** out_k = nullptr,
+                           uint32_t** out_v = nullptr) {
+    if (out_k) *out_k = keys;
+    if (out_v) *out_v = vals;
     if (n <= 1) return RS3D_OK;
     if (n > 0xffffffffull) {
         set_error("radix sort supports at most 2^32-1 keys");
@@ -350,15 +418,20 @@ static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, i
     uint32_t* dst_v = alt_vals;
     int passes = 0;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        k_rs_hist<K><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, n, shift, hist, nb);
+        k_rs_hist<K><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, n, shift, hist);
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(scan_u32_exclusive(ctx, hist, hist, (uint64_t)nb * 256));
+        RS3D_TRY(rs_offsets(ctx, hist, nb));
         k_rs_scatter<K, HAS_VAL><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, src_v, dst_k, dst_v, n,
-                                                                   shift, hist, nb);
+                                                                   shift, hist);
         RS3D_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
         ++passes;
+    }
+    if (out_k) {
+        *out_k = src_k;
+        if (out_v) *out_v = src_v;
+        return RS3D_OK;
     }
     if (passes & 1) {
         RS3D_HIP(hipMemcpyAsync(keys, src_k, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
@@ -393,6 +466,20 @@ int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t
     return radix_sort_impl<uint32_t, true>(ctx, keys, vals, n, 0, end_bit,
                                            ctx->sort_keys.as<uint32_t>(),
                                            ctx->sort_vals.as<uint32_t>());
+}
+
+int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit,
+                            uint32_t** out_keys, uint32_t** out_vals) {
+    *out_keys = keys;
+    *out_vals = vals;
+    if (n <= 1) return RS3D_OK;
+    RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint32_t)));
+    RS3D_TRY(ctx->sort_vals.ensure(n * sizeof(uint32_t)));
+    if (end_bit < 1) end_bit = 1;
+    if (end_bit > 32) end_bit = 32;
+    return radix_sort_impl<uint32_t, true>(ctx, keys, vals, n, 0, end_bit,
+                                           ctx->sort_keys.as<uint32_t>(),
+                                           ctx->sort_vals.as<uint32_t>(), out_keys, out_vals);
 }
 
 // Bitwise OR / AND of all keys, to skip digit positions where every key agrees.
@@ -443,11 +530,11 @@ int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
     uint32_t* hist = ctx->sort_hist.as<uint32_t>();
     for (int shift = 0; shift < 64; shift += 8) {
         if (((varying >> shift) & 0xffull) == 0) continue;
-        k_rs_hist<uint64_t><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, n, shift, hist, nbt);
+        k_rs_hist<uint64_t><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, n, shift, hist);
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(scan_u32_exclusive(ctx, hist, hist, (uint64_t)nbt * 256));
+        RS3D_TRY(rs_offsets(ctx, hist, nbt));
         k_rs_scatter<uint64_t, false><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, nullptr, dst, nullptr,
-                                                                         n, shift, hist, nbt);
+                                                                         n, shift, hist);
         RS3D_HIP(hipGetLastError());
         std::swap(src, dst);
     }
@@ -566,6 +653,7 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx) {
     ctx->sort_keys.release();
     ctx->sort_vals.release();
     ctx->sort_hist.release();
+    ctx->sort_csum.release();
     ctx->scan_tmp.release();
     ctx->tmp_a.release();
     ctx->tmp_b.release();

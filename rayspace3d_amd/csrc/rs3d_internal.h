@@ -160,7 +160,7 @@ struct rs3d_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     // Scratch, reused across calls on this context (calls on one context are serial).
-    rs3d::DevBuf sort_keys, sort_vals, sort_hist, scan_tmp, tmp_a, tmp_b, tmp_c, small;
+    rs3d::DevBuf sort_keys, sort_vals, sort_hist, sort_csum, scan_tmp, tmp_a, tmp_b, tmp_c, small;
     rs3d::DevBuf ray_keys, ray_vals;  // ray reordering (coherence) keys and permutation
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
     int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
@@ -195,6 +195,10 @@ int scan_i32_to_i64_exclusive(rs3d_ctx* ctx, const int32_t* in, int64_t* out, ui
 int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, uint64_t n,
                              uint32_t* total_dev);
 int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit);
+// Same sort, result left in place or in the context's alternate buffers (no copy back);
+// *out_keys / *out_vals point at the sorted arrays.
+int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit,
+                            uint32_t** out_keys, uint32_t** out_vals);
 int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
