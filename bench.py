@@ -133,6 +133,20 @@ def main():
     print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, traversal kernel "
           f"{kern_avg_ms:.3f} ms avg over {kern_n} launches", file=sys.stderr, flush=True)
     result = to_host(out)
+    # the same step with the points handed over in pinned host memory and the ids copied
+    # back (the rate a caller with host buffers sees; not the headline value)
+    hP = torch.from_numpy(pts).pin_memory()
+    hOut = torch.empty(len(pts), dtype=torch.int32).pin_memory()
+    dP2 = torch.empty_like(dP)
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    for _ in range(5):
+        dP2.copy_(hP, non_blocking=False)
+        pipe.execute(bvh, dP2, axis=2, out=out, sync=False)
+        ctx.sync()
+        hOut.copy_(out)
+    pcie_ms = (time.perf_counter() - tp0) * 1e3 / 5
+    del dP2
     inside = int((result >= 0).sum())
 
     # Canonical algorithmic bytes of the traversal (counted on the refine candidates over
@@ -212,6 +226,9 @@ def main():
                      "alg_bytes_per_ray": round(bytes_per_ray, 1)},
         "cpu_baseline": cpu,
         "build_ms": round(min(builds), 3),
+        "pcie_inclusive": {"ms_per_step": round(pcie_ms, 3),
+                           "mrays_s": round(len(pts) / (pcie_ms / 1e3) / 1e6, 1),
+                           "note": "points from pinned host memory, ids back to host"},
         "candidates": ncand,
         "inside": inside,
         "bvh": {"nodes": int(info.num_nodes), "depth": int(info.max_depth), "leaf_size": leaf},
