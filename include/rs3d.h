@@ -344,6 +344,22 @@ int rs3d_grid_stats(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
                     uint64_t num_objects, int resolution, float world_size, void* cells,
                     float* bounds_out);
 
+/* Unique-edge extraction on the GPU (replaces EdgePreprocessor::extractEdges,
+ * preprocess/src/EdgePreprocessor.cpp:52-130, the producer of the .pre edge payload):
+ * edges keyed by endpoints quantised with llround(x * 1e5), in lexicographic order of
+ * (smaller, larger) quantised endpoint; start/end = endpoints of the edge's first
+ * occurrence in triangle order; sources = distinct objects (tri_to_obj; the triangle index
+ * when tri_to_obj is null) in order of first appearance. Triangles with an index out of
+ * range are skipped. Outputs (device): starts/ends float3[max_edges], src_offsets /
+ * src_counts int32[max_edges], src_objects int32[max_sources]. num_edges/num_sources
+ * (host) are always set; RS3D_E_CAPACITY when they exceed the capacities (call with zero
+ * capacities to size the buffers). Synchronous. */
+int rs3d_extract_edges(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
+                       const uint32_t* indices, uint64_t num_triangles, const int32_t* tri_to_obj,
+                       float* starts, float* ends, int32_t* src_offsets, int32_t* src_counts,
+                       int32_t* src_objects, uint64_t max_edges, uint64_t max_sources,
+                       uint64_t* num_edges, uint64_t* num_sources);
+
 /* Estimated overlap pairs from two grids (host sizing of
  * query/src/applications/raytracer_overlap_estimated_main.cpp:133-195): raw estimate / alpha
  * (alpha_correction = 0: raw). grid_min/max/resolution: host arrays of grid A. */

@@ -140,6 +140,7 @@ SIGNATURES = {
     "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),
+    "rs3d_extract_edges": (I32, [P, P, U64, P, U64, P, P, P, P, P, P, U64, U64, PU64, PU64]),
     "rs3d_filter_refine": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
     "rs3d_filter_box": (I32, [P, C.POINTER(C.c_float), P, U64, I32, P]),
     "rs3d_scan_i32_i64": (I32, [P, P, P, U64, PI64]),

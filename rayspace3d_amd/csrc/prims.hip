@@ -507,9 +507,17 @@ __global__ void k_init_or_and(unsigned long long* acc) {
     acc[1] = ~0ull;
 }
 
-int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
+// Stable sort of u64 keys (with u32 values when vals is given), in place. Byte positions
+// where every key agrees are skipped.
+template <bool HAS_VAL>
+static int radix_sort_u64_impl(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n) {
     if (n <= 1) return RS3D_OK;
+    if (n > 0xffffffffull) {
+        set_error("radix sort supports at most 2^32-1 keys");
+        return RS3D_E_ARG;
+    }
     RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint64_t)));
+    if (HAS_VAL) RS3D_TRY(ctx->sort_vals.ensure(n * sizeof(uint32_t)));
     RS3D_TRY(ctx->small.ensure(64));
     unsigned long long* acc = ctx->small.as<unsigned long long>();
     k_init_or_and<<<1, 1, 0, ctx->stream>>>(acc);
@@ -524,6 +532,8 @@ int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
     // skipped; the result is identical because those digits are equal for all keys.
     uint64_t* src = keys;
     uint64_t* dst = ctx->sort_keys.as<uint64_t>();
+    uint32_t* vsrc = vals;
+    uint32_t* vdst = HAS_VAL ? ctx->sort_vals.as<uint32_t>() : nullptr;
     unsigned nbt;
     RS3D_TRY(grid_blocks(n, RS_TILE, &nbt));
     RS3D_TRY(ctx->sort_hist.ensure((size_t)nbt * 256 * sizeof(uint32_t)));
@@ -533,14 +543,26 @@ int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
         k_rs_hist<uint64_t><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, n, shift, hist);
         RS3D_HIP(hipGetLastError());
         RS3D_TRY(rs_offsets(ctx, hist, nbt));
-        k_rs_scatter<uint64_t, false><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, nullptr, dst, nullptr,
-                                                                         n, shift, hist);
+        k_rs_scatter<uint64_t, HAS_VAL><<<nbt, RS_BLOCK, 0, ctx->stream>>>(src, vsrc, dst, vdst, n,
+                                                                           shift, hist);
         RS3D_HIP(hipGetLastError());
         std::swap(src, dst);
+        std::swap(vsrc, vdst);
     }
-    if (src != keys)
+    if (src != keys) {
         RS3D_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+        if (HAS_VAL)
+            RS3D_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+    }
     return RS3D_OK;
+}
+
+int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n) {
+    return radix_sort_u64_impl<false>(ctx, keys, nullptr, n);
+}
+
+int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n) {
+    return radix_sort_u64_impl<true>(ctx, keys, vals, n);
 }
 
 // ------------------------------------------------------------------------------------

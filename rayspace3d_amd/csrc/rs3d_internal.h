@@ -200,6 +200,7 @@ int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t
 int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit,
                             uint32_t** out_keys, uint32_t** out_vals);
 int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
+int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n);  // stable
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
 int grid_blocks(uint64_t n, int block, unsigned* out);

@@ -12,6 +12,30 @@ from . import _lib, gen, preio
 from .device import Context, to_dev, to_host
 
 
+def extract_edges_gpu(ctx: Context, mesh: gen.Mesh) -> gen.EdgeSet:
+    """Unique edges of a mesh on the GPU (rs3d_extract_edges): the same records, in the
+    same order, as gen.extract_edges."""
+    dV, dI, dO = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
+    ne, ns = C.c_uint64(), C.c_uint64()
+    torch.cuda.synchronize()
+    args = (ctx.handle, C.c_void_p(dV.data_ptr()), len(mesh.vertices), C.c_void_p(dI.data_ptr()),
+            len(mesh.indices), C.c_void_p(dO.data_ptr()))
+    rc = ctx.lib.rs3d_extract_edges(*args, None, None, None, None, None, 0, 0, C.byref(ne), C.byref(ns))
+    if rc != _lib.RS3D_E_CAPACITY:
+        _lib.check(rc, "rs3d_extract_edges")
+    n_e, n_s = ne.value, ns.value
+    st = torch.empty((max(n_e, 1), 3), dtype=torch.float32, device="cuda")
+    en = torch.empty_like(st)
+    off = torch.empty(max(n_e, 1), dtype=torch.int32, device="cuda")
+    cnt = torch.empty_like(off)
+    src = torch.empty(max(n_s, 1), dtype=torch.int32, device="cuda")
+    p = lambda t: C.c_void_p(t.data_ptr())
+    _lib.check(ctx.lib.rs3d_extract_edges(*args, p(st), p(en), p(off), p(cnt), p(src), n_e, n_s,
+                                          C.byref(ne), C.byref(ns)), "rs3d_extract_edges")
+    return gen.EdgeSet(to_host(st[:n_e]).copy(), to_host(en[:n_e]).copy(), to_host(off[:n_e]).copy(),
+                       to_host(cnt[:n_e]).copy(), to_host(src[:n_s]).copy())
+
+
 def grid_stats_gpu(ctx: Context, mesh: gen.Mesh, res: int, world_size: float = -1.0) -> preio.Grid:
     n_obj = int(mesh.tri_to_obj.max()) + 1 if len(mesh.tri_to_obj) else 0
     cells = torch.zeros(res ** 3 * 16, dtype=torch.uint8, device="cuda")
@@ -28,9 +52,10 @@ def grid_stats_gpu(ctx: Context, mesh: gen.Mesh, res: int, world_size: float = -
 
 def write_dataset(path: str, mesh: gen.Mesh, ctx: Context | None = None, grid_res: int = 0,
                   world_size: float = -1.0) -> None:
-    """Writes mesh (+ unique edges, + grid histogram when grid_res > 0) as .pre v2."""
+    """Writes mesh (+ unique edges, + grid histogram when grid_res > 0) as .pre v2. Edges
+    and grid come from the GPU when a context is given."""
     if mesh.edges is None:
-        mesh.edges = gen.extract_edges(mesh)
+        mesh.edges = extract_edges_gpu(ctx, mesh) if ctx is not None else gen.extract_edges(mesh)
     grid = grid_stats_gpu(ctx, mesh, grid_res, world_size) if grid_res > 0 else None
     preio.write_pre(path, mesh, grid)
 

@@ -499,3 +499,53 @@ def test_server_alone_equals_mixed(ctx):
     srv.close()
     for b in (bA, bB, bC):
         b.free()
+
+
+def _edges_equal(e, ref):
+    S, E, off, cnt, src = ref
+    return (np.array_equal(e.starts, S) and np.array_equal(e.ends, E)
+            and np.array_equal(e.src_offsets, off) and np.array_equal(e.src_counts, cnt)
+            and np.array_equal(e.src_objects, src))
+
+
+def test_gpu_edge_extraction_matches_oracle(ctx):
+    import os
+    from rayspace3d_amd import preio
+    from rayspace3d_amd.prep import extract_edges_gpu
+    # the reference's committed fixture (540 closed objects at large coordinates)
+    golden = os.path.join(os.path.dirname(__file__), "golden", "check_n_geometry.pre")
+    m = preio.read_pre(golden).mesh
+    print(f"edge extraction: fixture {len(m.indices)} triangles", flush=True)
+    assert _edges_equal(extract_edges_gpu(ctx, m), orc.extract_edges(m.vertices, m.indices, m.tri_to_obj))
+    # mixed icosphere levels, rotated, anisotropic
+    m2, lv = gen.overlap_dataset(23, n_objects=120, world=90.0, levels=(2, 3, 4), return_levels=True)
+    print(f"edge extraction: {len(m2.indices)} triangles, mixed levels", flush=True)
+    e2 = extract_edges_gpu(ctx, m2)
+    assert _edges_equal(e2, orc.extract_edges(m2.vertices, m2.indices, m2.tri_to_obj))
+    ref_np = gen.extract_edges(m2)
+    assert np.array_equal(e2.starts, ref_np.starts) and np.array_equal(e2.src_objects, ref_np.src_objects)
+
+
+def test_gpu_edge_extraction_edge_cases(ctx):
+    from rayspace3d_amd.prep import extract_edges_gpu
+    rng = np.random.Generator(np.random.PCG64(77))
+    # a coarse lattice of positions (many exact duplicates under llround(x * 1e5)), negative
+    # coordinates, positions duplicated under different vertex indices, degenerate
+    # triangles (repeated index, zero-length edges), one edge shared by several objects in
+    # interleaved triangle order, and triangles with an out-of-range index (skipped)
+    base = rng.integers(-40, 40, size=(300, 3)).astype(np.float32) * np.float32(0.37)
+    V = np.concatenate([base, base[:150], base + np.float32(1e-7)]).astype(np.float32)
+    nV = len(V)
+    I = rng.integers(0, nV, size=(4000, 3)).astype(np.uint32)
+    I[::37, 1] = I[::37, 0]                      # degenerate
+    I[5::101, 2] = nV + 3                        # out of range
+    O = rng.integers(0, 9, size=len(I)).astype(np.int32)
+    m = gen.Mesh(V, I, O, 9)
+    print(f"edge extraction edge cases: {len(I)} triangles, {nV} vertices", flush=True)
+    ref = orc.extract_edges(V, I, O)
+    e = extract_edges_gpu(ctx, m)
+    assert _edges_equal(e, ref)
+    assert e.num_edges > 100 and (e.src_counts > 1).any()
+    # empty mesh
+    em = gen.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.int32), 0)
+    assert extract_edges_gpu(ctx, em).num_edges == 0
