@@ -27,12 +27,14 @@ __device__ __forceinline__ uint32_t spread2_11(uint32_t v) {  // 11 bits -> ever
     return v;
 }
 
-// Sort keys of containment rays: (descriptor index << 16) | 16-bit Morton of (u, w) on the
-// query's own grid. One workgroup per containment task (the first tasks of the table).
+// Sort keys of containment rays: 16-bit Morton of (u, w) on the query's own grid, with the
+// descriptor index in the bits above unless each query is sorted in its own segment. One
+// workgroup per containment task (the first tasks of the table).
 __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __restrict__ qs,
                                                           const uint64_t* __restrict__ task_off,
                                                           const uint32_t* __restrict__ task_q,
                                                           const float* __restrict__ lo_sc,
+                                                          int query_bits,
                                                           uint32_t* __restrict__ keys,
                                                           uint32_t* __restrict__ vals) {
     const uint64_t t = blockIdx.x;
@@ -45,7 +47,8 @@ __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __res
     const float u = q.pts[3 * i + U], w = q.pts[3 * i + W];
     const float fu = fminf(fmaxf((u - lo_sc[4 * qa]) * lo_sc[4 * qa + 2], 0.0f), 255.0f);
     const float fw = fminf(fmaxf((w - lo_sc[4 * qa + 1]) * lo_sc[4 * qa + 3], 0.0f), 255.0f);
-    keys[q.ray_base + i] = (qa << 16) | (spread2_11((uint32_t)fu) << 1) | spread2_11((uint32_t)fw);
+    keys[q.ray_base + i] = (query_bits ? (qa << 16) : 0u) | (spread2_11((uint32_t)fu) << 1) |
+                           spread2_11((uint32_t)fw);
     vals[q.ray_base + i] = (uint32_t)i;
 }
 
@@ -257,14 +260,25 @@ int BatchPlan::prepare() {
         const int ka = a.kind == BQ_CONTAIN ? a.axis : 3, kb = b.kind == BQ_CONTAIN ? b.axis : 3;
         return ka < kb;
     });
+    // Reorder layout: when whole-chunk padding of every containment query costs at most a
+    // quarter more keys, each query gets its own segment (a multiple of RS_SEG_KEYS keys,
+    // padded with the largest key) and a segmented 16-bit sort replaces the global sort
+    // on (query, Morton) - one radix pass less.
     n_contain = 0;
-    uint64_t rb = 0;
+    uint64_t padded = 0;
     for (auto& d : items)
         if (d.kind == BQ_CONTAIN) {
             ++n_contain;
-            d.ray_base = rb;
-            rb += d.n;
+            padded += (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS;
         }
+    segmented = contain_rays > 0 && padded <= contain_rays + contain_rays / 4;
+    uint64_t rb = 0;
+    for (auto& d : items)
+        if (d.kind == BQ_CONTAIN) {
+            d.ray_base = rb;
+            rb += segmented ? (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS : d.n;
+        }
+    sorted_rays = rb;
     for (int a = 0; a <= 3; ++a) axis_first[a] = 0;
     {
         uint32_t k = 0;
@@ -278,7 +292,7 @@ int BatchPlan::prepare() {
         set_error("batch: at most 1023 containment queries per launch");
         return RS3D_E_ARG;
     }
-    if (contain_rays > 0xffffffffull) {
+    if (sorted_rays > 0xffffffffull) {
         set_error("batch: more than 2^32-1 containment rays");
         return RS3D_E_ARG;
     }
@@ -292,8 +306,20 @@ int BatchPlan::prepare() {
     }
     // one staging block: descriptors | Morton frames | task offsets | queue heads | flags
     const auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    // segment table of the segmented sort: per chunk {first chunk of its query, chunks}
+    std::vector<uint32_t> segtab;
+    if (segmented)
+        for (uint32_t k = 0; k < n_contain; ++k) {
+            const uint32_t c0 = (uint32_t)(items[k].ray_base / RS_SEG_KEYS);
+            const uint32_t nc = (uint32_t)((items[k].n + RS_SEG_KEYS - 1) / RS_SEG_KEYS);
+            for (uint32_t c = 0; c < nc; ++c) {
+                segtab.push_back(c0);
+                segtab.push_back(nc);
+            }
+        }
     o_losc = up(nq * sizeof(BatchDesc));
-    o_task = o_losc + up(4 * (size_t)n_contain * sizeof(float));
+    o_seg = o_losc + up(4 * (size_t)n_contain * sizeof(float));
+    o_task = o_seg + up(segtab.size() * sizeof(uint32_t));
     o_head = o_task + up((nq + 1) * sizeof(uint64_t));
     o_bad = o_head + 4 * NSTRIPE * STRIPE_PITCH * sizeof(unsigned int);  // one queue per launch
     meta_bytes = o_bad + 256;
@@ -304,6 +330,8 @@ int BatchPlan::prepare() {
     }
     for (uint32_t k = 0; k < n_contain; ++k)
         std::memcpy(host_meta.data() + o_losc + 4 * k * sizeof(float), items[k].lo_sc, 4 * sizeof(float));
+    if (!segtab.empty())
+        std::memcpy(host_meta.data() + o_seg, segtab.data(), segtab.size() * sizeof(uint32_t));
     std::memcpy(host_meta.data() + o_task, task_off.data(), (nq + 1) * sizeof(uint64_t));
     RS3D_TRY(meta.ensure(meta_bytes));
     RS3D_HIP(hipMemcpyAsync(meta.ptr, host_meta.data(), o_head, hipMemcpyHostToDevice, ctx->stream));
@@ -360,17 +388,25 @@ int BatchPlan::device_pass(uint64_t slots) {
     const uint64_t* tp = reinterpret_cast<const uint64_t*>(mb + o_task);
     const uint32_t* perm = nullptr;
     if (contain_rays > 0 && ctx->opt_ray_sort_min > 0) {
-        RS3D_TRY(ctx->ray_keys.ensure(contain_rays * sizeof(uint32_t)));
-        RS3D_TRY(ctx->ray_vals.ensure(contain_rays * sizeof(uint32_t)));
+        RS3D_TRY(ctx->ray_keys.ensure(sorted_rays * sizeof(uint32_t)));
+        RS3D_TRY(ctx->ray_vals.ensure(sorted_rays * sizeof(uint32_t)));
+        if (segmented)  // padding keys sort after every real key of their segment
+            RS3D_TRY(fill_u32(ctx, ctx->ray_keys.ptr, sorted_rays, 0xffffu));
         k_batch_keys<<<(unsigned)task_off[n_contain], TASK_RAYS, 0, ctx->stream>>>(
             dp, tp, task_q.as<uint32_t>(), reinterpret_cast<const float*>(mb + o_losc),
-            ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
+            segmented ? 0 : 1, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>());
         RS3D_HIP(hipGetLastError());
-        int qbits = 0;  // the descriptor index occupies the bits above the 16 Morton bits
-        while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
         uint32_t *sk = nullptr, *sv = nullptr;
-        RS3D_TRY(radix_sort_u32_pairs_to(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
-                                         contain_rays, 16 + qbits, &sk, &sv));
+        if (segmented) {
+            RS3D_TRY(radix_sort_u32_pairs_to(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
+                                             sorted_rays, 16, &sk, &sv,
+                                             reinterpret_cast<const uint2*>(mb + o_seg)));
+        } else {
+            int qbits = 0;  // the descriptor index occupies the bits above the 16 Morton bits
+            while (qbits < 10 && (1u << qbits) < n_contain) ++qbits;
+            RS3D_TRY(radix_sort_u32_pairs_to(ctx, ctx->ray_keys.as<uint32_t>(), ctx->ray_vals.as<uint32_t>(),
+                                             sorted_rays, 16 + qbits, &sk, &sv));
+        }
         perm = sv;
     }
     HashArgs H{};

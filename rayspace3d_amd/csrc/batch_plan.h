@@ -104,7 +104,9 @@ private:
     uint32_t nq = 0, n_contain = 0;
     uint32_t axis_first[4] = {0, 0, 0, 0};  // containment items of axis a: [axis_first[a], axis_first[a+1])
     uint64_t ntasks = 0, slots_used = 0;
-    size_t o_losc = 0, o_task = 0, o_head = 0, o_bad = 0, meta_bytes = 0;
+    size_t o_losc = 0, o_seg = 0, o_task = 0, o_head = 0, o_bad = 0, meta_bytes = 0;
+    bool segmented = false;   // per-query segments in the reorder sort
+    uint64_t sorted_rays = 0;  // keys in the reorder sort (with segment padding)
 };
 
 }  // namespace rs3d

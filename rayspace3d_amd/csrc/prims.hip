@@ -269,9 +269,21 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys
 // chunk), all counts of smaller digits plus this digit's counts in earlier chunks; each
 // chunk then rewrites its rows as running starts. Rows are read with independent loads.
 constexpr int RS_CHUNK = 16;
+static_assert(RS_SEG_KEYS == (uint64_t)RS_CHUNK * RS_TILE, "segment unit = one column-scan chunk");
+
+// Position of (digit d, chunk c) in the flat column-sum array: digit-major over all
+// chunks, or, for a segmented sort (seg[c] = {first chunk of c's segment, chunks in it}),
+// segment-major, then digit, then chunk, so each segment sorts on its own.
+__device__ __forceinline__ uint64_t csum_index(uint32_t d, uint32_t c, uint32_t nchunks,
+                                               const uint2* __restrict__ seg) {
+    if (!seg) return (uint64_t)d * nchunks + c;
+    const uint2 g = seg[c];
+    return (uint64_t)g.x * 256 + (uint64_t)d * g.y + (c - g.x);
+}
 
 __global__ __launch_bounds__(256) void k_rs_colsum(const uint32_t* __restrict__ hist, uint32_t nb,
-                                                   uint32_t nchunks, uint32_t* __restrict__ csum) {
+                                                   uint32_t nchunks, const uint2* __restrict__ seg,
+                                                   uint32_t* __restrict__ csum) {
     const uint32_t r0 = blockIdx.x * RS_CHUNK;
     uint32_t v[RS_CHUNK];
 #pragma unroll
@@ -280,18 +292,18 @@ __global__ __launch_bounds__(256) void k_rs_colsum(const uint32_t* __restrict__ 
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < RS_CHUNK; ++k) s += v[k];
-    csum[(uint64_t)threadIdx.x * nchunks + blockIdx.x] = s;
+    csum[csum_index(threadIdx.x, blockIdx.x, nchunks, seg)] = s;
 }
 
 __global__ __launch_bounds__(256) void k_rs_colapply(uint32_t* __restrict__ hist, uint32_t nb,
-                                                     uint32_t nchunks,
+                                                     uint32_t nchunks, const uint2* __restrict__ seg,
                                                      const uint32_t* __restrict__ csum) {
     const uint32_t r0 = blockIdx.x * RS_CHUNK;
     uint32_t v[RS_CHUNK];
 #pragma unroll
     for (int k = 0; k < RS_CHUNK; ++k)
         v[k] = r0 + k < nb ? hist[(uint64_t)(r0 + k) * 256 + threadIdx.x] : 0u;
-    uint32_t run = csum[(uint64_t)threadIdx.x * nchunks + blockIdx.x];
+    uint32_t run = csum[csum_index(threadIdx.x, blockIdx.x, nchunks, seg)];
 #pragma unroll
     for (int k = 0; k < RS_CHUNK; ++k) {
         if (r0 + k < nb) hist[(uint64_t)(r0 + k) * 256 + threadIdx.x] = run;
@@ -299,15 +311,15 @@ __global__ __launch_bounds__(256) void k_rs_colapply(uint32_t* __restrict__ hist
     }
 }
 
-// Tile-major counts -> tile-major global starts (in place).
-static int rs_offsets(rs3d_ctx* ctx, uint32_t* hist, uint32_t nb) {
+// Tile-major counts -> tile-major global starts (in place). seg: see csum_index.
+static int rs_offsets(rs3d_ctx* ctx, uint32_t* hist, uint32_t nb, const uint2* seg = nullptr) {
     const uint32_t nchunks = (nb + RS_CHUNK - 1) / RS_CHUNK;
     RS3D_TRY(ctx->sort_csum.ensure((size_t)nchunks * 256 * sizeof(uint32_t)));
     uint32_t* csum = ctx->sort_csum.as<uint32_t>();
-    k_rs_colsum<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, csum);
+    k_rs_colsum<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, seg, csum);
     RS3D_HIP(hipGetLastError());
     RS3D_TRY(scan_u32_exclusive(ctx, csum, csum, (uint64_t)nchunks * 256));
-    k_rs_colapply<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, csum);
+    k_rs_colapply<<<nchunks, 256, 0, ctx->stream>>>(hist, nb, nchunks, seg, csum);
     RS3D_HIP(hipGetLastError());
     return RS3D_OK;
 }
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ k
 template <typename K, bool HAS_VAL>
 static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, int begin_bit,
                            int end_bit, K* alt_keys, uint32_t* alt_vals, K** out_k = nullptr,
-                           uint32_t** out_v = nullptr) {
+                           uint32_t** out_v = nullptr, const uint2* seg = nullptr) {
     if (out_k) *out_k = keys;
     if (out_v) *out_v = vals;
     if (n <= 1) return RS3D_OK;
@@ -420,7 +432,7 @@ static int radix_sort_impl(rs3d_ctx* ctx, K* keys, uint32_t* vals, uint64_t n, i
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         k_rs_hist<K><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, n, shift, hist);
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(rs_offsets(ctx, hist, nb));
+        RS3D_TRY(rs_offsets(ctx, hist, nb, seg));
         k_rs_scatter<K, HAS_VAL><<<nb, RS_BLOCK, 0, ctx->stream>>>(src_k, src_v, dst_k, dst_v, n,
                                                                    shift, hist);
         RS3D_HIP(hipGetLastError());
@@ -469,7 +481,7 @@ int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t
 }
 
 int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit,
-                            uint32_t** out_keys, uint32_t** out_vals) {
+                            uint32_t** out_keys, uint32_t** out_vals, const uint2* seg) {
     *out_keys = keys;
     *out_vals = vals;
     if (n <= 1) return RS3D_OK;
@@ -479,7 +491,7 @@ int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint6
     if (end_bit > 32) end_bit = 32;
     return radix_sort_impl<uint32_t, true>(ctx, keys, vals, n, 0, end_bit,
                                            ctx->sort_keys.as<uint32_t>(),
-                                           ctx->sort_vals.as<uint32_t>(), out_keys, out_vals);
+                                           ctx->sort_vals.as<uint32_t>(), out_keys, out_vals, seg);
 }
 
 // Bitwise OR / AND of all keys, to skip digit positions where every key agrees.

@@ -196,9 +196,12 @@ int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, ui
                              uint32_t* total_dev);
 int radix_sort_u32_pairs(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit);
 // Same sort, result left in place or in the context's alternate buffers (no copy back);
-// *out_keys / *out_vals point at the sorted arrays.
+// *out_keys / *out_vals point at the sorted arrays. With seg (one entry per chunk of
+// RS_SEG_KEYS keys: {first chunk of its segment, chunks in the segment}) every segment of
+// whole chunks is sorted on its own.
+constexpr uint64_t RS_SEG_KEYS = 16 * 2048;
 int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, int end_bit,
-                            uint32_t** out_keys, uint32_t** out_vals);
+                            uint32_t** out_keys, uint32_t** out_vals, const uint2* seg = nullptr);
 int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
 int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n);  // stable
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);

@@ -410,6 +410,28 @@ def test_batched_equals_per_query(ctx):
             b.free()
 
 
+@pytest.mark.parametrize("sizes", [(98300, 65537, 131000), (40000, 3000, 70000)])
+def test_batched_containment_reorder_layouts(ctx, sizes):
+    # sizes whose whole-chunk padding is small (per-query segmented reorder sort) and
+    # large (one global sort on (query, Morton)); both must equal the single-query path
+    from rayspace3d_amd.device import batch_execute
+    queries, expect, keep = [], [], []
+    for k, npts in enumerate(sizes):
+        mesh, pts = gen.many_spheres_workload(seed=500 + k, n_objects=250, n_points=npts, world=70.0)
+        bvh = Bvh.from_mesh(ctx, mesh)
+        dp = to_dev(pts)
+        queries.append({"kind": "contain", "bvh": bvh, "points": dp, "axis": (2, 0, 2)[k],
+                        "out": torch.full((npts,), -7, dtype=torch.int32, device="cuda")})
+        expect.append(to_host(trace_points(ctx, bvh, dp, axis=(2, 0, 2)[k])))
+        keep.append(bvh)
+    print(f"batched containment, sizes {sizes}", flush=True)
+    batch_execute(ctx, queries)
+    for q, ref in zip(queries, expect):
+        assert np.array_equal(to_host(q["out"]), ref)
+    for b in keep:
+        b.free()
+
+
 def test_captured_batch_replays(ctx):
     # graph replays must equal eager batch execution, replay after replay, with eager work
     # and synchronisations in between, and must read inputs rewritten in place
