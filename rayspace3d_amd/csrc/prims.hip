@@ -245,11 +245,17 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    K kv[RS_ITEMS];  // all loads issued before the ranking
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
+        kv[j] = i < n ? keys[i] : (K)0;
+    }
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
         const bool valid = i < n;
-        const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & 255u : 0u;
+        const uint32_t d = (uint32_t)(kv[j] >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {

@@ -92,6 +92,7 @@ int rs3d_server_destroy(rs3d_server* s) {
     }
     s->keys.release();
     s->scratch.release();
+    s->plan.release();
     delete s;
     return RS3D_OK;
 }
