@@ -39,7 +39,7 @@ struct BatchDesc {
 // NSTRIPE queue heads (task t sits in stripe t % NSTRIPE), each on its own 256-byte line,
 // so the atomics of the persistent waves spread over many addresses; a wave starts on the
 // stripe of its workgroup index and moves on when a stripe runs dry.
-constexpr int TASK_RAYS = 128;
+constexpr int TASK_RAYS = 64;
 constexpr int NSTRIPE = 32;
 constexpr int STRIPE_PITCH = 64;  // 32-bit words between heads
 
