@@ -218,40 +218,6 @@ size_t countObjects(const GeometryData& g) {
     return (size_t)(mx + 1);
 }
 
-// WKT "POINT Z (x y z)" or "x y z" per line (query/src/runtime/PointIO.cpp:33-62).
-PointData loadPointDataset(const std::string& path) {
-    PointData pd;
-    FILE* f = std::fopen(path.c_str(), "r");
-    if (!f) {
-        std::cerr << "Error: Could not open point dataset file: " << path << std::endl;
-        return pd;
-    }
-    char line[1024];
-    while (std::fgets(line, sizeof line, f)) {
-        char* p = line;
-        while (*p == ' ' || *p == '\t') ++p;
-        if (*p == '\0' || *p == '\r' || *p == '\n' || *p == '#') continue;
-        char* end = nullptr;
-        if (std::strncmp(p, "POINT", 5) == 0) {
-            char* s = std::strchr(p, '(');
-            if (!s) continue;
-            ++s;
-            const float x = std::strtof(s, &end);
-            const float y = std::strtof(end, &end);
-            const float z = std::strtof(end, &end);
-            pd.positions.push_back({x, y, z});
-        } else if ((*p >= '0' && *p <= '9') || *p == '-') {
-            const float x = std::strtof(p, &end);
-            const float y = std::strtof(end, &end);
-            const float z = std::strtof(end, &end);
-            if (end != p) pd.positions.push_back({x, y, z});
-        }
-    }
-    std::fclose(f);
-    pd.numPoints = pd.positions.size();
-    return pd;
-}
-
 std::vector<std::string> splitPaths(const std::string& input) {
     std::vector<std::string> out;
     std::stringstream ss(input);

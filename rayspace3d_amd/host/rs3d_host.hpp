@@ -6,6 +6,7 @@
 //   readBinaryFile/writeBinaryFile (.pre)      common/include/BinaryIO.h:35-180
 //   loadGeometryFromFile/requirePrecomputedEdges  query/src/runtime/GeometryIO.cpp:6-20
 //   loadPointDataset                           query/src/runtime/PointIO.cpp:12-72
+//   IDatasetLoader/ObjMeshDatasetLoader/DatasetLoaderFactory  preprocess/src/*.cpp
 //   PerformanceTimer (timing JSON)             query/src/timer.h:9-49, timer.cpp:124-192
 //   DeviceContext       <- OptixContext        query/src/optix/OptixContext.cpp:4-8
 //   GeometryUploader                           query/src/geometry/GeometryUploader.h:8-43
@@ -22,6 +23,7 @@
 #include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -85,8 +87,38 @@ GeometryData readBinaryFile(const std::string& filename);  // v2 and v1; empty o
 bool writeBinaryFile(const std::string& filename, const GeometryData& geometry);
 GeometryData loadGeometryFromFile(const std::string& path);
 bool requirePrecomputedEdges(const GeometryData& g, const std::string& path, const char* label);
+// Text points (WKT "POINT Z (x y z)" or "x y z" per line), or the binary point file below
+// (recognised by its magic). An empty path gives the reference's three default points.
 PointData loadPointDataset(const std::string& path);
+// Binary point file: 32-byte header {u32 magic 'RS3P', u32 version 1, u64 count, u32 12,
+// u32[3] 0} followed by count packed float3 (x, y, z).
+bool writeBinaryPoints(const std::string& path, const PointData& points);
 size_t countObjects(const GeometryData& g);  // max(triangleToObject) + 1
+
+// ---- dataset loaders (preprocess/src/IDatasetLoader.h, DatasetLoaderFactory.cpp:24-38) ----
+class IDatasetLoader {
+public:
+    virtual ~IDatasetLoader() = default;
+    virtual GeometryData load(const std::string& path) = 0;
+};
+
+// Wavefront OBJ (preprocess/src/ObjMeshDatasetLoader.cpp:14-169): one object per shape
+// ('o'/'g' groups), shapes without faces skipped, quads split along the shorter diagonal.
+class ObjMeshDatasetLoader : public IDatasetLoader {
+public:
+    GeometryData load(const std::string& path) override;
+};
+
+// Existing .pre file (v1 or v2) as input: re-preprocessing turns v1 files into v2.
+class PreDatasetLoader : public IDatasetLoader {
+public:
+    GeometryData load(const std::string& path) override;
+};
+
+struct DatasetLoaderFactory {
+    // ".obj" -> OBJ, ".pre" -> .pre; other extensions (WKT polygons, .dt) have no loader here.
+    static std::unique_ptr<IDatasetLoader> createFromPath(const std::string& path);
+};
 
 // ---- timing JSON ---------------------------------------------------------------------
 class PerformanceTimer {
@@ -289,6 +321,14 @@ struct DeviceHashTable {
     void clear(DeviceContext& ctx);
     uint64_t failedInserts(DeviceContext& ctx) const;
 };
+
+// ---- preprocessing on the device (preprocess/src/preprocess_dataset.cpp:37-179, 346-353) ----
+// Unique-edge payload of the geometry (rs3d_extract_edges), copied back to the host.
+EdgeData extractEdgesOnDevice(DeviceContext& ctx, const GeometryData& geometry);
+// Grid histogram (rs3d_grid_stats) at resolution^3 over the vertex bounds, or [0, worldSize]^3
+// when worldSize > 0.
+void generateGridStatsOnDevice(DeviceContext& ctx, const GeometryData& geometry, GridData& grid,
+                               int resolution, float worldSize);
 
 // Selectivity estimate of two .pre grids (raw / alpha), or -1 when grids are missing.
 long long estimatePairs(DeviceContext& ctx, const GeometryData& a, const GeometryData& b,

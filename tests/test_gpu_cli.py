@@ -108,3 +108,70 @@ def test_containment_cli(datasets):
     assert {r[:2] for r in rows if r[2] == "containment"} == keys_to_set(ct)
     assert {r[:2] for r in rows if r[2] == "overlap"} == keys_to_set(ov)
     assert len(ct) >= 40
+
+
+def _obj_text(mesh):
+    """OBJ of a mesh: one 'o' group per object holding the vertices it uses."""
+    out = []
+    nxt = 1
+    for k in range(int(mesh.tri_to_obj.max()) + 1):
+        tris = mesh.indices[mesh.tri_to_obj == k]
+        used = np.unique(tris)
+        out.append(f"o object_{k}")
+        out += [f"v {x:.9g} {y:.9g} {z:.9g}" for x, y, z in mesh.vertices[used].astype(np.float64)]
+        pos = {int(v): nxt + i for i, v in enumerate(used)}
+        out += ["f " + " ".join(str(pos[int(v)]) for v in t) for t in tris]
+        nxt += len(used)
+    return "\n".join(out) + "\n"
+
+
+def _assert_edges(pre_edges, V, I, O):
+    S, E, off, cnt, src = orc.extract_edges(V, I, O)
+    assert np.array_equal(pre_edges.starts.view(np.uint32), S.view(np.uint32))
+    assert np.array_equal(pre_edges.ends.view(np.uint32), E.view(np.uint32))
+    assert np.array_equal(pre_edges.src_offsets, off) and np.array_equal(pre_edges.src_counts, cnt)
+    assert np.array_equal(pre_edges.src_objects, src)
+
+
+def test_preprocess_dataset(datasets):
+    import hostio
+    from rayspace3d_amd import preio
+    d, a, b, pts = datasets
+    text = _obj_text(a)
+    (d / "a.obj").write_text(text)
+    V, I, O = hostio.load_obj(text)
+    assert len(I) == len(a.indices)
+    run("preprocess_dataset", ["--dataset", "a.obj", "--output-geometry", "a_obj.pre", "--generate-grid",
+                               "--grid-resolution", "16", "--world-size", "150", "--output-timing", "pp.json"], d)
+    pre = preio.read_pre(str(d / "a_obj.pre"))
+    assert pre.version == 2
+    assert np.array_equal(pre.mesh.vertices.view(np.uint32), V.view(np.uint32))
+    assert np.array_equal(pre.mesh.indices, I) and np.array_equal(pre.mesh.tri_to_obj, O)
+    _assert_edges(pre.mesh.edges, V, I, O)
+    cells, mn, mx = orc.grid_stats(V, I, O, 16, 150.0)
+    g = pre.grid
+    assert g.resolution == (16, 16, 16) and np.allclose(g.min_bound, mn) and np.allclose(g.max_bound, mx)
+    assert np.array_equal(g.cells["center"], cells["center"]) and np.array_equal(g.cells["touch"], cells["touch"])
+    assert np.allclose(g.cells["avg"], cells["avg"], rtol=1e-5)
+    assert np.allclose(g.cells["ratio"], cells["ratio"], rtol=1e-4)
+    t = json.load(open(d / "pp.json"))
+    for k in ("loading dataset_1", "generating grid statistics_1", "extracting edges_1", "writing geometry data_1"):
+        assert k in t["phases"], k
+    # v1 -> v2: the reference's committed v1 fixture gains the edge payload, grid unchanged
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "check_n_geometry.pre")
+    run("preprocess_dataset", ["--mode", "pre", "--dataset", golden, "--output-geometry", "g2.pre",
+                               "--output-timing", "pp2.json"], d)
+    g1, g2 = preio.read_pre(golden), preio.read_pre(str(d / "g2.pre"))
+    assert g1.version == 1 and g2.version == 2
+    m = g1.mesh
+    assert np.array_equal(g2.mesh.vertices, m.vertices) and np.array_equal(g2.mesh.indices, m.indices)
+    _assert_edges(g2.mesh.edges, m.vertices, m.indices, m.tri_to_obj)
+    assert g2.grid.resolution == g1.grid.resolution and g2.grid.cells.tobytes() == g1.grid.cells.tobytes()
+    # the converted .pre runs through the query executables; binary points give the WKT answer
+    run("preprocess_dataset", ["--mode", "points", "--dataset", "pts.wkt", "--output-geometry", "pts.rs3p",
+                               "--output-timing", "pp3.json"], d)
+    ref = orc.points_contain(V, I, O, pts, axis=2)
+    exp = {(i, int(o)) for i, o in enumerate(ref) if o >= 0}
+    run("raytracer", ["--geometry", "a_obj.pre", "--points", "pts.rs3p", "--warmup-runs", "0"], d)
+    hdr, rows = read_pairs(d / "ray_results.csv")
+    assert set(rows) == exp
