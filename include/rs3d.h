@@ -360,6 +360,30 @@ int rs3d_extract_edges(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertic
                        int32_t* src_objects, uint64_t max_edges, uint64_t max_sources,
                        uint64_t* num_edges, uint64_t* num_sources);
 
+/* ---- exact triangle-triangle intersection --------------------------------------------
+ * Predicate: Guigue-Devillers ("Fast and Robust Triangle-Triangle Overlap Test Using
+ * Orientation Predicates", JGT 8(1), 2003) with the decisions of the reference's TriInt
+ * (preprocess/src/tdbase_lib/geometry/TriInt.cpp:400-492), bit-identical in fp32; closed
+ * triangles (touching and coplanar overlap count).
+ * rs3d_tri_intersect replaces TriInt / TriInt_batch_gpu (tdbase_lib/include/geometry.h:164-166):
+ * a, b = device float[9 n] (three vertices per triangle), out = device uint8[n]. Async. */
+int rs3d_tri_intersect(rs3d_ctx* ctx, const float* a, const float* b, uint64_t n, uint8_t* out);
+/* Exact intersection join: pairs (object of bvh_a, object of bvh_b) having a triangle pair
+ * that intersects, the exact counterpart of the edge-ray overlap join (edge rays miss
+ * touching and coplanar contacts). Every triangle of bvh_a queries bvh_b with its bounding
+ * box. _hash: inserts into ht ((b, a) when swap), async. rs3d_tri_join: sorted unique keys
+ * (a << 32 | b); RS3D_E_CAPACITY when they exceed max_out (count = required size). */
+int rs3d_tri_join_hash(rs3d_ctx* ctx, const rs3d_bvh* bvh_a, const rs3d_bvh* bvh_b, int swap,
+                       rs3d_hash_table* ht);
+int rs3d_tri_join(rs3d_ctx* ctx, const rs3d_bvh* bvh_a, const rs3d_bvh* bvh_b, uint64_t* keys_out,
+                  uint64_t max_out, uint64_t* count);
+/* Work counters of one tri join (for the bytes-per-query roofline). Synchronous. */
+typedef struct {
+    uint64_t queries, nodes_visited, tris_loaded, predicate_tests, hits;
+} rs3d_tri_join_stats_t;
+int rs3d_tri_join_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh_a, const rs3d_bvh* bvh_b,
+                        rs3d_tri_join_stats_t* stats /* host */);
+
 /* Estimated overlap pairs from two grids (host sizing of
  * query/src/applications/raytracer_overlap_estimated_main.cpp:133-195): raw estimate / alpha
  * (alpha_correction = 0: raw). grid_min/max/resolution: host arrays of grid A. */

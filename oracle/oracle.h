@@ -102,6 +102,19 @@ int64_t orc_extract_edges(const float* V, int64_t nV, const uint32_t* I, int64_t
                           const int32_t* tri_obj, float* starts, float* ends, int32_t* offsets,
                           int32_t* counts, int32_t* sources, int64_t* n_sources);
 
+/* Exact triangle-triangle predicate (Guigue-Devillers, decision form of the reference's
+ * TriInt): t1, t2 = 9 floats (three vertices). 1 when the closed triangles intersect. */
+int orc_tri_tri(const float* t1, const float* t2);
+/* orc_tri_tri on n pairs (A[i], B[i]), 9 floats each. */
+void orc_tri_tri_many(const float* A, const float* B, int64_t n, uint8_t* out);
+
+/* Exact intersection join: sorted unique keys (a << 32 | b) of (object of A, object of B)
+ * with a triangle pair whose closed boxes overlap and orc_tri_tri reports intersecting.
+ * Returns the pair count (keys written up to cap). */
+int64_t orc_tri_join(const float* Va, int64_t nVa, const uint32_t* Ia, const int32_t* Oa, int64_t nTa,
+                     const float* Vb, int64_t nVb, const uint32_t* Ib, const int32_t* Ob, int64_t nTb,
+                     uint64_t* out_keys, int64_t cap, int threads);
+
 #ifdef __cplusplus
 }
 #endif

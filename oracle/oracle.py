@@ -31,6 +31,8 @@ _sig = {
     "orc_estimate_pairs": (_I64, [_P, _P, _I64, _P, _P, _P, C.c_float, C.c_float, C.c_int]),
     "orc_grid_stats": (None, [_P, _I64, _P, _I64, _P, C.c_int, C.c_float, _P, _P]),
     "orc_extract_edges": (_I64, [_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_tri_tri_many": (None, [_P, _P, _I64, _P]),
+    "orc_tri_join": (_I64, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I64, _P, _I64, C.c_int]),
 }
 
 _lib = None
@@ -184,3 +186,46 @@ def extract_edges(V, I, tri_obj):
     lib().orc_extract_edges(_p(V), len(V), _p(I), len(I), _p(O), _p(S), _p(E), _p(off), _p(cnt),
                             _p(src), C.cast(C.pointer(ns), C.c_void_p))
     return S, E, off, cnt, src[:ns.value]
+
+
+def tri_tri(A, B):
+    """Exact triangle-triangle predicate on pairs: A, B float32 [n, 3, 3] -> bool [n]."""
+    A = _c(np.asarray(A, np.float32).reshape(-1, 9), np.float32)
+    B = _c(np.asarray(B, np.float32).reshape(-1, 9), np.float32)
+    out = np.zeros(len(A), np.uint8)
+    lib().orc_tri_tri_many(_p(A), _p(B), len(A), _p(out))
+    return out.astype(bool)
+
+
+def tri_join(ma, mb, threads=1):
+    """Exact intersection join of two meshes: sorted unique keys (a << 32 | b)."""
+    args = []
+    for m in (ma, mb):
+        V = _c(m.vertices, np.float32); I = _c(m.indices, np.uint32); O = _c(m.tri_to_obj, np.int32)
+        args.append((V, I, O))
+    (Va, Ia, Oa), (Vb, Ib, Ob) = args
+    def call(out, cap):
+        return lib().orc_tri_join(_p(Va), len(Va), _p(Ia), _p(Oa), len(Ia), _p(Vb), len(Vb), _p(Ib), _p(Ob),
+                                  len(Ib), _p(out), cap, threads)
+    n = call(None, 0)
+    out = np.empty(max(n, 1), np.uint64)
+    call(out, n)
+    return out[:n]
+
+
+def ref_tri_int():
+    """The reference's own TriInt compiled from its source (oracle/_ref/libtriint.so), or
+    None when it was not built. Returns f(t1, t2) -> bool on 9-float triangles."""
+    path = os.path.join(HERE, "_ref", "libtriint.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    f = getattr(L, "_ZN6tdbase6TriIntEPKfS1_")
+    f.restype = C.c_bool
+    f.argtypes = [_P, _P]
+
+    def call(t1, t2):
+        a = _c(np.asarray(t1, np.float32).reshape(9), np.float32)
+        b = _c(np.asarray(t2, np.float32).reshape(9), np.float32)
+        return bool(f(_p(a), _p(b)))
+    return call

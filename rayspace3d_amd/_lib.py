@@ -59,6 +59,11 @@ class EdgeStats(C.Structure):
                                            "restarts", "pair_emits", "source_refs")]
 
 
+class TriJoinStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("queries", "nodes_visited", "tris_loaded", "predicate_tests",
+                                           "hits")]
+
+
 class Edges(C.Structure):
     _fields_ = [
         ("starts", C.c_void_p),
@@ -141,6 +146,10 @@ SIGNATURES = {
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),
     "rs3d_extract_edges": (I32, [P, P, U64, P, U64, P, P, P, P, P, P, U64, U64, PU64, PU64]),
+    "rs3d_tri_intersect": (I32, [P, P, P, U64, P]),
+    "rs3d_tri_join_hash": (I32, [P, P, P, I32, C.POINTER(HashTable)]),
+    "rs3d_tri_join": (I32, [P, P, P, P, U64, PU64]),
+    "rs3d_tri_join_stats": (I32, [P, P, P, C.POINTER(TriJoinStats)]),
     "rs3d_filter_refine": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
     "rs3d_filter_box": (I32, [P, C.POINTER(C.c_float), P, U64, I32, P]),
     "rs3d_scan_i32_i64": (I32, [P, P, P, U64, PI64]),

@@ -236,6 +236,41 @@ def overlap_join(ctx, bvh1, dedges1, bvh2, dedges2, mode: int = 0, max_out: int 
         return to_host(out[:cnt.value], np.uint64)
 
 
+def tri_intersect(ctx, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """Exact triangle-triangle predicate on pairs (rs3d_tri_intersect): a, b float32
+    device tensors [n, 3, 3] -> uint8 [n]."""
+    a = a.reshape(-1, 9).contiguous()
+    b = b.reshape(-1, 9).contiguous()
+    if a.shape != b.shape:
+        raise ValueError("tri_intersect: a and b must hold the same number of triangles")
+    out = torch.empty(max(a.shape[0], 1), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_tri_intersect(ctx.handle, _ptr(a), _ptr(b), a.shape[0], _ptr(out)), "rs3d_tri_intersect")
+    ctx.sync()
+    return out[:a.shape[0]]
+
+
+def tri_join(ctx, bvh_a, bvh_b, max_out: int = 1 << 22) -> np.ndarray:
+    """Exact intersection join (rs3d_tri_join): sorted unique (a << 32 | b) keys."""
+    while True:
+        out = torch.empty(max(max_out, 1), dtype=torch.int64, device="cuda")
+        cnt = C.c_uint64()
+        torch.cuda.synchronize()
+        rc = ctx.lib.rs3d_tri_join(ctx.handle, bvh_a.handle, bvh_b.handle, _ptr(out), max_out, C.byref(cnt))
+        if rc == _lib.RS3D_E_CAPACITY and cnt.value > max_out:
+            max_out = int(cnt.value)
+            continue
+        check(rc, "rs3d_tri_join")
+        return to_host(out[:cnt.value], np.uint64)
+
+
+def tri_join_stats(ctx, bvh_a, bvh_b) -> dict:
+    st = _lib.TriJoinStats()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_tri_join_stats(ctx.handle, bvh_a.handle, bvh_b.handle, C.byref(st)), "rs3d_tri_join_stats")
+    return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+
 def sort_unique_u64(ctx, keys: torch.Tensor) -> torch.Tensor:
     cnt = C.c_uint64()
     torch.cuda.synchronize()
