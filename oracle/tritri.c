@@ -16,6 +16,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -236,16 +237,19 @@ static int cmp_u64(const void* a, const void* b) {
     return x < y ? -1 : x > y;
 }
 
+/* Uniform grid over B's triangle boxes. Every A triangle visits the cells its box touches;
+ * a candidate pair is tested only in the cell that holds the lower corner of the two
+ * boxes' intersection, so each pair is tested once. */
 typedef struct {
-    float lo0;
-    int64_t idx;
-} sweep_ent;
+    double lo[3], inv;
+    int64_t n[3];
+    int64_t* start; /* cell -> first entry (n cells + 1) */
+    int64_t* items; /* B triangle indices */
+} tgrid;
 
-static int cmp_sweep(const void* a, const void* b) {
-    const sweep_ent* x = (const sweep_ent*)a;
-    const sweep_ent* y = (const sweep_ent*)b;
-    if (x->lo0 != y->lo0) return x->lo0 < y->lo0 ? -1 : 1;
-    return x->idx < y->idx ? -1 : x->idx > y->idx;
+static int64_t cell_of(const tgrid* g, int a, float x) {
+    int64_t c = (int64_t)(((double)x - g->lo[a]) * g->inv);
+    return c < 0 ? 0 : (c >= g->n[a] ? g->n[a] - 1 : c);
 }
 
 int64_t orc_tri_join(const float* Va, int64_t nVa, const uint32_t* Ia, const int32_t* Oa, int64_t nTa,
@@ -255,53 +259,90 @@ int64_t orc_tri_join(const float* Va, int64_t nVa, const uint32_t* Ia, const int
     tri_box* B = (tri_box*)malloc(sizeof(tri_box) * (size_t)(nTb > 0 ? nTb : 1));
     make_boxes(Va, nVa, Ia, Oa, nTa, A);
     make_boxes(Vb, nVb, Ib, Ob, nTb, B);
-    /* B sorted by box minimum along x; every A triangle scans the B boxes whose x
-     * minimum is <= its own x maximum, skipping those that end before it starts. */
-    sweep_ent* S = (sweep_ent*)malloc(sizeof(sweep_ent) * (size_t)(nTb > 0 ? nTb : 1));
-    int64_t nS = 0;
-    double max_ext = 0.0;
-    for (int64_t t = 0; t < nTb; ++t)
-        if (B[t].valid) {
-            S[nS].lo0 = B[t].lo[0];
-            S[nS].idx = t;
-            ++nS;
-            const double e = (double)B[t].hi[0] - (double)B[t].lo[0];
-            max_ext = e > max_ext ? e : max_ext;
+    tgrid g;
+    double hi[3], ext = 0.0;
+    int64_t nb = 0;
+    for (int a = 0; a < 3; ++a) g.lo[a] = 1e300, hi[a] = -1e300;
+    for (int64_t t = 0; t < nTb; ++t) {
+        if (!B[t].valid) continue;
+        ++nb;
+        for (int a = 0; a < 3; ++a) {
+            g.lo[a] = B[t].lo[a] < g.lo[a] ? B[t].lo[a] : g.lo[a];
+            hi[a] = B[t].hi[a] > hi[a] ? B[t].hi[a] : hi[a];
+            ext += (double)B[t].hi[a] - (double)B[t].lo[a];
         }
-    qsort(S, (size_t)nS, sizeof(sweep_ent), cmp_sweep);
+    }
     int64_t raw_cap = 1 << 16, raw_n = 0;
     uint64_t* raw = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)raw_cap);
-#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
-    for (int64_t i = 0; i < nTa; ++i) {
-        const tri_box* a = &A[i];
-        if (!a->valid) continue;
-        /* first S entry with lo0 >= a.lo0 - max_ext (those before it end before a starts;
-         * the margin keeps the cut conservative under rounding) */
-        const double start = (double)a->lo[0] - max_ext * 1.000001 - 1e-30;
-        int64_t lo = 0, hi = nS;
-        while (lo < hi) {
-            const int64_t m = (lo + hi) / 2;
-            if ((double)S[m].lo0 < start) lo = m + 1;
-            else hi = m;
+    if (nb > 0) {
+        /* cell edge: twice the mean box extent, and no more cells than 2 per triangle */
+        double h = 2.0 * ext / (3.0 * (double)nb);
+        const double vol = (hi[0] - g.lo[0] + 1e-30) * (hi[1] - g.lo[1] + 1e-30) * (hi[2] - g.lo[2] + 1e-30);
+        const double hmin = cbrt(vol / (2.0 * (double)nb));
+        h = h > hmin ? h : hmin;
+        h = h > 0 ? h : 1.0;
+        g.inv = 1.0 / h;
+        int64_t ncell = 1;
+        for (int a = 0; a < 3; ++a) {
+            g.n[a] = (int64_t)((hi[a] - g.lo[a]) * g.inv) + 1;
+            ncell *= g.n[a];
         }
-        int32_t last = -1;
-        for (int64_t s = lo; s < nS && S[s].lo0 <= a->hi[0]; ++s) {
-            const tri_box* b = &B[S[s].idx];
-            int ov = 1;
-            for (int k = 0; k < 3; ++k) ov &= a->lo[k] <= b->hi[k] && b->lo[k] <= a->hi[k];
-            if (!ov || b->obj == last) continue;
-            if (!orc_tri_tri(a->v, b->v)) continue;
-            last = b->obj;
-            const uint64_t key = ((uint64_t)(uint32_t)a->obj << 32) | (uint32_t)b->obj;
-#pragma omp critical(orc_tri_join_raw)
-            {
-                if (raw_n == raw_cap) {
-                    raw_cap *= 2;
-                    raw = (uint64_t*)realloc(raw, sizeof(uint64_t) * (size_t)raw_cap);
-                }
-                raw[raw_n++] = key;
+        g.start = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                for (int64_t c = 0; c < ncell; ++c) g.start[c + 1] += g.start[c];
+                g.items = (int64_t*)malloc(sizeof(int64_t) * (size_t)(g.start[ncell] > 0 ? g.start[ncell] : 1));
+                for (int64_t c = ncell; c > 0; --c) g.start[c] = g.start[c - 1];
+                g.start[0] = 0;
+            }
+            for (int64_t t = 0; t < nTb; ++t) {
+                if (!B[t].valid) continue;
+                int64_t c0[3], c1[3];
+                for (int a = 0; a < 3; ++a) c0[a] = cell_of(&g, a, B[t].lo[a]), c1[a] = cell_of(&g, a, B[t].hi[a]);
+                for (int64_t z = c0[2]; z <= c1[2]; ++z)
+                    for (int64_t y = c0[1]; y <= c1[1]; ++y)
+                        for (int64_t x = c0[0]; x <= c1[0]; ++x) {
+                            const int64_t c = (z * g.n[1] + y) * g.n[0] + x;
+                            if (pass == 0) g.start[c + 1]++;
+                            else g.items[g.start[c + 1]++] = t;
+                        }
             }
         }
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads > 0 ? threads : 1)
+        for (int64_t i = 0; i < nTa; ++i) {
+            const tri_box* a = &A[i];
+            if (!a->valid) continue;
+            int64_t c0[3], c1[3];
+            for (int k = 0; k < 3; ++k) c0[k] = cell_of(&g, k, a->lo[k]), c1[k] = cell_of(&g, k, a->hi[k]);
+            for (int64_t z = c0[2]; z <= c1[2]; ++z)
+                for (int64_t y = c0[1]; y <= c1[1]; ++y)
+                    for (int64_t x = c0[0]; x <= c1[0]; ++x) {
+                        const int64_t c = (z * g.n[1] + y) * g.n[0] + x;
+                        for (int64_t q = g.start[c]; q < g.start[c + 1]; ++q) {
+                            const tri_box* b = &B[g.items[q]];
+                            int ov = 1;
+                            for (int k = 0; k < 3; ++k) ov &= a->lo[k] <= b->hi[k] && b->lo[k] <= a->hi[k];
+                            if (!ov) continue;
+                            /* test the pair only in the cell of its boxes' common lower corner */
+                            const int64_t rx = cell_of(&g, 0, a->lo[0] > b->lo[0] ? a->lo[0] : b->lo[0]);
+                            const int64_t ry = cell_of(&g, 1, a->lo[1] > b->lo[1] ? a->lo[1] : b->lo[1]);
+                            const int64_t rz = cell_of(&g, 2, a->lo[2] > b->lo[2] ? a->lo[2] : b->lo[2]);
+                            if (rx != x || ry != y || rz != z) continue;
+                            if (!orc_tri_tri(a->v, b->v)) continue;
+                            const uint64_t key = ((uint64_t)(uint32_t)a->obj << 32) | (uint32_t)b->obj;
+#pragma omp critical(orc_tri_join_raw)
+                            {
+                                if (raw_n == raw_cap) {
+                                    raw_cap *= 2;
+                                    raw = (uint64_t*)realloc(raw, sizeof(uint64_t) * (size_t)raw_cap);
+                                }
+                                raw[raw_n++] = key;
+                            }
+                        }
+                    }
+        }
+        free(g.start);
+        free(g.items);
     }
     qsort(raw, (size_t)raw_n, sizeof(uint64_t), cmp_u64);
     int64_t u = 0;
@@ -311,7 +352,6 @@ int64_t orc_tri_join(const float* Va, int64_t nVa, const uint32_t* Ia, const int
             ++u;
         }
     free(raw);
-    free(S);
     free(A);
     free(B);
     return u;
