@@ -94,6 +94,14 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
                    const rs3d_bvh_opts* opts, rs3d_bvh** out);
 int rs3d_bvh_free(rs3d_bvh* bvh);
 int rs3d_bvh_info_get(const rs3d_bvh* bvh, rs3d_bvh_info* info); /* info: host */
+/* Copy of the tree's device arrays (no counterpart in the reference, whose OptiX handle is
+ * opaque; for inspection and structural checks): nodes = num_nodes records of 64 bytes
+ * (child 0 box x lo/hi, y lo/hi, z lo/hi, child 1 box likewise, as 12 floats, then int32
+ * link 0, link 1, count 0, count 1: count > 0 = leaf of that many triangles starting at the
+ * link, count 0 with link >= 0 = internal node, link < 0 = empty child); tris =
+ * num_triangles records of 48 bytes (3 vertices as float4; v0.w = original triangle index
+ * and v1.w = object id, as int32 bits). Either output may be null. Device pointers. */
+int rs3d_bvh_export(const rs3d_bvh* bvh, void* nodes, void* tris);
 
 /* ---- point containment (replaces __raygen__rg, query/src/cuda/raytracing.cu:9-78, and
  *      RayLauncher::launch, query/src/raytracing/RayLauncher.cpp:44-58) ------------------

@@ -32,11 +32,12 @@ __device__ __forceinline__ float3 ld3(const float* p, uint64_t i) {
 }
 
 // bounds[0..2] = centroid min, [3..5] = centroid max, [6..8] = box min, [9..11] = box max
-// bounds[16..18] = vertex min, [19..21] = vertex max; bounds[12] = bad-index flag
+// bounds[16..18] = vertex min, [19..21] = vertex max; bounds[12] = bad-index flag,
+// [13] = phase-3 refit records, [14] = refit record overflow
 __global__ void k_init_bounds(int* bounds) {
     int t = threadIdx.x;
     if (t < 12) bounds[t] = ((t / 3) % 2 == 0) ? 0x7fffffff : (int)0x80000000;
-    if (t == 12 || t == 13) bounds[t] = 0;
+    if (t >= 12 && t <= 14) bounds[t] = 0;
     if (t >= 16 && t < 22) bounds[t] = t < 19 ? 0x7fffffff : (int)0x80000000;
 }
 
@@ -252,14 +253,19 @@ __global__ __launch_bounds__(BVH_BLOCK) void k_karras(const uint32_t* __restrict
     if (i == 0) parent_node[0] = -1;
 }
 
-// Bottom-up refit in two phases, so that threads only ever synchronise with threads of
+// Bottom-up refit in three phases, so that threads only ever synchronise with threads of
 // their own workgroup (workgroup-scope fences; no device-wide fence per tree level).
 // A Karras node covers a contiguous leaf range [first, last] and its index is one end of
 // that range. Phase 1: a workgroup owns a window of REFIT_WIN consecutive leaves and refits
 // every node whose range lies inside the window; a thread that reaches a node crossing
-// the window's edge records it and stops. Phase 2: one workgroup takes those records and
-// refits the crossing nodes, which form the top of the tree.
+// the window's edge records it and stops. Phase 2: a workgroup owns MID_WINS consecutive
+// windows and continues from their records, refitting every node inside its span; nodes
+// crossing the span are recorded again. Phase 3: one workgroup finishes from those records
+// (the top of the tree). A record means "one child of this node is complete"; each node's
+// second arrival refits it, whichever phase that happens in.
 constexpr int REFIT_WIN = 1024;
+constexpr int MID_WINS = 256;   // phase-2 span: 256 windows = 262,144 leaves
+constexpr int SLOT_CAP = 64;    // phase-2 records per window (more go to phase 3)
 
 __device__ __forceinline__ void refit_node(const int2* __restrict__ child,
                                            const float4* __restrict__ sbox, float4* nbox,
@@ -290,32 +296,87 @@ __device__ __forceinline__ void refit_node(const int2* __restrict__ child,
     height[p] = (h0 > h1 ? h0 : h1) + 1;
 }
 
+// Record for phase 3 (capacity cap; overflow raises *bad, which fails the build).
+__device__ __forceinline__ void push_top(int p, int* __restrict__ top, int* __restrict__ num_top, int64_t cap,
+                                         int* __restrict__ bad) {
+    const int k = atomicAdd(num_top, 1);
+    if (k < cap) top[k] = p;
+    else atomicExch(bad, 1);
+}
+
 __global__ __launch_bounds__(REFIT_WIN) void k_refit_local(const int2* __restrict__ child,
                                                            const int2* __restrict__ range,
                                                            const int* __restrict__ parent_node,
                                                            const int* __restrict__ parent_leaf,
                                                            const float4* __restrict__ sbox,
                                                            int64_t n, float4* nbox, int* height,
-                                                           int* __restrict__ pending,
-                                                           int* __restrict__ num_pending) {
+                                                           int* __restrict__ slots,
+                                                           int* __restrict__ win_count,
+                                                           int* __restrict__ top, int* __restrict__ num_top,
+                                                           int* __restrict__ bad) {
     __shared__ int arrived[REFIT_WIN];
+    __shared__ int nslot;
     const int64_t base = (int64_t)blockIdx.x * REFIT_WIN;
+    const int64_t base2 = (int64_t)(blockIdx.x / MID_WINS) * ((int64_t)REFIT_WIN * MID_WINS);
+    const int64_t end2 = base2 + (int64_t)REFIT_WIN * MID_WINS;
     arrived[threadIdx.x] = 0;
+    if (threadIdx.x == 0) nslot = 0;
     __syncthreads();
     const int64_t j = base + threadIdx.x;
-    if (j >= n) return;
-    int p = parent_leaf[j];
-    while (p >= 0) {
-        const int2 r = range[p];
-        if (r.x < base || r.y >= base + REFIT_WIN) {
-            pending[atomicAdd(num_pending, 1)] = p;
-            return;
+    if (j < n) {
+        int p = parent_leaf[j];
+        while (p >= 0) {
+            const int2 r = range[p];
+            if (r.x < base || r.y >= base + REFIT_WIN) {
+                // crossing node: phase 2 when inside this span, else phase 3
+                bool mid = r.x >= base2 && r.y < end2;
+                if (mid) {
+                    const int k = atomicAdd(&nslot, 1);
+                    if (k < SLOT_CAP) slots[(int64_t)blockIdx.x * SLOT_CAP + k] = p;
+                    else mid = false;
+                }
+                if (!mid) push_top(p, top, num_top, n, bad);
+                break;
+            }
+            __threadfence_block();
+            if (atomicAdd(&arrived[p - base], 1) == 0) break;  // sibling still in progress
+            __threadfence_block();
+            refit_node(child, sbox, nbox, height, p);
+            p = parent_node[p];
         }
-        __threadfence_block();
-        if (atomicAdd(&arrived[p - base], 1) == 0) return;  // sibling still in progress
-        __threadfence_block();
-        refit_node(child, sbox, nbox, height, p);
-        p = parent_node[p];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && win_count) win_count[blockIdx.x] = nslot < SLOT_CAP ? nslot : SLOT_CAP;
+}
+
+__global__ __launch_bounds__(MID_WINS) void k_refit_mid(const int2* __restrict__ child,
+                                                        const int2* __restrict__ range,
+                                                        const int* __restrict__ parent_node,
+                                                        const float4* __restrict__ sbox, float4* nbox,
+                                                        int* height, int* flags, const int* __restrict__ slots,
+                                                        const int* __restrict__ win_count, int64_t num_win,
+                                                        int64_t n, int* __restrict__ top,
+                                                        int* __restrict__ num_top, int* __restrict__ bad) {
+    const int64_t w = (int64_t)blockIdx.x * MID_WINS + threadIdx.x;
+    const int64_t base2 = (int64_t)blockIdx.x * ((int64_t)REFIT_WIN * MID_WINS);
+    const int64_t end2 = base2 + (int64_t)REFIT_WIN * MID_WINS;
+    if (w >= num_win) return;
+    const int cnt = win_count[w];
+    for (int k = 0; k < cnt; ++k) {
+        int p = slots[w * SLOT_CAP + k];
+        while (p >= 0) {
+            const int2 r = range[p];
+            if (r.x < base2 || r.y >= end2) {
+                push_top(p, top, num_top, n, bad);
+                break;
+            }
+            __threadfence_block();
+            if (__hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+            __threadfence_block();
+            refit_node(child, sbox, nbox, height, p);
+            p = parent_node[p];
+        }
     }
 }
 
@@ -324,8 +385,8 @@ __global__ __launch_bounds__(REFIT_WIN) void k_refit_top(const int2* __restrict_
                                                          const float4* __restrict__ sbox,
                                                          float4* nbox, int* height, int* flags,
                                                          const int* __restrict__ pending,
-                                                         const int* __restrict__ num_pending) {
-    const int m = *num_pending;
+                                                         const int* __restrict__ num_pending, int64_t cap) {
+    const int m = (int)min((int64_t)*num_pending, cap);
     for (int it = threadIdx.x; it < m; it += REFIT_WIN) {
         int p = pending[it];
         while (p >= 0) {
@@ -508,13 +569,24 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
             set_error("hipMemsetAsync: %s", hipGetErrorString(e));
             return fail(RS3D_E_HIP);
         }
-        int* num_pending = bounds + 13;
-        int* pending = reinterpret_cast<int*>(vals);  // the sort permutation is dead here
-        const unsigned nw = (unsigned)((n + REFIT_WIN - 1) / REFIT_WIN);
-        k_refit_local<<<nw, REFIT_WIN, 0, s>>>(child, range, parent_node, parent_leaf, sbox, n,
-                                               nbox, height, pending, num_pending);
-        k_refit_top<<<1, REFIT_WIN, 0, s>>>(child, parent_node, sbox, nbox, height, flags,
-                                            pending, num_pending);
+        int* num_top = bounds + 13;
+        int* refit_bad = bounds + 14;
+        int* top = reinterpret_cast<int*>(vals);  // the sort permutation is dead here
+        const int64_t nw = (n + REFIT_WIN - 1) / REFIT_WIN;
+        // phase-2 records in the Morton keys, dead after k_karras (nw * (SLOT_CAP + 1) <= n
+        // whenever there is more than one window)
+        int* slots = nw > 1 ? reinterpret_cast<int*>(keys) : nullptr;
+        int* win_count = nw > 1 ? slots + nw * SLOT_CAP : nullptr;
+        k_refit_local<<<(unsigned)nw, REFIT_WIN, 0, s>>>(child, range, parent_node, parent_leaf, sbox, n,
+                                                         nbox, height, slots, win_count, top, num_top,
+                                                         refit_bad);
+        if (nw > 1) {
+            const unsigned nm = (unsigned)((nw + MID_WINS - 1) / MID_WINS);
+            k_refit_mid<<<nm, MID_WINS, 0, s>>>(child, range, parent_node, sbox, nbox, height, flags, slots,
+                                                win_count, nw, n, top, num_top, refit_bad);
+            k_refit_top<<<1, REFIT_WIN, 0, s>>>(child, parent_node, sbox, nbox, height, flags, top, num_top,
+                                                n);
+        }
         k_emit_nodes<<<nbi, BVH_BLOCK, 0, s>>>(child, range, sbox, nbox, n, leaf, bvh->nodes);
     } else {
         k_emit_nodes<<<1, 64, 0, s>>>(child, range, sbox, nbox, n, leaf, bvh->nodes);
@@ -536,6 +608,10 @@ int rs3d_bvh_build(rs3d_ctx* ctx, const float* vertices, uint64_t num_vertices,
                   (unsigned long long)num_vertices);
         return fail(RS3D_E_ARG);
     }
+    if (hb[14]) {
+        set_error("rs3d_bvh_build: refit records overflowed");
+        return fail(RS3D_E_HIP);
+    }
     for (int k = 0; k < 3; ++k) {
         bvh->bmin[k] = ord2f(hb[6 + k]);
         bvh->bmax[k] = ord2f(hb[9 + k]);
@@ -553,6 +629,21 @@ int rs3d_bvh_free(rs3d_bvh* bvh) {
     if (bvh->nodes) (void)hipFree(bvh->nodes);
     if (bvh->tris) (void)hipFree(bvh->tris);
     delete bvh;
+    return RS3D_OK;
+}
+
+int rs3d_bvh_export(const rs3d_bvh* bvh, void* nodes, void* tris) {
+    if (!bvh || !bvh->ctx) {
+        set_error("rs3d_bvh_export: bad arguments");
+        return RS3D_E_ARG;
+    }
+    RS3D_HIP(hipSetDevice(bvh->ctx->device));
+    hipStream_t s = bvh->ctx->stream;
+    if (nodes && bvh->num_nodes && bvh->nodes)
+        RS3D_HIP(hipMemcpyAsync(nodes, bvh->nodes, bvh->num_nodes * sizeof(BNode), hipMemcpyDeviceToDevice, s));
+    if (tris && bvh->num_tris && bvh->tris)
+        RS3D_HIP(hipMemcpyAsync(tris, bvh->tris, bvh->num_tris * sizeof(TriRec), hipMemcpyDeviceToDevice, s));
+    RS3D_HIP(hipStreamSynchronize(s));
     return RS3D_OK;
 }
 

@@ -124,6 +124,19 @@ class Bvh:
         return _lib.MeshView(v.data_ptr(), v.shape[0], i.data_ptr(), i.shape[0],
                              o.data_ptr() if o is not None else None, n_obj)
 
+    def export(self):
+        """Host copy of the tree: (nodes float32 [m, 12] child boxes as (lo.x, hi.x, lo.y,
+        hi.y, lo.z, hi.z) of child 0 then child 1, links int32 [m, 4] = (link 0, link 1,
+        count 0, count 1), triangles float32 [n, 3, 4] whose w lanes carry the original
+        index and the object id as int32 bits)."""
+        inf = self.info()
+        nodes = torch.empty((max(inf.num_nodes, 1), 16), dtype=torch.int32, device="cuda")
+        tris = torch.empty((max(inf.num_triangles, 1), 3, 4), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        check(self.ctx.lib.rs3d_bvh_export(self.handle, _ptr(nodes), _ptr(tris)), "rs3d_bvh_export")
+        nh = to_host(nodes[:inf.num_nodes]).copy()
+        return nh[:, :12].view(np.float32), nh[:, 12:], to_host(tris[:inf.num_triangles]).copy()
+
     def info(self) -> _lib.BvhInfo:
         inf = _lib.BvhInfo()
         check(self.ctx.lib.rs3d_bvh_info_get(self.handle, C.byref(inf)), "rs3d_bvh_info_get")

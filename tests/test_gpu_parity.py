@@ -571,3 +571,56 @@ def test_gpu_edge_extraction_edge_cases(ctx):
     # empty mesh
     em = gen.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), np.zeros(0, np.int32), 0)
     assert extract_edges_gpu(ctx, em).num_edges == 0
+
+
+def _check_bvh(bvh, mesh):
+    """Local invariants that make a BVH correct, over the nodes reachable from the root
+    (records inside collapsed leaves are never visited): leaves hold every triangle once,
+    each leaf box holds its triangles, each inner child box holds both boxes of that child."""
+    boxes, links, tris = bvh.export()
+    n = len(mesh.indices)
+    orig = tris[:, 0, 3].view(np.int32)
+    assert np.array_equal(np.sort(orig), np.arange(n))
+    V = mesh.vertices[mesh.indices[orig]]                                # [n, 3, 3]
+    assert np.array_equal(tris[:, :, :3], V)
+    assert np.array_equal(tris[:, 1, 3].view(np.int32), mesh.tri_to_obj[orig])
+    reach = np.zeros(len(links), bool)
+    frontier = np.array([0])
+    depth = 0
+    while len(frontier):
+        assert not reach[frontier].any()                                 # a tree: no node twice
+        reach[frontier] = True
+        nxt = [links[frontier, c][(links[frontier, 2 + c] == 0) & (links[frontier, c] >= 0)] for c in (0, 1)]
+        frontier = np.concatenate(nxt)
+        depth += 1
+    assert depth <= bvh.info().max_depth
+    covered = np.zeros(n, np.int64)
+    for c in (0, 1):
+        lo, hi = boxes[:, 6 * c:6 * c + 6:2], boxes[:, 6 * c + 1:6 * c + 6:2]
+        link, cnt = links[:, c], links[:, 2 + c]
+        leaf = reach & (cnt > 0)
+        for k in range(int(cnt[leaf].max()) if leaf.any() else 0):
+            sel = leaf & (cnt > k)
+            t = link[sel] + k
+            np.add.at(covered, t, 1)
+            tv = tris[t, :, :3]
+            assert np.all(tv.min(axis=1) >= lo[sel]) and np.all(tv.max(axis=1) <= hi[sel])
+        inner = reach & (cnt == 0) & (link >= 0)
+        kids = link[inner]
+        for cc in (0, 1):
+            klo, khi = boxes[kids, 6 * cc:6 * cc + 6:2], boxes[kids, 6 * cc + 1:6 * cc + 6:2]
+            nonempty = klo[:, 0] <= khi[:, 0]
+            assert np.all(klo[nonempty] >= lo[inner][nonempty]) and np.all(khi[nonempty] <= hi[inner][nonempty])
+    assert np.all(covered == 1)
+
+
+@pytest.mark.parametrize("n_obj", [300, 5000])
+def test_bvh_structure(ctx, n_obj):
+    # 5000 mixed-level objects: ~4 M triangles, so the refit runs all three phases with
+    # 16 phase-2 spans of 262,144 leaves
+    mesh = gen.overlap_dataset(71, n_objects=n_obj, world=400.0, levels=(2, 3))
+    print(f"bvh structure: {len(mesh.indices)} triangles", flush=True)
+    for leaf in (1, 4):
+        b = Bvh.from_mesh(ctx, mesh, leaf_size=leaf)
+        _check_bvh(b, mesh)
+        b.free()
