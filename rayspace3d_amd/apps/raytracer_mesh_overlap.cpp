@@ -1,7 +1,10 @@
 // raytracer_mesh_overlap -- edge-ray mesh overlap join, two-pass count/scan/write with GPU
 // sort/unique (query/src/applications/rayspace_mesh_overlap_main.cpp:42-146, 275-495).
 // Output order is the reference's: pairs sorted as the little-endian u64 of the struct,
-// i.e. by (object_id_mesh2, object_id_mesh1).
+// i.e. by (object_id_mesh2, object_id_mesh1). --exact replaces the edge rays with the exact
+// triangle-triangle intersection join (rs3d_tri_join), which also reports touching and
+// coplanar contacts and needs no edge payload.
+#include <algorithm>
 #include <chrono>
 #include <fstream>
 #include <iostream>
@@ -15,6 +18,44 @@ struct QueryResults {
     DeviceBuffer merged;
     long long numUnique = 0;
 };
+
+struct OverlapOptions : rs3d_cli::MeshPairOptions {
+    bool exact = false;
+    OverlapOptions() : rs3d_cli::MeshPairOptions("mesh_overlap_timing.json") {}
+    bool parseApplicationOption(const std::string& a, int&, int, char**) override {
+        if (a == "--exact") {
+            exact = true;
+            return true;
+        }
+        return false;
+    }
+};
+
+// Exact join: sorted unique (mesh1, mesh2) keys, as pairs in the output order.
+static QueryResults exactJoin(DeviceContext& ctx, AccelerationStructure& as1, AccelerationStructure& as2,
+                              bool verbose) {
+    uint64_t count = 0;
+    int rc = rs3d_tri_join(ctx.get(), as1.getHandle(), as2.getHandle(), nullptr, 0, &count);
+    if (rc != RS3D_E_CAPACITY) check(rc, "rs3d_tri_join");
+    QueryResults q;
+    if (count) {
+        DeviceBuffer keys(count * sizeof(uint64_t));
+        check(rs3d_tri_join(ctx.get(), as1.getHandle(), as2.getHandle(), keys.as<uint64_t>(), count, &count),
+              "rs3d_tri_join");
+        std::vector<uint64_t> k(count);
+        keys.download(k.data(), count * sizeof(uint64_t));
+        std::vector<MeshQueryResult> pairs = rs3d_cli::keysToPairs(k);
+        std::sort(pairs.begin(), pairs.end(), [](const MeshQueryResult& x, const MeshQueryResult& y) {
+            return x.object_id_mesh2 != y.object_id_mesh2 ? x.object_id_mesh2 < y.object_id_mesh2
+                                                          : x.object_id_mesh1 < y.object_id_mesh1;
+        });
+        q.merged.alloc(count * sizeof(MeshQueryResult));
+        q.merged.upload(pairs.data(), count * sizeof(MeshQueryResult));
+    }
+    q.numUnique = (long long)count;
+    if (verbose) std::cout << "Exact triangle join: Found " << q.numUnique << " unique object pairs." << std::endl;
+    return q;
+}
 
 static long long us_since(std::chrono::high_resolution_clock::time_point t0) {
     return std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::high_resolution_clock::now() - t0).count();
@@ -63,7 +104,7 @@ static QueryResults twoPass(DeviceContext& ctx, MeshOverlapEdgesLauncher& L, Mes
 int main(int argc, char* argv[]) {
     PerformanceTimer timer;
     timer.start("Data Reading");
-    rs3d_cli::MeshPairOptions opt("mesh_overlap_timing.json");
+    OverlapOptions opt;
     opt.allowNoExport = true;
     opt.parse(argc, argv);
     if (opt.helpRequested) {
@@ -71,6 +112,7 @@ int main(int argc, char* argv[]) {
         rs3d_cli::MeshPairOptions::meshPairHelp(o);
         rs3d_cli::MeshPairOptions::runHelp(o);
         o.emplace_back("--no-export", "Do not export results to CSV");
+        o.emplace_back("--exact", "Exact triangle-triangle intersection join instead of edge rays");
         rs3d_cli::printHelp(argv[0], "--mesh1 <path> --mesh2 <path> [options]",
                             "Mesh overlap join with edge rays in both directions.", o);
         return 0;
@@ -84,10 +126,10 @@ int main(int argc, char* argv[]) {
     DeviceContext ctx(0);
     timer.next("Load Mesh1");
     GeometryData m1 = loadGeometryFromFile(opt.mesh1Path);
-    if (m1.vertices.empty() || !requirePrecomputedEdges(m1, opt.mesh1Path, "Mesh1")) return 1;
+    if (m1.vertices.empty() || (!opt.exact && !requirePrecomputedEdges(m1, opt.mesh1Path, "Mesh1"))) return 1;
     timer.next("Load Mesh2");
     GeometryData m2 = loadGeometryFromFile(opt.mesh2Path);
-    if (m2.vertices.empty() || !requirePrecomputedEdges(m2, opt.mesh2Path, "Mesh2")) return 1;
+    if (m2.vertices.empty() || (!opt.exact && !requirePrecomputedEdges(m2, opt.mesh2Path, "Mesh2"))) return 1;
     timer.next("Upload Mesh1");
     GeometryUploader u1;
     u1.upload(m1);
@@ -100,10 +142,13 @@ int main(int argc, char* argv[]) {
     timer.next("Build Mesh2 Index");
     AccelerationStructure as2(ctx, u2);
     as2.build();
-    timer.next("Upload Mesh1");
-    EdgeMeshData e1 = PrecomputedEdgeData::uploadFromGeometry(m1);
-    timer.next("Upload Mesh2");
-    EdgeMeshData e2 = PrecomputedEdgeData::uploadFromGeometry(m2);
+    EdgeMeshData e1, e2;
+    if (!opt.exact) {
+        timer.next("Upload Mesh1");
+        e1 = PrecomputedEdgeData::uploadFromGeometry(m1);
+        timer.next("Upload Mesh2");
+        e2 = PrecomputedEdgeData::uploadFromGeometry(m2);
+    }
     timer.next("Create Launcher");
     MeshOverlapEdgesLauncher L(ctx);
     timer.next("Prepare Kernel Parameters");
@@ -115,11 +160,16 @@ int main(int argc, char* argv[]) {
     p2.target = as1.getHandle();
     p2.swap_pair_order = 1;
     timer.next("Warmup");
-    for (int w = 0; w < opt.warmupRuns; ++w) (void)twoPass(ctx, L, p1, p2, e1.num_edges, e2.num_edges, nullptr, false);
+    for (int w = 0; w < opt.warmupRuns; ++w) {
+        if (opt.exact) (void)exactJoin(ctx, as1, as2, false);
+        else (void)twoPass(ctx, L, p1, p2, e1.num_edges, e2.num_edges, nullptr, false);
+    }
     timer.next("Query");
     QueryResults q;
-    for (int r = 0; r < opt.numberOfRuns; ++r)
-        q = twoPass(ctx, L, p1, p2, e1.num_edges, e2.num_edges, r + 1 == opt.numberOfRuns ? &timer : nullptr, true);
+    for (int r = 0; r < opt.numberOfRuns; ++r) {
+        if (opt.exact) q = exactJoin(ctx, as1, as2, true);
+        else q = twoPass(ctx, L, p1, p2, e1.num_edges, e2.num_edges, r + 1 == opt.numberOfRuns ? &timer : nullptr, true);
+    }
     timer.next("Download Results");
     std::vector<MeshQueryResult> uniq(q.numUnique);
     q.merged.download(uniq.data(), q.numUnique * sizeof(MeshQueryResult));

@@ -175,3 +175,17 @@ def test_preprocess_dataset(datasets):
     run("raytracer", ["--geometry", "a_obj.pre", "--points", "pts.rs3p", "--warmup-runs", "0"], d)
     hdr, rows = read_pairs(d / "ray_results.csv")
     assert set(rows) == exp
+
+
+def test_mesh_overlap_exact(datasets):
+    d, a, b, pts = datasets
+    ref = orc.tri_join(a, b, threads=8)
+    exp = sorted(keys_to_set(ref), key=lambda p: (p[1], p[0]))
+    run("raytracer_mesh_overlap", ["--mesh1", "a.pre", "--mesh2", "b.pre", "--exact", "--warmup-runs", "1"], d)
+    hdr, rows = read_pairs(d / "mesh_overlap_results.csv")
+    assert hdr == ["object_id_mesh1", "object_id_mesh2"] and rows == exp and len(rows) > 10
+    # the exact join needs no edge payload: a v1 file without edges is accepted
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "check_n_geometry.pre")
+    run("raytracer_mesh_overlap", ["--mesh1", golden, "--mesh2", golden, "--exact", "--warmup-runs", "0"], d)
+    hdr, rows = read_pairs(d / "mesh_overlap_results.csv")
+    assert len(rows) >= 540 and all((i, i) in set(rows) for i in range(540))
