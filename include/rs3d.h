@@ -183,6 +183,17 @@ int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
 int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
                     const int64_t* offsets, int swap, rs3d_pair* out);
 
+/* Spatial order of an edge payload (no counterpart in the reference; for callers that keep
+ * edges resident): writes the edge records (start, end, source offset, source count)
+ * permuted along a 30-bit Morton curve of the segment midpoints, stable for equal codes;
+ * src_objects is shared unchanged, so the reordered view = the outputs + edges->src_objects.
+ * Edge rays traced in this order walk neighbouring nodes in neighbouring lanes (the
+ * lexicographic .pre order spreads a wave over a thin slab); every join result is a sorted
+ * set and does not change. Outputs: device arrays of num_edges records, not aliasing the
+ * inputs. Synchronous. */
+int rs3d_edges_spatial_order(rs3d_ctx* ctx, const rs3d_edges* edges, float* starts_out, float* ends_out,
+                             int32_t* src_offsets_out, int32_t* src_counts_out);
+
 /* Open-addressing pair hash table of u64 keys (a << 32 | b), empty = ~0. */
 typedef struct {
     uint64_t* slots;

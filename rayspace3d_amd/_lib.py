@@ -148,6 +148,7 @@ SIGNATURES = {
     "rs3d_extract_edges": (I32, [P, P, U64, P, U64, P, P, P, P, P, P, U64, U64, PU64, PU64]),
     "rs3d_tri_intersect": (I32, [P, P, P, U64, P]),
     "rs3d_bvh_export": (I32, [P, P, P]),
+    "rs3d_edges_spatial_order": (I32, [P, C.POINTER(Edges), P, P, P, P]),
     "rs3d_tri_join_hash": (I32, [P, P, P, I32, C.POINTER(HashTable)]),
     "rs3d_tri_join": (I32, [P, P, P, P, U64, PU64]),
     "rs3d_tri_join_stats": (I32, [P, P, P, C.POINTER(TriJoinStats)]),

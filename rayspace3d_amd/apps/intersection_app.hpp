@@ -117,8 +117,8 @@ inline int runIntersection(int argc, char** argv, bool estimated) {
     AccelerationStructure as2(ctx, u2);
     as2.build();
     timer.next("Prepare Kernel Parameters");
-    EdgeMeshData e1 = PrecomputedEdgeData::uploadFromGeometry(m1);
-    EdgeMeshData e2 = PrecomputedEdgeData::uploadFromGeometry(m2);
+    EdgeMeshData e1 = PrecomputedEdgeData::uploadFromGeometry(m1, ctx);
+    EdgeMeshData e2 = PrecomputedEdgeData::uploadFromGeometry(m2, ctx);
     const rs3d_mesh_view v1 = u1.view(), v2 = u2.view();
     timer.next("Query");
     std::vector<uint64_t> keys;

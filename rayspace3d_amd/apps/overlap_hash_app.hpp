@@ -145,8 +145,8 @@ inline int runOverlapHash(int argc, char** argv, bool direct) {
     AccelerationStructure as2(ctx, u2);
     as2.build();
     timer.next("Prepare Kernel Parameters");
-    EdgeMeshData e1 = PrecomputedEdgeData::uploadFromGeometry(m1);
-    EdgeMeshData e2 = PrecomputedEdgeData::uploadFromGeometry(m2);
+    EdgeMeshData e1 = PrecomputedEdgeData::uploadFromGeometry(m1, ctx);
+    EdgeMeshData e2 = PrecomputedEdgeData::uploadFromGeometry(m2, ctx);
 
     auto query = [&](unsigned long long size, PerformanceTimer* t, std::vector<uint64_t>& keys) {
         while (true) {

@@ -61,8 +61,8 @@ int main(int argc, char* argv[]) {
     AccelerationStructure asb(ctx, ub);
     asb.build();
     timer.next("Prepare Kernel Parameters");
-    EdgeMeshData ea = PrecomputedEdgeData::uploadFromGeometry(ma);
-    EdgeMeshData eb = PrecomputedEdgeData::uploadFromGeometry(mb);
+    EdgeMeshData ea = PrecomputedEdgeData::uploadFromGeometry(ma, ctx);
+    EdgeMeshData eb = PrecomputedEdgeData::uploadFromGeometry(mb, ctx);
     const rs3d_mesh_view va = ua.view(), vb = ub.view();
     std::vector<uint64_t> contain, overlap;
     auto runOnce = [&]() {

@@ -145,9 +145,9 @@ int main(int argc, char* argv[]) {
     EdgeMeshData e1, e2;
     if (!opt.exact) {
         timer.next("Upload Mesh1");
-        e1 = PrecomputedEdgeData::uploadFromGeometry(m1);
+        e1 = PrecomputedEdgeData::uploadFromGeometry(m1, ctx);
         timer.next("Upload Mesh2");
-        e2 = PrecomputedEdgeData::uploadFromGeometry(m2);
+        e2 = PrecomputedEdgeData::uploadFromGeometry(m2, ctx);
     }
     timer.next("Create Launcher");
     MeshOverlapEdgesLauncher L(ctx);

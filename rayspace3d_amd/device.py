@@ -215,6 +215,19 @@ class DevEdges:
         self.s = _lib.Edges(self.t[0].data_ptr(), self.t[1].data_ptr(), self.t[2].data_ptr(),
                             self.t[3].data_ptr(), self.t[4].data_ptr(), edges.num_edges)
 
+    def spatial(self, ctx) -> "DevEdges":
+        """The same edges in Morton order of their midpoints (rs3d_edges_spatial_order):
+        faster edge-ray joins, identical join results. Shares the source list."""
+        out = DevEdges.__new__(DevEdges)
+        out.t = [torch.empty_like(self.t[0]), torch.empty_like(self.t[1]), torch.empty_like(self.t[2]),
+                 torch.empty_like(self.t[3]), self.t[4]]
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_edges_spatial_order(ctx.handle, C.byref(self.s), _ptr(out.t[0]), _ptr(out.t[1]),
+                                               _ptr(out.t[2]), _ptr(out.t[3])), "rs3d_edges_spatial_order")
+        out.s = _lib.Edges(out.t[0].data_ptr(), out.t[1].data_ptr(), out.t[2].data_ptr(), out.t[3].data_ptr(),
+                           out.t[4].data_ptr(), self.s.num_edges)
+        return out
+
 
 def edge_count(ctx, bvh, dedges: DevEdges) -> torch.Tensor:
     counts = torch.empty(max(dedges.s.num_edges, 1), dtype=torch.int32, device="cuda")

@@ -520,6 +520,25 @@ EdgeMeshData PrecomputedEdgeData::uploadFromGeometry(const GeometryData& g) {
     return d;
 }
 
+EdgeMeshData PrecomputedEdgeData::uploadFromGeometry(const GeometryData& g, DeviceContext& ctx) {
+    EdgeMeshData d = uploadFromGeometry(g);
+    const size_t ne = (size_t)d.num_edges;
+    if (ne < 2) return d;
+    DeviceBuffer st(ne * sizeof(Vec3)), en(ne * sizeof(Vec3)), off(ne * sizeof(int)), cnt(ne * sizeof(int));
+    check(rs3d_edges_spatial_order(ctx.get(), &d.view, st.as<float>(), en.as<float>(), off.as<int32_t>(),
+                                   cnt.as<int32_t>()),
+          "rs3d_edges_spatial_order");
+    d.starts = std::move(st);
+    d.ends = std::move(en);
+    d.offsets = std::move(off);
+    d.counts = std::move(cnt);
+    d.view.starts = d.starts.as<float>();
+    d.view.ends = d.ends.as<float>();
+    d.view.src_offsets = d.offsets.as<int32_t>();
+    d.view.src_counts = d.counts.as<int32_t>();
+    return d;
+}
+
 void PrecomputedEdgeData::freeEdgeData(EdgeMeshData& d) {
     d.starts.release();
     d.ends.release();

@@ -287,6 +287,9 @@ struct EdgeMeshData {
 class PrecomputedEdgeData {
 public:
     static EdgeMeshData uploadFromGeometry(const GeometryData& geometry);
+    // Upload, then the edge records in Morton order of their midpoints
+    // (rs3d_edges_spatial_order): faster edge-ray joins, the same join results.
+    static EdgeMeshData uploadFromGeometry(const GeometryData& geometry, DeviceContext& ctx);
     static void freeEdgeData(EdgeMeshData& data);
 };
 

@@ -161,6 +161,18 @@ def test_edge_counts_and_overlap_join(ctx, leaf):
     for mode in (0, 1):
         got = overlap_join(ctx, ba, ea, bb, eb, mode=mode)
         assert np.array_equal(got, ref), f"mode {mode}: {len(got)} vs {len(ref)} pairs"
+    # the same edges in spatial order: a permutation of the records, the same join
+    sa, sb = ea.spatial(ctx), eb.spatial(ctx)
+    rec = lambda d: np.concatenate([to_host(d.t[0]).view(np.uint32), to_host(d.t[1]).view(np.uint32),
+                                    to_host(d.t[2])[:, None].view(np.uint32), to_host(d.t[3])[:, None].view(np.uint32)], 1)
+    for d, s_ in ((ea, sa), (eb, sb)):
+        r0, r1 = rec(d), rec(s_)
+        assert not np.array_equal(r0, r1)
+        assert np.array_equal(r0[np.lexsort(r0.T[::-1])], r1[np.lexsort(r1.T[::-1])])
+    s_counts = to_host(edge_count(ctx, bb, sa))
+    assert np.array_equal(np.sort(s_counts), np.sort(counts))
+    for mode in (0, 1):
+        assert np.array_equal(overlap_join(ctx, ba, sa, bb, sb, mode=mode), ref)
     ba.free()
     bb.free()
 
@@ -248,6 +260,8 @@ def test_intersection_join(ctx):
     assert len(ref) > len(ov) >= 5  # the fallback adds the nested pairs
     ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
     got, over = intersection_join(ctx, ba, DevEdges(a.edges), bb, DevEdges(b.edges))
+    got_s, over_s = intersection_join(ctx, ba, DevEdges(a.edges).spatial(ctx), bb, DevEdges(b.edges).spatial(ctx))
+    assert np.array_equal(got_s, got) and over_s == over
     assert over == 0
     assert np.array_equal(got, ref), f"{len(got)} vs {len(ref)}"
     ba.free()
@@ -260,6 +274,8 @@ def test_containment_join(ctx):
     assert len(ct_ref) >= 50 and len(ov_ref) > 5
     ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
     ct, ov, over = containment_join(ctx, ba, DevEdges(a.edges), bb, DevEdges(b.edges))
+    ct_s, ov_s, over_s = containment_join(ctx, ba, DevEdges(a.edges).spatial(ctx), bb, DevEdges(b.edges).spatial(ctx))
+    assert np.array_equal(ct_s, ct) and np.array_equal(ov_s, ov) and over_s == over
     assert over == 0
     assert np.array_equal(ov, ov_ref)
     assert np.array_equal(ct, ct_ref)
