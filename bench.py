@@ -534,10 +534,13 @@ def run_overlap(args):
     dA = [to_dev(A.vertices), to_dev(A.indices), to_dev(A.tri_to_obj)]
     dB = [to_dev(B.vertices), to_dev(B.indices), to_dev(B.tri_to_obj)]
     # resident edges in Morton order of their midpoints (one-time, like the BVH builds)
+    raw_a, raw_b = DevEdges(ea), DevEdges(eb)
+    torch.cuda.synchronize()
     te = time.perf_counter()
-    dea, deb = DevEdges(ea).spatial(ctx), DevEdges(eb).spatial(ctx)
+    dea, deb = raw_a.spatial(ctx), raw_b.spatial(ctx)
     torch.cuda.synchronize()
     edge_order_ms = (time.perf_counter() - te) * 1e3
+    del raw_a, raw_b
     builds = []
     ba = bb = None
     for _ in range(2):
