@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--rays-per-wave", type=int, default=64,
                     help="lane-refill run length of the traversal (0: one ray per lane)")
     ap.add_argument("--leaf-size", type=int, default=1, help="BVH leaf size")
+    ap.add_argument("--ray-sort-min", type=int, default=None,
+                    help="containment: reorder launches of at least this many rays (0: never)")
+    ap.add_argument("--axis-grid", type=int, default=1, choices=[0, 1],
+                    help="containment: let point launches use the axis grid (1) or the BVH walk (0)")
     ap.add_argument("--autotune", action="store_true",
                     help="containment: pick leaf size, reordering and run length with the "
                          "autotuner on the workload's own rays first (the defaults are its "
@@ -80,8 +84,8 @@ def main():
     import torch
 
     from rayspace3d_amd import _lib, gen
-    from rayspace3d_amd.device import (Bvh, Context, FilterRefinePipeline, to_dev, to_host,
-                                       trace_stats)
+    from rayspace3d_amd.device import (Bvh, Context, FilterRefinePipeline, axis_grid_prepare,
+                                       axis_grid_stats, to_dev, to_host, trace_stats)
 
     t0 = time.time()
     mesh, pts = gen.many_spheres_workload(seed=args.seed, n_objects=args.objects,
@@ -105,6 +109,8 @@ def main():
     else:
         ctx.set_option(_lib.OPT_RAYS_PER_WAVE, args.rays_per_wave)
         leaf = args.leaf_size
+    if args.ray_sort_min is not None:
+        ctx.set_option(_lib.OPT_RAY_SORT_MIN, args.ray_sort_min)
     # BVH build (outside the timed query, like the reference's "Build Index" phase)
     builds = []
     bvh = None
@@ -118,6 +124,13 @@ def main():
     info = bvh.info()
     print(f"[bench] BVH: {info.num_nodes} nodes, depth {info.max_depth}, "
           f"build {min(builds):.2f} ms", file=sys.stderr, flush=True)
+    # axis grid for +z rays (built once, outside the timed query like the BVH)
+    ctx.set_option(_lib.OPT_AXIS_GRID, args.axis_grid)
+    grid = None
+    if args.axis_grid:
+        grid = axis_grid_prepare(ctx, bvh, 2)
+        print(f"[bench] axis grid: {grid}", file=sys.stderr, flush=True)
+    use_grid = bool(grid and grid["state"] == 1)
 
     pipe = FilterRefinePipeline(ctx)
     out = torch.empty(len(pts), dtype=torch.int32, device="cuda")
@@ -166,13 +179,26 @@ def main():
     st = trace_stats(ctx, canon, to_dev(pts[cand]), axis=2)
     canon.free()
     ncand = int(res.candidate_count)
-    alg_bytes = 16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"]
-    bytes_per_ray = alg_bytes / max(st["rays"], 1)
+    if use_grid:
+        # the grid kernel's own bytes: 12 B origin + 4 B id + 8 B cell range per lookup +
+        # 64 B per record read (counted on the refine candidates)
+        gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
+        alg_bytes = 16 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
+        bytes_per_ray = alg_bytes / max(gst["rays"], 1)
+        kernel_name = "k_points_grid (refine, axis grid)"
+    else:
+        gst = None
+        alg_bytes = 16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"]
+        bytes_per_ray = alg_bytes / max(st["rays"], 1)
+        kernel_name = ("k_points_refill" if ctx.get_option(_lib.OPT_RAYS_PER_WAVE) else "k_points") + \
+            " (refine traversal)"
     achieved_gbs = alg_bytes / (kern_avg_ms / 1e3) / 1e9 if kern_avg_ms > 0 else 0.0
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tf = json.load(f)
+        if kernel_name.startswith(tf.get("kernel", "")):
+            traffic = tf.get("hbm_bytes_per_launch")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -227,8 +253,7 @@ def main():
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved_gbs / 8000.0, 4), "traffic": traffic,
-                     "kernel": ("k_points_refill" if ctx.get_option(_lib.OPT_RAYS_PER_WAVE)
-                                else "k_points") + " (refine traversal)",
+                     "kernel": kernel_name,
                      "kernel_avg_ms": round(kern_avg_ms, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "alg_bytes_per_ray": round(bytes_per_ray, 1)},
@@ -244,6 +269,10 @@ def main():
                               "rays_per_wave": ctx.get_option(_lib.OPT_RAYS_PER_WAVE),
                               "autotuned": tuned is not None},
         "trace_stats": st,
+        "axis_grid": grid,
+        "axis_grid_stats": gst,
+        "bvh_canonical_bytes_per_ray": round((16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"])
+                                             / max(st["rays"], 1), 1),
     }
     print(json.dumps(line), flush=True)
     bvh.free()

@@ -57,8 +57,14 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * ray still writes its own slot). 0 disables; default 65536.
  * RS3D_OPT_RAYS_PER_WAVE: point launches give each 64-lane wave a run of this many
  * consecutive rays and refill a lane from the run as soon as its ray finishes; 0 uses one
- * ray per lane. Results are unchanged. Default 64; 0 or 64..2^20. */
-enum { RS3D_OPT_RAY_SORT_MIN = 1, RS3D_OPT_RAYS_PER_WAVE = 2 };
+ * ray per lane. Results are unchanged. Default 64; 0 or 64..2^20.
+ * RS3D_OPT_AXIS_GRID: 1 (default) lets point launches use an axis grid of the BVH's
+ * triangles (see rs3d_axis_grid_prepare) where one is accepted for the ray axis; 0 keeps
+ * them on the BVH. Results are unchanged either way. Grid launches take the rays in the
+ * caller's order (no reordering) with RS3D_OPT_GRID_RAYS_PER_WAVE rays per wave run
+ * (default 1024; 64..2^20). */
+enum { RS3D_OPT_RAY_SORT_MIN = 1, RS3D_OPT_RAYS_PER_WAVE = 2, RS3D_OPT_AXIS_GRID = 3,
+       RS3D_OPT_GRID_RAYS_PER_WAVE = 4 };
 int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value);
 int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value);
 
@@ -127,6 +133,30 @@ typedef struct {
 } rs3d_trace_stats;
 int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
                             int axis, rs3d_trace_stats* stats /* host */);
+
+/* Axis grid (no counterpart in the reference, whose rays run on RT cores): for rays along
+ * +axis, a uniform 2-D grid over the plane perpendicular to the axis lists, per cell, the
+ * triangles whose padded box (the BVH's leaf box) overlaps the cell, ordered by the box's
+ * upper end along the axis. A ray reads its cell and tests the listed triangles whose box
+ * contains its line and reaches its origin -- the same candidates as the BVH walk, so the
+ * same hits and results -- in one lookup instead of a walk down the tree. The cell edge is
+ * the triangles' mean extent in the plane. A grid is built once per (BVH, axis), lazily by
+ * the first point launch along that axis or here, and rejected (the BVH is used) when its
+ * records would exceed 6 per triangle + 2^20, or the lists average more than 64 records
+ * per cell, or more than 256 per record (dense spots). Freed with the BVH. */
+typedef struct {
+    int32_t state;            /* 0 not built yet, 1 built, 2 rejected */
+    int32_t cells_u, cells_w; /* cells along the two perpendicular axes */
+    uint64_t entries;         /* (triangle, cell) records */
+    double expected_list;     /* records per cell: the mean list a uniformly placed ray reads */
+    uint64_t device_bytes;
+    double build_ms;          /* device time of the build (0 when not built here) */
+} rs3d_axis_grid_info;
+int rs3d_axis_grid_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info /* host, may be null */);
+/* Work counters of a point launch on the axis grid (nodes_visited = cell lookups,
+ * tris_tested = 64-byte records read); RS3D_E_ARG when no grid is accepted for the axis. */
+int rs3d_axis_grid_stats(rs3d_ctx* ctx, rs3d_bvh* bvh, const float* points, uint64_t n, int axis,
+                         rs3d_trace_stats* stats);
 /* ---- filter-refine (replaces FilterRefinePipeline::execute,
  *      query/src/raytracing/FilterRefine.cpp:81-122) --------------------------------------
  * Filter: +axis ray parity against the 12-triangle box mesh of the dataset AABB

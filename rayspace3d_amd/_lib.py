@@ -23,6 +23,8 @@ OUT_FLAG_U8 = 1
 OUT_OBJ_I32 = 2
 OPT_RAY_SORT_MIN = 1
 OPT_RAYS_PER_WAVE = 2
+OPT_AXIS_GRID = 3
+OPT_GRID_RAYS_PER_WAVE = 4
 REQ_CONTAIN = 0
 REQ_OVERLAP = 1
 REQ_INTERSECT = 2
@@ -42,6 +44,11 @@ class BvhInfo(C.Structure):
         ("bounds_max", C.c_float * 3),
         ("device_bytes", C.c_uint64),
     ]
+
+
+class AxisGridInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("cells_u", C.c_int32), ("cells_w", C.c_int32), ("entries", C.c_uint64),
+                ("expected_list", C.c_double), ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
 
 
 class TraceStats(C.Structure):
@@ -144,6 +151,8 @@ SIGNATURES = {
     "rs3d_bvh_info_get": (I32, [P, C.POINTER(BvhInfo)]),
     "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
+    "rs3d_axis_grid_prepare": (I32, [P, P, I32, C.POINTER(AxisGridInfo)]),
+    "rs3d_axis_grid_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),
     "rs3d_extract_edges": (I32, [P, P, U64, P, U64, P, P, P, P, P, P, U64, U64, PU64, PU64]),
     "rs3d_tri_intersect": (I32, [P, P, P, U64, P]),

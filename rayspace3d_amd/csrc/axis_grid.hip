@@ -1,0 +1,535 @@
+// axis_grid.hip -- 2-D grid of padded triangle boxes for axis-aligned point rays.
+//
+// Point containment casts rays along one axis, so every ray is a vertical half-line over a
+// point of the perpendicular plane. The grid covers that plane (the BVH's padded bounds)
+// with square cells; cell c lists every triangle whose padded box -- the BVH's leaf box,
+// computed by the same prim_padded_box -- overlaps c in the plane, as a self-contained
+// 64-byte record (vertices, original index, object, the box's upper end along the axis and
+// its extent in the plane), ordered by that upper end, largest first. A ray reads its
+// cell's range, stops at the first record whose box ends below its origin, and tests the
+// records whose box contains its line. Those are exactly the triangles whose leaf box the
+// BVH walk would accept (ancestor boxes contain the leaf boxes), the padding guarantees
+// that no other triangle can be hit, and the hit buffer, restarts and chain are the BVH
+// kernel's: the results are the same. The gain is the walk: one range lookup and a few
+// records instead of ~30 dependent node fetches.
+//
+// Build (per BVH and axis, once): the cell edge is the triangles' mean padded extent in the
+// plane; records per triangle from the cell span of its box, budget and list-length checks
+// (rejected grids leave the launches on the BVH), one
+// (cell << 32 | descending box top) key per record, a stable radix sort, per-cell counts
+// and their scan as the cell ranges, and a gather of the records.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "rs3d_internal.h"
+#include "traverse.h"
+
+namespace rs3d {
+
+struct AxisGrid {
+    int axis = 2, nu = 1, nw = 1;
+    float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
+    uint64_t entries = 0;
+    uint32_t* start = nullptr;  // nu * nw + 1 record offsets
+    float4* rec = nullptr;      // 4 float4 per record
+    size_t bytes = 0;
+    double expected = 0.0;
+    double build_ms = 0.0;
+};
+
+void axis_grid_free(AxisGrid* g) {
+    if (!g) return;
+    if (g->start) (void)hipFree(g->start);
+    if (g->rec) (void)hipFree(g->rec);
+    delete g;
+}
+
+struct AGArgs {
+    const uint32_t* start;
+    const float4* rec;
+    float lo_u, lo_w, inv_u, inv_w;
+    int nu, nw;
+};
+
+constexpr uint64_t AG_MAX_CELLS = 1ull << 26;
+constexpr int AG_MAX_DIM = 32768;
+constexpr double AG_MAX_UNIFORM_LIST = 64.0;    // records per cell, averaged over cells
+constexpr double AG_MAX_WEIGHTED_LIST = 256.0;  // averaged over records (dense spots)
+
+// Cell coordinate along one axis: monotone in x (clamped), NaN goes to cell 0.
+__device__ __forceinline__ int ag_cell(float x, float lo, float inv, int n) {
+    const float f = fminf(fmaxf((x - lo) * inv, 0.0f), (float)(n - 1));
+    return (int)f;
+}
+
+// Unsigned key whose order is the float order.
+__device__ __forceinline__ uint32_t ag_ord(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u);
+}
+
+template <int AX>
+__device__ __forceinline__ void ag_cells(const TriRec& r, const AGArgs& G, float (&plo)[3], float (&phi)[3],
+                                         int& cu0, int& cu1, int& cw0, int& cw1) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                    make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
+    cu0 = ag_cell(plo[U], G.lo_u, G.inv_u, G.nu);
+    cu1 = ag_cell(phi[U], G.lo_u, G.inv_u, G.nu);
+    cw0 = ag_cell(plo[W], G.lo_w, G.inv_w, G.nw);
+    cw1 = ag_cell(phi[W], G.lo_w, G.inv_w, G.nw);
+}
+
+// Sum of the triangles' padded extents in the plane (the larger of the two), for the cell size.
+template <int AX>
+__global__ __launch_bounds__(256) void k_ag_size(const TriRec* __restrict__ tris, uint64_t n, double* __restrict__ sum) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    double v = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const TriRec r = tris[i];
+        float plo[3], phi[3];
+        prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                        make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
+        const float e = fmaxf(phi[U] - plo[U], phi[W] - plo[W]);
+        if (e < 3.0e38f) v += (double)e;  // infinite or NaN extents do not set the cell size
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sum, v);
+}
+
+template <int AX>
+__global__ __launch_bounds__(256) void k_ag_span(const TriRec* __restrict__ tris, uint64_t n, AGArgs G,
+                                                 uint32_t* __restrict__ span, unsigned long long* __restrict__ total) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (i < n) {
+        float plo[3], phi[3];
+        int cu0, cu1, cw0, cw1;
+        ag_cells<AX>(tris[i], G, plo, phi, cu0, cu1, cw0, cw1);
+        v = (unsigned long long)(cu1 - cu0 + 1) * (unsigned long long)(cw1 - cw0 + 1);
+        span[i] = v > 0xffffffffull ? 0xffffffffu : (uint32_t)v;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(total, v);
+}
+
+template <int AX>
+__global__ __launch_bounds__(256) void k_ag_emit(const TriRec* __restrict__ tris, uint64_t n, AGArgs G,
+                                                 const uint32_t* __restrict__ off, uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float plo[3], phi[3];
+    int cu0, cu1, cw0, cw1;
+    ag_cells<AX>(tris[i], G, plo, phi, cu0, cu1, cw0, cw1);
+    const uint64_t low = 0xffffffffu - ag_ord(phi[AX]);  // box top, descending
+    uint64_t k = off[i];
+    for (int cw = cw0; cw <= cw1; ++cw)
+        for (int cu = cu0; cu <= cu1; ++cu) {
+            const uint64_t cell = (uint64_t)cw * G.nu + cu;
+            keys[k] = (cell << 32) | low;
+            vals[k] = (uint32_t)i;
+            ++k;
+            atomicAdd(&counts[cell], 1u);
+        }
+}
+
+// Sum of count^2 over the cells (the expected list length times the record count).
+__global__ __launch_bounds__(256) void k_ag_sq(const uint32_t* __restrict__ counts, uint64_t nc,
+                                               unsigned long long* __restrict__ acc) {
+    unsigned long long s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long c = counts[i];
+        s += c * c;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, s);
+}
+
+template <int AX>
+__global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tris, const uint32_t* __restrict__ vals,
+                                                   uint64_t m, AGArgs G, float4* __restrict__ rec) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const TriRec r = tris[vals[i]];
+    float plo[3], phi[3];
+    prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                    make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
+    rec[4 * i] = r.v0;      // .w: original triangle index
+    rec[4 * i + 1] = r.v1;  // .w: object id
+    rec[4 * i + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
+    rec[4 * i + 3] = make_float4(plo[U], phi[U], plo[W], phi[W]);
+}
+
+__global__ void k_ag_last(uint32_t* __restrict__ start, uint64_t nc, uint32_t total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) start[nc] = total;
+}
+
+// Point rays on the grid, lane-refill form: wave b owns rays [b * per_wave, ...).
+// Counters (STATS): cell lookups, records read, hits, restarts.
+template <int AX, bool STATS>
+__global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float* __restrict__ pts, uint64_t n,
+                                                          const uint32_t* __restrict__ imap, int out_kind,
+                                                          void* __restrict__ out, uint32_t per_wave,
+                                                          unsigned long long* __restrict__ stats) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float eps = 1e-6f, tmax = 1e16f;
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
+    const uint64_t run_hi = lo + per_wave < n ? lo + per_wave : n;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint64_t next = lo;
+    bool active = false;
+    uint32_t pi = 0, j = 0, je = 0, js = 0;
+    float3 o = make_float3(0.0f, 0.0f, 0.0f);
+    float thr = eps;
+    int count = 0, first_obj = -1;
+    uint32_t total = 0;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    kbuf_clear(bt, bi, bo);
+    TraceCounters ctr;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (idle && next < run_hi) {
+            const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
+            if (!active && r < run_hi) {
+                pi = imap ? imap[r] : (uint32_t)r;
+                o = ld3f(pts, pi);
+                const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
+                const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
+                const uint64_t c = (uint64_t)cw * G.nu + cu;
+                js = G.start[c];
+                je = G.start[c + 1];
+                j = js;
+                if (STATS) ctr.nodes++;
+                thr = eps;
+                count = 0;
+                first_obj = -1;
+                total = 0;
+                kbuf_clear(bt, bi, bo);
+                active = true;
+            }
+            next += (uint64_t)__popcll(idle);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+        if (j < je) {
+            const float4 e0 = G.rec[4 * (uint64_t)j], e1 = G.rec[4 * (uint64_t)j + 1];
+            const float4 e2 = G.rec[4 * (uint64_t)j + 2], e3 = G.rec[4 * (uint64_t)j + 3];
+            if (STATS) ctr.tris++;
+            if (e2.w < oa) {  // records are ordered by box top: the rest end below the origin
+                j = je;
+                continue;
+            }
+            ++j;
+            if (e2.w >= oa && ou >= e3.x && ou <= e3.y && ow >= e3.z && ow <= e3.w) {
+                float t;
+                const uint32_t orig = (uint32_t)__float_as_int(e0.w);
+                if (tri_hit(o, rp, make_float3(e0.x, e0.y, e0.z), make_float3(e1.x, e1.y, e1.z),
+                            make_float3(e2.x, e2.y, e2.z), t) &&
+                    t >= thr && t < tmax) {
+                    ++total;
+                    if (STATS) ctr.hits++;
+                    if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                        kbuf_insert(bt, bi, bo, t, orig, __float_as_int(e1.w));
+                }
+            }
+            continue;
+        }
+        // list complete: walk the chain inside the buffer (as the BVH kernels do)
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                if (count == 0) first_obj = bo[k];
+                ++count;
+                float nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                thr = nthr;
+            }
+        }
+        if (total > (uint32_t)KBUF && count < 10000) {
+            kbuf_clear(bt, bi, bo);
+            total = 0;
+            j = js;
+            if (STATS) ctr.restarts++;
+            continue;
+        }
+        const int res = (count & 1) ? first_obj : -1;
+        if (out_kind == RS3D_OUT_RAY_RESULT) {
+            rs3d_ray_result rr;
+            rr.ray_id = (int32_t)pi;
+            rr.polygon_index = res;
+            static_cast<rs3d_ray_result*>(out)[pi] = rr;
+        } else if (out_kind == RS3D_OUT_FLAG_U8) {
+            static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
+        } else {
+            static_cast<int32_t*>(out)[pi] = res;
+        }
+        active = false;
+    }
+    if (STATS) flush_counters(ctr, stats);
+}
+
+namespace {
+
+struct AgTmp {
+    std::vector<void*> ptrs;
+    ~AgTmp() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(T** out, size_t count) {
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+        if (e != hipSuccess) {
+            set_error("axis grid: hipMalloc(%zu) failed: %s", count * sizeof(T), hipGetErrorString(e));
+            return RS3D_E_NOMEM;
+        }
+        ptrs.push_back(p);
+        *out = static_cast<T*>(p);
+        return RS3D_OK;
+    }
+};
+
+template <int AX>
+int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    *out = nullptr;
+    const uint64_t n = bvh->num_tris;
+    hipStream_t s = ctx->stream;
+    AGArgs G{};
+    const double lo_u = bvh->bmin[U], lo_w = bvh->bmin[W];
+    const double eu = (double)bvh->bmax[U] - lo_u, ew = (double)bvh->bmax[W] - lo_w;
+    // Cell edge: the mean padded extent of the triangles (each then spans ~4 cells), but no
+    // more than 4 cells per triangle, 2^26 cells or 32768 cells along an axis.
+    RS3D_TRY(ctx->small.ensure(256));
+    double* dsum = reinterpret_cast<double*>(ctx->small.as<unsigned long long>() + 28);
+    RS3D_HIP(hipMemsetAsync(dsum, 0, sizeof(double), s));
+    {
+        unsigned nbs;
+        RS3D_TRY(grid_blocks(n, 256, &nbs));
+        k_ag_size<AX><<<nbs < 1024 ? nbs : 1024, 256, 0, s>>>(bvh->tris, n, dsum);
+        RS3D_HIP(hipGetLastError());
+    }
+    double size_sum = 0.0;
+    RS3D_HIP(hipMemcpyAsync(&size_sum, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+    RS3D_HIP(hipStreamSynchronize(s));
+    const double max_cells = std::min<double>(4.0 * (double)n + 1024.0, (double)AG_MAX_CELLS);
+    double h = size_sum / (double)n;
+    if (eu > 0 && ew > 0) h = std::max(h, std::sqrt(eu * ew / max_cells));
+    h = std::max({h, eu / AG_MAX_DIM, ew / AG_MAX_DIM});
+    int nu = 1, nw = 1;
+    if (h > 0 && std::isfinite(h)) {
+        nu = (int)std::min<double>(std::max<double>(std::ceil(eu / h), 1.0), AG_MAX_DIM);
+        nw = (int)std::min<double>(std::max<double>(std::ceil(ew / h), 1.0), AG_MAX_DIM);
+    }
+    G.lo_u = (float)lo_u;
+    G.lo_w = (float)lo_w;
+    G.inv_u = eu > 0 ? (float)(nu / eu) : 0.0f;
+    G.inv_w = ew > 0 ? (float)(nw / ew) : 0.0f;
+    G.nu = nu;
+    G.nw = nw;
+    const uint64_t nc = (uint64_t)nu * nw;
+    AgTmp tmp;
+    uint32_t *span, *off;
+    RS3D_TRY(tmp.get(&span, n));
+    RS3D_TRY(tmp.get(&off, n));
+    unsigned long long* acc = ctx->small.as<unsigned long long>() + 24;
+    RS3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), s));
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
+    RS3D_HIP(hipGetLastError());
+    uint64_t total = 0;
+    RS3D_TRY(read_u64(ctx, acc, &total));
+    const uint64_t budget = 6 * n + (1ull << 20);
+    if (total > budget || total > 0xffffffffull) return RS3D_OK;  // rejected: too many records
+    if ((double)total / (double)nc > AG_MAX_UNIFORM_LIST) return RS3D_OK;  // rejected: lists too long
+    RS3D_TRY(scan_u32_exclusive(ctx, span, off, n));
+    uint64_t* keys;
+    uint32_t *vals, *counts;
+    RS3D_TRY(tmp.get(&keys, total));
+    RS3D_TRY(tmp.get(&vals, total));
+    RS3D_TRY(tmp.get(&counts, nc));
+    RS3D_HIP(hipMemsetAsync(counts, 0, nc * sizeof(uint32_t), s));
+    k_ag_emit<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, off, keys, vals, counts);
+    unsigned nbc;
+    RS3D_TRY(grid_blocks(nc, 256, &nbc));
+    k_ag_sq<<<nbc < 1024 ? nbc : 1024, 256, 0, s>>>(counts, nc, acc + 1);
+    RS3D_HIP(hipGetLastError());
+    uint64_t sq = 0;
+    RS3D_TRY(read_u64(ctx, acc + 1, &sq));
+    const double weighted = total ? (double)sq / (double)total : 0.0;
+    if (weighted > AG_MAX_WEIGHTED_LIST) return RS3D_OK;  // rejected: dense spots too long
+    const double expected = (double)total / (double)nc;
+    AxisGrid* g = new AxisGrid();
+    g->axis = AX;
+    g->nu = nu;
+    g->nw = nw;
+    g->lo_u = G.lo_u;
+    g->lo_w = G.lo_w;
+    g->inv_u = G.inv_u;
+    g->inv_w = G.inv_w;
+    g->entries = total;
+    g->expected = expected;
+    hipError_t e;
+    if ((e = hipMalloc(&g->start, (nc + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->rec, std::max<uint64_t>(total, 1) * 4 * sizeof(float4))) != hipSuccess) {
+        set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
+        axis_grid_free(g);
+        return RS3D_E_NOMEM;
+    }
+    g->bytes = (nc + 1) * sizeof(uint32_t) + total * 4 * sizeof(float4);
+    const int rc = [&]() -> int {
+        RS3D_TRY(scan_u32_exclusive(ctx, counts, g->start, nc));
+        k_ag_last<<<1, 64, 0, s>>>(g->start, nc, (uint32_t)total);
+        if (total) {
+            RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
+            unsigned nbe;
+            RS3D_TRY(grid_blocks(total, 256, &nbe));
+            k_ag_gather<AX><<<nbe, 256, 0, s>>>(bvh->tris, vals, total, G, g->rec);
+        }
+        RS3D_HIP(hipGetLastError());
+        RS3D_HIP(hipStreamSynchronize(s));
+        return RS3D_OK;
+    }();
+    if (rc != RS3D_OK) {
+        axis_grid_free(g);
+        return rc;
+    }
+    *out = g;
+    return RS3D_OK;
+}
+
+int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
+    if (bvh->agrid_state[axis] != 0) return RS3D_OK;
+    if (bvh->num_tris == 0 || bvh->num_tris > 0x7fffffffull) {
+        bvh->agrid_state[axis] = 2;
+        return RS3D_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    AxisGrid* g = nullptr;
+    switch (axis) {
+        case 0: RS3D_TRY(build_ax<0>(ctx, bvh, &g)); break;
+        case 1: RS3D_TRY(build_ax<1>(ctx, bvh, &g)); break;
+        default: RS3D_TRY(build_ax<2>(ctx, bvh, &g)); break;
+    }
+    if (g) g->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    bvh->agrid[axis] = g;
+    bvh->agrid_state[axis] = g ? 1 : 2;
+    return RS3D_OK;
+}
+
+template <int AX, bool STATS>
+int launch_grid_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
+                   int out_kind, void* out, unsigned long long* stats) {
+    const AGArgs G{g->start, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, pw, &nb));
+    k_points_grid<AX, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(G, pts, n, imap, out_kind, out, pw, stats);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+template <bool STATS>
+int launch_grid(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
+                int out_kind, void* out, unsigned long long* stats) {
+    switch (g->axis) {
+        case 0: return launch_grid_ax<0, STATS>(ctx, g, pts, n, imap, out_kind, out, stats);
+        case 1: return launch_grid_ax<1, STATS>(ctx, g, pts, n, imap, out_kind, out, stats);
+        default: return launch_grid_ax<2, STATS>(ctx, g, pts, n, imap, out_kind, out, stats);
+    }
+}
+
+}  // namespace
+
+int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready) {
+    *ready = false;
+    if (!ctx->opt_axis_grid || axis < 0 || axis > 2) return RS3D_OK;
+    RS3D_TRY(ensure_grid(ctx, bvh, axis));
+    *ready = bvh->agrid[axis] != nullptr;
+    return RS3D_OK;
+}
+
+int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,
+                     int out_kind, void* out) {
+    const AxisGrid* g = bvh->agrid[axis];
+    if (!g || n > 0xffffffffull) {
+        set_error("axis grid launch without an accepted grid");
+        return RS3D_E_ARG;
+    }
+    return launch_grid<false>(ctx, g, pts, n, imap, out_kind, out, nullptr);
+}
+
+}  // namespace rs3d
+
+using namespace rs3d;
+
+extern "C" {
+
+int rs3d_axis_grid_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info) {
+    if (!ctx || !bvh || axis < 0 || axis > 2) {
+        set_error("rs3d_axis_grid_prepare: bad arguments");
+        return RS3D_E_ARG;
+    }
+    RS3D_HIP(hipSetDevice(ctx->device));
+    RS3D_TRY(ensure_grid(ctx, bvh, axis));
+    if (info) {
+        memset(info, 0, sizeof *info);
+        info->state = bvh->agrid_state[axis];
+        const AxisGrid* g = bvh->agrid[axis];
+        if (g) {
+            info->cells_u = g->nu;
+            info->cells_w = g->nw;
+            info->entries = g->entries;
+            info->expected_list = g->expected;
+            info->device_bytes = g->bytes;
+            info->build_ms = g->build_ms;
+        }
+    }
+    return RS3D_OK;
+}
+
+int rs3d_axis_grid_stats(rs3d_ctx* ctx, rs3d_bvh* bvh, const float* points, uint64_t n, int axis,
+                         rs3d_trace_stats* stats) {
+    if (!ctx || !bvh || !stats || (n && !points) || axis < 0 || axis > 2 || n > 0xffffffffull) {
+        set_error("rs3d_axis_grid_stats: bad arguments");
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(ensure_grid(ctx, bvh, axis));
+    const AxisGrid* g = bvh->agrid[axis];
+    if (!g) {
+        set_error("rs3d_axis_grid_stats: no axis grid accepted for axis %d", axis);
+        return RS3D_E_ARG;
+    }
+    memset(stats, 0, sizeof *stats);
+    stats->rays = n;
+    if (n == 0) return RS3D_OK;
+    RS3D_TRY(ctx->small.ensure(256));
+    RS3D_TRY(ctx->tmp_a.ensure(n));
+    unsigned long long* dstats = ctx->small.as<unsigned long long>() + 16;
+    RS3D_HIP(hipMemsetAsync(dstats, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    RS3D_TRY(launch_grid<true>(ctx, g, points, n, nullptr, RS3D_OUT_FLAG_U8, ctx->tmp_a.ptr, dstats));
+    unsigned long long h[4];
+    RS3D_HIP(hipMemcpyAsync(h, dstats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    stats->nodes_visited = h[0];
+    stats->tris_tested = h[1];
+    stats->hits = h[2];
+    stats->restarts = h[3];
+    return RS3D_OK;
+}
+
+}  // extern "C"

@@ -96,18 +96,10 @@ __global__ __launch_bounds__(BVH_BLOCK) void k_prim_bounds(const float* __restri
                              fminf(fminf(p0.z, p1.z), p2.z)};
         const float hi[3] = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y),
                              fmaxf(fmaxf(p0.z, p1.z), p2.z)};
-        float ext = 0.0f, mag = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            ext = fmaxf(ext, hi[k] - lo[k]);
-            mag = fmaxf(mag, fmaxf(fabsf(lo[k]), fabsf(hi[k])));
-        }
-        const float pad = box_pad(mag, ext);
         float plo[3], phi[3];
+        prim_padded_box(p0, p1, p2, plo, phi);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            plo[k] = lo[k] - pad;
-            phi[k] = hi[k] + pad;
             const float cc = 0.5f * lo[k] + 0.5f * hi[k];
             acc[k] = fminf(acc[k], cc);
             acc[3 + k] = fmaxf(acc[3 + k], cc);
@@ -628,6 +620,7 @@ int rs3d_bvh_free(rs3d_bvh* bvh) {
     if (bvh->ctx) (void)hipSetDevice(bvh->ctx->device);
     if (bvh->nodes) (void)hipFree(bvh->nodes);
     if (bvh->tris) (void)hipFree(bvh->tris);
+    for (int a = 0; a < 3; ++a) axis_grid_free(bvh->agrid[a]);
     delete bvh;
     return RS3D_OK;
 }

@@ -737,6 +737,22 @@ int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value) {
         ctx->opt_rays_per_wave = value;
         return RS3D_OK;
     }
+    if (option == RS3D_OPT_GRID_RAYS_PER_WAVE) {
+        if (value < 64 || value > (1 << 20)) {
+            set_error("RS3D_OPT_GRID_RAYS_PER_WAVE must be in [64, 2^20], got %lld", (long long)value);
+            return RS3D_E_ARG;
+        }
+        ctx->opt_grid_rays_per_wave = value;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_AXIS_GRID) {
+        if (value != 0 && value != 1) {
+            set_error("RS3D_OPT_AXIS_GRID must be 0 or 1, got %lld", (long long)value);
+            return RS3D_E_ARG;
+        }
+        ctx->opt_axis_grid = value;
+        return RS3D_OK;
+    }
     set_error("rs3d_ctx_set_option: unknown option %d", option);
     return RS3D_E_ARG;
 }
@@ -752,6 +768,14 @@ int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value) {
     }
     if (option == RS3D_OPT_RAYS_PER_WAVE) {
         *value = ctx->opt_rays_per_wave;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_AXIS_GRID) {
+        *value = ctx->opt_axis_grid;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_GRID_RAYS_PER_WAVE) {
+        *value = ctx->opt_grid_rays_per_wave;
         return RS3D_OK;
     }
     set_error("rs3d_ctx_get_option: unknown option %d", option);

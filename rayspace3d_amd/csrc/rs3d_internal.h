@@ -122,6 +122,28 @@ __host__ __device__ inline float box_pad(float mag, float ext) {
     return (mag + ext) * 3.814697265625e-6f /* 2^-18 */ + 1e-35f;
 }
 
+// Padded box of one triangle (the BVH's leaf boxes; every structure that culls triangles
+// for the same rays uses this one function, so they all cull alike).
+__device__ __forceinline__ void prim_padded_box(float3 p0, float3 p1, float3 p2, float (&plo)[3],
+                                                float (&phi)[3]) {
+    const float lo[3] = {fminf(fminf(p0.x, p1.x), p2.x), fminf(fminf(p0.y, p1.y), p2.y),
+                         fminf(fminf(p0.z, p1.z), p2.z)};
+    const float hi[3] = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y),
+                         fmaxf(fmaxf(p0.z, p1.z), p2.z)};
+    float ext = 0.0f, mag = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ext = fmaxf(ext, hi[k] - lo[k]);
+        mag = fmaxf(mag, fmaxf(fabsf(lo[k]), fabsf(hi[k])));
+    }
+    const float pad = box_pad(mag, ext);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        plo[k] = lo[k] - pad;
+        phi[k] = hi[k] + pad;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Host-side objects
 // ------------------------------------------------------------------------------------
@@ -164,6 +186,8 @@ struct rs3d_ctx {
     rs3d::DevBuf ray_keys, ray_vals;  // ray reordering (coherence) keys and permutation
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
     int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
+    int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid
+    int64_t opt_grid_rays_per_wave = 1024;  // run length of grid point launches
     uint64_t* pinned = nullptr;  // small pinned host buffer for counts
     // traversal-kernel timing (rs3d_ctx_profile)
     int profile = 0;
@@ -173,8 +197,15 @@ struct rs3d_ctx {
     uint64_t prof_launches = 0;
 };
 
+namespace rs3d {
+struct AxisGrid;
+void axis_grid_free(AxisGrid* g);
+}  // namespace rs3d
+
 struct rs3d_bvh {
     rs3d_ctx* ctx = nullptr;
+    rs3d::AxisGrid* agrid[3] = {nullptr, nullptr, nullptr};  // per ray axis, built lazily
+    int agrid_state[3] = {0, 0, 0};                           // 0 untried, 1 built, 2 rejected
     uint64_t num_tris = 0;
     uint64_t num_nodes = 0;
     int leaf_size = 4;
@@ -204,6 +235,12 @@ int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint6
                             uint32_t** out_keys, uint32_t** out_vals, const uint2* seg = nullptr);
 int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
 int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n);  // stable
+// Whether point launches along +axis go to the axis grid of the BVH: the context allows it
+// and a grid is accepted (built here on first use).
+int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready);
+// Point launch on the grid (axis_grid_ready said yes).
+int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
+                     const uint32_t* imap, int out_kind, void* out);
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
 int grid_blocks(uint64_t n, int block, unsigned* out);

@@ -308,6 +308,23 @@ int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, 
         return RS3D_OK;
     }
     if (stats) return launch_points<true>(ctx, bvh, pts, n, imap, axis, out_kind, out, stats);
+    // The axis grid, when one is accepted, gives the same results with one lookup per ray
+    // (built on first use; the cache on the BVH is not part of its value). Its rays run in
+    // the caller's order: reordering them costs more than it saves there.
+    bool grid = false;
+    if (n <= 0xffffffffull) RS3D_TRY(axis_grid_ready(ctx, const_cast<rs3d_bvh*>(bvh), axis, &grid));
+    if (grid) {
+        if (ctx->profile) {
+            RS3D_TRY(prof_collect(ctx));
+            RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        }
+        RS3D_TRY(axis_grid_launch(ctx, const_cast<rs3d_bvh*>(bvh), axis, pts, n, imap, out_kind, out));
+        if (ctx->profile) {
+            RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            ctx->prof_pending = true;
+        }
+        return RS3D_OK;
+    }
     if (ctx->opt_ray_sort_min > 0 && (int64_t)n >= ctx->opt_ray_sort_min && n <= 0xffffffffull) {
         // reorder rays along a Morton curve of their origins (identical results, coherent waves)
         RS3D_TRY(ctx->ray_keys.ensure(n * sizeof(uint32_t)));

@@ -177,6 +177,25 @@ def trace_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> 
     return {k: int(getattr(st, k)) for k, _ in st._fields_}
 
 
+def axis_grid_prepare(ctx: Context, bvh: Bvh, axis: int = 2) -> dict:
+    """Builds (once) the axis grid of `bvh` for rays along +axis (rs3d_axis_grid_prepare);
+    state 1 = accepted (point launches use it), 2 = rejected (they stay on the BVH)."""
+    info = _lib.AxisGridInfo()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_axis_grid_prepare(ctx.handle, bvh.handle, axis, C.byref(info)), "rs3d_axis_grid_prepare")
+    return {k: getattr(info, k) for k, _ in info._fields_}
+
+
+def axis_grid_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> dict:
+    """Work counters of a point launch on the axis grid: nodes_visited = cell lookups,
+    tris_tested = 64-byte records read."""
+    st = _lib.TraceStats()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_axis_grid_stats(ctx.handle, bvh.handle, _ptr(points), points.shape[0], axis, C.byref(st)),
+          "rs3d_axis_grid_stats")
+    return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+
 @dataclass
 class FilterRefineResult:
     final_results: torch.Tensor   # int32 per point, -1 outside

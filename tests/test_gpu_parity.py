@@ -8,6 +8,7 @@ import torch
 
 import oracle as orc
 from rayspace3d_amd import _lib, gen
+from rayspace3d_amd.device import axis_grid_prepare, axis_grid_stats
 from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, containment_join,
                                    edge_count, intersection_join, overlap_join, sort_unique_u64,
                                    to_dev, to_host, trace_points, trace_stats)
@@ -361,19 +362,23 @@ def test_autotune_restores_options(ctx):
     from rayspace3d_amd import autotune
     mesh, pts = gen.many_spheres_workload(seed=41, n_objects=200, n_points=40_000, world=60.0)
     dv, di, do = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj)
-    before = (ctx.get_option(_lib.OPT_RAY_SORT_MIN), ctx.get_option(_lib.OPT_RAYS_PER_WAVE))
+    opts = (_lib.OPT_RAY_SORT_MIN, _lib.OPT_RAYS_PER_WAVE, _lib.OPT_AXIS_GRID, _lib.OPT_GRID_RAYS_PER_WAVE)
+    before = tuple(ctx.get_option(o) for o in opts)
     best, trials = autotune.tune_traversal(ctx, dv, di, do, to_dev(pts), leaf_sizes=(2, 4),
-                                           rays_per_wave=(0, 128), reps=1)
-    assert len(trials) == 8 and best in trials and all(t["ms"] > 0 for t in trials)
-    assert (ctx.get_option(_lib.OPT_RAY_SORT_MIN), ctx.get_option(_lib.OPT_RAYS_PER_WAVE)) == before
-    autotune.apply(ctx, best)
-    try:
-        out, _ = _contain(ctx, mesh, pts, 2, leaf=best["leaf_size"])
-    finally:
-        ctx.set_option(_lib.OPT_RAY_SORT_MIN, before[0])
-        ctx.set_option(_lib.OPT_RAYS_PER_WAVE, before[1])
+                                           rays_per_wave=(0, 128), reps=1, grid_rays_per_wave=(256, 1024))
+    walk = [t for t in trials if not t["axis_grid"]]
+    assert len(walk) == 8 and len(trials) - len(walk) in (0, 2)
+    assert best in trials and all(t["ms"] > 0 for t in trials)
+    assert tuple(ctx.get_option(o) for o in opts) == before
     ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
-    assert np.array_equal(out, ref)
+    for trial in (best, min(walk, key=lambda t: t["ms"])):
+        autotune.apply(ctx, trial)
+        try:
+            out, _ = _contain(ctx, mesh, pts, 2, leaf=trial["leaf_size"])
+        finally:
+            for o, v in zip(opts, before):
+                ctx.set_option(o, v)
+        assert np.array_equal(out, ref), trial
 
 
 def test_edge_stats_counters(ctx):
@@ -640,3 +645,67 @@ def test_bvh_structure(ctx, n_obj):
         b = Bvh.from_mesh(ctx, mesh, leaf_size=leaf)
         _check_bvh(b, mesh)
         b.free()
+
+
+def _grid_and_bvh(ctx, mesh, pts, axis, leaf=1):
+    """Point containment with the axis grid and with the BVH walk on the same tree."""
+    b = Bvh.from_mesh(ctx, mesh, leaf_size=leaf)
+    info = axis_grid_prepare(ctx, b, axis)
+    dp = to_dev(pts)
+    try:
+        ctx.set_option(_lib.OPT_AXIS_GRID, 1)
+        g = to_host(trace_points(ctx, b, dp, axis=axis, out_kind=_lib.OUT_OBJ_I32)).copy()
+        ctx.set_option(_lib.OPT_AXIS_GRID, 0)
+        v = to_host(trace_points(ctx, b, dp, axis=axis, out_kind=_lib.OUT_OBJ_I32)).copy()
+    finally:
+        ctx.set_option(_lib.OPT_AXIS_GRID, 1)
+    return b, info, g, v
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_axis_grid_matches_bvh_and_oracle(ctx, axis):
+    mesh, pts = gen.many_spheres_workload(seed=31, n_objects=3000, n_points=400_000, world=300.0)
+    print(f"axis grid: {len(mesh.indices)} triangles, {len(pts)} points, axis {axis}", flush=True)
+    b, info, g, v = _grid_and_bvh(ctx, mesh, pts, axis)
+    assert info["state"] == 1 and info["entries"] >= len(mesh.indices)
+    assert np.array_equal(g, v)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=axis, threads=8)
+    assert np.array_equal(g, ref)
+    assert (g >= 0).sum() > 100
+    st = axis_grid_stats(ctx, b, to_dev(pts), axis)
+    assert st["rays"] == len(pts) and st["nodes_visited"] >= len(pts) and st["tris_tested"] > 0
+    b.free()
+
+
+def test_axis_grid_restarts_and_nesting(ctx):
+    # 6 nested spheres (12 surfaces per column, more than the 8-entry hit buffer) plus
+    # points on lattice coordinates shared with vertices, at leaf sizes 1 and 4
+    mesh = gen.spheres(3, np.zeros((6, 3)), np.linspace(1.0, 2.5, 6))
+    rng = np.random.Generator(np.random.PCG64(8))
+    pts = np.concatenate([gen.uniform_points(rng, 30000, -2.7, 2.7),
+                          mesh.vertices[rng.integers(0, len(mesh.vertices), 3000)] * np.float32(0.5)])
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    for leaf in (1, 4):
+        b, info, g, v = _grid_and_bvh(ctx, mesh, pts, 2, leaf)
+        assert info["state"] == 1, info
+        assert np.array_equal(g, v) and np.array_equal(g, ref)
+        assert axis_grid_stats(ctx, b, to_dev(pts), 2)["restarts"] > 0
+        b.free()
+
+
+def test_axis_grid_rejected_falls_back(ctx):
+    # 150 stacked copies of one large square: every ray's cell would list 300 records, so
+    # the grid is rejected and launches stay on the BVH with the same results
+    n = 150
+    base = np.array([[0, 0, 0], [10, 0, 0], [10, 10, 0], [0, 10, 0]], np.float32)
+    V = np.concatenate([base + np.float32([0, 0, 0.05 * k]) for k in range(n)]).astype(np.float32)
+    I = np.concatenate([np.array([[0, 1, 2], [0, 2, 3]], np.uint32) + 4 * k for k in range(n)])
+    O = np.repeat(np.arange(n, dtype=np.int32), 2)
+    mesh = gen.Mesh(V, I, O, num_objects=n)
+    rng = np.random.Generator(np.random.PCG64(3))
+    pts = gen.uniform_points(rng, 5000, -1.0, 11.0)
+    b, info, g, v = _grid_and_bvh(ctx, mesh, pts, 2)
+    assert info["state"] == 2 and info["entries"] == 0
+    assert np.array_equal(g, v)
+    assert np.array_equal(g, orc.points_contain(V, I, O, pts, axis=2))
+    b.free()
