@@ -1,0 +1,144 @@
+// axis_grid.h -- the axis grid of a BVH (a 2-D grid of padded triangle boxes for rays
+// along one axis) and the per-ray program on it, shared by the single-query kernel
+// (axis_grid.hip) and the batch engine.
+#pragma once
+
+#include "rs3d_internal.h"
+#include "traverse.h"
+
+namespace rs3d {
+
+struct AxisGrid {
+    int axis = 2, nu = 1, nw = 1;
+    float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
+    uint64_t entries = 0;
+    uint32_t* start = nullptr;  // nu * nw + 1 record offsets
+    float4* rec = nullptr;      // 4 float4 per record
+    size_t bytes = 0;
+    double expected = 0.0;
+    double build_ms = 0.0;
+};
+
+struct AGArgs {
+    const uint32_t* start;
+    const float4* rec;
+    float lo_u, lo_w, inv_u, inv_w;
+    int nu, nw;
+};
+
+// Cell coordinate along one axis: monotone in x (clamped), NaN goes to cell 0.
+__device__ __forceinline__ int ag_cell(float x, float lo, float inv, int n) {
+    const float f = fminf(fmaxf((x - lo) * inv, 0.0f), (float)(n - 1));
+    return (int)f;
+}
+
+// Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
+// calling wave take rays from the run as they finish. The per-ray program (hit buffer,
+// restarts, chain, output) is the BVH kernels'; only the candidate walk differs.
+template <int AX, bool STATS>
+__device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __restrict__ pts,
+                                                const uint32_t* __restrict__ imap, uint64_t lo, uint64_t run_hi,
+                                                int out_kind, void* __restrict__ out, TraceCounters& ctr) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float eps = 1e-6f, tmax = 1e16f;
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint64_t next = lo;
+    bool active = false;
+    uint32_t pi = 0, j = 0, je = 0, js = 0;
+    float3 o = make_float3(0.0f, 0.0f, 0.0f);
+    float thr = eps;
+    int count = 0, first_obj = -1;
+    uint32_t total = 0;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    kbuf_clear(bt, bi, bo);
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (idle && next < run_hi) {
+            const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
+            if (!active && r < run_hi) {
+                pi = imap ? imap[r] : (uint32_t)r;
+                o = ld3f(pts, pi);
+                const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
+                const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
+                const uint64_t c = (uint64_t)cw * G.nu + cu;
+                js = G.start[c];
+                je = G.start[c + 1];
+                j = js;
+                if (STATS) ctr.nodes++;
+                thr = eps;
+                count = 0;
+                first_obj = -1;
+                total = 0;
+                kbuf_clear(bt, bi, bo);
+                active = true;
+            }
+            next += (uint64_t)__popcll(idle);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+        if (j < je) {
+            const float4 e0 = G.rec[4 * (uint64_t)j], e1 = G.rec[4 * (uint64_t)j + 1];
+            const float4 e2 = G.rec[4 * (uint64_t)j + 2], e3 = G.rec[4 * (uint64_t)j + 3];
+            if (STATS) ctr.tris++;
+            if (e2.w < oa) {  // records are ordered by box top: the rest end below the origin
+                j = je;
+                continue;
+            }
+            ++j;
+            if (e2.w >= oa && ou >= e3.x && ou <= e3.y && ow >= e3.z && ow <= e3.w) {
+                float t;
+                const uint32_t orig = (uint32_t)__float_as_int(e0.w);
+                if (tri_hit(o, rp, make_float3(e0.x, e0.y, e0.z), make_float3(e1.x, e1.y, e1.z),
+                            make_float3(e2.x, e2.y, e2.z), t) &&
+                    t >= thr && t < tmax) {
+                    ++total;
+                    if (STATS) ctr.hits++;
+                    if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                        kbuf_insert(bt, bi, bo, t, orig, __float_as_int(e1.w));
+                }
+            }
+            continue;
+        }
+        // list complete: walk the chain inside the buffer (as the BVH kernels do)
+#pragma unroll
+        for (int k = 0; k < KBUF; ++k) {
+            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                if (count == 0) first_obj = bo[k];
+                ++count;
+                float nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                thr = nthr;
+            }
+        }
+        if (total > (uint32_t)KBUF && count < 10000) {
+            kbuf_clear(bt, bi, bo);
+            total = 0;
+            j = js;
+            if (STATS) ctr.restarts++;
+            continue;
+        }
+        const int res = (count & 1) ? first_obj : -1;
+        if (out_kind == RS3D_OUT_RAY_RESULT) {
+            rs3d_ray_result rr;
+            rr.ray_id = (int32_t)pi;
+            rr.polygon_index = res;
+            static_cast<rs3d_ray_result*>(out)[pi] = rr;
+        } else if (out_kind == RS3D_OUT_FLAG_U8) {
+            static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
+        } else {
+            static_cast<int32_t*>(out)[pi] = res;
+        }
+        active = false;
+    }
+}
+
+}  // namespace rs3d

@@ -40,6 +40,7 @@ __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __res
     const uint64_t t = blockIdx.x;
     const uint32_t qa = task_q[t];
     const BatchDesc& q = qs[qa];
+    if (q.grid.start) return;  // grid queries are not reordered
     const uint64_t i = (t - task_off[qa]) * TASK_RAYS + threadIdx.x;
     if (i >= q.n) return;
     const int U = q.axis == 0 ? 1 : (q.axis == 1 ? 2 : 0);
@@ -87,8 +88,15 @@ __device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
         }
         const uint32_t qa = task_q[t];
         const BatchDesc& q = qs[qa];
-        const uint64_t lo = (t - task_off[qa]) * TASK_RAYS;
-        const uint64_t hi = lo + TASK_RAYS < q.n ? lo + TASK_RAYS : q.n;
+        const uint64_t lo = (t - task_off[qa]) * q.task_rays;
+        const uint64_t hi = lo + q.task_rays < q.n ? lo + q.task_rays : q.n;
+        if ((KINDS & 1) && q.kind == BQ_CONTAIN && q.grid.start) {
+            const int ax = AX >= 0 ? AX : q.axis;
+            if (ax == 0) points_grid_run<0, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, q.out, ctr);
+            else if (ax == 1) points_grid_run<1, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, q.out, ctr);
+            else points_grid_run<2, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, q.out, ctr);
+            continue;
+        }
         if ((KINDS & 1) && q.kind == BQ_CONTAIN) {
             const uint32_t* im = perm ? perm + q.ray_base : nullptr;
             const int ax = AX >= 0 ? AX : q.axis;
@@ -188,6 +196,14 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
     d.out = out;
     d.n = n;
     d.axis = axis;
+    bool grid = false;
+    if (n <= 0xffffffffull) RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid));
+    if (grid) {  // rays in the caller's order on the grid: no reorder keys for this query
+        const AxisGrid* g = b->agrid[axis];
+        d.grid = AGArgs{g->start, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+        items.push_back(d);
+        return RS3D_OK;
+    }
     d.ray_base = contain_rays;
     contain_rays += n;
     const int U = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
@@ -264,17 +280,18 @@ int BatchPlan::prepare() {
     // quarter more keys, each query gets its own segment (a multiple of RS_SEG_KEYS keys,
     // padded with the largest key) and a segmented 16-bit sort replaces the global sort
     // on (query, Morton) - one radix pass less.
+    // (queries on an axis grid take no part in the reorder)
     n_contain = 0;
     uint64_t padded = 0;
     for (auto& d : items)
         if (d.kind == BQ_CONTAIN) {
             ++n_contain;
-            padded += (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS;
+            if (!d.grid.start) padded += (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS;
         }
     segmented = contain_rays > 0 && padded <= contain_rays + contain_rays / 4;
     uint64_t rb = 0;
     for (auto& d : items)
-        if (d.kind == BQ_CONTAIN) {
+        if (d.kind == BQ_CONTAIN && !d.grid.start) {
             d.ray_base = rb;
             rb += segmented ? (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS : d.n;
         }
@@ -298,7 +315,10 @@ int BatchPlan::prepare() {
     }
     nq = (uint32_t)items.size();
     task_off.assign(nq + 1, 0);
-    for (uint32_t k = 0; k < nq; ++k) task_off[k + 1] = task_off[k] + (items[k].n + TASK_RAYS - 1) / TASK_RAYS;
+    for (uint32_t k = 0; k < nq; ++k) {
+        items[k].task_rays = items[k].grid.start ? GRID_TASK_RAYS : TASK_RAYS;
+        task_off[k + 1] = task_off[k] + (items[k].n + items[k].task_rays - 1) / items[k].task_rays;
+    }
     ntasks = task_off[nq];
     if (task_off[n_contain] > 0x7fffffffull) {
         set_error("batch: too many containment tasks");
@@ -310,6 +330,7 @@ int BatchPlan::prepare() {
     std::vector<uint32_t> segtab;
     if (segmented)
         for (uint32_t k = 0; k < n_contain; ++k) {
+            if (items[k].grid.start) continue;
             const uint32_t c0 = (uint32_t)(items[k].ray_base / RS_SEG_KEYS);
             const uint32_t nc = (uint32_t)((items[k].n + RS_SEG_KEYS - 1) / RS_SEG_KEYS);
             for (uint32_t c = 0; c < nc; ++c) {

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "axis_grid.h"
 #include "rs3d_internal.h"
 #include "traverse.h"
 
@@ -26,8 +27,10 @@ struct BatchDesc {
     int* cand_ids;
     uint8_t* cand_par;
     int max_targets;
+    AGArgs grid;        // containment on the BVH's axis grid when grid.start is set
     uint64_t n;         // rays
-    uint64_t ray_base;  // offset of this query's rays in the global ray order
+    uint64_t ray_base;  // offset of this query's rays in the reorder (walk queries only)
+    uint32_t task_rays; // rays per task (TASK_RAYS, or GRID_TASK_RAYS on an axis grid)
     int kind;
     int axis;
     int swap;
@@ -40,6 +43,7 @@ struct BatchDesc {
 // so the atomics of the persistent waves spread over many addresses; a wave starts on the
 // stripe of its workgroup index and moves on when a stripe runs dry.
 constexpr int TASK_RAYS = 64;
+constexpr int GRID_TASK_RAYS = 1024;  // containment on an axis grid: one lane-refill run per task
 constexpr int NSTRIPE = 32;
 constexpr int STRIPE_PITCH = 64;  // 32-bit words between heads
 
