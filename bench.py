@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_points_many_spheres.json")
 TRAFFIC_FILE_TRIJOIN = os.path.join(ROOT, "profiles", "traffic_k_tri_join_overlap.json")
+TRAFFIC_FILE_GRID = os.path.join(ROOT, "profiles", "traffic_k_points_grid_many_spheres.json")
 
 
 def main():
@@ -180,10 +181,11 @@ def main():
     canon.free()
     ncand = int(res.candidate_count)
     if use_grid:
-        # the grid kernel's own bytes: 12 B origin + 4 B id + 8 B cell range per lookup +
-        # 64 B per record read (counted on the refine candidates)
+        # the grid kernel's own bytes: 12 B origin + 4 B id + 4 B occupancy word per ray,
+        # 8 B per cell range read (occupied cells), 64 B per record read (counted on the
+        # refine candidates)
         gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
-        alg_bytes = 16 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
+        alg_bytes = 20 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
         bytes_per_ray = alg_bytes / max(gst["rays"], 1)
         kernel_name = "k_points_grid (refine, axis grid)"
     else:
@@ -194,11 +196,12 @@ def main():
             " (refine traversal)"
     achieved_gbs = alg_bytes / (kern_avg_ms / 1e3) / 1e9 if kern_avg_ms > 0 else 0.0
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            tf = json.load(f)
-        if kernel_name.startswith(tf.get("kernel", "")):
-            traffic = tf.get("hbm_bytes_per_launch")
+    for path in (TRAFFIC_FILE_GRID, TRAFFIC_FILE):
+        if os.path.exists(path) and traffic is None:
+            with open(path) as f:
+                tf = json.load(f)
+            if kernel_name.split(" ")[0] == tf.get("kernel") and args.objects == 10_000 and len(pts) == 10_000_000:
+                traffic = tf.get("hbm_bytes_per_launch")
 
     cpu = None
     if args.cpu_sample > 0:

@@ -153,8 +153,9 @@ typedef struct {
     double build_ms;          /* device time of the build (0 when not built here) */
 } rs3d_axis_grid_info;
 int rs3d_axis_grid_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info /* host, may be null */);
-/* Work counters of a point launch on the axis grid (nodes_visited = cell lookups,
- * tris_tested = 64-byte records read); RS3D_E_ARG when no grid is accepted for the axis. */
+/* Work counters of a point launch on the axis grid (every ray reads one occupancy bit;
+ * nodes_visited = 8-byte cell ranges read for occupied cells, tris_tested = 64-byte
+ * records read); RS3D_E_ARG when no grid is accepted for the axis. */
 int rs3d_axis_grid_stats(rs3d_ctx* ctx, rs3d_bvh* bvh, const float* points, uint64_t n, int axis,
                          rs3d_trace_stats* stats);
 /* ---- filter-refine (replaces FilterRefinePipeline::execute,

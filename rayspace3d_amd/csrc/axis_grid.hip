@@ -30,6 +30,7 @@ namespace rs3d {
 void axis_grid_free(AxisGrid* g) {
     if (!g) return;
     if (g->start) (void)hipFree(g->start);
+    if (g->occ) (void)hipFree(g->occ);
     if (g->rec) (void)hipFree(g->rec);
     delete g;
 }
@@ -137,6 +138,18 @@ __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tr
     rec[4 * i + 1] = r.v1;  // .w: object id
     rec[4 * i + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
     rec[4 * i + 3] = make_float4(plo[U], phi[U], plo[W], phi[W]);
+}
+
+__global__ __launch_bounds__(256) void k_ag_occ(const uint32_t* __restrict__ counts, uint64_t nc,
+                                                uint32_t* __restrict__ occ) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (nc + 31) / 32) return;
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+        const uint64_t c = w * 32 + b;
+        if (c < nc && counts[c]) bits |= 1u << b;
+    }
+    occ[w] = bits;
 }
 
 __global__ void k_ag_last(uint32_t* __restrict__ start, uint64_t nc, uint32_t total) {
@@ -260,16 +273,21 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->entries = total;
     g->expected = expected;
     hipError_t e;
+    const uint64_t nwords = (nc + 31) / 32;
     if ((e = hipMalloc(&g->start, (nc + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->occ, nwords * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&g->rec, std::max<uint64_t>(total, 1) * 4 * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint32_t) + total * 4 * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint32_t) + nwords * sizeof(uint32_t) + total * 4 * sizeof(float4);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, g->start, nc));
         k_ag_last<<<1, 64, 0, s>>>(g->start, nc, (uint32_t)total);
+        unsigned nbw;
+        RS3D_TRY(grid_blocks(nwords, 256, &nbw));
+        k_ag_occ<<<nbw, 256, 0, s>>>(counts, nc, g->occ);
         if (total) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
@@ -310,7 +328,7 @@ int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
 template <int AX, bool STATS>
 int launch_grid_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
                    int out_kind, void* out, unsigned long long* stats) {
-    const AGArgs G{g->start, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const AGArgs G{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
     const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));

@@ -200,7 +200,7 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
     if (n <= 0xffffffffull) RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid));
     if (grid) {  // rays in the caller's order on the grid: no reorder keys for this query
         const AxisGrid* g = b->agrid[axis];
-        d.grid = AGArgs{g->start, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+        d.grid = AGArgs{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
         items.push_back(d);
         return RS3D_OK;
     }

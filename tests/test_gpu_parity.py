@@ -673,7 +673,7 @@ def test_axis_grid_matches_bvh_and_oracle(ctx, axis):
     assert np.array_equal(g, ref)
     assert (g >= 0).sum() > 100
     st = axis_grid_stats(ctx, b, to_dev(pts), axis)
-    assert st["rays"] == len(pts) and st["nodes_visited"] >= len(pts) and st["tris_tested"] > 0
+    assert st["rays"] == len(pts) and 0 < st["nodes_visited"] <= len(pts) and st["tris_tested"] > 0
     b.free()
 
 
