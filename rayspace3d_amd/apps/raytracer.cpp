@@ -79,6 +79,7 @@ int main(int argc, char* argv[]) {
             timer.next("Build Index");
             as = std::make_unique<AccelerationStructure>(ctx, uploader);
             as->build();
+            as->prepareAxis(axis);  // the axis grid belongs to the index build
         }
         if (p != cachedPointsPath) {
             timer.next("Load Points");

@@ -365,6 +365,13 @@ void AccelerationStructure::build() {
           "rs3d_bvh_build");
 }
 
+bool AccelerationStructure::prepareAxis(int axis) {
+    if (!bvh_) return false;
+    rs3d_axis_grid_info info;
+    check(rs3d_axis_grid_prepare(ctx_.get(), bvh_, axis, &info), "rs3d_axis_grid_prepare");
+    return info.state == 1;
+}
+
 RayLauncher::RayLauncher(DeviceContext& ctx, GeometryUploader& g, int axis)
     : ctx_(ctx), geometry_(g), axis_(axis) {}
 

@@ -213,6 +213,9 @@ public:
     AccelerationStructure(const AccelerationStructure&) = delete;
     AccelerationStructure& operator=(const AccelerationStructure&) = delete;
     void build();
+    // Builds the axis grid for rays along +axis now (otherwise the first launch does);
+    // point launches use it when it is accepted. Returns whether it was accepted.
+    bool prepareAxis(int axis);
     rs3d_bvh* getHandle() const { return bvh_; }
 
 private:
