@@ -3,9 +3,10 @@
 Workload (the headline containment configuration): 10,000 closed icospheres (level 2,
 320 triangles each, 3.2 M triangles), radii log-uniform in [0.5, 5], centres uniform in
 a 1000^3 cube; 10,000,000 points uniform in the cube; one +z ray per point. A step is
-one FilterRefinePipeline pass (dataset-box filter, stable compaction, exact containment
-of the candidates, object id per point) with inputs and BVH resident in HBM. The BVH
-build is timed separately and reported as build_ms.
+one FilterRefinePipeline pass (dataset-box filter, exact containment of the candidates,
+object id per point; on an axis grid in one kernel, else filter, stable compaction and BVH
+walk) with inputs and BVH resident in HBM. The BVH and grid builds are timed separately
+and reported as build_ms and in axis_grid.
 
 Prints one JSON line. value = rays per second of the whole step, in Mrays/s.
 """
@@ -22,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_points_many_spheres.json")
 TRAFFIC_FILE_TRIJOIN = os.path.join(ROOT, "profiles", "traffic_k_tri_join_overlap.json")
-TRAFFIC_FILE_GRID = os.path.join(ROOT, "profiles", "traffic_k_points_grid_many_spheres.json")
+TRAFFIC_FILE_GRID = os.path.join(ROOT, "profiles", "traffic_k_points_grid_filtered_many_spheres.json")
 
 
 def main():
@@ -181,13 +182,13 @@ def main():
     canon.free()
     ncand = int(res.candidate_count)
     if use_grid:
-        # the grid kernel's own bytes: 12 B origin + 4 B id + 4 B occupancy word per ray,
-        # 8 B per cell range read (occupied cells), 64 B per record read (counted on the
-        # refine candidates)
+        # the one-pass kernel's bytes: 12 B origin + 4 B answer per point, then per filter
+        # candidate 4 B occupancy word, 8 B per cell range read (occupied cells) and 64 B per
+        # record read (counted on the candidates)
         gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
-        alg_bytes = 20 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
-        bytes_per_ray = alg_bytes / max(gst["rays"], 1)
-        kernel_name = "k_points_grid (refine, axis grid)"
+        alg_bytes = 16 * len(pts) + 4 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
+        bytes_per_ray = alg_bytes / len(pts)
+        kernel_name = "k_points_grid_filtered (filter + refine, axis grid)"
     else:
         gst = None
         alg_bytes = 16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"]

@@ -34,13 +34,35 @@ __device__ __forceinline__ int ag_cell(float x, float lo, float inv, int n) {
     return (int)f;
 }
 
+// The dataset-box filter fused into a grid run: points that fail box_filter_pass are
+// answered -1 without a grid walk; the lane counts candidates and points found inside.
+struct GridFilter {
+    FBox box;
+    uint32_t cand, inside;
+};
+
+__device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict__ out, uint32_t pi, int res) {
+    if (out_kind == RS3D_OUT_RAY_RESULT) {
+        rs3d_ray_result rr;
+        rr.ray_id = (int32_t)pi;
+        rr.polygon_index = res;
+        static_cast<rs3d_ray_result*>(out)[pi] = rr;
+    } else if (out_kind == RS3D_OUT_FLAG_U8) {
+        static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
+    } else {
+        static_cast<int32_t*>(out)[pi] = res;
+    }
+}
+
 // Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
-// restarts, chain, output) is the BVH kernels'; only the candidate walk differs.
-template <int AX, bool STATS>
+// restarts, chain, output) is the BVH kernels'; only the candidate walk differs. With FILT,
+// each ray first passes the dataset-box filter of *gf (filter-refine in one pass).
+template <int AX, bool STATS, bool FILT = false>
 __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __restrict__ pts,
                                                 const uint32_t* __restrict__ imap, uint64_t lo, uint64_t run_hi,
-                                                int out_kind, void* __restrict__ out, TraceCounters& ctr) {
+                                                int out_kind, void* __restrict__ out, TraceCounters& ctr,
+                                                GridFilter* gf = nullptr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float eps = 1e-6f, tmax = 1e16f;
     float3 d = make_float3(0.0f, 0.0f, 0.0f);
@@ -65,9 +87,19 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         const uint64_t idle = __ballot(!active);
         if (idle && next < run_hi) {
             const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
-            if (!active && r < run_hi) {
+            bool take = !active && r < run_hi;
+            if (take) {
                 pi = imap ? imap[r] : (uint32_t)r;
                 o = ld3f(pts, pi);
+                if (FILT) {
+                    take = box_filter_pass<AX>(o, gf->box);
+                    if (take)
+                        gf->cand++;
+                    else
+                        grid_write_result(out_kind, out, pi, -1);
+                }
+            }
+            if (take) {
                 const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
                 const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
                 const uint64_t c = (uint64_t)cw * G.nu + cu;
@@ -90,7 +122,10 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             }
             next += (uint64_t)__popcll(idle);
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active) == 0) {
+            if (FILT && next < run_hi) continue;  // every taken ray failed the filter
+            break;
+        }
         if (!active) continue;
         const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
         if (j < je) {
@@ -135,16 +170,8 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             continue;
         }
         const int res = (count & 1) ? first_obj : -1;
-        if (out_kind == RS3D_OUT_RAY_RESULT) {
-            rs3d_ray_result rr;
-            rr.ray_id = (int32_t)pi;
-            rr.polygon_index = res;
-            static_cast<rs3d_ray_result*>(out)[pi] = rr;
-        } else if (out_kind == RS3D_OUT_FLAG_U8) {
-            static_cast<uint8_t*>(out)[pi] = res >= 0 ? 1 : 0;
-        } else {
-            static_cast<int32_t*>(out)[pi] = res;
-        }
+        grid_write_result(out_kind, out, pi, res);
+        if (FILT && res >= 0) gf->inside++;
         active = false;
     }
 }

@@ -170,6 +170,24 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
     if (STATS) flush_counters(ctr, stats);
 }
 
+// Filter-refine in one pass: every point is read once, filtered against the dataset box,
+// and traced on the grid when it passes; out gets -1 for the rest. counts (device u64):
+// candidates in the high 32 bits, points inside in the low 32 (n < 2^32, so no carry).
+template <int AX>
+__global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, const float* __restrict__ pts,
+                                                                   uint64_t n, int32_t* __restrict__ out,
+                                                                   uint32_t per_wave, FBox box,
+                                                                   unsigned long long* __restrict__ counts) {
+    const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
+    const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
+    TraceCounters ctr;
+    GridFilter gf{box, 0u, 0u};
+    points_grid_run<AX, false, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, ctr, &gf);
+    unsigned long long v = ((unsigned long long)gf.cand << 32) | gf.inside;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(counts, v);
+}
+
 namespace {
 
 struct AgTmp {
@@ -347,7 +365,34 @@ int launch_grid(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, 
     }
 }
 
+template <int AX>
+int launch_grid_filtered_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const FBox& box,
+                            int32_t* out, unsigned long long* counts) {
+    const AGArgs G{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, pw, &nb));
+    k_points_grid_filtered<AX><<<nb, TR_BLOCK, 0, ctx->stream>>>(G, pts, n, out, pw, box, counts);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
 }  // namespace
+
+int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
+                            const float* box, int32_t* out, unsigned long long* counts) {
+    const AxisGrid* g = axis >= 0 && axis <= 2 ? bvh->agrid[axis] : nullptr;
+    if (!g || n > 0xffffffffull) {
+        set_error("axis grid launch without an accepted grid");
+        return RS3D_E_ARG;
+    }
+    const FBox fb{box[0], box[1], box[2], box[3], box[4], box[5]};
+    switch (axis) {
+        case 0: return launch_grid_filtered_ax<0>(ctx, g, pts, n, fb, out, counts);
+        case 1: return launch_grid_filtered_ax<1>(ctx, g, pts, n, fb, out, counts);
+        default: return launch_grid_filtered_ax<2>(ctx, g, pts, n, fb, out, counts);
+    }
+}
 
 int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready) {
     *ready = false;

@@ -238,6 +238,11 @@ int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
 // Whether point launches along +axis go to the axis grid of the BVH: the context allows it
 // and a grid is accepted (built here on first use).
 int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready);
+// Filter-refine on the grid in one launch (axis_grid_ready said yes): box = dataset box
+// (host float[6]); out_obj gets every point's answer; *counts (device, zeroed by the
+// caller) accumulates candidates << 32 | inside.
+int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
+                            const float* box, int32_t* out_obj, unsigned long long* counts);
 // Point launch on the grid (axis_grid_ready said yes).
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                      const uint32_t* imap, int out_kind, void* out);

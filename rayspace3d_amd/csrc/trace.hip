@@ -59,78 +59,40 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_refill(const BNode* __restr
     points_refill<AX, DEPTH>(nodes, tris, pts, imap, lo, hi, out_kind, out, stack_mem + threadIdx.x);
 }
 
-// ------------------------------------------------------------------------------------
-// Filter: parity of the +AX ray against the 12-triangle box mesh of
-// query/src/geometry/BoundingBox.cpp:26-49 (same vertex order and triangulation).
-// ------------------------------------------------------------------------------------
+// Filter flags of the filter-refine pipeline (the per-point test is box_filter_pass).
 template <int AX>
-__global__ __launch_bounds__(256) void k_filter_box(const float* __restrict__ pts, uint64_t n,
-                                                    float x0, float y0, float z0, float x1,
-                                                    float y1, float z1, uint8_t* __restrict__ flags) {
+__global__ __launch_bounds__(256) void k_filter_box(const float* __restrict__ pts, uint64_t n, FBox box,
+                                                    uint8_t* __restrict__ flags) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float3 o = ld3f(pts, i);
-    const float3 bv[8] = {make_float3(x0, y0, z0), make_float3(x1, y0, z0), make_float3(x1, y1, z0),
-                          make_float3(x0, y1, z0), make_float3(x0, y0, z1), make_float3(x1, y0, z1),
-                          make_float3(x1, y1, z1), make_float3(x0, y1, z1)};
-    const int tri[12][3] = {{0, 2, 1}, {0, 3, 2}, {4, 5, 6}, {4, 6, 7}, {0, 1, 5}, {0, 5, 4},
-                            {3, 6, 2}, {3, 7, 6}, {0, 4, 7}, {0, 7, 3}, {1, 2, 6}, {1, 6, 5}};
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
-    constexpr int NB = 12;
-    float bt[NB];
-    uint32_t bi[NB];
-#pragma unroll
-    for (int s = 0; s < NB; ++s) {
-        bt[s] = INFINITY;
-        bi[s] = 0xffffffffu;
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        float t;
-        if (tri_hit(o, rp, bv[tri[k][0]], bv[tri[k][1]], bv[tri[k][2]], t) && t >= 1e-6f &&
-            t < 1e16f) {
-            float ct = t;
-            uint32_t ci = (uint32_t)k;
-#pragma unroll
-            for (int s = 0; s < NB; ++s) {
-                const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
-                if (less) {
-                    const float tt = bt[s];
-                    const uint32_t ti = bi[s];
-                    bt[s] = ct;
-                    bi[s] = ci;
-                    ct = tt;
-                    ci = ti;
-                }
-            }
-        }
-    }
-    float thr = 1e-6f;
-    int count = 0;
-#pragma unroll
-    for (int s = 0; s < NB; ++s) {
-        if (bt[s] < INFINITY && bt[s] >= thr) {
-            ++count;
-            float nthr = bt[s] + 1e-6f;
-            if (nthr <= bt[s]) nthr = nextafterf(bt[s], INFINITY);
-            thr = nthr;
-        }
-    }
-    flags[i] = (count & 1) ? 1 : 0;
+    flags[i] = box_filter_pass<AX>(ld3f(pts, i), box) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_count_nonneg(const int32_t* __restrict__ p, uint64_t n,
                                                       unsigned long long* __restrict__ cnt) {
-    uint64_t c = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        c += p[i] >= 0 ? 1 : 0;
+    // 16-byte loads over the aligned body; the first `head` values (up to the first 16-byte
+    // boundary) and the tail one at a time
+    const uint64_t head = min(n, (uint64_t)((16u - ((uint32_t)(uintptr_t)p & 15u)) & 15u) / 4);
+    const uint64_t n4 = (n - head) / 4;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t c = 0;
+    const int4* p4 = reinterpret_cast<const int4*>(p + head);
+    for (uint64_t i = tid; i < n4; i += stride) {
+        const int4 v = p4[i];
+        c += (v.x >= 0) + (v.y >= 0) + (v.z >= 0) + (v.w >= 0);
+    }
+    if (tid < head) c += p[tid] >= 0 ? 1 : 0;
+    for (uint64_t i = head + n4 * 4 + tid; i < n; i += stride) c += p[i] >= 0 ? 1 : 0;
+    // one atomic per block: same-address atomics serialise in L2
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+    __shared__ uint32_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long w = (unsigned long long)part[0] + part[1] + part[2] + part[3];
+        if (w) atomicAdd(cnt, w);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, uint64_t n, int32_t v) {
@@ -512,12 +474,13 @@ int rs3d_filter_box(rs3d_ctx* ctx, const float* box, const float* points, uint64
     if (n == 0) return RS3D_OK;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, 256, &nb));
+    const FBox fb{box[0], box[1], box[2], box[3], box[4], box[5]};
     if (axis == 0)
-        k_filter_box<0><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+        k_filter_box<0><<<nb, 256, 0, ctx->stream>>>(points, n, fb, flags);
     else if (axis == 1)
-        k_filter_box<1><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+        k_filter_box<1><<<nb, 256, 0, ctx->stream>>>(points, n, fb, flags);
     else
-        k_filter_box<2><<<nb, 256, 0, ctx->stream>>>(points, n, box[0], box[1], box[2], box[3], box[4], box[5], flags);
+        k_filter_box<2><<<nb, 256, 0, ctx->stream>>>(points, n, fb, flags);
     RS3D_HIP(hipGetLastError());
     return RS3D_OK;
 }
@@ -535,6 +498,30 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
     if (n == 0) return RS3D_OK;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, 256, &nb));
+    const float box[6] = {bvh->vmin[0], bvh->vmin[1], bvh->vmin[2],
+                          bvh->vmax[0], bvh->vmax[1], bvh->vmax[2]};
+    bool grid = false;
+    if (bvh->num_tris) RS3D_TRY(axis_grid_ready(ctx, const_cast<rs3d_bvh*>(bvh), axis, &grid));
+    if (grid) {
+        // one pass: filter, trace and count per point (same answers as the three phases)
+        RS3D_TRY(ctx->small.ensure(256));
+        unsigned long long* cnt = ctx->small.as<unsigned long long>() + 24;
+        RS3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
+        if (ctx->profile) {
+            RS3D_TRY(prof_collect(ctx));
+            RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        }
+        RS3D_TRY(axis_grid_filter_refine(ctx, const_cast<rs3d_bvh*>(bvh), axis, points, n, box, out_obj, cnt));
+        if (ctx->profile) {
+            RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            ctx->prof_pending = true;
+        }
+        uint64_t h = 0;
+        RS3D_TRY(read_u64(ctx, cnt, &h));
+        if (candidate_count) *candidate_count = h >> 32;
+        if (inside_count) *inside_count = h & 0xffffffffull;
+        return RS3D_OK;
+    }
     k_fill_i32<<<nb, 256, 0, ctx->stream>>>(out_obj, n, -1);
     RS3D_HIP(hipGetLastError());
     if (bvh->num_tris == 0) return RS3D_OK;
@@ -542,8 +529,6 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
     RS3D_TRY(ctx->tmp_c.ensure(n + n * sizeof(uint32_t) + 256));
     uint8_t* flags = ctx->tmp_c.as<uint8_t>();
     uint32_t* cand = reinterpret_cast<uint32_t*>(ctx->tmp_c.as<uint8_t>() + ((n + 255) & ~255ull));
-    const float box[6] = {bvh->vmin[0], bvh->vmin[1], bvh->vmin[2],
-                          bvh->vmax[0], bvh->vmax[1], bvh->vmax[2]};
     RS3D_TRY(rs3d_filter_box(ctx, box, points, n, axis, flags));
     uint64_t nc = 0;
     RS3D_TRY(rs3d_compact_flags(ctx, flags, n, cand, &nc));
@@ -554,7 +539,7 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
         RS3D_TRY(ctx->small.ensure(256));
         unsigned long long* cnt = ctx->small.as<unsigned long long>() + 24;
         RS3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
-        k_count_nonneg<<<std::min(nb, 4096u), 256, 0, ctx->stream>>>(out_obj, n, cnt);
+        k_count_nonneg<<<std::min(nb, 2048u), 256, 0, ctx->stream>>>(out_obj, n, cnt);
         RS3D_HIP(hipGetLastError());
         uint64_t h = 0;
         RS3D_TRY(read_u64(ctx, cnt, &h));

@@ -46,6 +46,80 @@ __device__ __forceinline__ void kbuf_clear(float (&bt)[KBUF], uint32_t (&bi)[KBU
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Dataset-box filter of the filter-refine pipeline: parity of the +AX ray against the
+// 12-triangle box mesh of query/src/geometry/BoundingBox.cpp:26-49 (same vertex order and
+// triangulation), the chain of the point kernels over its hits.
+// Only the 4 triangles of the two faces perpendicular to AX are tested. A triangle of any
+// other face never counts: its three vertices share one sheared coordinate a (the face's
+// plane; the shear of an axis ray is +0), and two of its three corners share the other one,
+// so the edge function of that pair is fl(a*y) - fl(y*a) = 0. That forces the double
+// recomputation, in which the remaining two are fl(a*p - a*q) and fl(a*q - a*p), exact
+// negatives under round-to-nearest: mixed strict signs (miss) or all zero (det = 0, miss).
+// Overflowed or NaN coordinates give t = NaN or inf, which the t window rejects, as it would
+// on the full mesh. Hits keep their triangle index for ties.
+// ------------------------------------------------------------------------------------
+struct FBox {
+    float x0, y0, z0, x1, y1, z1;
+};
+
+template <int AX>
+__device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
+    const float3 bv[8] = {make_float3(b.x0, b.y0, b.z0), make_float3(b.x1, b.y0, b.z0),
+                          make_float3(b.x1, b.y1, b.z0), make_float3(b.x0, b.y1, b.z0),
+                          make_float3(b.x0, b.y0, b.z1), make_float3(b.x1, b.y0, b.z1),
+                          make_float3(b.x1, b.y1, b.z1), make_float3(b.x0, b.y1, b.z1)};
+    const int tri[12][3] = {{0, 2, 1}, {0, 3, 2}, {4, 5, 6}, {4, 6, 7}, {0, 1, 5}, {0, 5, 4},
+                            {3, 6, 2}, {3, 7, 6}, {0, 4, 7}, {0, 7, 3}, {1, 2, 6}, {1, 6, 5}};
+    // faces perpendicular to AX: z -> triangles 0-3, y -> 4-7, x -> 8-11
+    constexpr int K0 = AX == 2 ? 0 : (AX == 1 ? 4 : 8);
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    if (AX == 0) d.x = 1.0f;
+    if (AX == 1) d.y = 1.0f;
+    if (AX == 2) d.z = 1.0f;
+    const RayPre rp = ray_pre(d);
+    constexpr int NB = 4;
+    float bt[NB];
+    uint32_t bi[NB];
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        bt[s] = INFINITY;
+        bi[s] = 0xffffffffu;
+    }
+#pragma unroll
+    for (int k = K0; k < K0 + 4; ++k) {
+        float t;
+        if (tri_hit(o, rp, bv[tri[k][0]], bv[tri[k][1]], bv[tri[k][2]], t) && t >= 1e-6f && t < 1e16f) {
+            float ct = t;
+            uint32_t ci = (uint32_t)k;
+#pragma unroll
+            for (int s = 0; s < NB; ++s) {
+                const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
+                if (less) {
+                    const float tt = bt[s];
+                    const uint32_t ti = bi[s];
+                    bt[s] = ct;
+                    bi[s] = ci;
+                    ct = tt;
+                    ci = ti;
+                }
+            }
+        }
+    }
+    float thr = 1e-6f;
+    int count = 0;
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        if (bt[s] >= thr && bt[s] < INFINITY) {
+            ++count;
+            float nthr = bt[s] + 1e-6f;
+            if (nthr <= bt[s]) nthr = nextafterf(bt[s], INFINITY);
+            thr = nthr;
+        }
+    }
+    return (count & 1) != 0;
+}
+
 struct NodeBoxes {
     float lo0[3], hi0[3], lo1[3], hi1[3];
     int l0, l1, n0, n1;

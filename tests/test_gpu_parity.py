@@ -83,21 +83,61 @@ def test_degenerate_sizes(ctx):
     assert list(oute) == [-1, -1, -1]
 
 
-def test_filter_refine(ctx):
+@pytest.mark.parametrize("grid", [1, 0])
+@pytest.mark.parametrize("axis", [2, 0])
+def test_filter_refine(ctx, grid, axis):
+    # grid 1: the one-pass filter + axis-grid kernel; grid 0: filter, compaction, BVH walk
     mesh, pts = gen.many_spheres_workload(seed=31, n_objects=1500, n_points=200_000, world=150.0)
-    # points beyond the dataset box on every side
+    # points beyond the dataset box on every side, and on its faces
     rng = np.random.Generator(np.random.PCG64(3))
-    pts = np.concatenate([pts, gen.uniform_points(rng, 20000, -30.0, 180.0)])
-    bvh = Bvh.from_mesh(ctx, mesh)
-    res = FilterRefinePipeline(ctx).execute(bvh, to_dev(pts), axis=2)
-    out = to_host(res.final_results)
-    bvh.free()
-    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
-    assert np.array_equal(out, ref)
-    assert res.inside_count == int((ref >= 0).sum())
     vmin, vmax = mesh.vertices.min(0), mesh.vertices.max(0)
+    face = gen.uniform_points(rng, 6000, -10.0, 160.0)
+    for k in range(3):
+        face[2000 * k: 2000 * k + 1000, k] = vmin[k]
+        face[2000 * k + 1000: 2000 * k + 2000, k] = vmax[k]
+    pts = np.concatenate([pts, gen.uniform_points(rng, 20000, -30.0, 180.0), face]).astype(np.float32)
+    print(f"filter-refine grid {grid} axis {axis}: {len(pts)} points", flush=True)
+    saved = ctx.get_option(_lib.OPT_AXIS_GRID)
+    ctx.set_option(_lib.OPT_AXIS_GRID, grid)
+    try:
+        bvh = Bvh.from_mesh(ctx, mesh)
+        res = FilterRefinePipeline(ctx).execute(bvh, to_dev(pts), axis=axis)
+        out = to_host(res.final_results)
+        bvh.free()
+    finally:
+        ctx.set_option(_lib.OPT_AXIS_GRID, saved)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=axis)
+    assert np.array_equal(out, ref), np.count_nonzero(out != ref)
+    assert res.inside_count == int((ref >= 0).sum())
+    import boxcases as bc
+    V, T = bc.box_mesh(tuple(vmin) + tuple(vmax))
+    passes = orc.points_contain(V, T, np.zeros(12, np.int32), pts, axis=axis, use_grid=False) >= 0
+    assert res.candidate_count == int(passes.sum())
+    # off the face planes the filter is the open box
+    on_face = np.any((pts == vmin) | (pts == vmax), axis=1)
     in_box = np.all((pts > vmin) & (pts < vmax), axis=1)
-    assert res.candidate_count == int(in_box.sum())
+    assert np.array_equal(passes[~on_face], in_box[~on_face])
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_filter_box_flags(ctx, axis):
+    # rs3d_filter_box tests the 4 triangles perpendicular to the axis; the oracle parity runs
+    # over the whole 12-triangle box mesh, on points on and one ulp off every face plane
+    import boxcases as bc
+    for box in bc.BOXES:
+        pts = bc.box_points(box)
+        print(f"filter box {box}, axis {axis}, {len(pts)} points", flush=True)
+        V, T = bc.box_mesh(box)
+        ref = orc.points_contain(V, T, np.zeros(12, np.int32), pts, axis=axis, use_grid=False) >= 0
+        dp = to_dev(pts)
+        flags = torch.zeros(len(pts), dtype=torch.uint8, device="cuda")
+        hb = (C.c_float * 6)(*[float(np.float32(v)) for v in box])
+        torch.cuda.synchronize()
+        _lib.check(ctx.lib.rs3d_filter_box(ctx.handle, hb, C.c_void_p(dp.data_ptr()), len(pts), axis,
+                                           C.c_void_p(flags.data_ptr())), "filter_box")
+        ctx.sync()
+        got = to_host(flags).astype(bool)
+        assert np.array_equal(got, ref), (box, np.count_nonzero(got != ref))
 
 
 def test_scan_compact_sort(ctx):

@@ -311,3 +311,27 @@ def test_threaded_oracle_matches_single_thread():
     a = orc.points_contain(m.vertices, m.indices, m.tri_to_obj, p, 2)
     b = orc.points_contain(m.vertices, m.indices, m.tri_to_obj, p, 2, threads=4)
     assert np.array_equal(a, b) and (a >= 0).any()
+
+
+# ---- box filter: side faces never count -----------------------------------------------
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_box_side_faces_never_count(axis):
+    """The containment filter tests only the 4 triangles perpendicular to the ray axis. On
+    the oracle, the parity over those 4 equals the parity over the full 12-triangle box mesh
+    for points on, one ulp off and between every face plane, signed-zero and subnormal boxes
+    included -- a side face that could count a hit fails this."""
+    import boxcases as bc
+    for box in bc.BOXES:
+        pts = bc.box_points(box)
+        V, T = bc.box_mesh(box)
+        full = orc.points_contain(V, T, np.zeros(12, np.int32), pts, axis=axis, use_grid=False)
+        Vp, Tp = bc.perp_mesh(box, axis)
+        perp = orc.points_contain(Vp, Tp, np.zeros(4, np.int32), pts, axis=axis, use_grid=False)
+        assert np.array_equal(full >= 0, perp >= 0), box
+        # an interior point is inside when the far face lies inside the [1e-6, 1e16) t window
+        if all(box[3 + k] > box[k] for k in range(3)) and 1e-3 < box[3 + axis] - box[axis] < 1e15:
+            mid = np.array([[0.5 * box[k] + 0.5 * box[3 + k] for k in range(3)]], np.float32)
+            if np.all((mid > np.float32(box[:3])) & (mid < np.float32(box[3:]))):
+                assert orc.points_contain(V, T, np.zeros(12, np.int32), mid, axis=axis,
+                                          use_grid=False)[0] == 0, box
