@@ -79,9 +79,9 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
     float thr = eps;
     int count = 0, first_obj = -1;
     uint32_t total = 0;
-    float bt[KBUF];
-    uint32_t bi[KBUF];
-    int bo[KBUF];
+    float bt[KBUF_GRID];
+    uint32_t bi[KBUF_GRID];
+    int bo[KBUF_GRID];
     kbuf_clear(bt, bi, bo);
     while (true) {
         const uint64_t idle = __ballot(!active);
@@ -145,7 +145,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                     t >= thr && t < tmax) {
                     ++total;
                     if (STATS) ctr.hits++;
-                    if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                    if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
                         kbuf_insert(bt, bi, bo, t, orig, __float_as_int(e1.w));
                 }
             }
@@ -153,7 +153,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         }
         // list complete: walk the chain inside the buffer (as the BVH kernels do)
 #pragma unroll
-        for (int k = 0; k < KBUF; ++k) {
+        for (int k = 0; k < KBUF_GRID; ++k) {
             if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
                 if (count == 0) first_obj = bo[k];
                 ++count;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 thr = nthr;
             }
         }
-        if (total > (uint32_t)KBUF && count < 10000) {
+        if (total > (uint32_t)KBUF_GRID && count < 10000) {
             kbuf_clear(bt, bi, bo);
             total = 0;
             j = js;

@@ -9,19 +9,23 @@ namespace rs3d {
 
 constexpr int TR_BLOCK = 64;
 constexpr int KBUF = 8;
+// Hit buffer of the axis-grid walk: a grid ray meets few triangles (a sparse scene's rays
+// cross a handful of surfaces), so a smaller buffer saves registers; more hits restart.
+constexpr int KBUF_GRID = 4;
 
 __device__ __forceinline__ float3 ld3f(const float* p, uint64_t i) {
     return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
 
-// Sorted insertion of (t, tri, obj) into the KBUF smallest, ordered by (t, tri).
-__device__ __forceinline__ void kbuf_insert(float (&bt)[KBUF], uint32_t (&bi)[KBUF],
-                                            int (&bo)[KBUF], float t, uint32_t tri, int obj) {
+// Sorted insertion of (t, tri, obj) into the N smallest, ordered by (t, tri).
+template <int N>
+__device__ __forceinline__ void kbuf_insert(float (&bt)[N], uint32_t (&bi)[N], int (&bo)[N], float t,
+                                            uint32_t tri, int obj) {
     float ct = t;
     uint32_t ci = tri;
     int co = obj;
 #pragma unroll
-    for (int s = 0; s < KBUF; ++s) {
+    for (int s = 0; s < N; ++s) {
         const bool less = ct < bt[s] || (ct == bt[s] && ci < bi[s]);
         if (less) {
             const float tt = bt[s];
@@ -37,9 +41,10 @@ __device__ __forceinline__ void kbuf_insert(float (&bt)[KBUF], uint32_t (&bi)[KB
     }
 }
 
-__device__ __forceinline__ void kbuf_clear(float (&bt)[KBUF], uint32_t (&bi)[KBUF], int (&bo)[KBUF]) {
+template <int N>
+__device__ __forceinline__ void kbuf_clear(float (&bt)[N], uint32_t (&bi)[N], int (&bo)[N]) {
 #pragma unroll
-    for (int s = 0; s < KBUF; ++s) {
+    for (int s = 0; s < N; ++s) {
         bt[s] = INFINITY;
         bi[s] = 0xffffffffu;
         bo[s] = -1;
