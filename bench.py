@@ -53,6 +53,11 @@ def main():
                     help="containment: reorder launches of at least this many rays (0: never)")
     ap.add_argument("--axis-grid", type=int, default=1, choices=[0, 1],
                     help="containment: let point launches use the axis grid (1) or the BVH walk (0)")
+    ap.add_argument("--grid-rays-per-wave", type=int, default=None,
+                    help="containment: rays per wave run on the axis grid (library default if unset)")
+    ap.add_argument("--grid-cell-pct", type=int, default=None,
+                    help="containment: axis grid cell edge in %% of the mean triangle extent "
+                         "(library default if unset)")
     ap.add_argument("--autotune", action="store_true",
                     help="containment: pick leaf size, reordering and run length with the "
                          "autotuner on the workload's own rays first (the defaults are its "
@@ -128,6 +133,10 @@ def main():
           f"build {min(builds):.2f} ms", file=sys.stderr, flush=True)
     # axis grid for +z rays (built once, outside the timed query like the BVH)
     ctx.set_option(_lib.OPT_AXIS_GRID, args.axis_grid)
+    if args.grid_rays_per_wave is not None:
+        ctx.set_option(_lib.OPT_GRID_RAYS_PER_WAVE, args.grid_rays_per_wave)
+    if args.grid_cell_pct is not None:
+        ctx.set_option(_lib.OPT_GRID_CELL_PCT, args.grid_cell_pct)
     grid = None
     if args.axis_grid:
         grid = axis_grid_prepare(ctx, bvh, 2)
@@ -271,6 +280,8 @@ def main():
         "bvh": {"nodes": int(info.num_nodes), "depth": int(info.max_depth), "leaf_size": leaf},
         "traversal_options": {"ray_sort_min": ctx.get_option(_lib.OPT_RAY_SORT_MIN),
                               "rays_per_wave": ctx.get_option(_lib.OPT_RAYS_PER_WAVE),
+                              "grid_rays_per_wave": ctx.get_option(_lib.OPT_GRID_RAYS_PER_WAVE),
+                              "grid_cell_pct": ctx.get_option(_lib.OPT_GRID_CELL_PCT),
                               "autotuned": tuned is not None},
         "trace_stats": st,
         "axis_grid": grid,

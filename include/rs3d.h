@@ -62,9 +62,11 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * triangles (see rs3d_axis_grid_prepare) where one is accepted for the ray axis; 0 keeps
  * them on the BVH. Results are unchanged either way. Grid launches take the rays in the
  * caller's order (no reordering) with RS3D_OPT_GRID_RAYS_PER_WAVE rays per wave run
- * (default 1024; 64..2^20). */
+ * (default 1024; 64..2^20). RS3D_OPT_GRID_CELL_PCT: cell edge of grids built afterwards,
+ * in percent of the triangles' mean extent (default 50; 10..1000; widened while the grid
+ * would hold more than 12 records per triangle). Results are unchanged. */
 enum { RS3D_OPT_RAY_SORT_MIN = 1, RS3D_OPT_RAYS_PER_WAVE = 2, RS3D_OPT_AXIS_GRID = 3,
-       RS3D_OPT_GRID_RAYS_PER_WAVE = 4 };
+       RS3D_OPT_GRID_RAYS_PER_WAVE = 4, RS3D_OPT_GRID_CELL_PCT = 5 };
 int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value);
 int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value);
 

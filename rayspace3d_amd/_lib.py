@@ -25,6 +25,7 @@ OPT_RAY_SORT_MIN = 1
 OPT_RAYS_PER_WAVE = 2
 OPT_AXIS_GRID = 3
 OPT_GRID_RAYS_PER_WAVE = 4
+OPT_GRID_CELL_PCT = 5
 REQ_CONTAIN = 0
 REQ_OVERLAP = 1
 REQ_INTERSECT = 2

@@ -13,8 +13,9 @@
 // kernel's: the results are the same. The gain is the walk: one range lookup and a few
 // records instead of ~30 dependent node fetches.
 //
-// Build (per BVH and axis, once): the cell edge is the triangles' mean padded extent in the
-// plane; records per triangle from the cell span of its box, budget and list-length checks
+// Build (per BVH and axis, once): the cell edge is a fraction of the triangles' mean padded
+// extent in the plane (RS3D_OPT_GRID_CELL_PCT, widened while the records exceed their
+// budget); records per triangle from the cell span of its box, budget and list-length checks
 // (rejected grids leave the launches on the BVH), one
 // (cell << 32 | descending box top) key per record, a stable radix sort, per-cell counts
 // and their scan as the cell ranges, and a gather of the records.
@@ -232,36 +233,48 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     double size_sum = 0.0;
     RS3D_HIP(hipMemcpyAsync(&size_sum, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
     RS3D_HIP(hipStreamSynchronize(s));
-    const double max_cells = std::min<double>(4.0 * (double)n + 1024.0, (double)AG_MAX_CELLS);
-    double h = size_sum / (double)n;
-    if (eu > 0 && ew > 0) h = std::max(h, std::sqrt(eu * ew / max_cells));
-    h = std::max({h, eu / AG_MAX_DIM, ew / AG_MAX_DIM});
-    int nu = 1, nw = 1;
-    if (h > 0 && std::isfinite(h)) {
-        nu = (int)std::min<double>(std::max<double>(std::ceil(eu / h), 1.0), AG_MAX_DIM);
-        nw = (int)std::min<double>(std::max<double>(std::ceil(ew / h), 1.0), AG_MAX_DIM);
-    }
-    G.lo_u = (float)lo_u;
-    G.lo_w = (float)lo_w;
-    G.inv_u = eu > 0 ? (float)(nu / eu) : 0.0f;
-    G.inv_w = ew > 0 ? (float)(nw / ew) : 0.0f;
-    G.nu = nu;
-    G.nw = nw;
-    const uint64_t nc = (uint64_t)nu * nw;
+    // Cells smaller than the mean extent shorten the lists a ray reads (fewer boxes that
+    // overlap its cell but not its line) at the cost of more records: the requested edge
+    // (a percentage of the mean extent) is widened by 1.5x steps until the records fit the
+    // budget of their edge (12 per triangle below the mean extent, 6 at or above it).
+    const double mean_ext = size_sum / (double)n;
+    const double max_cells = std::min<double>(16.0 * (double)n + 1024.0, (double)AG_MAX_CELLS);
+    double h_min = 0.0;
+    if (eu > 0 && ew > 0) h_min = std::sqrt(eu * ew / max_cells);
+    h_min = std::max({h_min, eu / AG_MAX_DIM, ew / AG_MAX_DIM});
+    double h = std::max(mean_ext * (double)ctx->opt_grid_cell_pct / 100.0, h_min);
     AgTmp tmp;
     uint32_t *span, *off;
     RS3D_TRY(tmp.get(&span, n));
     RS3D_TRY(tmp.get(&off, n));
     unsigned long long* acc = ctx->small.as<unsigned long long>() + 24;
-    RS3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), s));
     unsigned nb;
     RS3D_TRY(grid_blocks(n, 256, &nb));
-    k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
-    RS3D_HIP(hipGetLastError());
     uint64_t total = 0;
-    RS3D_TRY(read_u64(ctx, acc, &total));
-    const uint64_t budget = 6 * n + (1ull << 20);
-    if (total > budget || total > 0xffffffffull) return RS3D_OK;  // rejected: too many records
+    int nu = 1, nw = 1;
+    for (;;) {
+        nu = nw = 1;
+        if (h > 0 && std::isfinite(h)) {
+            nu = (int)std::min<double>(std::max<double>(std::ceil(eu / h), 1.0), AG_MAX_DIM);
+            nw = (int)std::min<double>(std::max<double>(std::ceil(ew / h), 1.0), AG_MAX_DIM);
+        }
+        G.lo_u = (float)lo_u;
+        G.lo_w = (float)lo_w;
+        G.inv_u = eu > 0 ? (float)(nu / eu) : 0.0f;
+        G.inv_w = ew > 0 ? (float)(nw / ew) : 0.0f;
+        G.nu = nu;
+        G.nw = nw;
+        RS3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), s));
+        k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(read_u64(ctx, acc, &total));
+        const bool fine = h < mean_ext;
+        const uint64_t budget = (fine ? 12 : 6) * n + (1ull << 20);
+        if (total <= budget && total <= 0xffffffffull) break;
+        if (!fine || !std::isfinite(h) || !(h > 0)) return RS3D_OK;  // rejected: too many records
+        h = std::min(h * 1.5, std::max(mean_ext, h_min));
+    }
+    const uint64_t nc = (uint64_t)nu * nw;
     if ((double)total / (double)nc > AG_MAX_UNIFORM_LIST) return RS3D_OK;  // rejected: lists too long
     RS3D_TRY(scan_u32_exclusive(ctx, span, off, n));
     uint64_t* keys;

@@ -718,7 +718,7 @@ def test_axis_grid_matches_bvh_and_oracle(ctx, axis):
 
 
 def test_axis_grid_restarts_and_nesting(ctx):
-    # 6 nested spheres (12 surfaces per column, more than the 8-entry hit buffer) plus
+    # 6 nested spheres (12 surfaces per column, more than either hit buffer) plus
     # points on lattice coordinates shared with vertices, at leaf sizes 1 and 4
     mesh = gen.spheres(3, np.zeros((6, 3)), np.linspace(1.0, 2.5, 6))
     rng = np.random.Generator(np.random.PCG64(8))
@@ -731,6 +731,26 @@ def test_axis_grid_restarts_and_nesting(ctx):
         assert np.array_equal(g, v) and np.array_equal(g, ref)
         assert axis_grid_stats(ctx, b, to_dev(pts), 2)["restarts"] > 0
         b.free()
+
+
+@pytest.mark.parametrize("pct", [10, 50, 200])
+def test_axis_grid_cell_size_invariant(ctx, pct):
+    # 10 % of the mean extent would list ~100 records per triangle: the edge is widened
+    # until the 12-per-triangle budget holds; every cell size gives the same answers
+    mesh, pts = gen.many_spheres_workload(seed=33, n_objects=1000, n_points=100_000, world=100.0)
+    n = len(mesh.indices)
+    saved = ctx.get_option(_lib.OPT_GRID_CELL_PCT)
+    ctx.set_option(_lib.OPT_GRID_CELL_PCT, pct)
+    try:
+        b, info, g, v = _grid_and_bvh(ctx, mesh, pts, 2)
+    finally:
+        ctx.set_option(_lib.OPT_GRID_CELL_PCT, saved)
+    print(f"cell pct {pct}: {info}", flush=True)
+    assert info["state"] == 1
+    assert n <= info["entries"] <= (12 if pct < 100 else 6) * n + (1 << 20)
+    assert np.array_equal(g, v)
+    assert np.array_equal(g, orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2))
+    b.free()
 
 
 def test_axis_grid_rejected_falls_back(ctx):
