@@ -56,9 +56,10 @@ __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict
 
 // Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
-// restarts, chain, output) is the BVH kernels'; only the candidate walk differs. With FILT,
-// each ray first passes the dataset-box filter of *gf (filter-refine in one pass).
-template <int AX, bool STATS, bool FILT = false>
+// restarts, chain, output) is the BVH kernels'; only the candidate walk differs: a lane
+// reads two records per step (the second is in flight while the first is tested). With
+// gf, the lane counts the points it finds inside.
+template <int AX, bool STATS>
 __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __restrict__ pts,
                                                 const uint32_t* __restrict__ imap, uint64_t lo, uint64_t run_hi,
                                                 int out_kind, void* __restrict__ out, TraceCounters& ctr,
@@ -87,19 +88,9 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         const uint64_t idle = __ballot(!active);
         if (idle && next < run_hi) {
             const uint64_t r = next + (uint64_t)__popcll(idle & lt_mask);
-            bool take = !active && r < run_hi;
-            if (take) {
+            if (!active && r < run_hi) {
                 pi = imap ? imap[r] : (uint32_t)r;
                 o = ld3f(pts, pi);
-                if (FILT) {
-                    take = box_filter_pass<AX>(o, gf->box);
-                    if (take)
-                        gf->cand++;
-                    else
-                        grid_write_result(out_kind, out, pi, -1);
-                }
-            }
-            if (take) {
                 const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
                 const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
                 const uint64_t c = (uint64_t)cw * G.nu + cu;
@@ -122,33 +113,48 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             }
             next += (uint64_t)__popcll(idle);
         }
-        if (__ballot(active) == 0) {
-            if (FILT && next < run_hi) continue;  // every taken ray failed the filter
-            break;
-        }
+        if (__ballot(active) == 0) break;
         if (!active) continue;
         const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
         if (j < je) {
-            const float4 e0 = G.rec[4 * (uint64_t)j], e1 = G.rec[4 * (uint64_t)j + 1];
-            const float4 e2 = G.rec[4 * (uint64_t)j + 2], e3 = G.rec[4 * (uint64_t)j + 3];
+            const bool two = j + 1 < je;
+            const uint64_t b0 = 4 * (uint64_t)j, b1 = 4 * (uint64_t)(two ? j + 1 : j);
+            const float4 e0 = G.rec[b0], e1 = G.rec[b0 + 1], e2 = G.rec[b0 + 2], e3 = G.rec[b0 + 3];
+            const float4 f0 = G.rec[b1], f1 = G.rec[b1 + 1], f2 = G.rec[b1 + 2], f3 = G.rec[b1 + 3];
+            // one record: box test in the plane, then the triangle
+            auto test = [&](const float4& a0, const float4& a1, const float4& a2, const float4& a3) {
+                if (ou >= a3.x && ou <= a3.y && ow >= a3.z && ow <= a3.w) {
+                    float t;
+                    const uint32_t orig = (uint32_t)__float_as_int(a0.w);
+                    if (tri_hit(o, rp, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                                make_float3(a2.x, a2.y, a2.z), t) &&
+                        t >= thr && t < tmax) {
+                        ++total;
+                        if (STATS) ctr.hits++;
+                        if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
+                            kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
+                    }
+                }
+            };
+            // records are ordered by box top: from the first that ends below the origin on,
+            // none can be hit
             if (STATS) ctr.tris++;
-            if (e2.w < oa) {  // records are ordered by box top: the rest end below the origin
+            if (e2.w < oa) {
                 j = je;
                 continue;
             }
-            ++j;
-            if (e2.w >= oa && ou >= e3.x && ou <= e3.y && ow >= e3.z && ow <= e3.w) {
-                float t;
-                const uint32_t orig = (uint32_t)__float_as_int(e0.w);
-                if (tri_hit(o, rp, make_float3(e0.x, e0.y, e0.z), make_float3(e1.x, e1.y, e1.z),
-                            make_float3(e2.x, e2.y, e2.z), t) &&
-                    t >= thr && t < tmax) {
-                    ++total;
-                    if (STATS) ctr.hits++;
-                    if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
-                        kbuf_insert(bt, bi, bo, t, orig, __float_as_int(e1.w));
-                }
+            test(e0, e1, e2, e3);
+            if (!two) {
+                ++j;
+                continue;
             }
+            if (STATS) ctr.tris++;
+            if (f2.w < oa) {
+                j = je;
+                continue;
+            }
+            test(f0, f1, f2, f3);
+            j += 2;
             continue;
         }
         // list complete: walk the chain inside the buffer (as the BVH kernels do)
@@ -171,7 +177,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         }
         const int res = (count & 1) ? first_obj : -1;
         grid_write_result(out_kind, out, pi, res);
-        if (FILT && res >= 0) gf->inside++;
+        if (gf && res >= 0) gf->inside++;
         active = false;
     }
 }
