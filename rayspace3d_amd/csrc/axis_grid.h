@@ -58,12 +58,13 @@ __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
 // restarts, chain, output) is the BVH kernels'; only the candidate walk differs: a lane
 // reads two records per step (the second is in flight while the first is tested). With
-// gf, the lane counts the points it finds inside.
-template <int AX, bool STATS>
+// gf, the lane counts the points it finds inside. With PRE, the list holds points whose
+// record range was looked up already: qrange[r] is ray imap[r]'s (first, end) record.
+template <int AX, bool STATS, bool PRE = false>
 __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __restrict__ pts,
                                                 const uint32_t* __restrict__ imap, uint64_t lo, uint64_t run_hi,
                                                 int out_kind, void* __restrict__ out, TraceCounters& ctr,
-                                                GridFilter* gf = nullptr) {
+                                                GridFilter* gf = nullptr, const uint2* qrange = nullptr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float eps = 1e-6f, tmax = 1e16f;
     float3 d = make_float3(0.0f, 0.0f, 0.0f);
@@ -91,17 +92,23 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             if (!active && r < run_hi) {
                 pi = imap ? imap[r] : (uint32_t)r;
                 o = ld3f(pts, pi);
-                const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
-                const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
-                const uint64_t c = (uint64_t)cw * G.nu + cu;
-                // most rays of a sparse scene fall in empty cells: the occupancy bits
-                // (a few hundred KB, L2-resident) answer those without the range fetch
-                if ((G.occ[c >> 5] >> (c & 31)) & 1u) {
-                    js = G.start[c];
-                    je = G.start[c + 1];
-                    if (STATS) ctr.nodes++;
+                if (PRE) {
+                    const uint2 rg = qrange[r];
+                    js = rg.x;
+                    je = rg.y;
                 } else {
-                    js = je = 0;
+                    const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
+                    const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
+                    const uint64_t c = (uint64_t)cw * G.nu + cu;
+                    // most rays of a sparse scene fall in empty cells: the occupancy bits
+                    // (L2-resident) answer those without the range fetch
+                    if ((G.occ[c >> 5] >> (c & 31)) & 1u) {
+                        js = G.start[c];
+                        je = G.start[c + 1];
+                        if (STATS) ctr.nodes++;
+                    } else {
+                        js = je = 0;
+                    }
                 }
                 j = js;
                 thr = eps;

@@ -171,14 +171,15 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
     if (STATS) flush_counters(ctr, stats);
 }
 
-// Filter-refine in one pass over a run of at most AG_FR_RUN points per wave, in two phases.
+// Filter-refine in one pass over a run of points per wave, in two alternating phases.
 // Phase 1 streams the run 256 points at a time (four independent loads per lane in flight):
-// origin, dataset-box filter, cell, occupancy bit; points that fail the filter or fall in
-// an empty cell are answered -1 at once, the others are queued in LDS. Phase 2 walks the
-// queued points on the grid with lane refill, so the lanes of the walk only hold points
-// with records to read. counts (device u64): candidates in the high 32 bits, points inside
-// in the low 32 (n < 2^32, so no carry).
-constexpr uint32_t AG_FR_RUN = 1024;
+// origin, dataset-box filter, cell, occupancy bit, and for points in occupied cells the
+// cell's record range; points that fail the filter or fall in an empty cell are answered
+// -1 at once, the others are queued in LDS with their range. Phase 2 walks the queue on
+// the grid with lane refill whenever it could not take another 256 points, and at the end,
+// so the lanes of the walk only hold points with records to read. counts (device u64):
+// candidates in the high 32 bits, points inside in the low 32 (n < 2^32, so no carry).
+constexpr uint32_t AG_FR_QCAP = 512;
 
 template <int AX>
 __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, const float* __restrict__ pts,
@@ -186,46 +187,62 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
                                                                    uint32_t per_wave, FBox box,
                                                                    unsigned long long* __restrict__ counts) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    __shared__ uint32_t queue[AG_FR_RUN];
+    __shared__ uint32_t qpi[AG_FR_QCAP];
+    __shared__ uint2 qrg[AG_FR_QCAP];
     const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
     const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint32_t nq = 0, cand = 0;
-    for (uint64_t base = lo; base < hi; base += 4 * 64) {
-        float3 o[4];
-        bool ok[4];
-        uint32_t cell[4], word[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t r = base + 64 * k + lane;
-            ok[k] = r < hi;
-            o[k] = ok[k] ? ld3f(pts, r) : make_float3(0.0f, 0.0f, 0.0f);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            ok[k] = ok[k] && box_filter_pass<AX>(o[k], box);
-            cand += ok[k] ? 1u : 0u;
-            const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
-            const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
-            cell[k] = (uint32_t)cw * (uint32_t)G.nu + (uint32_t)cu;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) word[k] = ok[k] ? G.occ[cell[k] >> 5] : 0u;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t r = base + 64 * k + lane;
-            const bool queued = (word[k] >> (cell[k] & 31)) & 1u;
-            if (r < hi && !queued) out[r] = -1;
-            const uint64_t m = __ballot(queued);
-            if (queued) queue[nq + (uint32_t)__popcll(m & lt_mask)] = (uint32_t)r;
-            nq += (uint32_t)__popcll(m);
-        }
-    }
-    __syncthreads();
+    uint32_t cand = 0;
     TraceCounters ctr;
     GridFilter gf{box, 0u, 0u};
-    points_grid_run<AX, false>(G, pts, queue, 0, nq, RS3D_OUT_OBJ_I32, out, ctr, &gf);
+    uint64_t base = lo;
+    while (true) {
+        uint32_t nq = 0;
+        for (; base < hi && nq + 4 * 64 <= AG_FR_QCAP; base += 4 * 64) {
+            float3 o[4];
+            bool ok[4];
+            uint32_t cell[4], word[4];
+            uint2 rg[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t r = base + 64 * k + lane;
+                ok[k] = r < hi;
+                o[k] = ok[k] ? ld3f(pts, r) : make_float3(0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ok[k] = ok[k] && box_filter_pass<AX>(o[k], box);
+                cand += ok[k] ? 1u : 0u;
+                const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
+                const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
+                cell[k] = (uint32_t)cw * (uint32_t)G.nu + (uint32_t)cu;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) word[k] = ok[k] ? G.occ[cell[k] >> 5] : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ok[k] = (word[k] >> (cell[k] & 31)) & 1u;
+                rg[k] = ok[k] ? make_uint2(G.start[cell[k]], G.start[cell[k] + 1]) : make_uint2(0u, 0u);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t r = base + 64 * k + lane;
+                if (r < hi && !ok[k]) out[r] = -1;
+                const uint64_t m = __ballot(ok[k]);
+                if (ok[k]) {
+                    const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
+                    qpi[slot] = (uint32_t)r;
+                    qrg[slot] = rg[k];
+                }
+                nq += (uint32_t)__popcll(m);
+            }
+        }
+        __syncthreads();
+        points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, RS3D_OUT_OBJ_I32, out, ctr, &gf, qrg);
+        __syncthreads();
+        if (base >= hi) break;
+    }
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(counts, v);
@@ -424,7 +441,7 @@ template <int AX>
 int launch_grid_filtered_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const FBox& box,
                             int32_t* out, unsigned long long* counts) {
     const AGArgs G{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
-    const uint32_t pw = std::min<uint32_t>((uint32_t)ctx->opt_grid_rays_per_wave, AG_FR_RUN);
+    const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
     k_points_grid_filtered<AX><<<nb, TR_BLOCK, 0, ctx->stream>>>(G, pts, n, out, pw, box, counts);
