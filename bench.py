@@ -192,10 +192,10 @@ def main():
     ncand = int(res.candidate_count)
     if use_grid:
         # the one-pass kernel's bytes: 12 B origin + 4 B answer per point, then per filter
-        # candidate 4 B occupancy word, 8 B per cell range read (occupied cells) and 64 B per
-        # record read (counted on the candidates)
+        # candidate 4 B occupancy word, 16 B per cell-table read (two 8 B entries, occupied
+        # cells) and 64 B per record read (counted on the candidates)
         gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
-        alg_bytes = 16 * len(pts) + 4 * gst["rays"] + 8 * gst["nodes_visited"] + 64 * gst["tris_tested"]
+        alg_bytes = 16 * len(pts) + 4 * gst["rays"] + 16 * gst["nodes_visited"] + 64 * gst["tris_tested"]
         bytes_per_ray = alg_bytes / len(pts)
         kernel_name = "k_points_grid_filtered (filter + refine, axis grid)"
     else:

@@ -12,7 +12,7 @@ struct AxisGrid {
     int axis = 2, nu = 1, nw = 1;
     float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
     uint64_t entries = 0;
-    uint32_t* start = nullptr;  // nu * nw + 1 record offsets
+    uint2* cells = nullptr;     // nu * nw + 1 entries: first record, box top of that record
     uint32_t* occ = nullptr;    // one bit per cell: the cell lists records
     float4* rec = nullptr;      // 4 float4 per record
     size_t bytes = 0;
@@ -21,7 +21,7 @@ struct AxisGrid {
 };
 
 struct AGArgs {
-    const uint32_t* start;
+    const uint2* cells;
     const uint32_t* occ;
     const float4* rec;
     float lo_u, lo_w, inv_u, inv_w;
@@ -102,12 +102,15 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                     const uint64_t c = (uint64_t)cw * G.nu + cu;
                     // most rays of a sparse scene fall in empty cells: the occupancy bits
                     // (L2-resident) answer those without the range fetch
+                    js = je = 0;
                     if ((G.occ[c >> 5] >> (c & 31)) & 1u) {
-                        js = G.start[c];
-                        je = G.start[c + 1];
+                        const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
                         if (STATS) ctr.nodes++;
-                    } else {
-                        js = je = 0;
+                        // the first record's box top decides as the walk's exit test would
+                        if (!(__uint_as_float(c0.y) < get3(o, AX))) {
+                            js = c0.x;
+                            je = c1.x;
+                        }
                     }
                 }
                 j = js;

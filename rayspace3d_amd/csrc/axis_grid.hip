@@ -6,8 +6,9 @@
 // computed by the same prim_padded_box -- overlaps c in the plane, as a self-contained
 // 64-byte record (vertices, original index, object, the box's upper end along the axis and
 // its extent in the plane), ordered by that upper end, largest first. A ray reads its
-// cell's range, stops at the first record whose box ends below its origin, and tests the
-// records whose box contains its line. Those are exactly the triangles whose leaf box the
+// cell's entry in the cell table (first record, and that record's box top, so that a ray
+// above it reads no record at all), stops at the first record whose box ends below its
+// origin, and tests the records whose box contains its line. Those are exactly the triangles whose leaf box the
 // BVH walk would accept (ancestor boxes contain the leaf boxes), the padding guarantees
 // that no other triangle can be hit, and the hit buffer, restarts and chain are the BVH
 // kernel's: the results are the same. The gain is the walk: one range lookup and a few
@@ -30,7 +31,7 @@ namespace rs3d {
 
 void axis_grid_free(AxisGrid* g) {
     if (!g) return;
-    if (g->start) (void)hipFree(g->start);
+    if (g->cells) (void)hipFree(g->cells);
     if (g->occ) (void)hipFree(g->occ);
     if (g->rec) (void)hipFree(g->rec);
     delete g;
@@ -153,8 +154,20 @@ __global__ __launch_bounds__(256) void k_ag_occ(const uint32_t* __restrict__ cou
     occ[w] = bits;
 }
 
-__global__ void k_ag_last(uint32_t* __restrict__ start, uint64_t nc, uint32_t total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) start[nc] = total;
+// Cell table: (first record, box top of the first record -- the largest of the cell) per
+// cell, and (record count, -inf) after the last cell.
+__global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ start,
+                                                  const uint32_t* __restrict__ counts,
+                                                  const float4* __restrict__ rec, uint64_t nc, uint32_t total,
+                                                  uint2* __restrict__ cells) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nc) return;
+    if (c == nc) {
+        cells[c] = make_uint2(total, __float_as_uint(-INFINITY));
+        return;
+    }
+    const uint32_t s = start[c];
+    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? rec[4 * (uint64_t)s + 2].w : -INFINITY));
 }
 
 // Point rays on the grid, lane-refill form: wave b owns rays [b * per_wave, ...).
@@ -223,7 +236,15 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 ok[k] = (word[k] >> (cell[k] & 31)) & 1u;
-                rg[k] = ok[k] ? make_uint2(G.start[cell[k]], G.start[cell[k] + 1]) : make_uint2(0u, 0u);
+                uint2 c0 = make_uint2(0u, 0u), c1 = make_uint2(0u, 0u);
+                if (ok[k]) {
+                    c0 = G.cells[cell[k]];
+                    c1 = G.cells[cell[k] + 1];
+                }
+                // a point above the cell's highest box top has nothing to test (the walk's
+                // exit test on the first record)
+                ok[k] = ok[k] && !(__uint_as_float(c0.y) < get3(o[k], AX));
+                rg[k] = make_uint2(c0.x, c1.x);
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -364,17 +385,18 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->expected = expected;
     hipError_t e;
     const uint64_t nwords = (nc + 31) / 32;
-    if ((e = hipMalloc(&g->start, (nc + 1) * sizeof(uint32_t))) != hipSuccess ||
+    if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->occ, nwords * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&g->rec, std::max<uint64_t>(total, 1) * 4 * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint32_t) + nwords * sizeof(uint32_t) + total * 4 * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nwords * sizeof(uint32_t) + total * 4 * sizeof(float4);
     const int rc = [&]() -> int {
-        RS3D_TRY(scan_u32_exclusive(ctx, counts, g->start, nc));
-        k_ag_last<<<1, 64, 0, s>>>(g->start, nc, (uint32_t)total);
+        uint32_t* start;
+        RS3D_TRY(tmp.get(&start, nc));
+        RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         unsigned nbw;
         RS3D_TRY(grid_blocks(nwords, 256, &nbw));
         k_ag_occ<<<nbw, 256, 0, s>>>(counts, nc, g->occ);
@@ -384,6 +406,9 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
             RS3D_TRY(grid_blocks(total, 256, &nbe));
             k_ag_gather<AX><<<nbe, 256, 0, s>>>(bvh->tris, vals, total, G, g->rec);
         }
+        unsigned nbc1;
+        RS3D_TRY(grid_blocks(nc + 1, 256, &nbc1));
+        k_ag_cells<<<nbc1, 256, 0, s>>>(start, counts, g->rec, nc, (uint32_t)total, g->cells);
         RS3D_HIP(hipGetLastError());
         RS3D_HIP(hipStreamSynchronize(s));
         return RS3D_OK;
@@ -418,7 +443,7 @@ int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
 template <int AX, bool STATS>
 int launch_grid_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
                    int out_kind, void* out, unsigned long long* stats) {
-    const AGArgs G{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const AGArgs G{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
     const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
@@ -440,7 +465,7 @@ int launch_grid(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, 
 template <int AX>
 int launch_grid_filtered_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const FBox& box,
                             int32_t* out, unsigned long long* counts) {
-    const AGArgs G{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const AGArgs G{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
     const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));

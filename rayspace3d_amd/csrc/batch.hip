@@ -40,7 +40,7 @@ __global__ __launch_bounds__(TASK_RAYS) void k_batch_keys(const BatchDesc* __res
     const uint64_t t = blockIdx.x;
     const uint32_t qa = task_q[t];
     const BatchDesc& q = qs[qa];
-    if (q.grid.start) return;  // grid queries are not reordered
+    if (q.grid.cells) return;  // grid queries are not reordered
     const uint64_t i = (t - task_off[qa]) * TASK_RAYS + threadIdx.x;
     if (i >= q.n) return;
     const int U = q.axis == 0 ? 1 : (q.axis == 1 ? 2 : 0);
@@ -90,7 +90,7 @@ __device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
         const BatchDesc& q = qs[qa];
         const uint64_t lo = (t - task_off[qa]) * q.task_rays;
         const uint64_t hi = lo + q.task_rays < q.n ? lo + q.task_rays : q.n;
-        if ((KINDS & 1) && q.kind == BQ_CONTAIN && q.grid.start) {
+        if ((KINDS & 1) && q.kind == BQ_CONTAIN && q.grid.cells) {
             const int ax = AX >= 0 ? AX : q.axis;
             if (ax == 0) points_grid_run<0, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, q.out, ctr);
             else if (ax == 1) points_grid_run<1, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, q.out, ctr);
@@ -200,7 +200,7 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
     if (n <= 0xffffffffull) RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid));
     if (grid) {  // rays in the caller's order on the grid: no reorder keys for this query
         const AxisGrid* g = b->agrid[axis];
-        d.grid = AGArgs{g->start, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+        d.grid = AGArgs{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
         items.push_back(d);
         return RS3D_OK;
     }
@@ -286,12 +286,12 @@ int BatchPlan::prepare() {
     for (auto& d : items)
         if (d.kind == BQ_CONTAIN) {
             ++n_contain;
-            if (!d.grid.start) padded += (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS;
+            if (!d.grid.cells) padded += (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS;
         }
     segmented = contain_rays > 0 && padded <= contain_rays + contain_rays / 4;
     uint64_t rb = 0;
     for (auto& d : items)
-        if (d.kind == BQ_CONTAIN && !d.grid.start) {
+        if (d.kind == BQ_CONTAIN && !d.grid.cells) {
             d.ray_base = rb;
             rb += segmented ? (d.n + RS_SEG_KEYS - 1) / RS_SEG_KEYS * RS_SEG_KEYS : d.n;
         }
@@ -316,7 +316,7 @@ int BatchPlan::prepare() {
     nq = (uint32_t)items.size();
     task_off.assign(nq + 1, 0);
     for (uint32_t k = 0; k < nq; ++k) {
-        items[k].task_rays = items[k].grid.start ? GRID_TASK_RAYS : TASK_RAYS;
+        items[k].task_rays = items[k].grid.cells ? GRID_TASK_RAYS : TASK_RAYS;
         task_off[k + 1] = task_off[k] + (items[k].n + items[k].task_rays - 1) / items[k].task_rays;
     }
     ntasks = task_off[nq];
@@ -330,7 +330,7 @@ int BatchPlan::prepare() {
     std::vector<uint32_t> segtab;
     if (segmented)
         for (uint32_t k = 0; k < n_contain; ++k) {
-            if (items[k].grid.start) continue;
+            if (items[k].grid.cells) continue;
             const uint32_t c0 = (uint32_t)(items[k].ray_base / RS_SEG_KEYS);
             const uint32_t nc = (uint32_t)((items[k].n + RS_SEG_KEYS - 1) / RS_SEG_KEYS);
             for (uint32_t c = 0; c < nc; ++c) {
