@@ -62,7 +62,8 @@ __device__ __forceinline__ void ag_cells(const TriRec& r, const AGArgs& G, float
 
 // Sum of the triangles' padded extents in the plane (the larger of the two), for the cell size.
 template <int AX>
-__global__ __launch_bounds__(256) void k_ag_size(const TriRec* __restrict__ tris, uint64_t n, double* __restrict__ sum) {
+__global__ __launch_bounds__(256) void k_ag_size(const TriRec* __restrict__ tris, uint64_t n,
+                                                 unsigned long long* __restrict__ sum) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     double v = 0.0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -74,7 +75,11 @@ __global__ __launch_bounds__(256) void k_ag_size(const TriRec* __restrict__ tris
         if (e < 3.0e38f) v += (double)e;  // infinite or NaN extents do not set the cell size
     }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(sum, v);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<double*>(acc_stripe(sum)), part[0] + part[1] + part[2] + part[3]);
 }
 
 template <int AX>
@@ -89,8 +94,8 @@ __global__ __launch_bounds__(256) void k_ag_span(const TriRec* __restrict__ tris
         v = (unsigned long long)(cu1 - cu0 + 1) * (unsigned long long)(cw1 - cw0 + 1);
         span[i] = v > 0xffffffffull ? 0xffffffffu : (uint32_t)v;
     }
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(total, v);
+    v = block_sum_256(v);
+    if (threadIdx.x == 0 && v) atomicAdd(acc_stripe(total), v);
 }
 
 template <int AX>
@@ -122,8 +127,8 @@ __global__ __launch_bounds__(256) void k_ag_sq(const uint32_t* __restrict__ coun
         const unsigned long long c = counts[i];
         s += c * c;
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(acc, s);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0 && s) atomicAdd(acc_stripe(acc), s);
 }
 
 template <int AX>
@@ -190,8 +195,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
 // cell's record range; points that fail the filter or fall in an empty cell are answered
 // -1 at once, the others are queued in LDS with their range. Phase 2 walks the queue on
 // the grid with lane refill whenever it could not take another 256 points, and at the end,
-// so the lanes of the walk only hold points with records to read. counts (device u64):
-// candidates in the high 32 bits, points inside in the low 32 (n < 2^32, so no carry).
+// so the lanes of the walk only hold points with records to read. counts (striped
+// accumulator): candidates in the high 32 bits, points inside in the low 32 (n < 2^32).
 constexpr uint32_t AG_FR_QCAP = 512;
 
 template <int AX>
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
     }
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0 && v) atomicAdd(counts, v);
+    if (lane == 0 && v) atomicAdd(acc_stripe(counts), v);
 }
 
 namespace {
@@ -301,18 +306,22 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     const double eu = (double)bvh->bmax[U] - lo_u, ew = (double)bvh->bmax[W] - lo_w;
     // Cell edge: the mean padded extent of the triangles (each then spans ~4 cells), but no
     // more than 4 cells per triangle, 2^26 cells or 32768 cells along an axis.
-    RS3D_TRY(ctx->small.ensure(256));
-    double* dsum = reinterpret_cast<double*>(ctx->small.as<unsigned long long>() + 28);
-    RS3D_HIP(hipMemsetAsync(dsum, 0, sizeof(double), s));
+    double size_sum = 0.0;
     {
+        unsigned long long* dsum;
+        RS3D_TRY(acc_slot(ctx, 0, 0, &dsum));  // 0 bits = +0.0
         unsigned nbs;
         RS3D_TRY(grid_blocks(n, 256, &nbs));
         k_ag_size<AX><<<nbs < 1024 ? nbs : 1024, 256, 0, s>>>(bvh->tris, n, dsum);
         RS3D_HIP(hipGetLastError());
+        uint64_t st[ACC_STRIPES];
+        RS3D_TRY(acc_read(ctx, dsum, st));
+        for (int i = 0; i < ACC_STRIPES; ++i) {
+            double d;
+            memcpy(&d, &st[i], sizeof d);
+            size_sum += d;
+        }
     }
-    double size_sum = 0.0;
-    RS3D_HIP(hipMemcpyAsync(&size_sum, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
-    RS3D_HIP(hipStreamSynchronize(s));
     // Cells smaller than the mean extent shorten the lists a ray reads (fewer boxes that
     // overlap its cell but not its line) at the cost of more records: the requested edge
     // (a percentage of the mean extent) is widened by 1.5x steps until the records fit the
@@ -327,7 +336,6 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     uint32_t *span, *off;
     RS3D_TRY(tmp.get(&span, n));
     RS3D_TRY(tmp.get(&off, n));
-    unsigned long long* acc = ctx->small.as<unsigned long long>() + 24;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, 256, &nb));
     uint64_t total = 0;
@@ -344,10 +352,11 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         G.inv_w = ew > 0 ? (float)(nw / ew) : 0.0f;
         G.nu = nu;
         G.nw = nw;
-        RS3D_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), s));
+        unsigned long long* acc;
+        RS3D_TRY(acc_slot(ctx, 0, 0, &acc));
         k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(read_u64(ctx, acc, &total));
+        RS3D_TRY(acc_read_sum(ctx, acc, &total));
         const bool fine = h < mean_ext;
         const uint64_t budget = (fine ? 12 : 6) * n + (1ull << 20);
         if (total <= budget && total <= 0xffffffffull) break;
@@ -366,10 +375,12 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     k_ag_emit<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, off, keys, vals, counts);
     unsigned nbc;
     RS3D_TRY(grid_blocks(nc, 256, &nbc));
-    k_ag_sq<<<nbc < 1024 ? nbc : 1024, 256, 0, s>>>(counts, nc, acc + 1);
+    unsigned long long* acc_sq;
+    RS3D_TRY(acc_slot(ctx, 1, 0, &acc_sq));
+    k_ag_sq<<<nbc < 1024 ? nbc : 1024, 256, 0, s>>>(counts, nc, acc_sq);
     RS3D_HIP(hipGetLastError());
     uint64_t sq = 0;
-    RS3D_TRY(read_u64(ctx, acc + 1, &sq));
+    RS3D_TRY(acc_read_sum(ctx, acc_sq, &sq));
     const double weighted = total ? (double)sq / (double)total : 0.0;
     if (weighted > AG_MAX_WEIGHTED_LIST) return RS3D_OK;  // rejected: dense spots too long
     const double expected = (double)total / (double)nc;

@@ -66,6 +66,42 @@ int prof_collect(rs3d_ctx* ctx) {
     return RS3D_OK;
 }
 
+__global__ void k_acc_init(unsigned long long* __restrict__ acc, unsigned long long init) {
+    const int i = threadIdx.x;
+    if (i < ACC_STRIPES) acc[i * ACC_PITCH] = init;
+}
+
+int acc_slot(rs3d_ctx* ctx, int k, uint64_t init, unsigned long long** acc) {
+    static_assert(ACC_STRIPES == 64 && ACC_PITCH == 16, "acc_stripe assumes 64 stripes 128 B apart");
+    if (k < 0 || k >= ACC_SLOTS) {
+        set_error("accumulator slot %d out of range", k);
+        return RS3D_E_ARG;
+    }
+    RS3D_TRY(ctx->accs.ensure((size_t)ACC_SLOTS * ACC_STRIPES * ACC_PITCH * sizeof(unsigned long long)));
+    *acc = ctx->accs.as<unsigned long long>() + (size_t)k * ACC_STRIPES * ACC_PITCH;
+    k_acc_init<<<1, 64, 0, ctx->stream>>>(*acc, init);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+int acc_read(rs3d_ctx* ctx, const unsigned long long* acc, uint64_t (&stripes)[ACC_STRIPES]) {
+    // one contiguous copy of the 8 KB block (a strided copy is slower than the extra bytes)
+    RS3D_HIP(hipMemcpyAsync(ctx->pinned, acc, ACC_STRIPES * ACC_PITCH * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < ACC_STRIPES; ++i) stripes[i] = ctx->pinned[i * ACC_PITCH];
+    return RS3D_OK;
+}
+
+int acc_read_sum(rs3d_ctx* ctx, const unsigned long long* acc, uint64_t* sum) {
+    uint64_t v[ACC_STRIPES];
+    RS3D_TRY(acc_read(ctx, acc, v));
+    uint64_t t = 0;
+    for (int i = 0; i < ACC_STRIPES; ++i) t += v[i];
+    *sum = t;
+    return RS3D_OK;
+}
+
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host) {
     RS3D_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     RS3D_HIP(hipStreamSynchronize(ctx->stream));
@@ -502,7 +538,8 @@ int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint6
 
 // Bitwise OR / AND of all keys, to skip digit positions where every key agrees.
 __global__ __launch_bounds__(256) void k_or_and_u64(const uint64_t* __restrict__ keys, uint64_t n,
-                                                    unsigned long long* __restrict__ acc) {
+                                                    unsigned long long* __restrict__ acc_or,
+                                                    unsigned long long* __restrict__ acc_and) {
     uint64_t o = 0, a = ~0ull;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -514,15 +551,16 @@ __global__ __launch_bounds__(256) void k_or_and_u64(const uint64_t* __restrict__
         o |= __shfl_xor(o, off, 64);
         a &= __shfl_xor(a, off, 64);
     }
+    __shared__ uint64_t po[4], pa[4];
     if ((threadIdx.x & 63) == 0) {
-        atomicOr(&acc[0], (unsigned long long)o);
-        atomicAnd(&acc[1], (unsigned long long)a);
+        po[threadIdx.x >> 6] = o;
+        pa[threadIdx.x >> 6] = a;
     }
-}
-
-__global__ void k_init_or_and(unsigned long long* acc) {
-    acc[0] = 0ull;
-    acc[1] = ~0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicOr(acc_stripe(acc_or), (unsigned long long)(po[0] | po[1] | po[2] | po[3]));
+        atomicAnd(acc_stripe(acc_and), (unsigned long long)(pa[0] & pa[1] & pa[2] & pa[3]));
+    }
 }
 
 // Stable sort of u64 keys (with u32 values when vals is given), in place. Byte positions
@@ -536,15 +574,21 @@ static int radix_sort_u64_impl(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, ui
     }
     RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint64_t)));
     if (HAS_VAL) RS3D_TRY(ctx->sort_vals.ensure(n * sizeof(uint32_t)));
-    RS3D_TRY(ctx->small.ensure(64));
-    unsigned long long* acc = ctx->small.as<unsigned long long>();
-    k_init_or_and<<<1, 1, 0, ctx->stream>>>(acc);
+    // accumulator slots 6 and 7 belong to this sort (callers may hold the others)
+    unsigned long long *acc_or, *acc_and;
+    RS3D_TRY(acc_slot(ctx, 6, 0ull, &acc_or));
+    RS3D_TRY(acc_slot(ctx, 7, ~0ull, &acc_and));
     unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
-    k_or_and_u64<<<nb, 256, 0, ctx->stream>>>(keys, n, acc);
+    k_or_and_u64<<<nb, 256, 0, ctx->stream>>>(keys, n, acc_or, acc_and);
     RS3D_HIP(hipGetLastError());
-    unsigned long long h[2];
-    RS3D_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t so[ACC_STRIPES], sa[ACC_STRIPES];
+    RS3D_TRY(acc_read(ctx, acc_or, so));
+    RS3D_TRY(acc_read(ctx, acc_and, sa));
+    uint64_t h[2] = {0ull, ~0ull};
+    for (int i = 0; i < ACC_STRIPES; ++i) {
+        h[0] |= so[i];
+        h[1] &= sa[i];
+    }
     const uint64_t varying = h[0] & ~h[1];
     // Sort each varying byte, least significant first. Passes over constant bytes are
     // skipped; the result is identical because those digits are equal for all keys.
@@ -675,7 +719,7 @@ int rs3d_ctx_create(int device, rs3d_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
-    e = hipHostMalloc((void**)&c->pinned, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->pinned, ACC_STRIPES * ACC_PITCH * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -699,6 +743,7 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx) {
     ctx->tmp_b.release();
     ctx->tmp_c.release();
     ctx->small.release();
+    ctx->accs.release();
     ctx->ray_keys.release();
     ctx->ray_vals.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

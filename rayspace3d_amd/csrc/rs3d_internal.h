@@ -165,6 +165,21 @@ void set_error(const char* fmt, ...);
         if (r_ != RS3D_OK) return r_;        \
     } while (0)
 
+// Block-wide sum of a 64-bit value over a 256-thread block (every thread passes its
+// value; thread 0 gets the total). Uses 32 bytes of LDS.
+__device__ __forceinline__ unsigned long long block_sum_256(unsigned long long v) {
+    __shared__ unsigned long long part[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return threadIdx.x == 0 ? part[0] + part[1] + part[2] + part[3] : 0ull;
+}
+
+// The stripe of a striped accumulator this block adds into.
+__device__ __forceinline__ unsigned long long* acc_stripe(unsigned long long* acc) {
+    return acc + (blockIdx.x % 64u) * 16u;
+}
+
 // Grow-only device buffer owned by a context (scratch that persists across calls).
 struct DevBuf {
     void* ptr = nullptr;
@@ -184,6 +199,7 @@ struct rs3d_ctx {
     // Scratch, reused across calls on this context (calls on one context are serial).
     rs3d::DevBuf sort_keys, sort_vals, sort_hist, sort_csum, scan_tmp, tmp_a, tmp_b, tmp_c, small;
     rs3d::DevBuf ray_keys, ray_vals;  // ray reordering (coherence) keys and permutation
+    rs3d::DevBuf accs;                // striped accumulators (rs3d::acc_slot)
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
     int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
     int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid
@@ -240,14 +256,26 @@ int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
 // and a grid is accepted (built here on first use).
 int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready);
 // Filter-refine on the grid in one launch (axis_grid_ready said yes): box = dataset box
-// (host float[6]); out_obj gets every point's answer; *counts (device, zeroed by the
-// caller) accumulates candidates << 32 | inside.
+// (host float[6]); out_obj gets every point's answer; counts (a zeroed striped
+// accumulator) accumulates candidates << 32 | inside.
 int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                             const float* box, int32_t* out_obj, unsigned long long* counts);
 // Point launch on the grid (axis_grid_ready said yes).
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                      const uint32_t* imap, int out_kind, void* out);
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
+// Striped accumulators. Device-scope atomics on one address serialise (about 10 ns each on
+// this GPU, so ten thousand blocks adding into one counter cost ~0.1 ms), so reduction
+// kernels combine within the block and add into one of ACC_STRIPES words 128 B apart
+// (acc_stripe picks it from the block index); the reader combines the stripes.
+constexpr int ACC_STRIPES = 64;
+constexpr int ACC_PITCH = 16;  // 8-byte words from one stripe to the next
+constexpr int ACC_SLOTS = 8;   // accumulators per context (6 and 7: the u64 radix sort)
+// Slot k (< ACC_SLOTS) of the context, every stripe set to init (fill kernel, graph-safe).
+int acc_slot(rs3d_ctx* ctx, int k, uint64_t init, unsigned long long** acc);
+// The stripes of an accumulator, copied to host (synchronises the stream).
+int acc_read(rs3d_ctx* ctx, const unsigned long long* acc, uint64_t (&stripes)[ACC_STRIPES]);
+int acc_read_sum(rs3d_ctx* ctx, const unsigned long long* acc, uint64_t* sum);
 int read_u64(rs3d_ctx* ctx, const void* dev, uint64_t* host);
 int grid_blocks(uint64_t n, int block, unsigned* out);
 int fill_u32(rs3d_ctx* ctx, void* p, uint64_t n, uint32_t v);  // kernel fill (graph-safe)

@@ -84,15 +84,8 @@ __global__ __launch_bounds__(256) void k_count_nonneg(const int32_t* __restrict_
     }
     if (tid < head) c += p[tid] >= 0 ? 1 : 0;
     for (uint64_t i = head + n4 * 4 + tid; i < n; i += stride) c += p[i] >= 0 ? 1 : 0;
-    // one atomic per block: same-address atomics serialise in L2
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    __shared__ uint32_t part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long w = (unsigned long long)part[0] + part[1] + part[2] + part[3];
-        if (w) atomicAdd(cnt, w);
-    }
+    const unsigned long long w = block_sum_256(c);
+    if (threadIdx.x == 0 && w) atomicAdd(acc_stripe(cnt), w);
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, uint64_t n, int32_t v) {
@@ -149,8 +142,8 @@ __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ v, 
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
         s += (unsigned long long)(v[i] > 0 ? v[i] : 0);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(acc, s);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0 && s) atomicAdd(acc_stripe(acc), s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -443,19 +436,26 @@ int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
     RS3D_TRY(ctx->small.ensure(256));
     RS3D_TRY(ctx->tmp_a.ensure(n * sizeof(int32_t)));
     unsigned long long* dstats = ctx->small.as<unsigned long long>() + 16;
-    unsigned long long* dsum = dstats + 4;
-    RS3D_HIP(hipMemsetAsync(dstats, 0, 6 * sizeof(unsigned long long), ctx->stream));
+    RS3D_HIP(hipMemsetAsync(dstats, 0, 4 * sizeof(unsigned long long), ctx->stream));
     HashArgs H{};
     int32_t* counts = ctx->tmp_a.as<int32_t>();
     RS3D_TRY(launch_edges<EMODE_COUNT>(ctx, target, edges, 0, counts, nullptr, 0, nullptr, H,
                                        CHAIN_EPS, 100, dstats));
     unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
-    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(counts, n, dsum);
-    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(edges->src_counts, n, dsum + 1);
+    unsigned long long *emits, *refs;
+    RS3D_TRY(acc_slot(ctx, 0, 0, &emits));
+    RS3D_TRY(acc_slot(ctx, 1, 0, &refs));
+    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(counts, n, emits);
+    k_sum_i32<<<nb, 256, 0, ctx->stream>>>(edges->src_counts, n, refs);
     RS3D_HIP(hipGetLastError());
     unsigned long long h[6];
-    RS3D_HIP(hipMemcpyAsync(h, dstats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipMemcpyAsync(h, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t sums[2];
+    RS3D_TRY(acc_read_sum(ctx, emits, &sums[0]));
+    RS3D_TRY(acc_read_sum(ctx, refs, &sums[1]));
+    h[4] = sums[0];
+    h[5] = sums[1];
     stats->nodes_visited = h[0];
     stats->tris_tested = h[1];
     stats->hits = h[2];
@@ -504,9 +504,8 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
     if (bvh->num_tris) RS3D_TRY(axis_grid_ready(ctx, const_cast<rs3d_bvh*>(bvh), axis, &grid));
     if (grid) {
         // one pass: filter, trace and count per point (same answers as the three phases)
-        RS3D_TRY(ctx->small.ensure(256));
-        unsigned long long* cnt = ctx->small.as<unsigned long long>() + 24;
-        RS3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
+        unsigned long long* cnt;
+        RS3D_TRY(acc_slot(ctx, 0, 0, &cnt));
         if (ctx->profile) {
             RS3D_TRY(prof_collect(ctx));
             RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
@@ -517,7 +516,7 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
             ctx->prof_pending = true;
         }
         uint64_t h = 0;
-        RS3D_TRY(read_u64(ctx, cnt, &h));
+        RS3D_TRY(acc_read_sum(ctx, cnt, &h));
         if (candidate_count) *candidate_count = h >> 32;
         if (inside_count) *inside_count = h & 0xffffffffull;
         return RS3D_OK;
@@ -536,13 +535,12 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
     if (nc == 0) return RS3D_OK;
     RS3D_TRY(trace_points_internal(ctx, bvh, points, nc, cand, axis, RS3D_OUT_OBJ_I32, out_obj, nullptr));
     if (inside_count) {
-        RS3D_TRY(ctx->small.ensure(256));
-        unsigned long long* cnt = ctx->small.as<unsigned long long>() + 24;
-        RS3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
+        unsigned long long* cnt;
+        RS3D_TRY(acc_slot(ctx, 0, 0, &cnt));
         k_count_nonneg<<<std::min(nb, 2048u), 256, 0, ctx->stream>>>(out_obj, n, cnt);
         RS3D_HIP(hipGetLastError());
         uint64_t h = 0;
-        RS3D_TRY(read_u64(ctx, cnt, &h));
+        RS3D_TRY(acc_read_sum(ctx, cnt, &h));
         *inside_count = h;
     }
     return RS3D_OK;
