@@ -67,11 +67,6 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                                                 GridFilter* gf = nullptr, const uint2* qrange = nullptr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float eps = 1e-6f, tmax = 1e16f;
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint64_t next = lo;
@@ -136,8 +131,8 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 if (ou >= a3.x && ou <= a3.y && ow >= a3.z && ow <= a3.w) {
                     float t;
                     const uint32_t orig = (uint32_t)__float_as_int(a0.w);
-                    if (tri_hit(o, rp, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                                make_float3(a2.x, a2.y, a2.z), t) &&
+                    if (tri_hit_axis<AX>(o, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                                         make_float3(a2.x, a2.y, a2.z), t) &&
                         t >= thr && t < tmax) {
                         ++total;
                         if (STATS) ctr.hits++;

@@ -655,7 +655,7 @@ static void batch_release(rs3d_batch* b) {
     b->keys.release();
     rs3d_ctx& p = b->priv;
     for (DevBuf* d : {&p.sort_keys, &p.sort_vals, &p.sort_hist, &p.sort_csum, &p.scan_tmp, &p.tmp_a, &p.tmp_b,
-                      &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals})
+                      &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals, &p.accs})
         d->release();
     if (p.pinned) (void)hipHostFree(p.pinned);
     p.pinned = nullptr;
@@ -680,7 +680,10 @@ int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_qu
     p.num_cus = ctx->num_cus;
     p.opt_ray_sort_min = ctx->opt_ray_sort_min;
     p.opt_rays_per_wave = ctx->opt_rays_per_wave;
-    if (hipHostMalloc((void**)&p.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+    p.opt_axis_grid = ctx->opt_axis_grid;
+    p.opt_grid_rays_per_wave = ctx->opt_grid_rays_per_wave;
+    p.opt_grid_cell_pct = ctx->opt_grid_cell_pct;
+    if (hipHostMalloc((void**)&p.pinned, CTX_PINNED_BYTES, hipHostMallocDefault) != hipSuccess) {
         delete b;
         set_error("rs3d_batch_create: hipHostMalloc failed");
         return RS3D_E_NOMEM;

@@ -719,7 +719,7 @@ int rs3d_ctx_create(int device, rs3d_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
-    e = hipHostMalloc((void**)&c->pinned, ACC_STRIPES * ACC_PITCH * sizeof(uint64_t), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->pinned, CTX_PINNED_BYTES, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;

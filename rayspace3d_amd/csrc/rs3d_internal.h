@@ -116,6 +116,40 @@ __device__ __forceinline__ bool tri_hit(const float3 o, const RayPre& r, const f
     return true;
 }
 
+// tri_hit for the ray +AX (the only direction of point and object rays), without the shear.
+// For d = +e_AX, ray_pre gives kz = AX, kx = AX+1, ky = AX+2 (mod 3), Sx = Sy = +0, Sz = 1,
+// so the sheared coordinates are A[kx] - (+0 * A[kz]) and A[kz] * 1. With A[kz] finite the
+// first is A[kx] up to the sign of a zero, and a zero's sign changes none of the decisions
+// (U, V, W and det are compared with 0; a zero term leaves a nonzero sum unchanged; a zero
+// T gives t = +-0, outside every window). With A[kz] infinite tri_hit takes NaN coordinates
+// and returns t = NaN or a miss, and this form returns t = +-inf, NaN or a miss: no window
+// [thr, tmax) with tmax <= 1e16 admits any of them. The hits that count are identical.
+template <int AX>
+__device__ __forceinline__ bool tri_hit_axis(const float3 o, const float3 v0, const float3 v1, const float3 v2,
+                                             float& tout) {
+    constexpr int KX = (AX + 1) % 3, KY = (AX + 2) % 3;
+    const float Ax = get3(v0, KX) - get3(o, KX), Ay = get3(v0, KY) - get3(o, KY);
+    const float Bx = get3(v1, KX) - get3(o, KX), By = get3(v1, KY) - get3(o, KY);
+    const float Cx = get3(v2, KX) - get3(o, KX), Cy = get3(v2, KY) - get3(o, KY);
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = get3(v0, AX) - get3(o, AX);
+    const float Bz = get3(v1, AX) - get3(o, AX);
+    const float Cz = get3(v2, AX) - get3(o, AX);
+    const float T = U * Az + V * Bz + W * Cz;
+    tout = T / det;
+    return true;
+}
+
 // Outward padding of primitive boxes, relative to coordinate magnitude and extent, so
 // box culling can never remove a triangle the fp32 test would report (pruning only).
 __host__ __device__ inline float box_pad(float mag, float ext) {
@@ -271,6 +305,8 @@ int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uin
 constexpr int ACC_STRIPES = 64;
 constexpr int ACC_PITCH = 16;  // 8-byte words from one stripe to the next
 constexpr int ACC_SLOTS = 8;   // accumulators per context (6 and 7: the u64 radix sort)
+// Size of a context's pinned host buffer (`pinned`): one accumulator's stripes.
+constexpr size_t CTX_PINNED_BYTES = (size_t)ACC_STRIPES * ACC_PITCH * sizeof(uint64_t);
 // Slot k (< ACC_SLOTS) of the context, every stripe set to init (fill kernel, graph-safe).
 int acc_slot(rs3d_ctx* ctx, int k, uint64_t init, unsigned long long** acc);
 // The stripes of an accumulator, copied to host (synchronises the stream).

@@ -78,11 +78,6 @@ __device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
                             {3, 6, 2}, {3, 7, 6}, {0, 4, 7}, {0, 7, 3}, {1, 2, 6}, {1, 6, 5}};
     // faces perpendicular to AX: z -> triangles 0-3, y -> 4-7, x -> 8-11
     constexpr int K0 = AX == 2 ? 0 : (AX == 1 ? 4 : 8);
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
     constexpr int NB = 4;
     float bt[NB];
     uint32_t bi[NB];
@@ -94,7 +89,7 @@ __device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
 #pragma unroll
     for (int k = K0; k < K0 + 4; ++k) {
         float t;
-        if (tri_hit(o, rp, bv[tri[k][0]], bv[tri[k][1]], bv[tri[k][2]], t) && t >= 1e-6f && t < 1e16f) {
+        if (tri_hit_axis<AX>(o, bv[tri[k][0]], bv[tri[k][1]], bv[tri[k][2]], t) && t >= 1e-6f && t < 1e16f) {
             float ct = t;
             uint32_t ci = (uint32_t)k;
 #pragma unroll
@@ -167,11 +162,6 @@ __device__ __forceinline__ uint32_t collect_axis(const BNode* __restrict__ nodes
                                                  int (&bo)[KBUF], TraceCounters& ctr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
     uint32_t total = 0;
     int sp = 0;
     int node = 0;
@@ -197,7 +187,7 @@ __device__ __forceinline__ uint32_t collect_axis(const BNode* __restrict__ nodes
                     load_tri(tris, j, v0, v1, v2, orig, obj);
                     if (STATS) ctr.tris++;
                     float t;
-                    if (tri_hit(o, rp, v0, v1, v2, t) && (t > thr || (t == thr && orig >= thr_tri)) &&
+                    if (tri_hit_axis<AX>(o, v0, v1, v2, t) && (t > thr || (t == thr && orig >= thr_tri)) &&
                         t < tmax_excl) {
                         ++total;
                         if (STATS) ctr.hits++;
@@ -544,11 +534,6 @@ __device__ __forceinline__ void points_refill(const BNode* __restrict__ nodes,
                                               int* stack) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float eps = 1e-6f, tmax = 1e16f;
-    float3 d = make_float3(0.0f, 0.0f, 0.0f);
-    if (AX == 0) d.x = 1.0f;
-    if (AX == 1) d.y = 1.0f;
-    if (AX == 2) d.z = 1.0f;
-    const RayPre rp = ray_pre(d);
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t run_hi = hi;
@@ -601,7 +586,7 @@ __device__ __forceinline__ void points_refill(const BNode* __restrict__ nodes,
                     int obj;
                     load_tri(tris, j, v0, v1, v2, orig, obj);
                     float t;
-                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t < tmax) {
+                    if (tri_hit_axis<AX>(o, v0, v1, v2, t) && t >= thr && t < tmax) {
                         ++total;
                         if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
                             kbuf_insert(bt, bi, bo, t, orig, obj);
