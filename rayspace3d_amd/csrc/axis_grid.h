@@ -187,4 +187,85 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
     }
 }
 
+// Queue capacity of the two-phase grid form (points; 6 KB of LDS with their ranges).
+constexpr uint32_t AG_QCAP = 512;
+
+// Point rays [lo, hi) on the grid in two alternating phases. Phase 1 streams the points 256
+// at a time (four independent loads per lane in flight): origin, with FILT the dataset-box
+// filter of gf.box, cell, occupancy bit and the cell-table entries; points that fail the
+// filter, fall in an empty cell or lie above the cell's highest box top are answered -1 at
+// once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS).
+// Ray r of the list is point imap[r] (r itself without imap); answers go out as out_kind.
+// Phase 2 walks the queue with lane refill (points_grid_run, PRE) whenever phase 1 could
+// not take another 256 points, and at the end. cand counts filter passes (FILT), gf.inside
+// the points found inside. The whole wave calls it with the same lo, hi.
+template <int AX, bool FILT>
+__device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const float* __restrict__ pts,
+                                                      const uint32_t* __restrict__ imap, uint64_t lo, uint64_t hi,
+                                                      int out_kind, void* __restrict__ out, uint32_t* qpi,
+                                                      uint2* qrg, uint32_t& cand, GridFilter& gf,
+                                                      TraceCounters& ctr) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint64_t base = lo;
+    while (true) {
+        uint32_t nq = 0;
+        for (; base < hi && nq + 4 * 64 <= AG_QCAP; base += 4 * 64) {
+            float3 o[4];
+            bool ok[4];
+            uint32_t cell[4], word[4], pid[4];
+            uint2 rg[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t r = base + 64 * k + lane;
+                ok[k] = r < hi;
+                pid[k] = ok[k] ? (imap ? imap[r] : (uint32_t)r) : 0u;
+                o[k] = ok[k] ? ld3f(pts, pid[k]) : make_float3(0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (FILT) {
+                    ok[k] = ok[k] && box_filter_pass<AX>(o[k], gf.box);
+                    cand += ok[k] ? 1u : 0u;
+                }
+                const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
+                const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
+                cell[k] = (uint32_t)cw * (uint32_t)G.nu + (uint32_t)cu;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) word[k] = ok[k] ? G.occ[cell[k] >> 5] : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ok[k] = (word[k] >> (cell[k] & 31)) & 1u;
+                uint2 c0 = make_uint2(0u, 0u), c1 = make_uint2(0u, 0u);
+                if (ok[k]) {
+                    c0 = G.cells[cell[k]];
+                    c1 = G.cells[cell[k] + 1];
+                }
+                // a point above the cell's highest box top has nothing to test (the walk's
+                // exit test on the first record)
+                ok[k] = ok[k] && !(__uint_as_float(c0.y) < get3(o[k], AX));
+                rg[k] = make_uint2(c0.x, c1.x);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t r = base + 64 * k + lane;
+                if (r < hi && !ok[k]) grid_write_result(out_kind, out, pid[k], -1);
+                const uint64_t m = __ballot(ok[k]);
+                if (ok[k]) {
+                    const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
+                    qpi[slot] = pid[k];
+                    qrg[slot] = rg[k];
+                }
+                nq += (uint32_t)__popcll(m);
+            }
+        }
+        __syncthreads();
+        points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
+        __syncthreads();
+        if (base >= hi) break;
+    }
+}
+
 }  // namespace rs3d

@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ s
     cells[c] = make_uint2(s, __float_as_uint(counts[c] ? rec[4 * (uint64_t)s + 2].w : -INFINITY));
 }
 
-// Point rays on the grid, lane-refill form: wave b owns rays [b * per_wave, ...).
-// Counters (STATS): cell lookups, records read, hits, restarts.
+// Point rays on the grid: wave b owns rays [b * per_wave, ...), in the two-phase form; the
+// counting pass (STATS: cell lookups, records read, hits, restarts) walks them in one.
 template <int AX, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float* __restrict__ pts, uint64_t n,
                                                           const uint32_t* __restrict__ imap, int out_kind,
@@ -185,90 +185,35 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
     const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
     const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
     TraceCounters ctr;
-    points_grid_run<AX, STATS>(G, pts, imap, lo, hi, out_kind, out, ctr);
-    if (STATS) flush_counters(ctr, stats);
+    if (STATS) {
+        points_grid_run<AX, true>(G, pts, imap, lo, hi, out_kind, out, ctr);
+        flush_counters(ctr, stats);
+    } else {
+        __shared__ uint32_t qpi[AG_QCAP];
+        __shared__ uint2 qrg[AG_QCAP];
+        uint32_t cand = 0;
+        GridFilter gf{};
+        points_grid_two_phase<AX, false>(G, pts, imap, lo, hi, out_kind, out, qpi, qrg, cand, gf, ctr);
+    }
 }
 
-// Filter-refine in one pass over a run of points per wave, in two alternating phases.
-// Phase 1 streams the run 256 points at a time (four independent loads per lane in flight):
-// origin, dataset-box filter, cell, occupancy bit, and for points in occupied cells the
-// cell's record range; points that fail the filter or fall in an empty cell are answered
-// -1 at once, the others are queued in LDS with their range. Phase 2 walks the queue on
-// the grid with lane refill whenever it could not take another 256 points, and at the end,
-// so the lanes of the walk only hold points with records to read. counts (striped
-// accumulator): candidates in the high 32 bits, points inside in the low 32 (n < 2^32).
-constexpr uint32_t AG_FR_QCAP = 512;
-
+// Filter-refine in one pass: one wave per run of per_wave points, in the two-phase grid
+// form with the dataset-box filter. counts (striped accumulator): candidates in the high 32
+// bits, points inside in the low 32 (n < 2^32, so no carry).
 template <int AX>
 __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, const float* __restrict__ pts,
                                                                    uint64_t n, int32_t* __restrict__ out,
                                                                    uint32_t per_wave, FBox box,
                                                                    unsigned long long* __restrict__ counts) {
-    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    __shared__ uint32_t qpi[AG_FR_QCAP];
-    __shared__ uint2 qrg[AG_FR_QCAP];
+    __shared__ uint32_t qpi[AG_QCAP];
+    __shared__ uint2 qrg[AG_QCAP];
     const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
     const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
     const int lane = threadIdx.x & 63;
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t cand = 0;
     TraceCounters ctr;
     GridFilter gf{box, 0u, 0u};
-    uint64_t base = lo;
-    while (true) {
-        uint32_t nq = 0;
-        for (; base < hi && nq + 4 * 64 <= AG_FR_QCAP; base += 4 * 64) {
-            float3 o[4];
-            bool ok[4];
-            uint32_t cell[4], word[4];
-            uint2 rg[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t r = base + 64 * k + lane;
-                ok[k] = r < hi;
-                o[k] = ok[k] ? ld3f(pts, r) : make_float3(0.0f, 0.0f, 0.0f);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                ok[k] = ok[k] && box_filter_pass<AX>(o[k], box);
-                cand += ok[k] ? 1u : 0u;
-                const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
-                const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
-                cell[k] = (uint32_t)cw * (uint32_t)G.nu + (uint32_t)cu;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) word[k] = ok[k] ? G.occ[cell[k] >> 5] : 0u;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                ok[k] = (word[k] >> (cell[k] & 31)) & 1u;
-                uint2 c0 = make_uint2(0u, 0u), c1 = make_uint2(0u, 0u);
-                if (ok[k]) {
-                    c0 = G.cells[cell[k]];
-                    c1 = G.cells[cell[k] + 1];
-                }
-                // a point above the cell's highest box top has nothing to test (the walk's
-                // exit test on the first record)
-                ok[k] = ok[k] && !(__uint_as_float(c0.y) < get3(o[k], AX));
-                rg[k] = make_uint2(c0.x, c1.x);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t r = base + 64 * k + lane;
-                if (r < hi && !ok[k]) out[r] = -1;
-                const uint64_t m = __ballot(ok[k]);
-                if (ok[k]) {
-                    const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
-                    qpi[slot] = (uint32_t)r;
-                    qrg[slot] = rg[k];
-                }
-                nq += (uint32_t)__popcll(m);
-            }
-        }
-        __syncthreads();
-        points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, RS3D_OUT_OBJ_I32, out, ctr, &gf, qrg);
-        __syncthreads();
-        if (base >= hi) break;
-    }
+    points_grid_two_phase<AX, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(acc_stripe(counts), v);
