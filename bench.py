@@ -565,7 +565,9 @@ def run_overlap(args):
     import torch
 
     from rayspace3d_amd import gen
-    from rayspace3d_amd.device import Bvh, Context, DevEdges, edge_stats, overlap_join, to_dev
+    from rayspace3d_amd import _lib
+    from rayspace3d_amd.device import (Bvh, Context, DevEdges, edge_stats, overlap_join, seg_occupancy_prepare,
+                                       to_dev)
 
     n_obj = args.overlap_objects
     t0 = time.time()
@@ -598,6 +600,12 @@ def run_overlap(args):
         builds.append((time.perf_counter() - tb) * 1e3)
     build_ms = min(builds)
     print(f"[bench] BVHs built in {build_ms:.1f} ms (both sides)", file=sys.stderr, flush=True)
+    # segment occupancy bits of both sides (one-time per BVH, like the builds)
+    to = time.perf_counter()
+    occ_a, occ_b = seg_occupancy_prepare(ctx, ba), seg_occupancy_prepare(ctx, bb)
+    ctx.sync()
+    occ_ms = (time.perf_counter() - to) * 1e3
+    print(f"[bench] occupancy bits in {occ_ms:.1f} ms: {occ_a} / {occ_b}", file=sys.stderr, flush=True)
     for _ in range(max(args.warmup, 1)):
         pairs = overlap_join(ctx, ba, dea, bb, deb, mode=1)
     steps = max(1, min(args.steps, 10))
@@ -610,13 +618,21 @@ def run_overlap(args):
     kern_ms, kern_n = ctx.profile_read()
     ctx.profile(False)
     edge_ms = kern_ms / steps  # both directions
-    # canonical algorithmic bytes: counted over leaf-4 trees whatever tree the timed run used
+    # algorithmic bytes, counted over leaf-4 trees whatever tree the timed run used: per ray
+    # 24 B segment + 8 B source range + 4 B per occupancy word read; per walked ray 4 B per
+    # source reference, 64 B per node and per triangle visited, 16 B per hash insert. The
+    # same walk without the occupancy test (every ray walks) is reported beside it.
     ca, cb = Bvh(ctx, *dA, leaf_size=4), Bvh(ctx, *dB, leaf_size=4)
     s1, s2 = edge_stats(ctx, cb, dea), edge_stats(ctx, ca, deb)
+    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 0)
+    w1, w2 = edge_stats(ctx, cb, dea), edge_stats(ctx, ca, deb)
+    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 1)
     ca.free()
     cb.free()
-    alg = sum(32 * s["rays"] + 4 * s["source_refs"] + 64 * s["nodes_visited"] + 64 * s["tris_tested"]
-              + 16 * s["pair_emits"] for s in (s1, s2))
+    alg = sum(32 * s["rays"] + 4 * s["occ_words"] + 4 * s["source_refs"] + 64 * s["nodes_visited"]
+              + 64 * s["tris_tested"] + 16 * s["pair_emits"] for s in (s1, s2))
+    walk_canonical = sum(32 * s["rays"] + 4 * s["source_refs"] + 64 * s["nodes_visited"] + 64 * s["tris_tested"]
+                         + 16 * s["pair_emits"] for s in (w1, w2))
     rays = ea.num_edges + eb.num_edges
     achieved = alg / (edge_ms / 1e3) / 1e9
     # symmetry check at full size: every pair's objects exist on their sides
@@ -670,7 +686,11 @@ def run_overlap(args):
                    "edges": [ea.num_edges, eb.num_edges], "seed": args.seed + 10},
         "pairs": int(len(pairs)),
         "pairs_per_s": round(len(pairs) / t_join, 1),
-        "pairs_per_s_incl_build": round(len(pairs) / (t_join + build_ms / 1e3), 1),
+        "pairs_per_s_incl_build": round(len(pairs) / (t_join + (build_ms + occ_ms) / 1e3), 1),
+        "occupancy_ms": round(occ_ms, 3),
+        "occupancy": [occ_a, occ_b],
+        "rays_walked": [s1["rays_walked"], s2["rays_walked"]],
+        "walk_canonical_bytes_per_join": int(walk_canonical),
         "build_ms": round(build_ms, 3),
         "edge_order_ms": round(edge_order_ms, 3),
         "edge_kernel_ms": round(edge_ms, 4),

@@ -64,9 +64,14 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * caller's order (no reordering) with RS3D_OPT_GRID_RAYS_PER_WAVE rays per wave run
  * (default 1024; 64..2^20). RS3D_OPT_GRID_CELL_PCT: cell edge of grids built afterwards,
  * in percent of the triangles' mean extent (default 50; 10..1000; widened while the grid
- * would hold more than 12 records per triangle). Results are unchanged. */
+ * would hold more than 12 records per triangle). Results are unchanged.
+ * RS3D_OPT_SEG_OCCUPANCY: 1 (default) lets edge rays first test the target's segment
+ * occupancy bits (see rs3d_seg_occupancy_prepare); 0 always walks the BVH. Results are
+ * unchanged. RS3D_OPT_SEG_CELL_PCT: their cell edge for bits built afterwards, in percent
+ * of the triangles' mean extent (default 200; 25..10000). */
 enum { RS3D_OPT_RAY_SORT_MIN = 1, RS3D_OPT_RAYS_PER_WAVE = 2, RS3D_OPT_AXIS_GRID = 3,
-       RS3D_OPT_GRID_RAYS_PER_WAVE = 4, RS3D_OPT_GRID_CELL_PCT = 5 };
+       RS3D_OPT_GRID_RAYS_PER_WAVE = 4, RS3D_OPT_GRID_CELL_PCT = 5, RS3D_OPT_SEG_OCCUPANCY = 6,
+       RS3D_OPT_SEG_CELL_PCT = 7 };
 int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value);
 int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value);
 
@@ -209,12 +214,31 @@ int rs3d_edge_count(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
  * No reference counterpart (measurement of the algorithmic bytes). */
 typedef struct {
     uint64_t rays, nodes_visited, tris_tested, hits, restarts, pair_emits, source_refs;
+    uint64_t rays_walked;    /* rays that passed the occupancy test (all, without bits) */
+    uint64_t occ_words;      /* occupancy words read */
 } rs3d_edge_stats_t;
 int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
                     rs3d_edge_stats_t* stats /* host */);
 /* Pass 2: writes raw pairs at offsets[e] (exclusive scan of counts). swap: emit (tgt, src). */
 int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges,
                     const int64_t* offsets, int swap, rs3d_pair* out);
+
+/* Segment occupancy bits (no counterpart in the reference, whose rays run on RT cores): one
+ * bit per cube cell of a uniform 3-D grid over the target BVH's padded bounds, set where a
+ * padded triangle box (the BVH's leaf box) overlaps the cell. An edge ray whose segment
+ * box (widened by 2^-14 of its coordinates) covers no set bit cannot hit the target and
+ * skips the walk; results are unchanged. Built once per BVH by the first edge launch
+ * against it, or here; freed with the BVH. state: 1 built, 2 rejected (a triangle box over
+ * more than 4096 cells, or non-finite bounds: edge rays walk the tree). */
+typedef struct {
+    int state;
+    int cells_x, cells_y, cells_z;
+    double cell_edge;
+    uint64_t occupied_cells;
+    uint64_t device_bytes;
+    double build_ms;
+} rs3d_seg_occupancy_info;
+int rs3d_seg_occupancy_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, rs3d_seg_occupancy_info* info);
 
 /* Spatial order of an edge payload (no counterpart in the reference; for callers that keep
  * edges resident): writes the edge records (start, end, source offset, source count)

@@ -26,6 +26,8 @@ OPT_RAYS_PER_WAVE = 2
 OPT_AXIS_GRID = 3
 OPT_GRID_RAYS_PER_WAVE = 4
 OPT_GRID_CELL_PCT = 5
+OPT_SEG_OCCUPANCY = 6
+OPT_SEG_CELL_PCT = 7
 REQ_CONTAIN = 0
 REQ_OVERLAP = 1
 REQ_INTERSECT = 2
@@ -64,7 +66,14 @@ class TraceStats(C.Structure):
 
 class EdgeStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("rays", "nodes_visited", "tris_tested", "hits",
-                                           "restarts", "pair_emits", "source_refs")]
+                                           "restarts", "pair_emits", "source_refs", "rays_walked",
+                                           "occ_words")]
+
+
+class SegOccupancyInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("cells_x", C.c_int32), ("cells_y", C.c_int32), ("cells_z", C.c_int32),
+                ("cell_edge", C.c_double), ("occupied_cells", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("build_ms", C.c_double)]
 
 
 class TriJoinStats(C.Structure):
@@ -153,6 +162,7 @@ SIGNATURES = {
     "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_axis_grid_prepare": (I32, [P, P, I32, C.POINTER(AxisGridInfo)]),
+    "rs3d_seg_occupancy_prepare": (I32, [P, P, C.POINTER(SegOccupancyInfo)]),
     "rs3d_axis_grid_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),
     "rs3d_extract_edges": (I32, [P, P, U64, P, U64, P, P, P, P, P, P, U64, U64, PU64, PU64]),

@@ -141,7 +141,7 @@ __device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
                 const int nsrc = q.E.src_cnt[i];
                 const int soff = q.E.src_off[i];
                 const uint64_t qk = (uint64_t)q.qid << 48;
-                edge_ray<DEPTH, false>(q.nodes, q.tris, ld3f(q.E.starts, i), ld3f(q.E.ends, i), CHAIN_EPS,
+                edge_ray<DEPTH, false>(q.nodes, q.tris, q.occ, ld3f(q.E.starts, i), ld3f(q.E.ends, i), CHAIN_EPS,
                                        100, stack, ctr, [&](int tgt) {
                                            for (int s = 0; s < nsrc; ++s) {
                                                const int src = q.E.src_obj[soff + s];
@@ -233,6 +233,7 @@ int BatchPlan::add_edges(const rs3d_bvh* target, const rs3d_edges* e, int swap, 
     BatchDesc d{};
     RS3D_TRY(base(d, target, BQ_EDGES, qid));
     d.E = EdgeArgs{e->starts, e->ends, e->src_offsets, e->src_counts, e->src_objects, e->num_edges};
+    RS3D_TRY(seg_occ_args(ctx_, const_cast<rs3d_bvh*>(target), &d.occ));
     d.n = e->num_edges;
     d.swap = swap;
     items.push_back(d);
@@ -691,6 +692,8 @@ int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_qu
     p.opt_axis_grid = ctx->opt_axis_grid;
     p.opt_grid_rays_per_wave = ctx->opt_grid_rays_per_wave;
     p.opt_grid_cell_pct = ctx->opt_grid_cell_pct;
+    p.opt_seg_occupancy = ctx->opt_seg_occupancy;
+    p.opt_seg_cell_pct = ctx->opt_seg_cell_pct;
     if (hipHostMalloc((void**)&p.pinned, CTX_PINNED_BYTES, hipHostMallocDefault) != hipSuccess) {
         delete b;
         set_error("rs3d_batch_create: hipHostMalloc failed");

@@ -27,7 +27,8 @@ struct BatchDesc {
     int* cand_ids;
     uint8_t* cand_par;
     int max_targets;
-    AGArgs grid;        // containment on the BVH's axis grid when grid.start is set
+    AGArgs grid;        // containment on the BVH's axis grid when grid.cells is set
+    OccArgs occ;        // edge rays: the target's segment occupancy bits (bits null: none)
     uint64_t n;         // rays
     uint64_t ray_base;  // offset of this query's rays in the reorder (walk queries only)
     uint32_t task_rays; // rays per task (TASK_RAYS, or GRID_TASK_RAYS on an axis grid)

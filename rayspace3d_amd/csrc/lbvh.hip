@@ -621,6 +621,7 @@ int rs3d_bvh_free(rs3d_bvh* bvh) {
     if (bvh->nodes) (void)hipFree(bvh->nodes);
     if (bvh->tris) (void)hipFree(bvh->tris);
     for (int a = 0; a < 3; ++a) axis_grid_free(bvh->agrid[a]);
+    seg_occ_free(bvh->socc);
     delete bvh;
     return RS3D_OK;
 }

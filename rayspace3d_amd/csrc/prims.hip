@@ -790,6 +790,22 @@ int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value) {
         ctx->opt_grid_rays_per_wave = value;
         return RS3D_OK;
     }
+    if (option == RS3D_OPT_SEG_OCCUPANCY) {
+        if (value != 0 && value != 1) {
+            set_error("RS3D_OPT_SEG_OCCUPANCY must be 0 or 1, got %lld", (long long)value);
+            return RS3D_E_ARG;
+        }
+        ctx->opt_seg_occupancy = value;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_SEG_CELL_PCT) {
+        if (value < 25 || value > 10000) {
+            set_error("RS3D_OPT_SEG_CELL_PCT must be in [25, 10000], got %lld", (long long)value);
+            return RS3D_E_ARG;
+        }
+        ctx->opt_seg_cell_pct = value;
+        return RS3D_OK;
+    }
     if (option == RS3D_OPT_GRID_CELL_PCT) {
         if (value < 10 || value > 1000) {
             set_error("RS3D_OPT_GRID_CELL_PCT must be in [10, 1000], got %lld", (long long)value);
@@ -833,6 +849,14 @@ int rs3d_ctx_get_option(const rs3d_ctx* ctx, int option, int64_t* value) {
     }
     if (option == RS3D_OPT_GRID_CELL_PCT) {
         *value = ctx->opt_grid_cell_pct;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_SEG_OCCUPANCY) {
+        *value = ctx->opt_seg_occupancy;
+        return RS3D_OK;
+    }
+    if (option == RS3D_OPT_SEG_CELL_PCT) {
+        *value = ctx->opt_seg_cell_pct;
         return RS3D_OK;
     }
     set_error("rs3d_ctx_get_option: unknown option %d", option);

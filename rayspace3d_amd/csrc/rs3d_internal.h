@@ -239,6 +239,8 @@ struct rs3d_ctx {
     int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid
     int64_t opt_grid_rays_per_wave = 1024;  // run length of grid point launches
     int64_t opt_grid_cell_pct = 50;         // grid cell edge, % of the mean triangle extent
+    int64_t opt_seg_occupancy = 1;          // edge rays test the target's occupancy bits first
+    int64_t opt_seg_cell_pct = 200;         // occupancy cell edge, % of the mean triangle extent
     uint64_t* pinned = nullptr;  // small pinned host buffer for counts
     // traversal-kernel timing (rs3d_ctx_profile)
     int profile = 0;
@@ -251,12 +253,16 @@ struct rs3d_ctx {
 namespace rs3d {
 struct AxisGrid;
 void axis_grid_free(AxisGrid* g);
+struct SegOcc;
+void seg_occ_free(SegOcc* g);
 }  // namespace rs3d
 
 struct rs3d_bvh {
     rs3d_ctx* ctx = nullptr;
     rs3d::AxisGrid* agrid[3] = {nullptr, nullptr, nullptr};  // per ray axis, built lazily
     int agrid_state[3] = {0, 0, 0};                           // 0 untried, 1 built, 2 rejected
+    rs3d::SegOcc* socc = nullptr;  // segment occupancy bits, built lazily
+    int socc_state = 0;            // 0 untried, 1 built, 2 rejected
     uint64_t num_tris = 0;
     uint64_t num_nodes = 0;
     int leaf_size = 4;
@@ -294,6 +300,10 @@ int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready);
 // accumulator) accumulates candidates << 32 | inside.
 int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                             const float* box, int32_t* out_obj, unsigned long long* counts);
+// Segment occupancy bits of a BVH for edge rays (built here on first use when the context
+// allows them); *args gets bits = null when there are none.
+struct OccArgs;
+int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args);
 // Point launch on the grid (axis_grid_ready said yes).
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                      const uint32_t* imap, int out_kind, void* out);

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, uint6
 // ------------------------------------------------------------------------------------
 template <int MODE, int DEPTH, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ nodes,
-                                                    const TriRec* __restrict__ tris, EdgeArgs E,
+                                                    const TriRec* __restrict__ tris, OccArgs occ, EdgeArgs E,
                                                     int swap, int chain_rule, int max_iter,
                                                     int32_t* __restrict__ counts,
                                                     const int64_t* __restrict__ offsets,
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
         int64_t cursor = 0;
         if (MODE == EMODE_WRITE_PAIRS || MODE == EMODE_WRITE_KEYS) cursor = offsets[e] + offset_bias;
         const int hits = edge_ray<DEPTH, STATS>(
-            nodes, tris, ld3f(E.starts, e), ld3f(E.ends, e), chain_rule, max_iter,
+            nodes, tris, occ, ld3f(E.starts, e), ld3f(E.ends, e), chain_rule, max_iter,
             stack_mem + threadIdx.x, ctr, [&](int tgt) {
                 for (int q = 0; q < nsrc; ++q) {
                     const int src = E.src_obj[soff + q];
@@ -133,7 +133,18 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
             });
         if (MODE == EMODE_COUNT) counts[e] = hits * nsrc;
     }
-    if (STATS) flush_counters(ctr, stats);
+    if (STATS) {
+        flush_counters(ctr, stats);
+        uint64_t w = ctr.walked, o = ctr.occ_words;
+        for (int off = 32; off > 0; off >>= 1) {
+            w += __shfl_xor(w, off, 64);
+            o += __shfl_xor(o, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats[4], (unsigned long long)w);
+            atomicAdd(&stats[5], (unsigned long long)o);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ v, uint64_t n,
@@ -332,12 +343,14 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     }
     int depth;
     RS3D_TRY(pick_depth(bvh, &depth));
+    OccArgs occ;
+    RS3D_TRY(seg_occ_args(ctx, const_cast<rs3d_bvh*>(bvh), &occ));
     if (stats) {
         switch (depth) {
-            case 24: k_edges<MODE, 24, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
-            case 32: k_edges<MODE, 32, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
-            case 48: k_edges<MODE, 48, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
-            default: k_edges<MODE, 64, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            case 24: k_edges<MODE, 24, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            case 32: k_edges<MODE, 32, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            case 48: k_edges<MODE, 48, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
+            default: k_edges<MODE, 64, true><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, stats); break;
         }
         RS3D_HIP(hipGetLastError());
         return RS3D_OK;
@@ -347,10 +360,10 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
         RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     }
     switch (depth) {
-        case 24: k_edges<MODE, 24, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
-        case 32: k_edges<MODE, 32, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
-        case 48: k_edges<MODE, 48, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
-        default: k_edges<MODE, 64, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
+        case 24: k_edges<MODE, 24, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
+        case 32: k_edges<MODE, 32, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
+        case 48: k_edges<MODE, 48, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
+        default: k_edges<MODE, 64, false><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule, max_iter, counts, offsets, bias, out, H, nullptr); break;
     }
     RS3D_HIP(hipGetLastError());
     if (ctx->profile) {
@@ -436,7 +449,7 @@ int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
     RS3D_TRY(ctx->small.ensure(256));
     RS3D_TRY(ctx->tmp_a.ensure(n * sizeof(int32_t)));
     unsigned long long* dstats = ctx->small.as<unsigned long long>() + 16;
-    RS3D_HIP(hipMemsetAsync(dstats, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    RS3D_HIP(hipMemsetAsync(dstats, 0, 6 * sizeof(unsigned long long), ctx->stream));
     HashArgs H{};
     int32_t* counts = ctx->tmp_a.as<int32_t>();
     RS3D_TRY(launch_edges<EMODE_COUNT>(ctx, target, edges, 0, counts, nullptr, 0, nullptr, H,
@@ -449,19 +462,19 @@ int rs3d_edge_stats(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
     k_sum_i32<<<nb, 256, 0, ctx->stream>>>(edges->src_counts, n, refs);
     RS3D_HIP(hipGetLastError());
     unsigned long long h[6];
-    RS3D_HIP(hipMemcpyAsync(h, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    RS3D_HIP(hipMemcpyAsync(h, dstats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     RS3D_HIP(hipStreamSynchronize(ctx->stream));
     uint64_t sums[2];
     RS3D_TRY(acc_read_sum(ctx, emits, &sums[0]));
     RS3D_TRY(acc_read_sum(ctx, refs, &sums[1]));
-    h[4] = sums[0];
-    h[5] = sums[1];
     stats->nodes_visited = h[0];
     stats->tris_tested = h[1];
     stats->hits = h[2];
     stats->restarts = h[3];
-    stats->pair_emits = h[4];
-    stats->source_refs = h[5];
+    stats->rays_walked = h[4];
+    stats->occ_words = h[5];
+    stats->pair_emits = sums[0];
+    stats->source_refs = sums[1];
     return RS3D_OK;
 }
 

@@ -147,7 +147,51 @@ __device__ __forceinline__ void load_tri(const TriRec* __restrict__ tris, int j,
 
 struct TraceCounters {
     uint64_t nodes = 0, tris = 0, hits = 0, restarts = 0;
+    uint64_t walked = 0, occ_words = 0;  // segment rays: walks after the occupancy test, words read
 };
+
+// ------------------------------------------------------------------------------------
+// Segment occupancy bits of a target BVH: one bit per cube cell of a uniform 3-D grid over
+// its padded bounds, set where some padded triangle box (the BVH's leaf box) overlaps the
+// cell. A segment whose (slightly widened) bounding box covers no set bit cannot hit any
+// triangle: a reported hit lies, up to rounding far below the widening, on its triangle,
+// inside the padded box, and on the segment, inside the segment's box, so the two boxes
+// meet and so do their cell ranges (cells come from one monotone clamped map for both).
+// ------------------------------------------------------------------------------------
+struct OccArgs {
+    const uint32_t* bits;  // null: no occupancy test
+    float org_x, org_y, org_z, inv;
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ int occ_cell(float x, float org, float inv, int n) {
+    const float f = fminf(fmaxf((x - org) * inv, 0.0f), (float)(n - 1));
+    return (int)f;
+}
+
+// Whether segment a-b may hit a triangle of the occupancy bits' BVH. Segments covering
+// more than 64 cells are not tested (answer true). words: occupancy words read.
+__device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b,
+                                                    uint64_t& words) {
+    const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + 1e-30f;  // 2^-14
+    const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + 1e-30f;
+    const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + 1e-30f;
+    const int x0 = occ_cell(fminf(a.x, b.x) - wa, g.org_x, g.inv, g.nx);
+    const int x1 = occ_cell(fmaxf(a.x, b.x) + wa, g.org_x, g.inv, g.nx);
+    const int y0 = occ_cell(fminf(a.y, b.y) - wb, g.org_y, g.inv, g.ny);
+    const int y1 = occ_cell(fmaxf(a.y, b.y) + wb, g.org_y, g.inv, g.ny);
+    const int z0 = occ_cell(fminf(a.z, b.z) - wc, g.org_z, g.inv, g.nz);
+    const int z1 = occ_cell(fmaxf(a.z, b.z) + wc, g.org_z, g.inv, g.nz);
+    if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 64) return true;
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) {
+                const uint64_t c = ((uint64_t)z * g.ny + y) * g.nx + x;
+                ++words;
+                if ((g.bits[c >> 5] >> (c & 31)) & 1u) return true;
+            }
+    return false;
+}
 
 // ------------------------------------------------------------------------------------
 // All-hits traversal for an axis-aligned ray (+AX). Hits with thr <= t < tmax_excl are
@@ -416,17 +460,24 @@ enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
 // the number of chain hits.
 template <int DEPTH, bool STATS, typename Emit>
 __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
-                                        const TriRec* __restrict__ tris, const float3 s,
-                                        const float3 en, int chain_rule, int max_iter, int* stack,
-                                        TraceCounters& ctr, Emit&& emit) {
+                                        const TriRec* __restrict__ tris, const OccArgs& occ,
+                                        const float3 s, const float3 en, int chain_rule, int max_iter,
+                                        int* stack, TraceCounters& ctr, Emit&& emit) {
     const float eps = 1e-6f;
     const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
     if (!(len >= eps)) return 0;
     const float3 d = make_float3(dx / len, dy / len, dz / len);
+    const float tmax = len + eps;
+    // hits lie on s + t d with t in [eps, tmax]: the segment s .. s + tmax d (its box is
+    // widened by 2^-14 of the coordinates' magnitude inside the test)
+    if (occ.bits &&
+        !occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax),
+                             ctr.occ_words))
+        return 0;
+    if (STATS) ctr.walked++;
     const RayPre rp = ray_pre(d);
     const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const float tmax = len + eps;
     float thr = eps;
     int iter = 0;
     float bt[KBUF];

@@ -186,6 +186,15 @@ def axis_grid_prepare(ctx: Context, bvh: Bvh, axis: int = 2) -> dict:
     return {k: getattr(info, k) for k, _ in info._fields_}
 
 
+def seg_occupancy_prepare(ctx: Context, bvh: Bvh) -> dict:
+    """Builds (once) the segment occupancy bits of `bvh` (rs3d_seg_occupancy_prepare);
+    state 1 = built (edge rays test them first), 2 = rejected (edge rays walk the tree)."""
+    info = _lib.SegOccupancyInfo()
+    torch.cuda.synchronize()
+    check(ctx.lib.rs3d_seg_occupancy_prepare(ctx.handle, bvh.handle, C.byref(info)), "rs3d_seg_occupancy_prepare")
+    return {k: getattr(info, k) for k, _ in info._fields_}
+
+
 def axis_grid_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> dict:
     """Work counters of a point launch on the axis grid: nodes_visited = cell lookups,
     tris_tested = 64-byte records read."""

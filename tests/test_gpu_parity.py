@@ -431,9 +431,73 @@ def test_edge_stats_counters(ctx):
     assert st["rays"] == a.edges.num_edges
     assert st["pair_emits"] == int(counts.sum()) > 0
     assert st["source_refs"] == int(a.edges.src_counts.sum())
-    assert st["nodes_visited"] >= st["rays"] and st["tris_tested"] >= st["hits"] > 0
+    assert 0 < st["rays_walked"] <= st["rays"] and st["occ_words"] > 0
+    assert st["nodes_visited"] >= st["rays_walked"] and st["tris_tested"] >= st["hits"] > 0
     ba.free()
     bb.free()
+
+
+def _crafted_segments(mesh, rng, n_tris=3000):
+    """Segments against the triangles of `mesh` that sit on the boundaries of the hit rules:
+    ending on a triangle (t = len, inside len + 1e-6), ending just short of it, starting on
+    it (t = 0, below 1e-6), along an edge (coplanar), through a vertex, grazing the plane,
+    plus short random segments near the surface and long ones across the whole field."""
+    T = mesh.vertices[mesh.indices[rng.integers(0, len(mesh.indices), n_tris)]].astype(np.float32)
+    v0, v1, v2 = T[:, 0], T[:, 1], T[:, 2]
+    c = ((v0 + v1 + v2) / np.float32(3)).astype(np.float32)
+    nrm = np.cross(v1 - v0, v2 - v0).astype(np.float32)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30).astype(np.float32)
+    off = np.float32(0.3)
+    S, E = [], []
+    S.append(c + off * nrm); E.append(c)                                  # ends on the triangle
+    S.append(c + off * nrm); E.append(c + np.float32(1e-7) * nrm)         # ends 1e-7 short
+    S.append(c); E.append(c + off * nrm)                                  # starts on it
+    S.append(v0); E.append(v1)                                            # along an edge
+    S.append(v0 + off * nrm); E.append(v0 - off * nrm)                    # through a vertex
+    S.append(c + np.float32(1e-6) * nrm); E.append(c + np.float32(1e-6) * nrm + (v1 - v0))  # grazing
+    S.append(c + rng.normal(0, 0.5, (n_tris, 3)).astype(np.float32))
+    E.append(S[-1] + rng.normal(0, 0.5, (n_tris, 3)).astype(np.float32))  # short, near the surface
+    lo, hi = mesh.vertices.min(0), mesh.vertices.max(0)
+    a = rng.uniform(lo, hi, (200, 3)).astype(np.float32)
+    S.append(a); E.append(rng.uniform(lo, hi, (200, 3)).astype(np.float32))  # long: no occupancy test
+    S = np.ascontiguousarray(np.concatenate(S), np.float32)
+    E = np.ascontiguousarray(np.concatenate(E), np.float32)
+    n = len(S)
+    return gen.EdgeSet(S, E, np.arange(n, dtype=np.int32), np.ones(n, np.int32), np.zeros(n, np.int32))
+
+
+@pytest.mark.parametrize("pct", [200, 25, 1000])
+def test_seg_occupancy_same_hits(ctx, pct):
+    # edge rays with the occupancy test (fine, default and coarse cells) against the walk
+    # alone and the oracle, on field edges and on segments at the hit rules' boundaries
+    from rayspace3d_amd.device import edge_stats, seg_occupancy_prepare
+    a, b = _overlap_case(73, 300, 120.0)
+    rng = np.random.Generator(np.random.PCG64(pct))
+    crafted = _crafted_segments(b, rng)
+    saved = (ctx.get_option(_lib.OPT_SEG_OCCUPANCY), ctx.get_option(_lib.OPT_SEG_CELL_PCT))
+    try:
+        ctx.set_option(_lib.OPT_SEG_CELL_PCT, pct)
+        bb = Bvh.from_mesh(ctx, b)
+        info = seg_occupancy_prepare(ctx, bb)
+        print(f"occupancy bits, cell {pct} %: {info}", flush=True)
+        assert info["state"] == 1 and 0 < info["occupied_cells"] <= info["cells_x"] * info["cells_y"] * info["cells_z"]
+        for es in (a.edges, crafted):
+            de = DevEdges(es)
+            ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 1)
+            on = to_host(edge_count(ctx, bb, de)).copy()
+            st = edge_stats(ctx, bb, de)
+            ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 0)
+            off = to_host(edge_count(ctx, bb, de)).copy()
+            _, ref = orc.edge_pairs(es, b.vertices, b.indices, b.tri_to_obj, with_counts=True)
+            assert np.array_equal(on, off) and np.array_equal(on, ref), np.count_nonzero(on != ref)
+            assert st["rays_walked"] <= st["rays"]
+            if pct <= 200:
+                assert st["rays_walked"] < st["rays"]
+        assert int(ref.sum()) > 100  # the crafted segments do hit
+        bb.free()
+    finally:
+        ctx.set_option(_lib.OPT_SEG_OCCUPANCY, saved[0])
+        ctx.set_option(_lib.OPT_SEG_CELL_PCT, saved[1])
 
 
 def test_batched_equals_per_query(ctx):
