@@ -58,6 +58,11 @@ def main():
     ap.add_argument("--grid-cell-pct", type=int, default=None,
                     help="containment: axis grid cell edge in %% of the mean triangle extent "
                          "(library default if unset)")
+    ap.add_argument("--seg-cell-pct", type=int, default=None,
+                    help="overlap: segment occupancy cell edge in %% of the mean triangle extent "
+                         "(library default if unset)")
+    ap.add_argument("--seg-occupancy", type=int, default=1, choices=[0, 1],
+                    help="overlap: edge rays test the target's occupancy bits first (1) or always walk (0)")
     ap.add_argument("--autotune", action="store_true",
                     help="containment: pick leaf size, reordering and run length with the "
                          "autotuner on the workload's own rays first (the defaults are its "
@@ -463,6 +468,9 @@ def run_trijoin(args):
     print(f"[bench] generated {A.num_triangles}+{B.num_triangles} triangles in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     ctx = Context(0)
+    if args.seg_cell_pct is not None:
+        ctx.set_option(_lib.OPT_SEG_CELL_PCT, args.seg_cell_pct)
+    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, args.seg_occupancy)
     dA = [to_dev(A.vertices), to_dev(A.indices), to_dev(A.tri_to_obj)]
     dB = [to_dev(B.vertices), to_dev(B.indices), to_dev(B.tri_to_obj)]
     torch.cuda.synchronize()
@@ -601,10 +609,13 @@ def run_overlap(args):
     build_ms = min(builds)
     print(f"[bench] BVHs built in {build_ms:.1f} ms (both sides)", file=sys.stderr, flush=True)
     # segment occupancy bits of both sides (one-time per BVH, like the builds)
-    to = time.perf_counter()
-    occ_a, occ_b = seg_occupancy_prepare(ctx, ba), seg_occupancy_prepare(ctx, bb)
-    ctx.sync()
-    occ_ms = (time.perf_counter() - to) * 1e3
+    occ_a = occ_b = None
+    occ_ms = 0.0
+    if args.seg_occupancy:
+        to = time.perf_counter()
+        occ_a, occ_b = seg_occupancy_prepare(ctx, ba), seg_occupancy_prepare(ctx, bb)
+        ctx.sync()
+        occ_ms = (time.perf_counter() - to) * 1e3
     print(f"[bench] occupancy bits in {occ_ms:.1f} ms: {occ_a} / {occ_b}", file=sys.stderr, flush=True)
     for _ in range(max(args.warmup, 1)):
         pairs = overlap_join(ctx, ba, dea, bb, deb, mode=1)
@@ -626,7 +637,7 @@ def run_overlap(args):
     s1, s2 = edge_stats(ctx, cb, dea), edge_stats(ctx, ca, deb)
     ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 0)
     w1, w2 = edge_stats(ctx, cb, dea), edge_stats(ctx, ca, deb)
-    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 1)
+    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, args.seg_occupancy)
     ca.free()
     cb.free()
     alg = sum(32 * s["rays"] + 4 * s["occ_words"] + 4 * s["source_refs"] + 64 * s["nodes_visited"]

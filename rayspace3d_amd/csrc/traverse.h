@@ -458,24 +458,32 @@ enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
 
 // A.2 edge ray from s towards en: calls emit(target object) once per chain hit; returns
 // the number of chain hits.
+// Whether edge s-en may have chain hits: its length reaches 1e-6 and, with occupancy bits,
+// the segment s .. s + tmax d that holds every hit (t in [1e-6, tmax]) covers a set bit
+// (its box is widened by 2^-14 of the coordinates' magnitude inside the test).
+__device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s, const float3 en, uint64_t& words) {
+    const float eps = 1e-6f;
+    const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (!(len >= eps)) return false;
+    if (!occ.bits) return true;
+    const float3 d = make_float3(dx / len, dy / len, dz / len);
+    const float tmax = len + eps;
+    return occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax), words);
+}
+
 template <int DEPTH, bool STATS, typename Emit>
 __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
                                         const TriRec* __restrict__ tris, const OccArgs& occ,
                                         const float3 s, const float3 en, int chain_rule, int max_iter,
                                         int* stack, TraceCounters& ctr, Emit&& emit) {
+    if (!edge_may_hit(occ, s, en, ctr.occ_words)) return 0;
+    if (STATS) ctr.walked++;
     const float eps = 1e-6f;
     const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (!(len >= eps)) return 0;
     const float3 d = make_float3(dx / len, dy / len, dz / len);
     const float tmax = len + eps;
-    // hits lie on s + t d with t in [eps, tmax]: the segment s .. s + tmax d (its box is
-    // widened by 2^-14 of the coordinates' magnitude inside the test)
-    if (occ.bits &&
-        !occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax),
-                             ctr.occ_words))
-        return 0;
-    if (STATS) ctr.walked++;
     const RayPre rp = ray_pre(d);
     const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float thr = eps;
