@@ -468,9 +468,6 @@ def run_trijoin(args):
     print(f"[bench] generated {A.num_triangles}+{B.num_triangles} triangles in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     ctx = Context(0)
-    if args.seg_cell_pct is not None:
-        ctx.set_option(_lib.OPT_SEG_CELL_PCT, args.seg_cell_pct)
-    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, args.seg_occupancy)
     dA = [to_dev(A.vertices), to_dev(A.indices), to_dev(A.tri_to_obj)]
     dB = [to_dev(B.vertices), to_dev(B.indices), to_dev(B.tri_to_obj)]
     torch.cuda.synchronize()
@@ -585,6 +582,9 @@ def run_overlap(args):
     print(f"[bench] generated {A.num_triangles}+{B.num_triangles} triangles, "
           f"{ea.num_edges}+{eb.num_edges} edges in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ctx = Context(0)
+    if args.seg_cell_pct is not None:
+        ctx.set_option(_lib.OPT_SEG_CELL_PCT, args.seg_cell_pct)
+    ctx.set_option(_lib.OPT_SEG_OCCUPANCY, args.seg_occupancy)
     dA = [to_dev(A.vertices), to_dev(A.indices), to_dev(A.tri_to_obj)]
     dB = [to_dev(B.vertices), to_dev(B.indices), to_dev(B.tri_to_obj)]
     # resident edges in Morton order of their midpoints (one-time, like the BVH builds)
