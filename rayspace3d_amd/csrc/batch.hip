@@ -664,7 +664,7 @@ static void batch_release(rs3d_batch* b) {
     b->keys.release();
     rs3d_ctx& p = b->priv;
     for (DevBuf* d : {&p.sort_keys, &p.sort_vals, &p.sort_hist, &p.sort_csum, &p.scan_tmp, &p.tmp_a, &p.tmp_b,
-                      &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals, &p.accs})
+                      &p.tmp_c, &p.small, &p.ray_keys, &p.ray_vals, &p.accs, &p.edge_sel})
         d->release();
     if (p.pinned) (void)hipHostFree(p.pinned);
     p.pinned = nullptr;

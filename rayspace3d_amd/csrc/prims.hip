@@ -744,6 +744,7 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx) {
     ctx->tmp_c.release();
     ctx->small.release();
     ctx->accs.release();
+    ctx->edge_sel.release();
     ctx->ray_keys.release();
     ctx->ray_vals.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

@@ -146,21 +146,33 @@ __device__ __forceinline__ void flush_edge_counters(const TraceCounters& ctr, un
 }
 
 // Edge rays (A.2), one per lane: segment from start towards end, hits in [thr, len + 1e-6].
+// With list, lane i runs edge list[i] of n_list; otherwise edge i of E.
 template <int MODE, int DEPTH, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, OccArgs occ, EdgeArgs E,
+                                                    const uint32_t* __restrict__ list, uint64_t n_list,
                                                     int swap, int chain_rule, int max_iter,
                                                     int32_t* __restrict__ counts,
                                                     const int64_t* __restrict__ offsets,
                                                     int64_t offset_bias, void* __restrict__ out,
                                                     HashArgs H, unsigned long long* __restrict__ stats) {
     __shared__ int stack_mem[DEPTH * TR_BLOCK];
-    const uint64_t e = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
     TraceCounters ctr;
-    if (e < E.num_edges)
-        edge_one<MODE, DEPTH, STATS>(nodes, tris, occ, E, e, swap, chain_rule, max_iter, counts, offsets,
-                                     offset_bias, out, H, stack_mem + threadIdx.x, ctr);
+    if (i < (list ? n_list : E.num_edges))
+        edge_one<MODE, DEPTH, STATS>(nodes, tris, occ, E, list ? list[i] : i, swap, chain_rule, max_iter, counts,
+                                     offsets, offset_bias, out, H, stack_mem + threadIdx.x, ctr);
     flush_edge_counters<STATS>(ctr, stats);
+}
+
+// Edge selection against occupancy bits: sel[e] = 1 where edge e may hit (edge_may_hit).
+// A light kernel (no LDS stack, few registers) keeps many waves of these random bit reads
+// in flight; the walk then runs only over the selected edges, in full waves.
+__global__ __launch_bounds__(256) void k_edge_select(EdgeArgs E, OccArgs occ, uint8_t* __restrict__ sel) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E.num_edges) return;
+    uint64_t words = 0;
+    sel[i] = edge_may_hit(occ, ld3f(E.starts, i), ld3f(E.ends, i), words) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ v, uint64_t n,
@@ -361,15 +373,20 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     RS3D_TRY(pick_depth(bvh, &depth));
     OccArgs occ;
     RS3D_TRY(seg_occ_args(ctx, const_cast<rs3d_bvh*>(bvh), &occ));
-    // One ray per lane, edges in the caller's order: in Morton order the rays that pass the
-    // occupancy test cluster in few waves (compacting them into full waves through an LDS
-    // queue was measured slower: 37.4 vs 34.0 ms on the overlap join).
+    // One ray per lane. With occupancy bits (timed launches): a selection pass, a stable
+    // compaction of the selected edges, and the walk over them alone (the counting pass
+    // keeps the one-kernel form, which counts the words read).
+    const uint32_t* list = nullptr;
+    uint64_t n_list = 0;
+    unsigned nbl = nb;
     auto launch = [&](auto depth_tag, auto stats_tag) {
         constexpr int D = decltype(depth_tag)::value;
         constexpr bool ST = decltype(stats_tag)::value;
         unsigned long long* st = ST ? stats : nullptr;
-        k_edges<MODE, D, ST><<<nb, TR_BLOCK, 0, ctx->stream>>>(bvh->nodes, bvh->tris, occ, E, swap, chain_rule,
-                                                              max_iter, counts, offsets, bias, out, H, st);
+        const OccArgs none{};
+        k_edges<MODE, D, ST><<<list ? nbl : nb, TR_BLOCK, 0, ctx->stream>>>(
+            bvh->nodes, bvh->tris, list ? none : occ, E, list, n_list, swap, chain_rule, max_iter, counts, offsets,
+            bias, out, H, st);
     };
     auto launch_depth = [&](auto stats_tag) {
         switch (depth) {
@@ -388,7 +405,22 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
         RS3D_TRY(prof_collect(ctx));
         RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     }
-    launch_depth(std::integral_constant<bool, false>{});
+    if (occ.bits && edges->num_edges <= 0xffffffffull) {
+        const uint64_t n = edges->num_edges;
+        const size_t sel_bytes = (n + 255) & ~(size_t)255;
+        RS3D_TRY(ctx->edge_sel.ensure(sel_bytes + n * sizeof(uint32_t)));
+        uint8_t* sel = ctx->edge_sel.as<uint8_t>();
+        uint32_t* idx = reinterpret_cast<uint32_t*>(sel + sel_bytes);
+        unsigned nbs;
+        RS3D_TRY(grid_blocks(n, 256, &nbs));
+        k_edge_select<<<nbs, 256, 0, ctx->stream>>>(E, occ, sel);
+        RS3D_HIP(hipGetLastError());
+        if (MODE == EMODE_COUNT) RS3D_TRY(fill_u32(ctx, counts, n, 0u));  // unselected edges: no hits
+        RS3D_TRY(rs3d_compact_flags(ctx, sel, n, idx, &n_list));
+        list = idx;
+        RS3D_TRY(grid_blocks(n_list ? n_list : 1, TR_BLOCK, &nbl));
+    }
+    if (!list || n_list) launch_depth(std::integral_constant<bool, false>{});
     RS3D_HIP(hipGetLastError());
     if (ctx->profile) {
         RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
