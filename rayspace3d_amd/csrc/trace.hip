@@ -373,9 +373,9 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     RS3D_TRY(pick_depth(bvh, &depth));
     OccArgs occ;
     RS3D_TRY(seg_occ_args(ctx, const_cast<rs3d_bvh*>(bvh), &occ));
-    // One ray per lane. With occupancy bits (timed launches): a selection pass, a stable
-    // compaction of the selected edges, and the walk over them alone (the counting pass
-    // keeps the one-kernel form, which counts the words read).
+    // One ray per lane. With occupancy bits (large timed launches): a selection pass, a
+    // stable compaction of the selected edges, and the walk over them alone (the counting
+    // pass keeps the one-kernel form, which counts the words read).
     const uint32_t* list = nullptr;
     uint64_t n_list = 0;
     unsigned nbl = nb;
@@ -405,7 +405,9 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
         RS3D_TRY(prof_collect(ctx));
         RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     }
-    if (occ.bits && edges->num_edges <= 0xffffffffull) {
+    // (below 2^20 edges the selection's extra launches and host read cost more than they
+    // save: the walking kernel tests the bits itself)
+    if (occ.bits && edges->num_edges >= (1ull << 20) && edges->num_edges <= 0xffffffffull) {
         const uint64_t n = edges->num_edges;
         const size_t sel_bytes = (n + 255) & ~(size_t)255;
         RS3D_TRY(ctx->edge_sel.ensure(sel_bytes + n * sizeof(uint32_t)));
