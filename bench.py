@@ -459,7 +459,8 @@ def run_trijoin(args):
     import torch
 
     from rayspace3d_amd import gen
-    from rayspace3d_amd.device import Bvh, Context, DevEdges, overlap_join, to_dev, tri_join, tri_join_stats
+    from rayspace3d_amd.device import (Bvh, Context, DevEdges, overlap_join, seg_occupancy_prepare, to_dev,
+                                       tri_join, tri_join_stats)
 
     n_obj = args.overlap_objects
     t0 = time.time()
@@ -474,6 +475,12 @@ def run_trijoin(args):
     ba = Bvh(ctx, *dA, leaf_size=args.leaf_size)
     bb = Bvh(ctx, *dB, leaf_size=args.leaf_size)
     ctx.sync()
+    # B's occupancy bits (one-time per BVH, outside the timed joins): A triangles whose box
+    # covers no set bit skip the walk
+    to = time.perf_counter()
+    occ_b = seg_occupancy_prepare(ctx, bb)
+    ctx.sync()
+    occ_ms = (time.perf_counter() - to) * 1e3
     for _ in range(max(args.warmup, 1)):
         pairs = tri_join(ctx, ba, bb)
     steps = max(1, min(args.steps, 10))
@@ -493,10 +500,12 @@ def run_trijoin(args):
     st = tri_join_stats(ctx, ca, cb)
     ca.free()
     cb.free()
-    # compulsory bytes: every query record, B's tree once, one hash probe per hit; the
-    # per-visit fetches (64 B per node or triangle record visited) mostly hit in L2/MALL
+    # compulsory bytes: every query record, B's occupancy bits and tree once, one hash probe
+    # per hit; the per-visit fetches (64 B per node or triangle record visited) mostly hit
+    # in L2/MALL
     inf_b = bb.info()
-    alg = 48 * A.num_triangles + 64 * inf_b.num_nodes + 48 * B.num_triangles + 16 * st["hits"]
+    alg = (48 * A.num_triangles + int(occ_b["device_bytes"]) + 64 * inf_b.num_nodes + 48 * B.num_triangles
+           + 16 * st["hits"])
     visit_bytes = 64 * st["nodes_visited"] + 64 * st["tris_loaded"]
     achieved = alg / (k_ms / 1e3) / 1e9
     traffic = None
@@ -542,6 +551,8 @@ def run_trijoin(args):
         "pairs_per_s": round(len(pairs) / t_join, 1),
         "kernel_ms": round(k_ms, 4),
         "bvh_leaf_size": args.leaf_size,
+        "occupancy_ms": round(occ_ms, 3),
+        "occupancy": occ_b,
         "edge_ray_join_pairs": int(len(epairs)),
         "pairs_only_in_tri_join": only_tri,
         "pairs_only_in_edge_join": only_edge,

@@ -169,19 +169,14 @@ __device__ __forceinline__ int occ_cell(float x, float org, float inv, int n) {
     return (int)f;
 }
 
-// Whether segment a-b may hit a triangle of the occupancy bits' BVH. Segments covering
-// more than 64 cells are not tested (answer true). words: occupancy words read.
-__device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b,
-                                                    uint64_t& words) {
-    const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + 1e-30f;  // 2^-14
-    const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + 1e-30f;
-    const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + 1e-30f;
-    const int x0 = occ_cell(fminf(a.x, b.x) - wa, g.org_x, g.inv, g.nx);
-    const int x1 = occ_cell(fmaxf(a.x, b.x) + wa, g.org_x, g.inv, g.nx);
-    const int y0 = occ_cell(fminf(a.y, b.y) - wb, g.org_y, g.inv, g.ny);
-    const int y1 = occ_cell(fmaxf(a.y, b.y) + wb, g.org_y, g.inv, g.ny);
-    const int z0 = occ_cell(fminf(a.z, b.z) - wc, g.org_z, g.inv, g.nz);
-    const int z1 = occ_cell(fmaxf(a.z, b.z) + wc, g.org_z, g.inv, g.nz);
+// Whether the box [lo, hi] covers a set bit, i.e. may meet a padded triangle box of the
+// bits' BVH. Boxes over more than 64 cells are not tested (answer true). words: occupancy
+// words read.
+__device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 lo, const float3 hi,
+                                                uint64_t& words) {
+    const int x0 = occ_cell(lo.x, g.org_x, g.inv, g.nx), x1 = occ_cell(hi.x, g.org_x, g.inv, g.nx);
+    const int y0 = occ_cell(lo.y, g.org_y, g.inv, g.ny), y1 = occ_cell(hi.y, g.org_y, g.inv, g.ny);
+    const int z0 = occ_cell(lo.z, g.org_z, g.inv, g.nz), z1 = occ_cell(hi.z, g.org_z, g.inv, g.nz);
     if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 64) return true;
     for (int z = z0; z <= z1; ++z)
         for (int y = y0; y <= y1; ++y)
@@ -191,6 +186,17 @@ __device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const floa
                 if ((g.bits[c >> 5] >> (c & 31)) & 1u) return true;
             }
     return false;
+}
+
+// Whether segment a-b may hit a triangle of the bits' BVH: its box, widened by 2^-14 of the
+// coordinates' magnitude, through occ_box_may_hit.
+__device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b,
+                                                    uint64_t& words) {
+    const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + 1e-30f;  // 2^-14
+    const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + 1e-30f;
+    const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + 1e-30f;
+    return occ_box_may_hit(g, make_float3(fminf(a.x, b.x) - wa, fminf(a.y, b.y) - wb, fminf(a.z, b.z) - wc),
+                           make_float3(fmaxf(a.x, b.x) + wa, fmaxf(a.y, b.y) + wb, fmaxf(a.z, b.z) + wc), words);
 }
 
 // ------------------------------------------------------------------------------------

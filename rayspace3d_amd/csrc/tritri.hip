@@ -39,24 +39,26 @@ template <int DEPTH, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) void k_tri_join(const BNode* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris_b,
                                                        const TriRec* __restrict__ tris_a, uint64_t n_a,
-                                                       int swap, HashArgs H,
-                                                       unsigned long long* __restrict__ stats) {
+                                                       const uint32_t* __restrict__ list, OccArgs occ, int swap,
+                                                       HashArgs H, unsigned long long* __restrict__ stats) {
     __shared__ int stack_mem[DEPTH * TR_BLOCK];
     int* stack = stack_mem + threadIdx.x;
     const uint64_t i = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
     TraceCounters ctr;
-    if (i < n_a) {
+    if (i < n_a) {  // with list: query list[i] of n_a listed ones
         float3 a0, a1, a2;
         uint32_t orig_a;
         int obj_a;
-        load_tri(tris_a, (int)i, a0, a1, a2, orig_a, obj_a);
+        load_tri(tris_a, list ? (int)list[i] : (int)i, a0, a1, a2, orig_a, obj_a);
         const float3 qlo = make_float3(fminf(fminf(a0.x, a1.x), a2.x), fminf(fminf(a0.y, a1.y), a2.y),
                                        fminf(fminf(a0.z, a1.z), a2.z));
         const float3 qhi = make_float3(fmaxf(fmaxf(a0.x, a1.x), a2.x), fmaxf(fmaxf(a0.y, a1.y), a2.y),
                                        fmaxf(fmaxf(a0.z, a1.z), a2.z));
         int last = -1;  // object of the last pair inserted by this lane
         int sp = 0, node = 0;
-        while (true) {
+        // a pair needs A's box to meet a B triangle's box, so to cover a set occupancy bit
+        const bool walk = !occ.bits || occ_box_may_hit(occ, qlo, qhi, ctr.occ_words);
+        while (walk) {
             // child 0 box: x (a.x, a.y), y (a.z, a.w), z (b.x, b.y); child 1: x (b.z, b.w),
             // y (c.x, c.y), z (c.z, c.w); link = (first 0, first 1, count 0, count 1)
             const BNode nd = nodes[node];
@@ -116,6 +118,24 @@ __global__ __launch_bounds__(TR_BLOCK) void k_tri_join(const BNode* __restrict__
     if (STATS) flush_counters(ctr, stats);
 }
 
+// Query selection against B's occupancy bits: sel[i] = 1 where A's triangle i has a box
+// covering a set bit (the only queries that can pair).
+__global__ __launch_bounds__(256) void k_tri_select(const TriRec* __restrict__ tris_a, uint64_t n_a, OccArgs occ,
+                                                    uint8_t* __restrict__ sel) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_a) return;
+    float3 a0, a1, a2;
+    uint32_t orig;
+    int obj;
+    load_tri(tris_a, (int)i, a0, a1, a2, orig, obj);
+    const float3 lo = make_float3(fminf(fminf(a0.x, a1.x), a2.x), fminf(fminf(a0.y, a1.y), a2.y),
+                                  fminf(fminf(a0.z, a1.z), a2.z));
+    const float3 hi = make_float3(fmaxf(fmaxf(a0.x, a1.x), a2.x), fmaxf(fmaxf(a0.y, a1.y), a2.y),
+                                  fmaxf(fmaxf(a0.z, a1.z), a2.z));
+    uint64_t words = 0;
+    sel[i] = occ_box_may_hit(occ, lo, hi, words) ? 1 : 0;
+}
+
 template <bool STATS>
 static int launch_tri_join(rs3d_ctx* ctx, const rs3d_bvh* a, const rs3d_bvh* b, int swap, const HashArgs& H,
                            unsigned long long* stats) {
@@ -128,16 +148,37 @@ static int launch_tri_join(rs3d_ctx* ctx, const rs3d_bvh* a, const rs3d_bvh* b, 
     RS3D_TRY(grid_blocks(a->num_tris, TR_BLOCK, &nb));
     int depth;
     RS3D_TRY(pick_depth(b, &depth));
+    OccArgs occ;  // B's segment occupancy bits (built here on first use)
+    RS3D_TRY(seg_occ_args(ctx, const_cast<rs3d_bvh*>(b), &occ));
     const bool timed = !STATS && ctx->profile;
     if (timed) {
         RS3D_TRY(prof_collect(ctx));
         RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     }
+    // Large timed joins with bits: select the queries whose box covers a set bit, compact
+    // them (stable: Morton order kept) and walk only those, in full waves.
+    const uint32_t* list = nullptr;
+    uint64_t n_q = a->num_tris;
+    if (!STATS && occ.bits && a->num_tris >= (1ull << 20)) {
+        const uint64_t n = a->num_tris;
+        const size_t sel_bytes = (n + 255) & ~(size_t)255;
+        RS3D_TRY(ctx->edge_sel.ensure(sel_bytes + n * sizeof(uint32_t)));
+        uint8_t* sel = ctx->edge_sel.as<uint8_t>();
+        uint32_t* idx = reinterpret_cast<uint32_t*>(sel + sel_bytes);
+        unsigned nbs;
+        RS3D_TRY(grid_blocks(n, 256, &nbs));
+        k_tri_select<<<nbs, 256, 0, ctx->stream>>>(a->tris, n, occ, sel);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(rs3d_compact_flags(ctx, sel, n, idx, &n_q));
+        list = idx;
+        occ = OccArgs{};  // tested already
+        RS3D_TRY(grid_blocks(n_q ? n_q : 1, TR_BLOCK, &nb));
+    }
     switch (depth) {
-        case 24: k_tri_join<24, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, a->num_tris, swap, H, stats); break;
-        case 32: k_tri_join<32, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, a->num_tris, swap, H, stats); break;
-        case 48: k_tri_join<48, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, a->num_tris, swap, H, stats); break;
-        default: k_tri_join<64, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, a->num_tris, swap, H, stats); break;
+        case 24: k_tri_join<24, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, n_q, list, occ, swap, H, stats); break;
+        case 32: k_tri_join<32, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, n_q, list, occ, swap, H, stats); break;
+        case 48: k_tri_join<48, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, n_q, list, occ, swap, H, stats); break;
+        default: k_tri_join<64, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(b->nodes, b->tris, a->tris, n_q, list, occ, swap, H, stats); break;
     }
     RS3D_HIP(hipGetLastError());
     if (timed) {

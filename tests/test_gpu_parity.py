@@ -833,3 +833,33 @@ def test_axis_grid_rejected_falls_back(ctx):
     assert np.array_equal(g, v)
     assert np.array_equal(g, orc.points_contain(V, I, O, pts, axis=2))
     b.free()
+
+
+def test_occupancy_selection_path_large(ctx):
+    # 1.3 M triangles and 2.0 M edges per side: edge launches and the tri join take the
+    # selection + compaction path (2^20 items or more); counts, pairs and the tri join must
+    # equal the oracle and the walk without bits
+    from rayspace3d_amd.device import tri_join
+    a = gen.overlap_dataset(91, n_objects=400, world=150.0, levels=(3, 4))
+    b = gen.overlap_dataset(92, n_objects=400, world=150.0, levels=(3, 4))
+    a.edges = gen.extract_edges(a)
+    print(f"selection path: {a.num_triangles} x {b.num_triangles} triangles, {a.edges.num_edges} edges",
+          flush=True)
+    assert a.edges.num_edges >= (1 << 20) and a.num_triangles >= (1 << 20)
+    ba, bb = Bvh.from_mesh(ctx, a), Bvh.from_mesh(ctx, b)
+    ea = DevEdges(a.edges)
+    keys, ref_counts = orc.edge_pairs(a.edges, b.vertices, b.indices, b.tri_to_obj, with_counts=True)
+    got = to_host(edge_count(ctx, bb, ea)).copy()
+    assert np.array_equal(got, ref_counts), np.count_nonzero(got != ref_counts)
+    ref_tri = orc.tri_join(a, b, threads=8)
+    assert len(ref_tri) > 100
+    assert np.array_equal(tri_join(ctx, ba, bb), ref_tri)
+    saved = ctx.get_option(_lib.OPT_SEG_OCCUPANCY)
+    try:
+        ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 0)
+        assert np.array_equal(to_host(edge_count(ctx, bb, ea)), ref_counts)
+        assert np.array_equal(tri_join(ctx, ba, bb), ref_tri)
+    finally:
+        ctx.set_option(_lib.OPT_SEG_OCCUPANCY, saved)
+    ba.free()
+    bb.free()
