@@ -270,7 +270,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     // Cells smaller than the mean extent shorten the lists a ray reads (fewer boxes that
     // overlap its cell but not its line) at the cost of more records: the requested edge
     // (a percentage of the mean extent) is widened by 1.5x steps until the records fit the
-    // budget of their edge (12 per triangle below the mean extent, 6 at or above it).
+    // budget of their edge (16 per triangle below the mean extent, 6 at or above it).
     const double mean_ext = size_sum / (double)n;
     const double max_cells = std::min<double>(16.0 * (double)n + 1024.0, (double)AG_MAX_CELLS);
     double h_min = 0.0;
@@ -303,7 +303,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         RS3D_HIP(hipGetLastError());
         RS3D_TRY(acc_read_sum(ctx, acc, &total));
         const bool fine = h < mean_ext;
-        const uint64_t budget = (fine ? 12 : 6) * n + (1ull << 20);
+        const uint64_t budget = (fine ? 16 : 6) * n + (1ull << 20);
         if (total <= budget && total <= 0xffffffffull) break;
         if (!fine || !std::isfinite(h) || !(h > 0)) return RS3D_OK;  // rejected: too many records
         h = std::min(h * 1.5, std::max(mean_ext, h_min));
