@@ -34,11 +34,11 @@ __device__ __forceinline__ int ag_cell(float x, float lo, float inv, int n) {
     return (int)f;
 }
 
-// The dataset-box filter fused into a grid run: points that fail box_filter_pass are
-// answered -1 without a grid walk; the lane counts candidates and points found inside.
+// The dataset-box filter of a filter-refine grid run (points that fail box_filter_pass are
+// answered -1 without a walk) and the lane's count of points found inside.
 struct GridFilter {
     FBox box;
-    uint32_t cand, inside;
+    uint32_t inside;
 };
 
 __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict__ out, uint32_t pi, int res) {

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
     const int lane = threadIdx.x & 63;
     uint32_t cand = 0;
     TraceCounters ctr;
-    GridFilter gf{box, 0u, 0u};
+    GridFilter gf{box, 0u};
     points_grid_two_phase<AX, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
