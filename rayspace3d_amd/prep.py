@@ -20,20 +20,27 @@ def extract_edges_gpu(ctx: Context, mesh: gen.Mesh) -> gen.EdgeSet:
     torch.cuda.synchronize()
     args = (ctx.handle, C.c_void_p(dV.data_ptr()), len(mesh.vertices), C.c_void_p(dI.data_ptr()),
             len(mesh.indices), C.c_void_p(dO.data_ptr()))
-    rc = ctx.lib.rs3d_extract_edges(*args, None, None, None, None, None, 0, 0, C.byref(ne), C.byref(ns))
-    if rc != _lib.RS3D_E_CAPACITY:
-        _lib.check(rc, "rs3d_extract_edges")
-    n_e, n_s = ne.value, ns.value
-    st = torch.empty((max(n_e, 1), 3), dtype=torch.float32, device="cuda")
-    en = torch.empty_like(st)
-    off = torch.empty(max(n_e, 1), dtype=torch.int32, device="cuda")
-    cnt = torch.empty_like(off)
-    src = torch.empty(max(n_s, 1), dtype=torch.int32, device="cuda")
+    # a mesh of t triangles has at most 3t edges and 3t edge sources: one pass with outputs
+    # of that size (HBM is plentiful), the sizing pass only if that were ever exceeded
+    n_e = n_s = 3 * len(mesh.indices)
     p = lambda t: C.c_void_p(t.data_ptr())
-    _lib.check(ctx.lib.rs3d_extract_edges(*args, p(st), p(en), p(off), p(cnt), p(src), n_e, n_s,
-                                          C.byref(ne), C.byref(ns)), "rs3d_extract_edges")
-    return gen.EdgeSet(to_host(st[:n_e]).copy(), to_host(en[:n_e]).copy(), to_host(off[:n_e]).copy(),
-                       to_host(cnt[:n_e]).copy(), to_host(src[:n_s]).copy())
+    while True:
+        st = torch.empty((max(n_e, 1), 3), dtype=torch.float32, device="cuda")
+        en = torch.empty_like(st)
+        off = torch.empty(max(n_e, 1), dtype=torch.int32, device="cuda")
+        cnt = torch.empty_like(off)
+        src = torch.empty(max(n_s, 1), dtype=torch.int32, device="cuda")
+        rc = ctx.lib.rs3d_extract_edges(*args, p(st), p(en), p(off), p(cnt), p(src), n_e, n_s,
+                                        C.byref(ne), C.byref(ns))
+        if rc == _lib.RS3D_E_CAPACITY and (ne.value > n_e or ns.value > n_s):
+            n_e, n_s = ne.value, ns.value
+            continue
+        _lib.check(rc, "rs3d_extract_edges")
+        break
+    n_e, n_s = ne.value, ns.value
+    # to_host returns fresh host arrays (no second copy needed)
+    return gen.EdgeSet(to_host(st[:n_e]), to_host(en[:n_e]), to_host(off[:n_e]), to_host(cnt[:n_e]),
+                       to_host(src[:n_s]))
 
 
 def grid_stats_gpu(ctx: Context, mesh: gen.Mesh, res: int, world_size: float = -1.0) -> preio.Grid:
