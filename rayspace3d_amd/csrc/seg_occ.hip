@@ -63,14 +63,24 @@ constexpr int SO_MAX_SPAN = 4096;  // cells of one triangle box; beyond, the bit
 __global__ __launch_bounds__(256) void k_so_mark(const TriRec* __restrict__ tris, uint64_t n, OccArgs g,
                                                  uint32_t* __restrict__ bits, unsigned long long* __restrict__ big) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const TriRec r = tris[i];
-    float plo[3], phi[3];
-    prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
-                    make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
-    const int x0 = occ_cell(plo[0], g.org_x, g.inv, g.nx), x1 = occ_cell(phi[0], g.org_x, g.inv, g.nx);
-    const int y0 = occ_cell(plo[1], g.org_y, g.inv, g.ny), y1 = occ_cell(phi[1], g.org_y, g.inv, g.ny);
-    const int z0 = occ_cell(plo[2], g.org_z, g.inv, g.nz), z1 = occ_cell(phi[2], g.org_z, g.inv, g.nz);
+    const bool valid = i < n;
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1, z0 = 0, z1 = -1;  // empty range for lanes past the end
+    if (valid) {
+        const TriRec r = tris[i];
+        float plo[3], phi[3];
+        prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                        make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
+        x0 = occ_cell(plo[0], g.org_x, g.inv, g.nx), x1 = occ_cell(phi[0], g.org_x, g.inv, g.nx);
+        y0 = occ_cell(plo[1], g.org_y, g.inv, g.ny), y1 = occ_cell(phi[1], g.org_y, g.inv, g.ny);
+        z0 = occ_cell(plo[2], g.org_z, g.inv, g.nz), z1 = occ_cell(phi[2], g.org_z, g.inv, g.nz);
+    }
+    // triangles in Morton order: a lane whose cell range equals the previous lane's leaves
+    // the marking to that lane (same cells)
+    const int lane = threadIdx.x & 63;
+    const bool same = lane > 0 && __shfl_up(x0, 1, 64) == x0 && __shfl_up(x1, 1, 64) == x1 &&
+                      __shfl_up(y0, 1, 64) == y0 && __shfl_up(y1, 1, 64) == y1 &&
+                      __shfl_up(z0, 1, 64) == z0 && __shfl_up(z1, 1, 64) == z1;
+    if (!valid || same) return;
     if ((uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) * (uint64_t)(z1 - z0 + 1) > (uint64_t)SO_MAX_SPAN) {
         atomicAdd(acc_stripe(big), 1ull);
         return;
@@ -80,7 +90,7 @@ __global__ __launch_bounds__(256) void k_so_mark(const TriRec* __restrict__ tris
             for (int x = x0; x <= x1; ++x) {
                 const uint64_t c = ((uint64_t)z * g.ny + y) * g.nx + x;
                 const uint32_t m = 1u << (c & 31);
-                if (!(bits[c >> 5] & m)) atomicOr(&bits[c >> 5], m);
+                if (!(bits[c >> 5] & m)) atomicOr(&bits[c >> 5], m);  // a read first: most bits are set already
             }
 }
 
