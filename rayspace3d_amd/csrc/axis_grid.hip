@@ -131,20 +131,53 @@ __global__ __launch_bounds__(256) void k_ag_sq(const uint32_t* __restrict__ coun
     if (threadIdx.x == 0 && s) atomicAdd(acc_stripe(acc), s);
 }
 
+// Record i of the sorted (cell, top) keys goes to its cell's padded list: pstart[cell] +
+// its rank in the cell (i - start[cell]).
 template <int AX>
 __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tris, const uint32_t* __restrict__ vals,
-                                                   uint64_t m, AGArgs G, float4* __restrict__ rec) {
+                                                   const uint64_t* __restrict__ keys, const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ pstart, uint64_t m, AGArgs G,
+                                                   float4* __restrict__ rec) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
+    const uint32_t cell = (uint32_t)(keys[i] >> 32);
+    const uint64_t d = (uint64_t)pstart[cell] + (i - start[cell]);
     const TriRec r = tris[vals[i]];
     float plo[3], phi[3];
     prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
                     make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
-    rec[4 * i] = r.v0;      // .w: original triangle index
-    rec[4 * i + 1] = r.v1;  // .w: object id
-    rec[4 * i + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
-    rec[4 * i + 3] = make_float4(plo[U], phi[U], plo[W], phi[W]);
+    rec[4 * d] = r.v0;      // .w: original triangle index
+    rec[4 * d + 1] = r.v1;  // .w: object id
+    rec[4 * d + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
+    rec[4 * d + 3] = make_float4(plo[U], phi[U], plo[W], phi[W]);
+}
+
+// Lists padded to an even number of records, so a two-record step reads one 128-byte line:
+// pcounts = counts rounded up to even; the pad is a sentinel record (NaN vertices and box,
+// box top -inf) after a cell's last record, where the walk's exit test stops every ray
+// (top -inf < origin) and a NaN origin's box test fails.
+__global__ __launch_bounds__(256) void k_ag_pcounts(const uint32_t* __restrict__ counts, uint64_t nc,
+                                                    uint32_t* __restrict__ pcounts,
+                                                    unsigned long long* __restrict__ odd) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = c < nc ? counts[c] : 0u;
+    if (c < nc) pcounts[c] = k + (k & 1u);
+    const unsigned long long o = block_sum_256(k & 1u);
+    if (threadIdx.x == 0 && o) atomicAdd(acc_stripe(odd), o);
+}
+
+__global__ __launch_bounds__(256) void k_ag_pad(const uint32_t* __restrict__ counts,
+                                                const uint32_t* __restrict__ pstart, uint64_t nc,
+                                                float4* __restrict__ rec) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc || !(counts[c] & 1u)) return;
+    const uint64_t d = (uint64_t)pstart[c] + counts[c];
+    const float nan = __int_as_float(0x7fc00000);
+    rec[4 * d] = make_float4(nan, nan, nan, __int_as_float(-1));
+    rec[4 * d + 1] = make_float4(nan, nan, nan, __int_as_float(-1));
+    rec[4 * d + 2] = make_float4(nan, nan, nan, -INFINITY);
+    rec[4 * d + 3] = make_float4(nan, nan, nan, nan);
 }
 
 __global__ __launch_bounds__(256) void k_ag_occ(const uint32_t* __restrict__ counts, uint64_t nc,
@@ -337,22 +370,36 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->lo_w = G.lo_w;
     g->inv_u = G.inv_u;
     g->inv_w = G.inv_w;
+    // padded lists (even record counts; see k_ag_pcounts)
+    uint32_t *pcounts, *pstart, *start;
+    RS3D_TRY(tmp.get(&pcounts, nc));
+    RS3D_TRY(tmp.get(&pstart, nc));
+    RS3D_TRY(tmp.get(&start, nc));
+    uint64_t odd = 0;
+    {
+        unsigned long long* acc_odd;
+        RS3D_TRY(acc_slot(ctx, 2, 0, &acc_odd));
+        k_ag_pcounts<<<nbc, 256, 0, s>>>(counts, nc, pcounts, acc_odd);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(acc_read_sum(ctx, acc_odd, &odd));
+    }
+    const uint64_t ptotal = total + odd;
+    if (ptotal > 0xffffffffull) return RS3D_OK;  // rejected: too many records
     g->entries = total;
     g->expected = expected;
     hipError_t e;
     const uint64_t nwords = (nc + 31) / 32;
     if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->occ, nwords * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&g->rec, std::max<uint64_t>(total, 1) * 4 * sizeof(float4))) != hipSuccess) {
+        (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * 4 * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nwords * sizeof(uint32_t) + total * 4 * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nwords * sizeof(uint32_t) + ptotal * 4 * sizeof(float4);
     const int rc = [&]() -> int {
-        uint32_t* start;
-        RS3D_TRY(tmp.get(&start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
+        RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
         unsigned nbw;
         RS3D_TRY(grid_blocks(nwords, 256, &nbw));
         k_ag_occ<<<nbw, 256, 0, s>>>(counts, nc, g->occ);
@@ -360,11 +407,12 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
             RS3D_TRY(grid_blocks(total, 256, &nbe));
-            k_ag_gather<AX><<<nbe, 256, 0, s>>>(bvh->tris, vals, total, G, g->rec);
+            k_ag_gather<AX><<<nbe, 256, 0, s>>>(bvh->tris, vals, keys, start, pstart, total, G, g->rec);
+            k_ag_pad<<<nbc, 256, 0, s>>>(counts, pstart, nc, g->rec);
         }
         unsigned nbc1;
         RS3D_TRY(grid_blocks(nc + 1, 256, &nbc1));
-        k_ag_cells<<<nbc1, 256, 0, s>>>(start, counts, g->rec, nc, (uint32_t)total, g->cells);
+        k_ag_cells<<<nbc1, 256, 0, s>>>(pstart, counts, g->rec, nc, (uint32_t)ptotal, g->cells);
         RS3D_HIP(hipGetLastError());
         RS3D_HIP(hipStreamSynchronize(s));
         return RS3D_OK;
