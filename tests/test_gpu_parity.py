@@ -863,3 +863,37 @@ def test_occupancy_selection_path_large(ctx):
         ctx.set_option(_lib.OPT_SEG_OCCUPANCY, saved)
     ba.free()
     bb.free()
+
+
+def test_seg_occupancy_flat_and_rejected(ctx):
+    # a flat target (zero extent along z) gets bits over one cell layer; a target with one
+    # triangle spanning more than 4096 cells has its bits rejected (edge rays walk); the
+    # counts equal the oracle either way
+    from rayspace3d_amd.device import seg_occupancy_prepare
+    rng = np.random.Generator(np.random.PCG64(17))
+    n = 400
+    base = rng.uniform(0, 50, (n, 2)).astype(np.float32)
+    quads = []
+    for k in range(n):
+        x, y = base[k]
+        quads.append([[x, y, 0], [x + 1, y, 0], [x + 1, y + 1, 0], [x, y + 1, 0]])
+    V = np.asarray(quads, np.float32).reshape(-1, 3)
+    I = np.concatenate([np.array([[0, 1, 2], [0, 2, 3]], np.uint32) + 4 * k for k in range(n)])
+    O = np.repeat(np.arange(n, dtype=np.int32), 2)
+    flat = gen.Mesh(V, I, O, num_objects=n)
+    big_v = np.array([[-1000, -1000, 5], [1000, -1000, 5], [0, 1000, 5]], np.float32)
+    spanning = gen.Mesh(np.concatenate([V, big_v]), np.concatenate([I, np.array([[4 * n, 4 * n + 1, 4 * n + 2]], np.uint32)]),
+                        np.concatenate([O, [n]]).astype(np.int32), num_objects=n + 1)
+    s = np.concatenate([rng.uniform(-5, 55, (20000, 2)), rng.uniform(-3, 8, (20000, 1))], 1).astype(np.float32)
+    e = (s + rng.normal(0, 2.0, s.shape)).astype(np.float32)
+    m = len(s)
+    segs = gen.EdgeSet(s, e, np.arange(m, dtype=np.int32), np.ones(m, np.int32), np.zeros(m, np.int32))
+    for mesh, state in ((flat, 1), (spanning, 2)):
+        b = Bvh.from_mesh(ctx, mesh)
+        info = seg_occupancy_prepare(ctx, b)
+        print(f"occupancy on {mesh.num_triangles} triangles: {info}", flush=True)
+        assert info["state"] == state
+        got = to_host(edge_count(ctx, b, DevEdges(segs)))
+        _, ref = orc.edge_pairs(segs, mesh.vertices, mesh.indices, mesh.tri_to_obj, with_counts=True)
+        assert np.array_equal(got, ref) and ref.sum() > 100
+        b.free()
