@@ -147,10 +147,12 @@ int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* poi
  * upper end along the axis. A ray reads its cell and tests the listed triangles whose box
  * contains its line and reaches its origin -- the same candidates as the BVH walk, so the
  * same hits and results -- in one lookup instead of a walk down the tree. The cell edge is
- * the triangles' mean extent in the plane. A grid is built once per (BVH, axis), lazily by
- * the first point launch along that axis or here, and rejected (the BVH is used) when its
- * records would exceed 6 per triangle + 2^20, or the lists average more than 64 records
- * per cell, or more than 256 per record (dense spots). Freed with the BVH. */
+ * RS3D_OPT_GRID_CELL_PCT percent of the triangles' mean extent in the plane (default 35),
+ * widened while the records would exceed 16 per triangle + 2^20. A grid is built once per
+ * (BVH, axis), lazily by the first point launch along that axis or here, and rejected (the
+ * BVH is used) when its records would exceed 6 per triangle + 2^20 even at the mean extent,
+ * or the lists average more than 64 records per cell, or more than 256 per record (dense
+ * spots). Freed with the BVH. */
 typedef struct {
     int32_t state;            /* 0 not built yet, 1 built, 2 rejected */
     int32_t cells_u, cells_w; /* cells along the two perpendicular axes */
