@@ -180,16 +180,18 @@ __global__ __launch_bounds__(256) void k_ag_pad(const uint32_t* __restrict__ cou
     rec[4 * d + 3] = make_float4(nan, nan, nan, nan);
 }
 
+// Occupancy bits from the counts: one cell per lane, a wave's ballot is the two words of
+// its 64 cells.
 __global__ __launch_bounds__(256) void k_ag_occ(const uint32_t* __restrict__ counts, uint64_t nc,
                                                 uint32_t* __restrict__ occ) {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= (nc + 31) / 32) return;
-    uint32_t bits = 0;
-    for (int b = 0; b < 32; ++b) {
-        const uint64_t c = w * 32 + b;
-        if (c < nc && counts[c]) bits |= 1u << b;
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t m = __ballot(c < nc && counts[c] != 0u);
+    if ((threadIdx.x & 63) == 0 && c < nc) {
+        const uint64_t w = c / 32;  // c is a multiple of 64 here
+        const uint64_t nwords = (nc + 31) / 32;
+        occ[w] = (uint32_t)m;
+        if (w + 1 < nwords) occ[w + 1] = (uint32_t)(m >> 32);
     }
-    occ[w] = bits;
 }
 
 // Cell table: (first record, box top of the first record -- the largest of the cell) per
@@ -400,9 +402,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
-        unsigned nbw;
-        RS3D_TRY(grid_blocks(nwords, 256, &nbw));
-        k_ag_occ<<<nbw, 256, 0, s>>>(counts, nc, g->occ);
+        k_ag_occ<<<nbc, 256, 0, s>>>(counts, nc, g->occ);
         if (total) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
