@@ -897,3 +897,37 @@ def test_seg_occupancy_flat_and_rejected(ctx):
         _, ref = orc.edge_pairs(segs, mesh.vertices, mesh.indices, mesh.tri_to_obj, with_counts=True)
         assert np.array_equal(got, ref) and ref.sum() > 100
         b.free()
+
+
+@pytest.mark.parametrize("grid", [1, 0])
+def test_index_map_launch(ctx, grid):
+    # rays given as an index list (a shuffled subset): every listed point gets its answer in
+    # each output form, the others keep what the caller wrote there
+    mesh, pts = gen.many_spheres_workload(seed=41, n_objects=800, n_points=120_000, world=120.0)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    rng = np.random.Generator(np.random.PCG64(41))
+    sel = rng.permutation(len(pts))[: len(pts) // 3].astype(np.int32)
+    saved = ctx.get_option(_lib.OPT_AXIS_GRID)
+    ctx.set_option(_lib.OPT_AXIS_GRID, grid)
+    try:
+        b = Bvh.from_mesh(ctx, mesh)
+        dp, dm = to_dev(pts), to_dev(sel)
+        for kind in (_lib.OUT_OBJ_I32, _lib.OUT_FLAG_U8, _lib.OUT_RAY_RESULT):
+            if kind == _lib.OUT_RAY_RESULT:
+                out = torch.full((len(pts), 2), -7, dtype=torch.int32, device="cuda")
+            elif kind == _lib.OUT_FLAG_U8:
+                out = torch.full((len(pts),), 9, dtype=torch.uint8, device="cuda")
+            else:
+                out = torch.full((len(pts),), -7, dtype=torch.int32, device="cuda")
+            got = to_host(trace_points(ctx, b, dp, axis=2, out_kind=kind, index_map=dm, out=out))
+            rest = np.setdiff1d(np.arange(len(pts)), sel)
+            if kind == _lib.OUT_RAY_RESULT:
+                assert np.array_equal(got[sel, 1], ref[sel]) and np.array_equal(got[sel, 0], sel)
+                assert (got[rest] == -7).all()
+            elif kind == _lib.OUT_FLAG_U8:
+                assert np.array_equal(got[sel], (ref[sel] >= 0).astype(np.uint8)) and (got[rest] == 9).all()
+            else:
+                assert np.array_equal(got[sel], ref[sel]) and (got[rest] == -7).all()
+        b.free()
+    finally:
+        ctx.set_option(_lib.OPT_AXIS_GRID, saved)
