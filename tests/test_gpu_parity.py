@@ -854,6 +854,12 @@ def test_occupancy_selection_path_large(ctx):
     ref_tri = orc.tri_join(a, b, threads=8)
     assert len(ref_tri) > 100
     assert np.array_equal(tri_join(ctx, ba, bb), ref_tri)
+    # both join forms (count + scan + write, and hash) through the selection path
+    b.edges = gen.extract_edges(b)
+    eb = DevEdges(b.edges)
+    ref_pairs = orc.overlap_pairs(a, a.edges, b, b.edges)
+    for mode in (0, 1):
+        assert np.array_equal(overlap_join(ctx, ba, ea, bb, eb, mode=mode), ref_pairs), mode
     saved = ctx.get_option(_lib.OPT_SEG_OCCUPANCY)
     try:
         ctx.set_option(_lib.OPT_SEG_OCCUPANCY, 0)
