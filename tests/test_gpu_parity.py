@@ -320,9 +320,10 @@ def test_containment_join(ctx):
     assert over == 0
     assert np.array_equal(ov, ov_ref)
     assert np.array_equal(ct, ct_ref)
-    # every nested B object (first half) is contained in the A object it was placed in
+    # every nested B object (first half: radius 2 within sqrt(3) of the centre of a
+    # radius-5 A object, so wholly inside it) is contained in the A object it was placed in
     pairs = {(int(k >> np.uint64(32)), int(k & np.uint64(0xffffffff))) for k in ct}
-    assert sum((i, i) in pairs for i in range(100)) >= 95
+    assert all((i, i) in pairs for i in range(100))
     ba.free()
     bb.free()
 
