@@ -175,6 +175,22 @@ int rs3d_axis_grid_stats(rs3d_ctx* ctx, rs3d_bvh* bvh, const float* points, uint
 int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
                        int axis, int32_t* out_obj, uint64_t* candidate_count,
                        uint64_t* inside_count);
+/* Filter-refine that allocates nothing. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, answers and counts as
+ * rs3d_filter_refine, which it shares kernels with. Its scratch is a fixed device array of
+ * RS3D_NOALLOC_SCRATCH_BYTES inside librs3d, held by every GPU the library runs on until
+ * the process ends; no call allocates or frees device or host memory. An axis grid is used
+ * only when rs3d_axis_grid_prepare (or an earlier point launch) has already built it for
+ * this BVH and axis; then n is limited only as in rs3d_filter_refine. Otherwise the
+ * three-phase BVH form runs with its flags, positions and candidates in the array, and
+ * n > RS3D_NOALLOC_MAX_POINTS returns RS3D_E_CAPACITY. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race and must be serialised by the caller. */
+#define RS3D_NOALLOC_MAX_POINTS (1ull << 23)
+#define RS3D_NOALLOC_SCRATCH_BYTES 75538688ull
+int rs3d_filter_refine_noalloc(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
+                               int axis, int32_t* out_obj, uint64_t* candidate_count,
+                               uint64_t* inside_count);
 /* Box-mesh filter flags only: flags[i] = 1 when point i passes. box: host float[6]. */
 int rs3d_filter_box(rs3d_ctx* ctx, const float* box_min_max, const float* points, uint64_t n,
                     int axis, uint8_t* flags);

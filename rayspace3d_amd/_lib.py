@@ -17,6 +17,7 @@ RS3D_E_HIP = -2
 RS3D_E_NOMEM = -3
 RS3D_E_CAPACITY = -4
 RS3D_E_DEPTH = -5
+NOALLOC_MAX_POINTS = 1 << 23
 
 OUT_RAY_RESULT = 0
 OUT_FLAG_U8 = 1
@@ -173,6 +174,7 @@ SIGNATURES = {
     "rs3d_tri_join": (I32, [P, P, P, P, U64, PU64]),
     "rs3d_tri_join_stats": (I32, [P, P, P, C.POINTER(TriJoinStats)]),
     "rs3d_filter_refine": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
+    "rs3d_filter_refine_noalloc": (I32, [P, P, P, U64, I32, P, PU64, PU64]),
     "rs3d_filter_box": (I32, [P, C.POINTER(C.c_float), P, U64, I32, P]),
     "rs3d_scan_i32_i64": (I32, [P, P, P, U64, PI64]),
     "rs3d_compact_hits": (I32, [P, P, U64, P, PU64]),

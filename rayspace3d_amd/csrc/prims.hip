@@ -258,6 +258,13 @@ int scan_u8_to_u32_exclusive(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, ui
     return scan_impl<uint8_t, uint32_t>(ctx, in, out, n, total_dev, ctx->scan_tmp.as<uint32_t>(), te);
 }
 
+uint64_t scan_scratch_elems(uint64_t n) { return scan_tmp_elems(n); }
+
+int scan_u8_to_u32_exclusive_in(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, uint64_t n,
+                                uint32_t* total_dev, uint32_t* tmp, uint64_t tmp_elems) {
+    return scan_impl<uint8_t, uint32_t>(ctx, in, out, n, total_dev, tmp, tmp_elems);
+}
+
 // ------------------------------------------------------------------------------------
 // Stable LSD radix sort, 8-bit digits. Per pass: (1) per-tile digit histograms, counted
 // with wave64 ballot matching (one LDS add per distinct digit per wave, so runs of equal
@@ -652,6 +659,15 @@ __global__ __launch_bounds__(256) void k_scatter_index(const uint8_t* __restrict
     if (i < n && flags[i]) out[pos[i]] = (uint32_t)i;
 }
 
+int scatter_index(rs3d_ctx* ctx, const uint8_t* flags, const uint32_t* pos, uint64_t n, uint32_t* out) {
+    if (n == 0) return RS3D_OK;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_scatter_index<<<nb, 256, 0, ctx->stream>>>(flags, pos, n, out);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host) {
     if (n == 0) {
         *count_host = 0;
@@ -725,6 +741,15 @@ int rs3d_ctx_create(int device, rs3d_ctx** out) {
         delete c;
         set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
         return RS3D_E_NOMEM;
+    }
+    // the device address of rs3d_filter_refine_noalloc's fixed scratch, looked up once per
+    // device here so that the entry point itself never makes the runtime materialise it
+    const int rc = noalloc_init(c);
+    if (rc != RS3D_OK) {
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
     }
     *out = c;
     return RS3D_OK;

@@ -309,6 +309,22 @@ int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args);
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                      const uint32_t* imap, int out_kind, void* out);
 int unique_u64(rs3d_ctx* ctx, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* count_host);
+// Scan and compaction pieces on caller-provided scratch (no allocation): the exclusive scan
+// of u8 flags needs scan_scratch_elems(n) u32 words of tmp; scatter_index writes i at
+// out[pos[i]] for every flagged i.
+uint64_t scan_scratch_elems(uint64_t n);
+int scan_u8_to_u32_exclusive_in(rs3d_ctx* ctx, const uint8_t* in, uint32_t* out, uint64_t n,
+                                uint32_t* total_dev, uint32_t* tmp, uint64_t tmp_elems);
+int scatter_index(rs3d_ctx* ctx, const uint8_t* flags, const uint32_t* pos, uint64_t n, uint32_t* out);
+// The BVH walk of point rays as given (no reordering, no axis grid; allocates nothing).
+int trace_points_walk(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, uint64_t n,
+                      const uint32_t* imap, int axis, int out_kind, void* out);
+int fill_i32(rs3d_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
+// Adds the number of p[i] >= 0 into the striped accumulator acc.
+int count_nonneg(rs3d_ctx* ctx, const int32_t* p, uint64_t n, unsigned long long* acc);
+// Looks up, once per device (with ctx's device current), the device address of the fixed
+// scratch array of rs3d_filter_refine_noalloc; called by rs3d_ctx_create.
+int noalloc_init(rs3d_ctx* ctx);
 // Striped accumulators. Device-scope atomics on one address serialise (about 10 ns each on
 // this GPU, so ten thousand blocks adding into one counter cost ~0.1 ms), so reduction
 // kernels combine within the block and add into one of ACC_STRIPES words 128 B apart

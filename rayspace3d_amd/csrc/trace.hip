@@ -354,6 +354,30 @@ int trace_points_internal(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, 
     return RS3D_OK;
 }
 
+int trace_points_walk(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* pts, uint64_t n,
+                      const uint32_t* imap, int axis, int out_kind, void* out) {
+    if (n == 0 || bvh->num_tris == 0) return trace_points_internal(ctx, bvh, pts, n, imap, axis, out_kind, out, nullptr);
+    return launch_points<false>(ctx, bvh, pts, n, imap, axis, out_kind, out, nullptr);
+}
+
+int fill_i32(rs3d_ctx* ctx, int32_t* p, uint64_t n, int32_t v) {
+    if (n == 0) return RS3D_OK;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_fill_i32<<<nb, 256, 0, ctx->stream>>>(p, n, v);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
+int count_nonneg(rs3d_ctx* ctx, const int32_t* p, uint64_t n, unsigned long long* acc) {
+    if (n == 0) return RS3D_OK;
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_count_nonneg<<<std::min(nb, 2048u), 256, 0, ctx->stream>>>(p, n, acc);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
+}
+
 template <int MODE>
 static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* edges, int swap,
                         int32_t* counts, const int64_t* offsets, int64_t bias, void* out,
