@@ -858,7 +858,7 @@ def run_serving(args):
     starts = rng.integers(0, pool.shape[0] - 1_000_000, size=n)
 
     def serve(rate, max_slots):
-        srv = Server(ctx, max_slots=max_slots, memory_budget=32 << 30)
+        srv = Server(ctx, max_slots=max_slots, memory_budget=240 << 30)
         ids = [srv.add_dataset(b, e, grid_res=128) for b, e in srv_datasets]
         qids = [srv.add_dataset(b, e, grid_res=128) for b, e in queries]
         arrivals = np.zeros(n) if rate is None else np.cumsum(rng.exponential(1.0 / rate, size=n))

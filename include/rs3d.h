@@ -154,7 +154,8 @@ int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* poi
  * or the lists average more than 64 records per cell, or more than 256 per record (dense
  * spots). Freed with the BVH. */
 typedef struct {
-    int32_t state;            /* 0 not built yet, 1 built, 2 rejected */
+    int32_t state;            /* 0 not built yet, 1 built, 2 rejected, 3 refused for lack of
+                                 device memory (launches use the BVH; prepare tries again) */
     int32_t cells_u, cells_w; /* cells along the two perpendicular axes */
     uint64_t entries;         /* (triangle, cell) records */
     double expected_list;     /* records per cell: the mean list a uniformly placed ray reads */
@@ -162,6 +163,8 @@ typedef struct {
     double build_ms;          /* device time of the build (0 when not built here) */
 } rs3d_axis_grid_info;
 int rs3d_axis_grid_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info /* host, may be null */);
+/* The grid's current state and sizes without building it (info: host). */
+int rs3d_axis_grid_query(const rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info);
 /* Work counters of a point launch on the axis grid (every ray reads one occupancy bit;
  * nodes_visited = 8-byte cell ranges read for occupied cells, tris_tested = 64-byte
  * records read); RS3D_E_ARG when no grid is accepted for the axis. */
@@ -247,7 +250,8 @@ int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
  * box (widened by 2^-14 of its coordinates) covers no set bit cannot hit the target and
  * skips the walk; results are unchanged. Built once per BVH by the first edge launch
  * against it, or here; freed with the BVH. state: 1 built, 2 rejected (a triangle box over
- * more than 4096 cells, or non-finite bounds: edge rays walk the tree). */
+ * more than 4096 cells, or non-finite bounds: edge rays walk the tree), 3 refused for lack
+ * of device memory (edge rays walk the tree; this call tries again). */
 typedef struct {
     int state;
     int cells_x, cells_y, cells_z;
@@ -397,9 +401,15 @@ int rs3d_batch_destroy(rs3d_batch* batch);
  * histogram of the given resolution for the estimator) stay resident. Requests queue FIFO;
  * each rs3d_server_step admits queued requests while slots are free and their estimated
  * memory fits the budget, runs them all in one persistent launch, and completes them.
- * Overlap / intersection pairs are (query-dataset object, resident-dataset object),
- * identical to rs3d_overlap_join / rs3d_intersection_join of the two datasets. Calls on
- * one server are serial (single host thread). */
+ * memory_budget (0: 8 GiB) bounds the device memory the server keeps resident (per dataset
+ * its first-triangle list and grid histogram, and the axis grids and segment occupancy
+ * bits of the datasets' BVHs) plus the per-step memory of admitted requests; the caller's
+ * BVHs and mesh arrays are not counted. A grid or occupancy bits are built, before a step,
+ * only when the upper bound of their size fits the budget; otherwise the request runs on
+ * the BVH walk with identical results. peak_memory (rs3d_server_stats) includes the
+ * resident bytes. Overlap / intersection pairs are (query-dataset object, resident-dataset
+ * object), identical to rs3d_overlap_join / rs3d_intersection_join of the two datasets.
+ * Calls on one server are serial (single host thread). */
 typedef struct rs3d_server rs3d_server;
 enum { RS3D_REQ_CONTAIN = 0, RS3D_REQ_OVERLAP = 1, RS3D_REQ_INTERSECT = 2 };
 typedef struct {

@@ -163,6 +163,7 @@ SIGNATURES = {
     "rs3d_trace_points": (I32, [P, P, P, U64, P, I32, I32, P]),
     "rs3d_trace_points_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_axis_grid_prepare": (I32, [P, P, I32, C.POINTER(AxisGridInfo)]),
+    "rs3d_axis_grid_query": (I32, [P, I32, C.POINTER(AxisGridInfo)]),
     "rs3d_seg_occupancy_prepare": (I32, [P, P, C.POINTER(SegOccupancyInfo)]),
     "rs3d_axis_grid_stats": (I32, [P, P, P, U64, I32, C.POINTER(TraceStats)]),
     "rs3d_edge_stats": (I32, [P, P, C.POINTER(Edges), C.POINTER(EdgeStats)]),

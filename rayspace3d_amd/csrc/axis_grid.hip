@@ -364,14 +364,6 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     const double weighted = total ? (double)sq / (double)total : 0.0;
     if (weighted > AG_MAX_WEIGHTED_LIST) return RS3D_OK;  // rejected: dense spots too long
     const double expected = (double)total / (double)nc;
-    AxisGrid* g = new AxisGrid();
-    g->axis = AX;
-    g->nu = nu;
-    g->nw = nw;
-    g->lo_u = G.lo_u;
-    g->lo_w = G.lo_w;
-    g->inv_u = G.inv_u;
-    g->inv_w = G.inv_w;
     // padded lists (even record counts; see k_ag_pcounts)
     uint32_t *pcounts, *pstart, *start;
     RS3D_TRY(tmp.get(&pcounts, nc));
@@ -387,6 +379,14 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     }
     const uint64_t ptotal = total + odd;
     if (ptotal > 0xffffffffull) return RS3D_OK;  // rejected: too many records
+    AxisGrid* g = new AxisGrid();
+    g->axis = AX;
+    g->nu = nu;
+    g->nw = nw;
+    g->lo_u = G.lo_u;
+    g->lo_w = G.lo_w;
+    g->inv_u = G.inv_u;
+    g->inv_w = G.inv_w;
     g->entries = total;
     g->expected = expected;
     hipError_t e;
@@ -433,11 +433,20 @@ int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
     }
     const auto t0 = std::chrono::steady_clock::now();
     AxisGrid* g = nullptr;
+    int rc;
     switch (axis) {
-        case 0: RS3D_TRY(build_ax<0>(ctx, bvh, &g)); break;
-        case 1: RS3D_TRY(build_ax<1>(ctx, bvh, &g)); break;
-        default: RS3D_TRY(build_ax<2>(ctx, bvh, &g)); break;
+        case 0: rc = build_ax<0>(ctx, bvh, &g); break;
+        case 1: rc = build_ax<1>(ctx, bvh, &g); break;
+        default: rc = build_ax<2>(ctx, bvh, &g); break;
     }
+    if (rc == RS3D_E_NOMEM) {
+        // the grid only speeds launches up: without memory for it they walk the BVH (same
+        // results); state 3 is not retried by launches, only by rs3d_axis_grid_prepare
+        (void)hipGetLastError();
+        bvh->agrid_state[axis] = 3;
+        return RS3D_OK;
+    }
+    if (rc != RS3D_OK) return rc;
     if (g) g->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     bvh->agrid[axis] = g;
     bvh->agrid_state[axis] = g ? 1 : 2;
@@ -495,12 +504,23 @@ int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float*
     }
 }
 
-int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready) {
+int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready, bool build) {
     *ready = false;
     if (!ctx->opt_axis_grid || axis < 0 || axis > 2) return RS3D_OK;
-    RS3D_TRY(ensure_grid(ctx, bvh, axis));
-    *ready = bvh->agrid[axis] != nullptr;
+    if (build) RS3D_TRY(ensure_grid(ctx, bvh, axis));
+    *ready = bvh->agrid_state[axis] == 1 && bvh->agrid[axis] != nullptr;
     return RS3D_OK;
+}
+
+uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
+    // Peak device bytes of a build: records are at most 16 per triangle + 2^20 (the budget
+    // the build widens its cells to meet), 64 B each plus up to 1/8 for the padding to even
+    // lists, and while the build sorts them 24 B more each (keys, values and the sort's
+    // alternate buffers); cells are within 16 per triangle + 1024, 8 B in the table and
+    // 16 B of counts and offsets during the build; 8 B per triangle of spans and offsets.
+    const uint64_t n = bvh->num_tris;
+    const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024;
+    return recs * (72 + 24) + cells * 24 + n * 8 + (64ull << 20);
 }
 
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,
@@ -525,8 +545,17 @@ int rs3d_axis_grid_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, rs3d_axis_gri
         return RS3D_E_ARG;
     }
     RS3D_HIP(hipSetDevice(ctx->device));
+    if (bvh->agrid_state[axis] == 3) bvh->agrid_state[axis] = 0;  // refused for memory: try again
     RS3D_TRY(ensure_grid(ctx, bvh, axis));
-    if (info) {
+    return info ? rs3d_axis_grid_query(bvh, axis, info) : RS3D_OK;
+}
+
+int rs3d_axis_grid_query(const rs3d_bvh* bvh, int axis, rs3d_axis_grid_info* info) {
+    if (!bvh || axis < 0 || axis > 2 || !info) {
+        set_error("rs3d_axis_grid_query: bad arguments");
+        return RS3D_E_ARG;
+    }
+    {
         memset(info, 0, sizeof *info);
         info->state = bvh->agrid_state[axis];
         const AxisGrid* g = bvh->agrid[axis];

@@ -205,7 +205,8 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
     d.n = n;
     d.axis = axis;
     bool grid = false;
-    if (n <= 0xffffffffull) RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid));
+    if (n <= 0xffffffffull)
+        RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid, build_structures));
     if (grid) {  // rays in the caller's order on the grid: no reorder keys for this query
         const AxisGrid* g = b->agrid[axis];
         d.grid = AGArgs{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
@@ -233,7 +234,7 @@ int BatchPlan::add_edges(const rs3d_bvh* target, const rs3d_edges* e, int swap, 
     BatchDesc d{};
     RS3D_TRY(base(d, target, BQ_EDGES, qid));
     d.E = EdgeArgs{e->starts, e->ends, e->src_offsets, e->src_counts, e->src_objects, e->num_edges};
-    RS3D_TRY(seg_occ_args(ctx_, const_cast<rs3d_bvh*>(target), &d.occ));
+    RS3D_TRY(seg_occ_args(ctx_, const_cast<rs3d_bvh*>(target), &d.occ, build_structures));
     d.n = e->num_edges;
     d.swap = swap;
     items.push_back(d);

@@ -87,6 +87,9 @@ public:
     int num_qids = 0;
     uint64_t slot_hint = 1u << 20;
     bool split_kinds = true;  // one persistent launch per ray-program kind
+    // false: queries use axis grids and occupancy bits that already exist but build none
+    // (the request server builds them itself when its memory budget allows)
+    bool build_structures = true;
 
 private:
     int base(BatchDesc& d, const rs3d_bvh* b, int kind, int qid);

@@ -1,6 +1,8 @@
 // join.hip -- pair hash table, overlap join driver, selectivity estimator and the
 // reference-named C entry points (query/src/cuda/mesh_query_deduplication.h,
 // result_compaction.h, scan_utils.h).
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -131,8 +133,13 @@ static std::mutex g_default_mu;
 static rs3d_ctx* g_default_ctx[64] = {nullptr};
 
 rs3d_ctx* default_ctx() {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    int ndev = 0, dev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e == hipSuccess && ndev > 0) e = hipGetDevice(&dev);
+    if (e != hipSuccess || ndev <= 0 || dev < 0 || dev >= 64) {
+        set_error("no HIP device is available (%s, %d devices)", hipGetErrorString(e), ndev);
+        return nullptr;
+    }
     std::lock_guard<std::mutex> lk(g_default_mu);
     if (!g_default_ctx[dev]) {
         rs3d_ctx* c = nullptr;
@@ -140,6 +147,96 @@ rs3d_ctx* default_ctx() {
         g_default_ctx[dev] = c;
     }
     return g_default_ctx[dev];
+}
+
+// Prints the calling thread's last error and ends the process with EXIT_FAILURE, as the
+// reference's CUDA_CHECK does (query/src/optix/OptixHelpers.h:15-21).
+[[noreturn]] void fail_fast(const char* fn) {
+    std::fprintf(stderr, "librs3d: %s failed: %s\n", fn, last_error());
+    std::fflush(stderr);
+    std::exit(EXIT_FAILURE);
+}
+
+rs3d_ctx* default_ctx_or_die(const char* fn) {
+    rs3d_ctx* c = default_ctx();
+    if (!c) {
+        if (!*last_error()) set_error("no HIP device is available");
+        fail_fast(fn);
+    }
+    return c;
+}
+
+// Moves count keys from src down to dst (dst < src, ranges may overlap) with one
+// workgroup walking tiles upward: each tile is loaded whole before any lane stores it, and
+// every later tile reads above everything stored so far.
+__global__ __launch_bounds__(1024) void k_move_down(uint64_t* __restrict__ base, uint64_t dst,
+                                                    uint64_t src, uint64_t count) {
+    constexpr int PER = 4;
+    constexpr uint64_t TILE = 1024 * PER;
+    for (uint64_t t = 0; t < count; t += TILE) {
+        uint64_t v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint64_t i = t + (uint64_t)j * 1024 + threadIdx.x;
+            if (i < count) v[j] = base[src + i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint64_t i = t + (uint64_t)j * 1024 + threadIdx.x;
+            if (i < count) base[dst + i] = v[j];
+        }
+        __syncthreads();
+    }
+}
+
+// Device bytes the dedup may count on: RS3D_DEDUP_FREE_BYTES when set (a cap callers use to
+// leave memory to others, and tests use to force the halving path), else the device's free
+// memory, plus the scratch the context already holds.
+static uint64_t dedup_budget(rs3d_ctx* ctx) {
+    uint64_t free_b = 0;
+    const char* env = std::getenv("RS3D_DEDUP_FREE_BYTES");
+    if (env && *env) {
+        free_b = std::strtoull(env, nullptr, 10);
+    } else {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) == hipSuccess) free_b = f;
+    }
+    return free_b + ctx->sort_keys.bytes + ctx->sort_hist.bytes + ctx->tmp_a.bytes + ctx->tmp_b.bytes +
+           ctx->scan_tmp.bytes;
+}
+
+// Scratch of one sort + unique of n keys (the alternate key buffer, flags, positions, tile
+// histograms and scan partials, with the buffers' growth margin), plus 512 MB of headroom
+// as the reference keeps.
+static uint64_t dedup_need(uint64_t n) { return n * 15 + n / 16 + (512ull << 20); }
+
+static int dedup_inplace(rs3d_ctx* ctx, uint64_t* keys, uint64_t n, uint64_t* unique_count) {
+    if (n <= 1) {
+        *unique_count = n;
+        return RS3D_OK;
+    }
+    if (n <= 0xffffffffull && dedup_need(n) <= dedup_budget(ctx)) {
+        const int rc = rs3d_sort_unique_u64(ctx, keys, n, unique_count);
+        if (rc != RS3D_E_NOMEM) return rc;
+        (void)hipGetLastError();  // an allocation that failed: halve instead
+    }
+    const uint64_t half = n / 2;
+    uint64_t u1 = 0, u2 = 0;
+    RS3D_TRY(dedup_inplace(ctx, keys, half, &u1));
+    RS3D_TRY(dedup_inplace(ctx, keys + half, n - half, &u2));
+    if (u2 && u1 < half) {
+        k_move_down<<<1, 1024, 0, ctx->stream>>>(keys, u1, half, u2);
+        RS3D_HIP(hipGetLastError());
+    }
+    const uint64_t m = u1 + u2;
+    if (m < n) return dedup_inplace(ctx, keys, m, unique_count);
+    // no duplicates within either half: the merged sort must run at this size
+    if (n > 0xffffffffull) {
+        set_error("dedup of %llu distinct keys exceeds the sort's 2^32-1", (unsigned long long)n);
+        return RS3D_E_CAPACITY;
+    }
+    return rs3d_sort_unique_u64(ctx, keys, n, unique_count);
 }
 
 }  // namespace rs3d
@@ -280,75 +377,113 @@ int rs3d_estimate_raw(rs3d_ctx* ctx, const void* cells_a, const void* cells_b, u
 }
 
 // ---- reference-named entry points (synchronous, like the reference) ----------------
+// Failures are not turned into counts: like the reference's CUDA_CHECK
+// (query/src/optix/OptixHelpers.h:15-29) they print the error and end the process with
+// EXIT_FAILURE. The dedup first halves its input when the sort's scratch would not fit
+// (query/src/cuda/mesh_query_deduplication.cu:56-90).
 
 int count_hits_gpu(const rs3d_ray_result* d_results, int num_results) {
-    rs3d_ctx* ctx = default_ctx();
-    if (!ctx || num_results <= 0 || !d_results) return 0;
+    if (num_results == 0) return 0;
+    rs3d_ctx* ctx = default_ctx_or_die("count_hits_gpu");
+    if (num_results < 0 || !d_results) {
+        set_error("count_hits_gpu: bad arguments (num_results %d)", num_results);
+        fail_fast("count_hits_gpu");
+    }
     static thread_local DevBuf out;
-    if (out.ensure((size_t)num_results * sizeof(rs3d_ray_result)) != RS3D_OK) return 0;
+    if (out.ensure((size_t)num_results * sizeof(rs3d_ray_result)) != RS3D_OK) fail_fast("count_hits_gpu");
     uint64_t c = 0;
-    if (rs3d_compact_hits(ctx, d_results, (uint64_t)num_results, out.as<rs3d_ray_result>(), &c) !=
-        RS3D_OK)
-        return 0;
+    if (rs3d_compact_hits(ctx, d_results, (uint64_t)num_results, out.as<rs3d_ray_result>(), &c) != RS3D_OK)
+        fail_fast("count_hits_gpu");
     return (int)c;
 }
 
 void compact_hits_gpu(const rs3d_ray_result* d_input, rs3d_ray_result* d_output, int num_results,
                       int* h_num_hits) {
-    rs3d_ctx* ctx = default_ctx();
+    if (num_results == 0) {
+        if (h_num_hits) *h_num_hits = 0;
+        return;
+    }
+    rs3d_ctx* ctx = default_ctx_or_die("compact_hits_gpu");
+    if (num_results < 0 || !d_input || !d_output) {
+        set_error("compact_hits_gpu: bad arguments (num_results %d)", num_results);
+        fail_fast("compact_hits_gpu");
+    }
     uint64_t c = 0;
-    if (ctx && num_results > 0 && d_input && d_output)
-        (void)rs3d_compact_hits(ctx, d_input, (uint64_t)num_results, d_output, &c);
+    if (rs3d_compact_hits(ctx, d_input, (uint64_t)num_results, d_output, &c) != RS3D_OK)
+        fail_fast("compact_hits_gpu");
     if (h_num_hits) *h_num_hits = (int)c;
 }
 
 long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements) {
-    rs3d_ctx* ctx = default_ctx();
-    if (!ctx || num_elements <= 0) return 0;
+    if (num_elements == 0) return 0;
+    rs3d_ctx* ctx = default_ctx_or_die("exclusive_scan_gpu");
+    if (num_elements < 0 || !d_input || !d_output) {
+        set_error("exclusive_scan_gpu: bad arguments (num_elements %d)", num_elements);
+        fail_fast("exclusive_scan_gpu");
+    }
     int64_t total = 0;
     if (rs3d_scan_i32_i64(ctx, d_input, reinterpret_cast<int64_t*>(d_output), (uint64_t)num_elements,
                           &total) != RS3D_OK)
-        return 0;
+        fail_fast("exclusive_scan_gpu");
     return (long long)total;
 }
 
 long long merge_and_deduplicate_pairs_gpu(const rs3d_pair* d_results1, long long num_results1,
                                           const rs3d_pair* d_results2, long long num_results2,
                                           rs3d_pair* d_merged_output) {
-    rs3d_ctx* ctx = default_ctx();
     const long long total = num_results1 + num_results2;
-    if (!ctx || total <= 0 || !d_merged_output) return 0;
-    if (d_results1 && num_results1 > 0 && d_results1 != d_merged_output)
-        (void)hipMemcpyAsync(d_merged_output, d_results1, num_results1 * sizeof(rs3d_pair),
-                             hipMemcpyDeviceToDevice, ctx->stream);
-    if (d_results2 && num_results2 > 0 && d_results2 != d_merged_output + num_results1)
-        (void)hipMemcpyAsync(d_merged_output + num_results1, d_results2,
-                             num_results2 * sizeof(rs3d_pair), hipMemcpyDeviceToDevice, ctx->stream);
+    if (num_results1 == 0 && num_results2 == 0) return 0;
+    rs3d_ctx* ctx = default_ctx_or_die("merge_and_deduplicate_pairs_gpu");
+    // a null input array means its results already lie in place in d_merged_output
+    if (num_results1 < 0 || num_results2 < 0 || !d_merged_output) {
+        set_error("merge_and_deduplicate_pairs_gpu: bad arguments (%lld, %lld)", num_results1, num_results2);
+        fail_fast("merge_and_deduplicate_pairs_gpu");
+    }
+    if (d_results1 && num_results1 > 0 && d_results1 != d_merged_output &&
+        hipMemcpyAsync(d_merged_output, d_results1, num_results1 * sizeof(rs3d_pair), hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess) {
+        set_error("merge_and_deduplicate_pairs_gpu: copy of the first result array failed");
+        fail_fast("merge_and_deduplicate_pairs_gpu");
+    }
+    if (d_results2 && num_results2 > 0 && d_results2 != d_merged_output + num_results1 &&
+        hipMemcpyAsync(d_merged_output + num_results1, d_results2, num_results2 * sizeof(rs3d_pair),
+                       hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        set_error("merge_and_deduplicate_pairs_gpu: copy of the second result array failed");
+        fail_fast("merge_and_deduplicate_pairs_gpu");
+    }
     // The pair struct read as a little-endian u64 is (mesh2 << 32 | mesh1): sorting it
     // reproduces the reference's (b, a) order.
     uint64_t u = 0;
-    if (rs3d_sort_unique_u64(ctx, reinterpret_cast<uint64_t*>(d_merged_output), (uint64_t)total, &u) !=
-        RS3D_OK)
-        return 0;
+    if (dedup_inplace(ctx, reinterpret_cast<uint64_t*>(d_merged_output), (uint64_t)total, &u) != RS3D_OK)
+        fail_fast("merge_and_deduplicate_pairs_gpu");
     return (long long)u;
 }
 
 int compact_hash_table_pairs(const unsigned long long* d_hash_table, unsigned long long table_size,
                              rs3d_pair* d_output, int max_output_size) {
-    rs3d_ctx* ctx = default_ctx();
-    if (!ctx || !d_hash_table || table_size == 0 || max_output_size <= 0 || !d_output) return 0;
+    if (table_size == 0 || max_output_size == 0) return 0;
+    rs3d_ctx* ctx = default_ctx_or_die("compact_hash_table_pairs");
+    if (!d_hash_table || max_output_size < 0 || !d_output) {
+        set_error("compact_hash_table_pairs: bad arguments (max_output_size %d)", max_output_size);
+        fail_fast("compact_hash_table_pairs");
+    }
     static thread_local DevBuf keys;
-    if (keys.ensure((size_t)max_output_size * sizeof(uint64_t)) != RS3D_OK) return 0;
+    if (keys.ensure((size_t)max_output_size * sizeof(uint64_t)) != RS3D_OK) fail_fast("compact_hash_table_pairs");
     uint64_t c = 0;
     if (ht_compact_internal(ctx, reinterpret_cast<const uint64_t*>(d_hash_table), table_size,
                             keys.as<uint64_t>(), (uint64_t)max_output_size, &c) != RS3D_OK)
-        return 0;
+        fail_fast("compact_hash_table_pairs");
     const uint64_t w = c < (uint64_t)max_output_size ? c : (uint64_t)max_output_size;
     if (w) {
         unsigned nb;
-        if (grid_blocks(w, 256, &nb) != RS3D_OK) return 0;
+        if (grid_blocks(w, 256, &nb) != RS3D_OK) fail_fast("compact_hash_table_pairs");
         k_keys_to_pairs<<<nb, 256, 0, ctx->stream>>>(keys.as<uint64_t>(), w, d_output);
-        (void)hipStreamSynchronize(ctx->stream);
+        const hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            set_error("compact_hash_table_pairs: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+            fail_fast("compact_hash_table_pairs");
+        }
     }
     return (int)w;
 }

@@ -998,12 +998,12 @@ int rs3d_sort_unique_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n, uint64_t* un
         if (unique_count) *unique_count = 0;
         return RS3D_OK;
     }
-    // unique into scratch, then copy back
-    RS3D_TRY(ctx->tmp_c.ensure(n * sizeof(uint64_t)));
+    // unique into the sort's alternate buffer (n keys, free again after the sort), then back
+    RS3D_TRY(ctx->sort_keys.ensure(n * sizeof(uint64_t)));
     uint64_t cnt = 0;
-    RS3D_TRY(unique_u64(ctx, keys, n, ctx->tmp_c.as<uint64_t>(), &cnt));
+    RS3D_TRY(unique_u64(ctx, keys, n, ctx->sort_keys.as<uint64_t>(), &cnt));
     if (cnt)
-        RS3D_HIP(hipMemcpyAsync(keys, ctx->tmp_c.ptr, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+        RS3D_HIP(hipMemcpyAsync(keys, ctx->sort_keys.ptr, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                                 ctx->stream));
     RS3D_HIP(hipStreamSynchronize(ctx->stream));
     if (unique_count) *unique_count = cnt;

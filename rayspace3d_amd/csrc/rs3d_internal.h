@@ -182,6 +182,7 @@ __device__ __forceinline__ void prim_padded_box(float3 p0, float3 p1, float3 p2,
 // Host-side objects
 // ------------------------------------------------------------------------------------
 void set_error(const char* fmt, ...);
+const char* last_error();
 
 #define RS3D_HIP(call)                                                                  \
     do {                                                                                \
@@ -261,9 +262,9 @@ void seg_occ_free(SegOcc* g);
 struct rs3d_bvh {
     rs3d_ctx* ctx = nullptr;
     rs3d::AxisGrid* agrid[3] = {nullptr, nullptr, nullptr};  // per ray axis, built lazily
-    int agrid_state[3] = {0, 0, 0};                           // 0 untried, 1 built, 2 rejected
+    int agrid_state[3] = {0, 0, 0};  // 0 untried, 1 built, 2 rejected, 3 refused for memory
     rs3d::SegOcc* socc = nullptr;  // segment occupancy bits, built lazily
-    int socc_state = 0;            // 0 untried, 1 built, 2 rejected
+    int socc_state = 0;            // 0 untried, 1 built, 2 rejected, 3 refused for memory
     uint64_t num_tris = 0;
     uint64_t num_nodes = 0;
     int leaf_size = 4;
@@ -294,8 +295,10 @@ int radix_sort_u32_pairs_to(rs3d_ctx* ctx, uint32_t* keys, uint32_t* vals, uint6
 int radix_sort_u64(rs3d_ctx* ctx, uint64_t* keys, uint64_t n);
 int radix_sort_u64_pairs(rs3d_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n);  // stable
 // Whether point launches along +axis go to the axis grid of the BVH: the context allows it
-// and a grid is accepted (built here on first use).
-int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready);
+// and a grid is accepted (built here on first use unless build is false).
+int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready, bool build = true);
+// Upper bound of the device bytes an axis grid build of this BVH takes at its peak.
+uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh);
 // Filter-refine on the grid in one launch (axis_grid_ready said yes): box = dataset box
 // (host float[6]); out_obj gets every point's answer; counts (a zeroed striped
 // accumulator) accumulates candidates << 32 | inside.
@@ -304,7 +307,10 @@ int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float*
 // Segment occupancy bits of a BVH for edge rays (built here on first use when the context
 // allows them); *args gets bits = null when there are none.
 struct OccArgs;
-int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args);
+int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args, bool build = true);
+// Upper bound of the device bytes of a BVH's segment occupancy bits (2^30 cells at most).
+uint64_t seg_occ_bytes_bound(const rs3d_bvh* bvh);
+uint64_t seg_occ_bytes(const rs3d_bvh* bvh);  // bytes of the built bits (0 when none)
 // Point launch on the grid (axis_grid_ready said yes).
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                      const uint32_t* imap, int out_kind, void* out);

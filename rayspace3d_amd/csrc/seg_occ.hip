@@ -201,7 +201,15 @@ int ensure(rs3d_ctx* ctx, rs3d_bvh* bvh) {
     }
     const auto t0 = std::chrono::steady_clock::now();
     SegOcc* g = nullptr;
-    RS3D_TRY(build(ctx, bvh, &g));
+    const int rc = build(ctx, bvh, &g);
+    if (rc == RS3D_E_NOMEM) {
+        // the bits only let edge rays skip the walk: without memory for them every edge
+        // walks the tree (same results); state 3 is retried only by the prepare call
+        (void)hipGetLastError();
+        bvh->socc_state = 3;
+        return RS3D_OK;
+    }
+    if (rc != RS3D_OK) return rc;
     if (g) g->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     bvh->socc = g;
     bvh->socc_state = g ? 1 : 2;
@@ -210,12 +218,19 @@ int ensure(rs3d_ctx* ctx, rs3d_bvh* bvh) {
 
 }  // namespace
 
-int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args) {
+int seg_occ_args(rs3d_ctx* ctx, rs3d_bvh* bvh, OccArgs* args, bool build) {
     *args = OccArgs{};
     if (!ctx->opt_seg_occupancy) return RS3D_OK;
-    RS3D_TRY(ensure(ctx, bvh));
-    if (bvh->socc) *args = bvh->socc->args;
+    if (build) RS3D_TRY(ensure(ctx, bvh));
+    if (bvh->socc_state == 1 && bvh->socc) *args = bvh->socc->args;
     return RS3D_OK;
+}
+
+uint64_t seg_occ_bytes(const rs3d_bvh* bvh) { return bvh->socc ? bvh->socc->bytes : 0; }
+
+uint64_t seg_occ_bytes_bound(const rs3d_bvh* bvh) {
+    (void)bvh;
+    return (1ull << 30) / 8 + (16ull << 20);  // bits of at most 2^30 cells, and slack
 }
 
 }  // namespace rs3d
@@ -230,6 +245,7 @@ int rs3d_seg_occupancy_prepare(rs3d_ctx* ctx, rs3d_bvh* bvh, rs3d_seg_occupancy_
         return RS3D_E_ARG;
     }
     RS3D_HIP(hipSetDevice(ctx->device));
+    if (bvh->socc_state == 3) bvh->socc_state = 0;  // refused for memory: try again
     RS3D_TRY(ensure(ctx, bvh));
     if (info) {
         memset(info, 0, sizeof *info);

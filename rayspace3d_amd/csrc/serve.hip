@@ -10,9 +10,19 @@
 // arrive while a step runs are admitted at the next iteration. Each request's result is
 // identical to running it alone (the batch engine runs the same per-ray programs and
 // keeps one pair set per request).
+//
+// The memory budget covers what the server keeps resident -- per dataset the first
+// triangle of each object and the grid histogram, and the axis grids and segment
+// occupancy bits of the datasets' BVHs -- plus the per-step memory of the admitted
+// requests. Grids and bits only speed requests up (results are identical on the BVH
+// walk), so the server builds one only when its upper-bound size fits what the budget
+// leaves, before the step's launch; otherwise the request runs on the walk. The BVHs and
+// mesh arrays themselves belong to the caller and are not counted.
 #include <algorithm>
 #include <cstring>
 #include <deque>
+#include <set>
+#include <utility>
 #include <vector>
 
 #include "batch_plan.h"
@@ -60,6 +70,23 @@ int batch_keys_to_pairs(rs3d_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_
 }  // namespace rs3d
 
 using namespace rs3d;
+
+// Device bytes resident for the server's datasets: its own per-dataset arrays and the
+// accelerating structures that exist for their BVHs (each BVH counted once).
+static uint64_t resident_bytes(const rs3d_server* s) {
+    uint64_t b = 0;
+    std::set<const rs3d_bvh*> seen;
+    for (const ServerDataset* d : s->ds) {
+        b += d->first_tri.bytes + d->grid.bytes;
+        if (!seen.insert(d->bvh).second) continue;
+        for (int ax = 0; ax < 3; ++ax) {
+            rs3d_axis_grid_info gi;
+            if (rs3d_axis_grid_query(d->bvh, ax, &gi) == RS3D_OK && gi.state == 1) b += gi.device_bytes;
+        }
+        if (d->bvh->socc_state == 1) b += seg_occ_bytes(d->bvh);
+    }
+    return b;
+}
 
 static uint64_t pow2_ge(uint64_t v) {
     uint64_t p = 1024;
@@ -195,12 +222,41 @@ int rs3d_server_step(rs3d_server* s, int* admitted, int* completed) {
     if (admitted) *admitted = 0;
     if (completed) *completed = 0;
     if (s->queue.empty()) return RS3D_OK;
-    // admission: FIFO, slots and the memory budget (the head request is always admitted)
+    // admission: FIFO, slots and the memory budget (the head request is always admitted).
+    // A request whose dataset lacks a grid (containment) or occupancy bits (overlap,
+    // intersection) plans their build when the bound of their size fits the budget next to
+    // what is resident and admitted; otherwise it runs without them.
     std::vector<int64_t> batch;
     uint64_t mem = 0, est_total = 0, scratch_bytes = 0;
+    const uint64_t resident = resident_bytes(s);
+    std::vector<std::pair<const rs3d_bvh*, int>> builds;  // (bvh, axis); axis -1: occupancy bits
+    uint64_t build_bytes = 0;
+    auto planned = [&](const rs3d_bvh* b, int ax) {
+        return std::find(builds.begin(), builds.end(), std::make_pair(b, ax)) != builds.end();
+    };
     while (!s->queue.empty() && (int)batch.size() < s->max_slots) {
         const ServerRequest& r = s->reqs[s->queue.front()];
-        if (!batch.empty() && mem + r.mem > s->budget) break;
+        if (!batch.empty() && resident + build_bytes + mem + r.mem > s->budget) break;
+        std::vector<std::pair<const rs3d_bvh*, int>> want;
+        uint64_t want_bytes = 0;
+        if (r.req.kind == RS3D_REQ_CONTAIN) {
+            const rs3d_bvh* b = s->ds[r.req.dataset]->bvh;
+            if (s->ctx->opt_axis_grid && b->num_tris && b->agrid_state[r.req.axis] == 0 && !planned(b, r.req.axis)) {
+                want.emplace_back(b, r.req.axis);
+                want_bytes += axis_grid_bytes_bound(b);
+            }
+        } else if (s->ctx->opt_seg_occupancy) {
+            for (const rs3d_bvh* b : {s->ds[r.req.dataset]->bvh, s->ds[r.req.query_dataset]->bvh})
+                if (b->num_tris && b->socc_state == 0 && !planned(b, -1) &&
+                    std::find(want.begin(), want.end(), std::make_pair(b, -1)) == want.end()) {
+                    want.emplace_back(b, -1);
+                    want_bytes += seg_occ_bytes_bound(b);
+                }
+        }
+        if (resident + build_bytes + want_bytes + mem + r.mem <= s->budget) {
+            builds.insert(builds.end(), want.begin(), want.end());
+            build_bytes += want_bytes;
+        }
         mem += r.mem;
         est_total += (uint64_t)r.estimate;
         if (r.req.kind == RS3D_REQ_INTERSECT)
@@ -208,12 +264,20 @@ int rs3d_server_step(rs3d_server* s, int* admitted, int* completed) {
         batch.push_back(s->queue.front());
         s->queue.pop_front();
     }
-    s->peak_mem = std::max(s->peak_mem, mem);
+    for (const auto& bw : builds) {
+        rs3d_bvh* b = const_cast<rs3d_bvh*>(bw.first);
+        if (bw.second >= 0)
+            RS3D_TRY(rs3d_axis_grid_prepare(s->ctx, b, bw.second, nullptr));
+        else
+            RS3D_TRY(rs3d_seg_occupancy_prepare(s->ctx, b, nullptr));
+    }
+    s->peak_mem = std::max(s->peak_mem, (builds.empty() ? resident : resident_bytes(s)) + mem);
     RS3D_TRY(s->scratch.ensure(scratch_bytes + 256));
     uint8_t* scr = s->scratch.as<uint8_t>();
     size_t so = 0;
     BatchPlan& P = s->plan;
     P.reset(s->ctx);
+    P.build_structures = false;  // only what exists now (built above when the budget allowed)
     for (size_t k = 0; k < batch.size(); ++k) {
         const rs3d_request& r = s->reqs[batch[k]].req;
         const int qid = (int)k;

@@ -51,6 +51,81 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+# ---- argument checks ------------------------------------------------------------------
+# The C ABI takes raw device pointers and cannot see dtypes, shapes or lengths, so the
+# wrappers check them first and raise ValueError: a wrong dtype or a short buffer would
+# otherwise be read or written out of bounds on the device. Checks run in the order dtype,
+# shape, layout, index values, device.
+
+_INT4 = tuple(t for t in (torch.int32, getattr(torch, "uint32", None)) if t is not None)
+
+
+def _need(cond, msg):
+    if not cond:
+        raise ValueError(msg)
+
+
+def _on_device(t, device, name):
+    _need(t.is_cuda and (t.device.index if t.device.index is not None else torch.cuda.current_device()) == device,
+          f"{name} must be on cuda:{device} (got {t.device})")
+
+
+def check_points(points, device: int, name: str = "points"):
+    _need(isinstance(points, torch.Tensor), f"{name} must be a torch tensor")
+    _need(points.dtype == torch.float32, f"{name} must be float32 (got {points.dtype})")
+    _need(points.dim() == 2 and points.shape[1] == 3, f"{name} must have shape (n, 3) (got {tuple(points.shape)})")
+    _need(points.is_contiguous(), f"{name} must be contiguous")
+    _on_device(points, device, name)
+
+
+def check_index_map(index_map, n_points: int, device: int):
+    """index_map: 4-byte integers, every value in [0, n_points)."""
+    _need(isinstance(index_map, torch.Tensor), "index_map must be a torch tensor")
+    _need(index_map.dtype in _INT4, f"index_map must hold 4-byte integers (int32), got {index_map.dtype}")
+    _need(index_map.dim() == 1 and index_map.is_contiguous(), "index_map must be a contiguous 1-D tensor")
+    if index_map.numel():
+        lo, hi = int(index_map.min().item()), int(index_map.max().item())
+        _need(0 <= lo and hi < n_points, f"index_map values must lie in [0, {n_points}) (got [{lo}, {hi}])")
+    _on_device(index_map, device, "index_map")
+
+
+_OUT_KINDS = {_lib.OUT_FLAG_U8: (torch.uint8, 1), _lib.OUT_OBJ_I32: (torch.int32, 1),
+              _lib.OUT_RAY_RESULT: (torch.int32, 2)}
+
+
+def check_out(out, out_kind: int, slots: int, device: int, name: str = "out"):
+    """out: the dtype of out_kind with at least `slots` result slots (ray results: (m, 2))."""
+    _need(out_kind in _OUT_KINDS, f"unknown out_kind {out_kind}")
+    dt, width = _OUT_KINDS[out_kind]
+    _need(isinstance(out, torch.Tensor), f"{name} must be a torch tensor")
+    _need(out.dtype == dt, f"{name} must be {dt} for out_kind {out_kind} (got {out.dtype})")
+    if width == 1:
+        _need(out.dim() == 1, f"{name} must be 1-D (got shape {tuple(out.shape)})")
+    else:
+        _need(out.dim() == 2 and out.shape[1] == 2, f"{name} must have shape (m, 2) (got {tuple(out.shape)})")
+    _need(out.shape[0] >= slots, f"{name} has {out.shape[0]} slots, {slots} needed")
+    _need(out.is_contiguous(), f"{name} must be contiguous")
+    _on_device(out, device, name)
+
+
+def check_mesh_arrays(vertices, indices, tri_to_obj, device: int):
+    _need(vertices.dtype == torch.float32 and vertices.dim() == 2 and vertices.shape[1] == 3,
+          f"vertices must be float32 (nV, 3) (got {vertices.dtype} {tuple(vertices.shape)})")
+    _need(indices.dtype in _INT4 and indices.dim() == 2 and indices.shape[1] == 3,
+          f"indices must be 4-byte integers (nT, 3) (got {indices.dtype} {tuple(indices.shape)})")
+    _need(vertices.is_contiguous() and indices.is_contiguous(), "vertices and indices must be contiguous")
+    if tri_to_obj is not None:
+        _need(tri_to_obj.dtype in _INT4 and tri_to_obj.dim() == 1 and tri_to_obj.shape[0] == indices.shape[0]
+              and tri_to_obj.is_contiguous(), "tri_to_obj must be a contiguous int32 array, one per triangle")
+    if indices.numel():
+        lo, hi = int(indices.min().item()), int(indices.max().item())
+        _need(0 <= lo and hi < vertices.shape[0],
+              f"indices must lie in [0, {vertices.shape[0]}) (got [{lo}, {hi}])")
+    for t, nm in ((vertices, "vertices"), (indices, "indices"), (tri_to_obj, "tri_to_obj")):
+        if t is not None:
+            _on_device(t, device, nm)
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.lib = _lib.load()
@@ -99,6 +174,7 @@ class Bvh:
                  tri_to_obj: torch.Tensor | None, leaf_size: int = 1):
         self.ctx = ctx
         self.handle = C.c_void_p()
+        check_mesh_arrays(vertices, indices, tri_to_obj, ctx.device)
         opts = _lib.BvhOpts()
         opts.leaf_size = leaf_size
         torch.cuda.synchronize()
@@ -151,7 +227,11 @@ class Bvh:
 def trace_points(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2,
                  out_kind: int = _lib.OUT_OBJ_I32, index_map: torch.Tensor | None = None,
                  out: torch.Tensor | None = None, sync: bool = True) -> torch.Tensor:
+    check_points(points, ctx.device)
     n = points.shape[0]
+    _need(out_kind in _OUT_KINDS, f"unknown out_kind {out_kind}")
+    if index_map is not None:
+        check_index_map(index_map, n, ctx.device)
     if out is None:
         if out_kind == _lib.OUT_FLAG_U8:
             out = torch.empty(n, dtype=torch.uint8, device=points.device)
@@ -159,6 +239,8 @@ def trace_points(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2,
             out = torch.empty(n, dtype=torch.int32, device=points.device)
         else:
             out = torch.empty((n, 2), dtype=torch.int32, device=points.device)
+    # results land at out[index_map[i]] (index_map values < n) or out[i]
+    check_out(out, out_kind, n, ctx.device)
     nrays = index_map.shape[0] if index_map is not None else n
     if sync:
         torch.cuda.synchronize()
@@ -170,6 +252,8 @@ def trace_points(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2,
 
 
 def trace_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> dict:
+    check_points(points, ctx.device)
+    _need(axis in (0, 1, 2), f"axis must be 0, 1 or 2 (got {axis})")
     st = _lib.TraceStats()
     torch.cuda.synchronize()
     check(ctx.lib.rs3d_trace_points_stats(ctx.handle, bvh.handle, _ptr(points), points.shape[0],
@@ -198,6 +282,7 @@ def seg_occupancy_prepare(ctx: Context, bvh: Bvh) -> dict:
 def axis_grid_stats(ctx: Context, bvh: Bvh, points: torch.Tensor, axis: int = 2) -> dict:
     """Work counters of a point launch on the axis grid: nodes_visited = cell lookups,
     tris_tested = 64-byte records read."""
+    check_points(points, ctx.device)
     st = _lib.TraceStats()
     torch.cuda.synchronize()
     check(ctx.lib.rs3d_axis_grid_stats(ctx.handle, bvh.handle, _ptr(points), points.shape[0], axis, C.byref(st)),
@@ -220,9 +305,11 @@ class FilterRefinePipeline:
 
     def execute(self, bvh: Bvh, points: torch.Tensor, axis: int = 2,
                 out: torch.Tensor | None = None, sync: bool = True) -> FilterRefineResult:
+        check_points(points, self.ctx.device)
         n = points.shape[0]
         if out is None:
             out = torch.empty(n, dtype=torch.int32, device=points.device)
+        check_out(out, _lib.OUT_OBJ_I32, n, self.ctx.device)
         cand = C.c_uint64()
         inside = C.c_uint64()
         if sync:
@@ -293,8 +380,12 @@ def overlap_join(ctx, bvh1, dedges1, bvh2, dedges2, mode: int = 0, max_out: int 
 def tri_intersect(ctx, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
     """Exact triangle-triangle predicate on pairs (rs3d_tri_intersect): a, b float32
     device tensors [n, 3, 3] -> uint8 [n]."""
+    _need(a.dtype == torch.float32 and b.dtype == torch.float32, "tri_intersect: a and b must be float32")
+    _need(a.numel() % 9 == 0 and b.numel() % 9 == 0, "tri_intersect: a and b must hold 9 floats per triangle")
     a = a.reshape(-1, 9).contiguous()
     b = b.reshape(-1, 9).contiguous()
+    _on_device(a, ctx.device, "a")
+    _on_device(b, ctx.device, "b")
     if a.shape != b.shape:
         raise ValueError("tri_intersect: a and b must hold the same number of triangles")
     out = torch.empty(max(a.shape[0], 1), dtype=torch.uint8, device="cuda")
@@ -326,6 +417,9 @@ def tri_join_stats(ctx, bvh_a, bvh_b) -> dict:
 
 
 def sort_unique_u64(ctx, keys: torch.Tensor) -> torch.Tensor:
+    _need(keys.dtype == torch.int64 and keys.dim() == 1 and keys.is_contiguous(),
+          "sort_unique_u64: keys must be a contiguous 1-D int64 tensor (u64 bit patterns)")
+    _on_device(keys, ctx.device, "keys")
     cnt = C.c_uint64()
     torch.cuda.synchronize()
     check(ctx.lib.rs3d_sort_unique_u64(ctx.handle, _ptr(keys), keys.shape[0], C.byref(cnt)),
@@ -380,12 +474,14 @@ def containment_join(ctx, bvha, dedgesa, bvhb, dedgesb, max_out=1 << 20):
         return (to_host(cont[:nc.value], np.uint64), to_host(ov[:no.value], np.uint64), over.value)
 
 
-def _batch_array(queries):
+def _batch_array(queries, device: int):
     n = len(queries)
     arr = (_lib.BatchQuery * max(n, 1))()
     for i, q in enumerate(queries):
         b = arr[i]
         if q["kind"] == "contain":
+            check_points(q["points"], device, f"queries[{i}]['points']")
+            check_out(q["out"], _lib.OUT_OBJ_I32, q["points"].shape[0], device, f"queries[{i}]['out']")
             b.kind = 0
             b.axis = q.get("axis", 2)
             b.bvh1 = q["bvh"].handle.value
@@ -408,7 +504,7 @@ def batch_execute(ctx, queries, max_pairs=1 << 22):
     Returns a list with the sorted pair keys (uint64 numpy) of each query (empty for
     containment queries, whose ids land in their "out" tensors)."""
     n = len(queries)
-    arr = _batch_array(queries)
+    arr = _batch_array(queries, ctx.device)
     offs = (C.c_uint64 * (n + 1))()
     torch.cuda.synchronize()
     while True:
@@ -429,7 +525,7 @@ class CapturedBatch:
 
     def __init__(self, ctx, queries):
         self.ctx, self.n = ctx, len(queries)
-        self._arr = _batch_array(queries)
+        self._arr = _batch_array(queries, ctx.device)
         self._keep = queries
         self.handle = C.c_void_p()
         torch.cuda.synchronize()
@@ -478,6 +574,9 @@ class Server:
 
     def submit(self, kind: int, dataset: int, query_dataset: int = -1, points=None, out=None,
                axis: int = 2) -> int:
+        if kind == _lib.REQ_CONTAIN:
+            check_points(points, self.ctx.device)
+            check_out(out, _lib.OUT_OBJ_I32, points.shape[0], self.ctx.device)
         r = _lib.Request(kind, dataset, query_dataset, axis,
                          points.data_ptr() if points is not None else None,
                          points.shape[0] if points is not None else 0,
