@@ -332,6 +332,83 @@ int rs3d_point_in_mesh_join(rs3d_ctx* ctx, const rs3d_mesh_view* src_b, const rs
                             int max_targets, const rs3d_hash_table* overlap_ht,
                             rs3d_hash_table* out_ht, uint64_t* overflow);
 
+/* Work counters of the intersection join, field for field the reference's
+ * MeshIntersectionProfilingStats (query/src/cuda/mesh_intersection.h:7-22), accumulated on
+ * the device (clear it before the first launch). */
+typedef struct {
+    uint64_t overlap_trace_calls;
+    uint64_t overlap_iterations_total;
+    uint64_t overlap_hits_total;
+    uint32_t overlap_max_iterations_per_trace;
+    uint64_t containment_rays_total;
+    uint64_t containment_iterations_total;
+    uint64_t containment_hits_total;
+    uint32_t containment_max_iterations_per_ray;
+    uint64_t containment_same_hit_suppressed;
+    uint64_t containment_candidate_additions;
+    uint64_t containment_candidate_toggles;
+    uint64_t containment_candidate_overflow;
+    uint64_t containment_targets_total;
+} rs3d_intersection_stats;
+
+/* Object rays with the reference's launch options (the containment fields of
+ * MeshIntersectionLaunchParams, query/src/cuda/mesh_intersection.h:54-88, and of
+ * MeshContainmentLaunchParams, query/src/cuda/mesh_containment.h:31-47).
+ * chain_mode RS3D_OBJ_ALL_HITS: every hit in (t, triangle) order over [1e-4, 1e10], a hit of
+ *   the same object within 1e-5 of the previous hit merged (the reference's default any-hit
+ *   forms, mesh_intersection_anyhit.cu:46-240 and mesh_containment.cu:196-256);
+ * RS3D_OBJ_CHAIN_NEXTAFTER: one closest hit per iteration, next tmin = nextafterf(t, 1e10),
+ *   at most max_iterations (0: 2048) (use_anyhit_containment = 0, mesh_intersection.cu:134-230);
+ * RS3D_OBJ_CHAIN_STEP: one closest hit per iteration over [tmin, 1e10), next tmin =
+ *   fl(t + 1e-4), at most max_iterations (0: 2000) (use_anyhit_point_in_mesh = 0,
+ *   mesh_containment.cu:336-366).
+ * origins: optional device float3 per source object (num_sources of them) instead of the
+ * first vertex of each object's first triangle (MeshContainmentLaunchParams::b_first_vertices).
+ * Optional per-source device outputs: pair_target_ids / pair_target_hits
+ * [num_sources * max_pair_targets] (odd targets in first-hit order and their hit counts, -1 /
+ * 0 beyond; PairHitTracking.h:28-110), iterations / candidate_count / candidate_overflow
+ * [num_sources] (ContainmentTracking.h:20-95); stats accumulates the containment counters.
+ * Two-pass output of the fallback form (instead of a hash table): counts (pass 1, odd
+ * targets per source, at most 256), offsets + results (pass 2). */
+enum { RS3D_OBJ_ALL_HITS = 0, RS3D_OBJ_CHAIN_NEXTAFTER = 1, RS3D_OBJ_CHAIN_STEP = 2 };
+enum { RS3D_OBJ_FALLBACK = 0, RS3D_OBJ_POINT_IN_MESH = 1 };
+typedef struct {
+    int32_t chain_mode;
+    int32_t max_iterations;
+    const float* origins;
+    uint64_t num_sources;
+    int32_t* pair_target_ids;
+    uint32_t* pair_target_hits;
+    int32_t max_pair_targets;
+    uint32_t* iterations;
+    uint32_t* candidate_count;
+    uint32_t* candidate_overflow;
+    rs3d_intersection_stats* stats;
+    int32_t* counts;
+    const int64_t* offsets;
+    rs3d_pair* results;
+} rs3d_object_ray_opts;
+/* form RS3D_OBJ_FALLBACK: (src, tgt), or (tgt, src) when swap, for every odd target (the
+ * first 256), into ht or the two-pass outputs; RS3D_OBJ_POINT_IN_MESH: (tgt, src) for every
+ * odd target whose pair is not in overlap_ht, into ht. src may be null when opts->origins is
+ * given. *overflow (host) counts targets beyond max_targets. */
+int rs3d_object_rays(rs3d_ctx* ctx, const rs3d_mesh_view* src, const rs3d_bvh* target, int form,
+                     int swap, int max_targets, const rs3d_object_ray_opts* opts, rs3d_hash_table* ht,
+                     const rs3d_hash_table* overlap_ht, uint64_t* overflow);
+
+/* Overlap counters of edge rays (the overlap fields of rs3d_intersection_stats, as
+ * trace_edge_multi_hits counts them per source object, query/src/cuda/mesh_intersection.cu:
+ * 16-91): one trace call per source object of every edge of length >= 1e-6, min(hits + 1,
+ * max_iterations) iterations each. Accumulates into stats (device). */
+int rs3d_edge_profile(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                      int max_iterations, rs3d_intersection_stats* stats);
+/* Counting and writing passes of edge rays with an explicit chain rule and cap (the two-pass
+ * form of the intersection module's overlap launches). */
+int rs3d_edge_count_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                       int max_iterations, int32_t* counts);
+int rs3d_edge_write_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                       int max_iterations, const int64_t* offsets, int swap, rs3d_pair* out);
+
 /* Whole intersection join (raytracer_intersection_estimated / raytracer_mesh_intersection,
  * query/src/applications/raytracer_intersection_estimated_main.cpp:77-176): overlap edge
  * rays 1->2 and 2->1 (cap overlap_max_iterations, reference default 100) plus the

@@ -746,76 +746,207 @@ int64_t orc_extract_edges(const float* V, int64_t nV, const uint32_t* I, int64_t
 /* 46-240; query/src/cuda/mesh_containment.cu:196-377)                                   */
 /* ------------------------------------------------------------------------------------ */
 
-int64_t orc_object_parity(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
-                          int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
-                          const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
-                          int swap, uint64_t* out_keys, int64_t cap, int64_t* overflow) {
+/* Per-candidate parity list of one object ray: the first max_targets distinct targets
+ * in hit order, with parity and hit counts (toggles + the addition). */
+typedef struct {
+    int32_t* ids;
+    unsigned char* par;
+    uint32_t* hits;
+    int count, max;
+    uint64_t additions, toggles, overflow;
+} cands_t;
+
+static void cands_hit(cands_t* c, int obj) {
+    for (int q = 0; q < c->count; ++q)
+        if (c->ids[q] == obj) {
+            c->par[q] ^= 1;
+            c->hits[q]++;
+            c->toggles++;
+            return;
+        }
+    if (c->count < c->max) {
+        c->ids[c->count] = obj;
+        c->par[c->count] = 1;
+        c->hits[c->count] = 1;
+        c->count++;
+        c->additions++;
+    } else {
+        c->overflow++;
+    }
+}
+
+/* Index of the first hit (in (t, tri) order) with t >= thr, or n. */
+static int64_t first_at_or_above(const hitvec* hv, float thr) {
+    int64_t lo = 0, hi = hv->n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (hv->h[mid].t < thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+int64_t orc_object_parity_ex(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
+                             int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
+                             const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
+                             int swap, orc_objray_opts* opt, uint64_t* out_keys, int64_t cap,
+                             int64_t* overflow) {
     (void)nV;
     const float d[3] = {0.0f, 0.0f, 1.0f};
     orc_raypre rp;
     orc_ray_pre(d, &rp);
+    const int mode = opt ? opt->chain_mode : 0;
+    const int max_iter = opt ? opt->max_iter : 0;
     int32_t* first = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_src_obj > 0 ? n_src_obj : 1));
     for (int64_t o = 0; o < n_src_obj; ++o) first[o] = -1;
     for (int64_t t = 0; t < src_nT; ++t) {
-        const int32_t o = src_tri_obj[t];
+        const int32_t o = src_tri_obj ? src_tri_obj[t] : -1;
         if (o >= 0 && o < n_src_obj && first[o] < 0) first[o] = (int32_t)t;
     }
-    int32_t* ids = (int32_t*)malloc(sizeof(int32_t) * (size_t)(max_targets > 0 ? max_targets : 1));
-    unsigned char* par = (unsigned char*)malloc((size_t)(max_targets > 0 ? max_targets : 1));
+    const int mt = max_targets > 0 ? max_targets : 1;
+    cands_t c;
+    c.ids = (int32_t*)malloc(sizeof(int32_t) * (size_t)mt);
+    c.par = (unsigned char*)malloc((size_t)mt);
+    c.hits = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)mt);
+    c.max = max_targets;
     grid_t G;
     const int axes[2] = {0, 1};
     if (nT > 0) grid_build(&G, 2, axes, V, I, nT, 1.0);
     hitvec hv = {0, 0, 0};
     int64_t total = 0, over = 0;
+    uint64_t* st = opt ? opt->stats : NULL;
     for (int64_t s = 0; s < n_src_obj; ++s) {
-        if (first[s] < 0 || nT == 0) continue;
-        const float* o = srcV + 3 * (int64_t)srcI[3 * (int64_t)first[s]];
+        const int has_origin = opt && opt->origins ? 1 : first[s] >= 0;
+        if (!has_origin) continue;
+        const float* o = opt && opt->origins ? opt->origins + 3 * s : srcV + 3 * (int64_t)srcI[3 * (int64_t)first[s]];
         hv.n = 0;
-        const int cu = cell_coord(&G, 0, o[0]), cv = cell_coord(&G, 1, o[1]);
-        if (cu >= 0 && cu < G.g[0] && cv >= 0 && cv < G.g[1]) {
-            const int64_t c = cu + (int64_t)G.g[0] * cv;
-            for (int64_t k = G.start[c]; k < G.start[c + 1]; ++k)
-                test_tri_collect(V, I, G.items[k], o, &rp, 1e-4f, 1e10f, 1, &hv);
+        if (nT > 0) {
+            const int cu = cell_coord(&G, 0, o[0]), cv = cell_coord(&G, 1, o[1]);
+            if (cu >= 0 && cu < G.g[0] && cv >= 0 && cv < G.g[1]) {
+                const int64_t cc = cu + (int64_t)G.g[0] * cv;
+                for (int64_t k = G.start[cc]; k < G.start[cc + 1]; ++k)
+                    test_tri_collect(V, I, G.items[k], o, &rp, 1e-4f, 1e10f, 1, &hv);
+            }
         }
         qsort(hv.h, (size_t)hv.n, sizeof(hit_t), hit_cmp);
-        int count = 0, has_last = 0, last_obj = -1;
-        float last_t = 0.0f;
-        for (int64_t j = 0; j < hv.n; ++j) {
-            const float t = hv.h[j].t;
-            const int obj = tri_obj[hv.h[j].tri];
-            const int same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
-            has_last = 1;
-            last_obj = obj;
-            last_t = t;
-            if (same) continue;
-            int pos = -1;
-            for (int q = 0; q < count; ++q)
-                if (ids[q] == obj) {
-                    pos = q;
-                    break;
+        c.count = 0;
+        c.additions = c.toggles = c.overflow = 0;
+        uint64_t iters = 0, hits = 0, suppressed = 0;
+        if (mode == 0) {
+            /* all hits in (t, tri) order, a same-object hit within 1e-5 of the previous merged */
+            int has_last = 0, last_obj = -1;
+            float last_t = 0.0f;
+            iters = 1;
+            for (int64_t j = 0; j < hv.n; ++j) {
+                const float t = hv.h[j].t;
+                const int obj = tri_obj[hv.h[j].tri];
+                const int same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
+                has_last = 1;
+                last_obj = obj;
+                last_t = t;
+                ++hits;
+                if (same) {
+                    ++suppressed;
+                    continue;
                 }
-            if (pos >= 0)
-                par[pos] ^= 1;
-            else if (count < max_targets) {
-                ids[count] = obj;
-                par[count] = 1;
-                ++count;
-            } else
-                ++over;
+                cands_hit(&c, obj);
+            }
+        } else if (mode == 1) {
+            /* closest-hit loop of mesh_intersection.cu:134-230: one trace per iteration over
+             * [thr, 1e10], next threshold nextafterf(t, 1e10) */
+            const float tmax = 1e10f;
+            float thr = 1e-4f, last_t = -1.0f;
+            uint32_t last_tri = 0xffffffffu;
+            const int cap_it = max_iter > 0 ? max_iter : 2048;
+            for (int it = 0; it < cap_it; ++it) {
+                ++iters;
+                if (thr > tmax) break;
+                const int64_t j = first_at_or_above(&hv, thr);
+                if (j == hv.n) break;
+                const float t = hv.h[j].t;
+                const uint32_t tri = hv.h[j].tri;
+                ++hits;
+                const int same = tri == last_tri && last_t >= 0.0f && fabsf(t - last_t) <= 1e-5f;
+                if (same) ++suppressed;
+                else cands_hit(&c, tri_obj[tri]);
+                last_t = t;
+                last_tri = tri;
+                float nx = nextafterf(t, tmax);
+                if (nx <= t) nx = t + 1e-6f;
+                if (nx <= thr) nx = nextafterf(thr, tmax);
+                if (nx <= thr) break;
+                thr = nx;
+            }
+        } else {
+            /* closest-hit loop of mesh_containment.cu:336-366: one trace per iteration over
+             * [thr, 1e10], a hit at 1e10 ends it, next threshold fl(t + 1e-4) (the same hit
+             * again when that add is absorbed) */
+            const float tmax = 1e10f;
+            float thr = 1e-4f;
+            const int cap_it = max_iter > 0 ? max_iter : 2000;
+            for (int it = 0; it < cap_it; ++it) {
+                ++iters;
+                const int64_t j = first_at_or_above(&hv, thr);
+                if (j == hv.n) break;
+                const float t = hv.h[j].t;
+                if (t >= tmax) break;
+                ++hits;
+                cands_hit(&c, tri_obj[hv.h[j].tri]);
+                thr = t + 1e-4f;
+            }
         }
-        for (int q = 0; q < count; ++q) {
-            if (!(par[q] & 1)) continue;
-            const uint32_t a = (uint32_t)(swap ? ids[q] : (int32_t)s);
-            const uint32_t b = (uint32_t)(swap ? (int32_t)s : ids[q]);
+        int odd = 0;
+        const int max_out = opt && opt->max_out > 0 ? opt->max_out : c.count;
+        for (int q = 0; q < c.count; ++q) {
+            if (!(c.par[q] & 1)) continue;
+            if (odd >= max_out) break;
+            if (opt && opt->pair_ids && odd < opt->max_pair) {
+                opt->pair_ids[s * (int64_t)opt->max_pair + odd] = c.ids[q];
+                opt->pair_hits[s * (int64_t)opt->max_pair + odd] = c.hits[q];
+            }
+            ++odd;
+            const uint32_t a = (uint32_t)(swap ? c.ids[q] : (int32_t)s);
+            const uint32_t b = (uint32_t)(swap ? (int32_t)s : c.ids[q]);
             if (total < cap) out_keys[total] = ((uint64_t)a << 32) | b;
             ++total;
         }
+        if (opt && opt->pair_ids)
+            for (int q = odd; q < opt->max_pair; ++q) {
+                opt->pair_ids[s * (int64_t)opt->max_pair + q] = -1;
+                opt->pair_hits[s * (int64_t)opt->max_pair + q] = 0;
+            }
+        if (opt && opt->iterations) {
+            opt->iterations[s] = (uint32_t)iters;
+            opt->cand_count[s] = (uint32_t)c.count;
+            opt->cand_overflow[s] = (uint32_t)c.overflow;
+        }
+        if (st) {
+            st[0] += 1;                  /* rays */
+            st[1] += iters;              /* iterations */
+            st[2] += hits;               /* hits */
+            if (iters > st[3]) st[3] = iters;  /* max iterations per ray */
+            st[4] += suppressed;
+            st[5] += c.additions;
+            st[6] += c.toggles;
+            st[7] += c.overflow;
+            st[8] += (uint64_t)(odd < 256 ? odd : 256);  /* targets (bounded as the reference) */
+        }
+        over += (int64_t)c.overflow;
     }
     free(hv.h);
     free(first);
-    free(ids);
-    free(par);
+    free(c.ids);
+    free(c.par);
+    free(c.hits);
     if (nT > 0) grid_free(&G);
     if (overflow) *overflow = over;
     return total;
+}
+
+int64_t orc_object_parity(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
+                          int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
+                          const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
+                          int swap, uint64_t* out_keys, int64_t cap, int64_t* overflow) {
+    return orc_object_parity_ex(srcV, srcI, src_tri_obj, src_nT, n_src_obj, V, nV, I, nT, tri_obj,
+                                max_targets, swap, NULL, out_keys, cap, overflow);
 }

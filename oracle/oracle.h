@@ -73,6 +73,35 @@ int64_t orc_object_parity(const float* srcV, const uint32_t* srcI, const int32_t
                           const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
                           int swap, uint64_t* out_keys, int64_t cap, int64_t* overflow);
 
+/* Options of orc_object_parity_ex. chain_mode 0: all hits in (t, tri) order with the
+ * same-object 1e-5 merge (the any-hit forms); 1: closest-hit loop with a nextafterf step
+ * (query/src/cuda/mesh_intersection.cu:134-230, cap max_iter, default 2048); 2: closest-hit
+ * loop with a t + 1e-4 step and t < 1e10 (query/src/cuda/mesh_containment.cu:336-366, cap
+ * max_iter, default 2000). origins: optional float3 per source object (instead of the first
+ * vertex of its first triangle). Optional per-source tracking: iterations, cand_count,
+ * cand_overflow [n_src]; pair_ids / pair_hits [n_src * max_pair] (odd targets in candidate
+ * order, -1 / 0 beyond). stats accumulates: rays, iterations, hits, max iterations per ray,
+ * suppressed, additions, toggles, overflow, odd targets (at most 256 per ray). The fallback
+ * form reports at most 256 odd targets per source (max_out = 256,
+ * query/src/cuda/mesh_intersection_anyhit.cu:175, mesh_intersection.cu:268). */
+typedef struct {
+    int chain_mode, max_iter;
+    const float* origins;
+    uint32_t* iterations;
+    uint32_t* cand_count;
+    uint32_t* cand_overflow;
+    int32_t* pair_ids;
+    uint32_t* pair_hits;
+    int max_pair;
+    int max_out;  /* odd targets reported per source, first ones in candidate order (0: all) */
+    uint64_t stats[9];
+} orc_objray_opts;
+int64_t orc_object_parity_ex(const float* srcV, const uint32_t* srcI, const int32_t* src_tri_obj,
+                             int64_t src_nT, int64_t n_src_obj, const float* V, int64_t nV,
+                             const uint32_t* I, int64_t nT, const int32_t* tri_obj, int max_targets,
+                             int swap, orc_objray_opts* opt, uint64_t* out_keys, int64_t cap,
+                             int64_t* overflow);
+
 /* Sorts ascending and removes duplicates in place; returns the unique count. */
 int64_t orc_sort_unique_u64(uint64_t* keys, int64_t n);
 

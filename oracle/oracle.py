@@ -24,6 +24,8 @@ _sig = {
                                  C.c_int, C.c_int, _P, _I64, _P]),
     "orc_object_parity": (_I64, [_P, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
                                  _P, _I64, _P]),
+    "orc_object_parity_ex": (_I64, [_P, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, C.c_int, C.c_int,
+                                    _P, _P, _I64, _P]),
     "orc_sort_unique_u64": (_I64, [_P, _I64]),
     "orc_fmix64": (C.c_uint64, [C.c_uint64]),
     "orc_ht_insert": (C.c_int, [_P, C.c_uint64, C.c_int, C.c_int, C.c_uint64]),
@@ -113,6 +115,58 @@ def object_parity(src_mesh, mesh, max_targets=256, swap=False):
     keys = np.empty(max(total, 1), np.uint64)
     lib().orc_object_parity(*args, _p(keys), total, C.cast(C.pointer(over), C.c_void_p))
     return keys[:total], over.value
+
+
+class ObjRayOpts(C.Structure):
+    _fields_ = [("chain_mode", C.c_int), ("max_iter", C.c_int), ("origins", _P),
+                ("iterations", _P), ("cand_count", _P), ("cand_overflow", _P),
+                ("pair_ids", _P), ("pair_hits", _P), ("max_pair", C.c_int), ("max_out", C.c_int),
+                ("stats", C.c_uint64 * 9)]
+
+
+STAT_NAMES = ("rays", "iterations", "hits", "max_iterations", "suppressed", "additions", "toggles",
+              "overflow", "targets")
+
+
+def object_parity_ex(src_mesh, mesh, max_targets=256, swap=False, chain_mode=0, max_iter=0,
+                     origins=None, n_src=None, track=False, max_pair=256, max_out=0):
+    """A.3/A.4 object rays with the reference's launch options (chain_mode 0 all hits, 1
+    nextafter closest-hit loop, 2 t + 1e-4 closest-hit loop). Returns (keys, overflow,
+    tracking dict or None, stats dict)."""
+    sV = _c(src_mesh.vertices, np.float32) if src_mesh is not None else np.zeros((1, 3), np.float32)
+    sI = _c(src_mesh.indices, np.uint32) if src_mesh is not None else np.zeros((1, 3), np.uint32)
+    sO = _c(src_mesh.tri_to_obj, np.int32) if src_mesh is not None else np.zeros(0, np.int32)
+    if n_src is None:
+        n_src = len(origins) if origins is not None else (int(sO.max()) + 1 if len(sO) else 0)
+    V = _c(mesh.vertices, np.float32); I = _c(mesh.indices, np.uint32); O = _c(mesh.tri_to_obj, np.int32)
+    org = _c(origins, np.float32) if origins is not None else None
+    tr = None
+    if track:
+        tr = {"iterations": np.zeros(n_src, np.uint32), "candidate_count": np.zeros(n_src, np.uint32),
+              "candidate_overflow": np.zeros(n_src, np.uint32),
+              "pair_target_ids": np.zeros((n_src, max_pair), np.int32),
+              "pair_target_hits": np.zeros((n_src, max_pair), np.uint32)}
+
+    def opts():
+        o = ObjRayOpts()
+        o.chain_mode, o.max_iter, o.origins, o.max_out = chain_mode, max_iter, _p(org), max_out
+        if tr is not None:
+            o.iterations, o.cand_count = _p(tr["iterations"]), _p(tr["candidate_count"])
+            o.cand_overflow = _p(tr["candidate_overflow"])
+            o.pair_ids, o.pair_hits, o.max_pair = _p(tr["pair_target_ids"]), _p(tr["pair_target_hits"]), max_pair
+        return o
+    over = C.c_int64(0)
+    args = (_p(sV), _p(sI), _p(sO), len(sO), n_src, _p(V), len(V), _p(I), len(I), _p(O),
+            int(max_targets), int(swap))
+    o1 = opts()
+    total = lib().orc_object_parity_ex(*args, C.cast(C.pointer(o1), C.c_void_p), None, 0,
+                                       C.cast(C.pointer(over), C.c_void_p))
+    keys = np.empty(max(total, 1), np.uint64)
+    o2 = opts()
+    lib().orc_object_parity_ex(*args, C.cast(C.pointer(o2), C.c_void_p), _p(keys), total,
+                               C.cast(C.pointer(over), C.c_void_p))
+    stats = {k: int(v) for k, v in zip(STAT_NAMES, o2.stats)}
+    return keys[:total], over.value, tr, stats
 
 
 def intersection_pairs(m1, m2, max_iter=100):

@@ -147,6 +147,32 @@ PU64 = C.POINTER(C.c_uint64)
 PI64 = C.POINTER(C.c_int64)
 
 # name -> (restype, argtypes); every symbol declared in include/rs3d.h
+class IntersectionStats(C.Structure):
+    _fields_ = [("overlap_trace_calls", C.c_uint64), ("overlap_iterations_total", C.c_uint64),
+                ("overlap_hits_total", C.c_uint64), ("overlap_max_iterations_per_trace", C.c_uint32),
+                ("containment_rays_total", C.c_uint64), ("containment_iterations_total", C.c_uint64),
+                ("containment_hits_total", C.c_uint64), ("containment_max_iterations_per_ray", C.c_uint32),
+                ("containment_same_hit_suppressed", C.c_uint64),
+                ("containment_candidate_additions", C.c_uint64),
+                ("containment_candidate_toggles", C.c_uint64),
+                ("containment_candidate_overflow", C.c_uint64), ("containment_targets_total", C.c_uint64)]
+
+
+class ObjectRayOpts(C.Structure):
+    _fields_ = [("chain_mode", C.c_int32), ("max_iterations", C.c_int32), ("origins", C.c_void_p),
+                ("num_sources", C.c_uint64), ("pair_target_ids", C.c_void_p), ("pair_target_hits", C.c_void_p),
+                ("max_pair_targets", C.c_int32), ("iterations", C.c_void_p), ("candidate_count", C.c_void_p),
+                ("candidate_overflow", C.c_void_p), ("stats", C.c_void_p), ("counts", C.c_void_p),
+                ("offsets", C.c_void_p), ("results", C.c_void_p)]
+
+
+OBJ_ALL_HITS = 0
+OBJ_CHAIN_NEXTAFTER = 1
+OBJ_CHAIN_STEP = 2
+OBJ_FALLBACK = 0
+OBJ_POINT_IN_MESH = 1
+
+
 SIGNATURES = {
     "rs3d_last_error": (C.c_char_p, []),
     "rs3d_ctx_create": (I32, [I32, C.POINTER(P)]),
@@ -189,6 +215,11 @@ SIGNATURES = {
     "rs3d_ht_compact": (I32, [P, C.POINTER(HashTable), P, U64, PU64]),
     "rs3d_overlap_join": (I32, [P, P, C.POINTER(Edges), P, C.POINTER(Edges), I32, P, U64, PU64]),
     "rs3d_edge_hash_ex": (I32, [P, P, C.POINTER(Edges), I32, I32, I32, C.POINTER(HashTable)]),
+    "rs3d_object_rays": (I32, [P, C.POINTER(MeshView), P, I32, I32, I32, C.POINTER(ObjectRayOpts),
+                               C.POINTER(HashTable), C.POINTER(HashTable), PU64]),
+    "rs3d_edge_profile": (I32, [P, P, C.POINTER(Edges), I32, I32, P]),
+    "rs3d_edge_count_ex": (I32, [P, P, C.POINTER(Edges), I32, I32, P]),
+    "rs3d_edge_write_ex": (I32, [P, P, C.POINTER(Edges), I32, I32, P, I32, P]),
     "rs3d_containment_fallback": (I32, [P, C.POINTER(MeshView), P, I32, I32, C.POINTER(HashTable), PU64]),
     "rs3d_point_in_mesh_join": (I32, [P, C.POINTER(MeshView), P, I32, C.POINTER(HashTable),
                                       C.POINTER(HashTable), PU64]),

@@ -464,6 +464,45 @@ int edge_count_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e, i
     HashArgs H{};
     return launch_edges<EMODE_COUNT>(ctx, t, e, 0, counts, nullptr, 0, nullptr, H);
 }
+// Overlap counters of rs3d_edge_profile from the per-edge counts of a counting pass (chain
+// hits x sources): per source object of an edge of length >= 1e-6 one trace call with
+// min(hits + 1, cap) iterations (trace_edge_multi_hits, query/src/cuda/mesh_intersection.cu:
+// 16-91); the sums are combined per block before one atomic per counter.
+__global__ __launch_bounds__(256) void k_edge_profile(const float* __restrict__ starts, const float* __restrict__ ends,
+                                                      const int32_t* __restrict__ src_cnt,
+                                                      const int32_t* __restrict__ counts, uint64_t n, int cap,
+                                                      rs3d_intersection_stats* __restrict__ st) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long calls = 0, iters = 0, hits = 0;
+    uint32_t mx = 0;
+    if (e < n) {
+        const float3 a = ld3f(starts, e), b = ld3f(ends, e);
+        const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        const int sc = src_cnt[e];
+        if (len >= 1e-6f && sc > 0) {
+            const uint32_t h = (uint32_t)(counts[e] / sc);
+            const uint32_t it = h + 1 < (uint32_t)cap ? h + 1 : (uint32_t)cap;
+            calls = (unsigned long long)sc;
+            iters = (unsigned long long)it * sc;
+            hits = (unsigned long long)counts[e];
+            mx = it;
+        }
+    }
+    const unsigned long long c_sum = block_sum_256(calls);
+    __syncthreads();
+    const unsigned long long i_sum = block_sum_256(iters);
+    __syncthreads();
+    const unsigned long long h_sum = block_sum_256(hits);
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(&st->overlap_max_iterations_per_trace, mx);
+    if (threadIdx.x == 0) {
+        if (c_sum) atomicAdd(reinterpret_cast<unsigned long long*>(&st->overlap_trace_calls), c_sum);
+        if (i_sum) atomicAdd(reinterpret_cast<unsigned long long*>(&st->overlap_iterations_total), i_sum);
+        if (h_sum) atomicAdd(reinterpret_cast<unsigned long long*>(&st->overlap_hits_total), h_sum);
+    }
+}
+
 int edge_write_keys_internal(rs3d_ctx* ctx, const rs3d_bvh* t, const rs3d_edges* e,
                              const int64_t* offsets, int64_t bias, int swap, uint64_t* out) {
     HashArgs H{};
@@ -667,6 +706,52 @@ int rs3d_edge_hash(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edge
         return RS3D_E_ARG;
     }
     return edge_hash_internal(ctx, target, edges, swap, ht, CHAIN_EPS, 100);
+}
+
+int rs3d_edge_count_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                       int max_iterations, int32_t* counts) {
+    if (!ctx || !target || !edges_ok(edges) || (edges->num_edges && !counts) || chain_rule < 0 ||
+        chain_rule > 1 || max_iterations <= 0) {
+        set_error("rs3d_edge_count_ex: bad arguments");
+        return RS3D_E_ARG;
+    }
+    HashArgs H{};
+    return launch_edges<EMODE_COUNT>(ctx, target, edges, 0, counts, nullptr, 0, nullptr, H, chain_rule,
+                                     max_iterations);
+}
+
+int rs3d_edge_write_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                       int max_iterations, const int64_t* offsets, int swap, rs3d_pair* out) {
+    if (!ctx || !target || !edges_ok(edges) || (edges->num_edges && (!offsets || !out)) || chain_rule < 0 ||
+        chain_rule > 1 || max_iterations <= 0) {
+        set_error("rs3d_edge_write_ex: bad arguments");
+        return RS3D_E_ARG;
+    }
+    HashArgs H{};
+    return launch_edges<EMODE_WRITE_PAIRS>(ctx, target, edges, swap, nullptr, offsets, 0, out, H, chain_rule,
+                                           max_iterations);
+}
+
+int rs3d_edge_profile(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int chain_rule,
+                      int max_iterations, rs3d_intersection_stats* stats) {
+    if (!ctx || !target || !edges_ok(edges) || !stats || chain_rule < 0 || chain_rule > 1 || max_iterations <= 0) {
+        set_error("rs3d_edge_profile: bad arguments");
+        return RS3D_E_ARG;
+    }
+    const uint64_t n = edges->num_edges;
+    if (n == 0) return RS3D_OK;
+    static thread_local DevBuf counts;
+    RS3D_TRY(counts.ensure(n * sizeof(int32_t)));
+    HashArgs H{};
+    RS3D_TRY(launch_edges<EMODE_COUNT>(ctx, target, edges, 0, counts.as<int32_t>(), nullptr, 0, nullptr, H,
+                                       chain_rule, max_iterations));
+    unsigned nb;
+    RS3D_TRY(grid_blocks(n, 256, &nb));
+    k_edge_profile<<<nb, 256, 0, ctx->stream>>>(edges->starts, edges->ends, edges->src_counts,
+                                                counts.as<int32_t>(), n, max_iterations, stats);
+    RS3D_HIP(hipGetLastError());
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    return RS3D_OK;
 }
 
 int rs3d_edge_hash_ex(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edges, int swap,

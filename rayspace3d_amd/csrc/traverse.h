@@ -523,63 +523,175 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
     return iter;
 }
 
-// A.3/A.4 object ray: +z from o over [1e-4, 1e10]; all hits in (t, triangle) order; a hit
-// of the same object as the previous hit within 1e-5 is skipped, every other hit toggles
-// its object's parity in (ids, par) (at most max_targets distinct objects, further new
-// objects counted in `over`). Returns the number of distinct objects recorded.
+// Work counters of one object ray (the containment fields of the reference's
+// MeshIntersectionProfilingStats, query/src/cuda/mesh_intersection.h:7-22).
+struct ObjRayCtr {
+    uint32_t iters = 0, hits = 0, suppressed = 0, additions = 0, toggles = 0, overflow = 0;
+};
+
+// One hit on target object obj: toggles its parity (times times) in the candidate list
+// (ids, par, hit counts hc when given), or adds it while fewer than max_targets are
+// listed, or counts an overflow.
+__device__ __forceinline__ void cand_hit(int* ids, uint8_t* par, uint32_t* hc, int& count, int max_targets,
+                                         int obj, uint32_t times, ObjRayCtr& c) {
+    for (int j = 0; j < count; ++j)
+        if (ids[j] == obj) {
+            par[j] ^= (uint8_t)(times & 1u);
+            if (hc) hc[j] += times;
+            c.toggles += times;
+            return;
+        }
+    if (count < max_targets) {
+        ids[count] = obj;
+        par[count] = (uint8_t)(times & 1u);
+        if (hc) hc[count] = times;
+        ++count;
+        c.additions += 1;
+        c.toggles += times - 1;
+    } else {
+        c.overflow += times;
+    }
+}
+
+enum { OBJ_ALL_HITS = 0, OBJ_CHAIN_NEXTAFTER = 1, OBJ_CHAIN_STEP = 2 };
+
+// A.3/A.4 object ray: +z from o, parity per target object in (ids, par) (at most
+// max_targets distinct objects, further new objects counted as overflow). Returns the
+// number of distinct objects recorded. Forms:
+//  OBJ_ALL_HITS: every hit in [1e-4, 1e10] in (t, triangle) order; a hit of the same object
+//    as the previous hit within 1e-5 of it is skipped (the any-hit forms,
+//    query/src/cuda/mesh_intersection_anyhit.cu:194-239, mesh_containment.cu:196-256);
+//  OBJ_CHAIN_NEXTAFTER: one closest hit per iteration over [thr, 1e10], thr from 1e-4 and
+//    then nextafterf(t, 1e10), at most max_iter iterations (default 2048), a repeat of the
+//    previous triangle within 1e-5 skipped (query/src/cuda/mesh_intersection.cu:134-230);
+//  OBJ_CHAIN_STEP: one closest hit per iteration over [thr, 1e10), thr from 1e-4 and then
+//    fl(t + 1e-4), at most max_iter iterations (default 2000); when that add is absorbed
+//    every remaining iteration finds the same hit again (query/src/cuda/mesh_containment.cu:336-366).
+// The closest hit of an iteration is the smallest (t, triangle index) at or above thr. The
+// walk keeps the sorted hit buffer of the traversal kernels and refills it from the
+// current threshold, so no hit is lost for any count.
+template <int DEPTH>
+__device__ __forceinline__ int object_ray_walk(const BNode* __restrict__ nodes,
+                                               const TriRec* __restrict__ tris, const float3 o,
+                                               int* stack, int* ids, uint8_t* par, uint32_t* hc,
+                                               int max_targets, int mode, int max_iter, ObjRayCtr& c) {
+    int count = 0;
+    float bt[KBUF];
+    uint32_t bi[KBUF];
+    int bo[KBUF];
+    TraceCounters ctr;
+    if (mode == OBJ_ALL_HITS) {
+        float lo_t = 1e-4f;
+        uint32_t lo_tri = 0;
+        const float tmax_excl = nextafterf(1e10f, INFINITY);
+        bool has_last = false;
+        int last_obj = -1;
+        float last_t = 0.0f;
+        c.iters = 1;
+        while (true) {
+            kbuf_clear(bt, bi, bo);
+            const uint32_t total = collect_axis<2, DEPTH, false>(nodes, tris, o, lo_t, lo_tri, tmax_excl,
+                                                                 stack, bt, bi, bo, ctr);
+#pragma unroll
+            for (int k = 0; k < KBUF; ++k) {
+                if ((uint32_t)k < total) {
+                    const float t = bt[k];
+                    const int obj = bo[k];
+                    const bool same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
+                    has_last = true;
+                    last_obj = obj;
+                    last_t = t;
+                    c.hits++;
+                    if (same) c.suppressed++;
+                    else cand_hit(ids, par, hc, count, max_targets, obj, 1u, c);
+                }
+            }
+            if (total <= (uint32_t)KBUF) break;
+            lo_t = bt[KBUF - 1];
+            lo_tri = bi[KBUF - 1] + 1;
+        }
+        return count;
+    }
+    const float tmax = 1e10f;
+    // window: t <= 1e10 for the nextafter loop; t < 1e10 for the step loop (a hit at 1e10
+    // ends it like a miss)
+    const float tmax_excl = mode == OBJ_CHAIN_NEXTAFTER ? nextafterf(tmax, INFINITY) : tmax;
+    const uint32_t cap = max_iter > 0 ? (uint32_t)max_iter : (mode == OBJ_CHAIN_NEXTAFTER ? 2048u : 2000u);
+    float thr = 1e-4f, last_t = -1.0f;
+    uint32_t last_tri = 0xffffffffu;
+    bool done = false;
+    while (!done) {
+        kbuf_clear(bt, bi, bo);
+        const uint32_t total = collect_axis<2, DEPTH, false>(nodes, tris, o, thr, 0u, tmax_excl, stack,
+                                                             bt, bi, bo, ctr);
+        const uint32_t nb = total < (uint32_t)KBUF ? total : (uint32_t)KBUF;
+        uint32_t k = 0;
+        while (true) {
+            if (c.iters >= cap) {
+                done = true;
+                break;
+            }
+            if (mode == OBJ_CHAIN_NEXTAFTER && thr > tmax) {
+                c.iters++;
+                done = true;
+                break;
+            }
+            while (k < nb && bt[k] < thr) ++k;
+            if (k == nb) {
+                if (total > (uint32_t)KBUF) break;  // more hits lie beyond the buffer: refill
+                c.iters++;                          // the iteration whose trace misses
+                done = true;
+                break;
+            }
+            c.iters++;
+            c.hits++;
+            const float t = bt[k];
+            const uint32_t tri = bi[k];
+            const int obj = bo[k];
+            if (mode == OBJ_CHAIN_NEXTAFTER) {
+                const bool same = tri == last_tri && last_t >= 0.0f && fabsf(t - last_t) <= 1e-5f;
+                if (same) c.suppressed++;
+                else cand_hit(ids, par, hc, count, max_targets, obj, 1u, c);
+                last_t = t;
+                last_tri = tri;
+                float nx = nextafterf(t, tmax);
+                if (nx <= t) nx = t + 1e-6f;
+                if (nx <= thr) nx = nextafterf(thr, tmax);
+                if (nx <= thr) {
+                    done = true;
+                    break;
+                }
+                thr = nx;
+            } else {
+                const float nx = t + 1e-4f;
+                if (nx <= t) {
+                    // absorbed: this hit is found again on each remaining iteration
+                    const uint32_t r = cap - c.iters + 1;
+                    c.hits += r - 1;
+                    c.iters = cap;
+                    cand_hit(ids, par, hc, count, max_targets, obj, r, c);
+                    done = true;
+                    break;
+                }
+                cand_hit(ids, par, hc, count, max_targets, obj, 1u, c);
+                thr = nx;
+            }
+            ++k;
+        }
+    }
+    return count;
+}
+
+// The all-hits object ray without per-candidate hit counts (the batch engine's form).
 template <int DEPTH>
 __device__ __forceinline__ int object_ray_parity(const BNode* __restrict__ nodes,
                                                  const TriRec* __restrict__ tris, const float3 o,
                                                  int* stack, int* ids, uint8_t* par,
                                                  int max_targets, unsigned long long& over) {
-    int count = 0;
-    float lo_t = 1e-4f;
-    uint32_t lo_tri = 0;
-    const float tmax_excl = nextafterf(1e10f, INFINITY);
-    bool has_last = false;
-    int last_obj = -1;
-    float last_t = 0.0f;
-    float bt[KBUF];
-    uint32_t bi[KBUF];
-    int bo[KBUF];
-    TraceCounters ctr;
-    while (true) {
-        kbuf_clear(bt, bi, bo);
-        const uint32_t total = collect_axis<2, DEPTH, false>(nodes, tris, o, lo_t, lo_tri, tmax_excl,
-                                                             stack, bt, bi, bo, ctr);
-#pragma unroll
-        for (int k = 0; k < KBUF; ++k) {
-            if ((uint32_t)k < total) {
-                const float t = bt[k];
-                const int obj = bo[k];
-                const bool same = has_last && obj == last_obj && fabsf(t - last_t) <= 1e-5f;
-                has_last = true;
-                last_obj = obj;
-                last_t = t;
-                if (!same) {
-                    int pos = -1;
-                    for (int j = 0; j < count; ++j)
-                        if (ids[j] == obj) {
-                            pos = j;
-                            break;
-                        }
-                    if (pos >= 0) {
-                        par[pos] ^= 1;
-                    } else if (count < max_targets) {
-                        ids[count] = obj;
-                        par[count] = 1;
-                        ++count;
-                    } else {
-                        ++over;
-                    }
-                }
-            }
-        }
-        if (total <= (uint32_t)KBUF) break;
-        lo_t = bt[KBUF - 1];
-        lo_tri = bi[KBUF - 1] + 1;
-    }
-    return count;
+    ObjRayCtr c;
+    const int n = object_ray_walk<DEPTH>(nodes, tris, o, stack, ids, par, nullptr, max_targets, OBJ_ALL_HITS, 0, c);
+    over += c.overflow;
+    return n;
 }
 
 

@@ -612,3 +612,85 @@ class Server:
         if self.handle:
             self.ctx.lib.rs3d_server_destroy(self.handle)
             self.handle = C.c_void_p()
+
+
+def object_rays(ctx, src: "Bvh | None", target: "Bvh", swap: bool = False, max_targets: int = 256,
+                chain_mode: int = _lib.OBJ_ALL_HITS, max_iter: int = 0, origins: torch.Tensor | None = None,
+                track: bool = False, max_pair: int = 256, with_stats: bool = False, two_pass: bool = False,
+                form: int = _lib.OBJ_FALLBACK, overlap_table=None):
+    """rs3d_object_rays: per-object +z containment rays with the reference's launch options.
+    Returns a dict: keys (sorted unique (a << 32 | b) as uint64), overflow, and when asked the
+    per-source tracking arrays and the counters (rs3d_intersection_stats as a dict)."""
+    n_src = origins.shape[0] if origins is not None else src.view().num_objects
+    if origins is not None:
+        check_points(origins, ctx.device, "origins")
+    o = _lib.ObjectRayOpts()
+    o.chain_mode, o.max_iterations = chain_mode, max_iter
+    keep = []
+
+    def dev(shape, dtype, fill=0):
+        t = torch.full(shape, fill, dtype=dtype, device="cuda")
+        keep.append(t)
+        return t
+    if origins is not None:
+        o.origins, o.num_sources = origins.data_ptr(), n_src
+    tr = None
+    if track:
+        tr = {"pair_target_ids": dev((max(n_src, 1), max_pair), torch.int32, -7),
+              "pair_target_hits": dev((max(n_src, 1), max_pair), torch.int32, 7),
+              "iterations": dev((max(n_src, 1),), torch.int32, 7), "candidate_count": dev((max(n_src, 1),), torch.int32, 7),
+              "candidate_overflow": dev((max(n_src, 1),), torch.int32, 7)}
+        o.pair_target_ids, o.pair_target_hits = tr["pair_target_ids"].data_ptr(), tr["pair_target_hits"].data_ptr()
+        o.max_pair_targets = max_pair
+        o.iterations, o.candidate_count = tr["iterations"].data_ptr(), tr["candidate_count"].data_ptr()
+        o.candidate_overflow = tr["candidate_overflow"].data_ptr()
+    st = None
+    if with_stats:
+        st = dev((13,), torch.int64, 0)
+        o.stats = st.data_ptr()
+    view = src.view() if src is not None else None
+    over = C.c_uint64()
+    ht = None
+    if two_pass:
+        counts = dev((max(n_src, 1),), torch.int32, -1)
+        o.counts = counts.data_ptr()
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_object_rays(ctx.handle, C.byref(view) if view is not None else None, target.handle, form,
+                                       int(swap), max_targets, C.byref(o), None, None, C.byref(over)), "rs3d_object_rays")
+        c = to_host(counts)[:n_src].astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(c)[:-1]]) if n_src else np.zeros(0, np.int64)
+        doffs = to_dev(offs.astype(np.int64)) if n_src else dev((1,), torch.int64)
+        res = dev((max(int(c.sum()), 1), 2), torch.int32)
+        o.counts, o.offsets, o.results = None, doffs.data_ptr(), res.data_ptr()
+        if st is not None:
+            st.zero_()
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_object_rays(ctx.handle, C.byref(view) if view is not None else None, target.handle, form,
+                                       int(swap), max_targets, C.byref(o), None, None, C.byref(over)), "rs3d_object_rays")
+        pr = to_host(res)[:int(c.sum())]
+        keys = (pr[:, 0].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | pr[:, 1].astype(np.uint32).astype(np.uint64)
+        keys = np.unique(keys)
+    else:
+        slots = 1 << 12
+        while slots < 4 * max(n_src, 1) * 8:
+            slots <<= 1
+        tab = dev((slots + 1,), torch.int64, -1)
+        tab[slots] = 0
+        ht = _lib.HashTable(tab.data_ptr(), slots, 1, 0, tab.data_ptr() + 8 * slots)
+        torch.cuda.synchronize()
+        check(ctx.lib.rs3d_object_rays(ctx.handle, C.byref(view) if view is not None else None, target.handle, form,
+                                       int(swap), max_targets, C.byref(o), C.byref(ht),
+                                       C.byref(overlap_table) if overlap_table is not None else None, C.byref(over)),
+              "rs3d_object_rays")
+        t = to_host(tab)
+        assert t[slots] == 0, "pair table overflow"
+        keys = np.unique(t[:slots][t[:slots] != -1].view(np.uint64))
+    out = {"keys": keys, "overflow": over.value}
+    if tr is not None:
+        out["tracking"] = {k: to_host(v)[:n_src].view(np.uint32 if k != "pair_target_ids" else np.int32)
+                           for k, v in tr.items()}
+    if st is not None:
+        raw = to_host(st).view(np.uint8).tobytes()
+        sv = _lib.IntersectionStats.from_buffer_copy(raw[:C.sizeof(_lib.IntersectionStats)])
+        out["stats"] = {k: int(getattr(sv, k)) for k, _ in sv._fields_}
+    return out

@@ -351,3 +351,78 @@ def test_estimate_pairs_alpha_hand_computed():
     # untouched grids: the estimate is 0
     z = np.zeros(2, orc.GRID_CELL)
     assert orc.estimate_pairs(z, b, mn, mx, res) == 0
+
+
+# ---- closest-hit forms of the object rays -------------------------------------------------
+
+def _modes(target, mode, max_iter=0, max_targets=256):
+    keys, over, tr, st = orc.object_parity_ex(_source_at_origin(), target, max_targets, False, mode,
+                                             max_iter, track=True)
+    return sorted(int(k & np.uint64(0xffffffff)) for k in keys), tr, st
+
+
+def test_closest_hit_nextafter_loop():
+    # mesh_intersection.cu:134-230: one closest hit per iteration, next tmin = nextafterf(t)
+    m = _planes([F32(1.0), _up(1.0)])
+    assert _modes(m, 1)[0] == [0, 1] and _modes(m, 0)[0] == [0, 1]
+    # two planes of one object 4e-6 apart: the loop counts both (even), all-hits merges them
+    m = _planes([F32(1.0), F32(1.000004)], [0, 0])
+    assert _modes(m, 1)[0] == [] and _modes(m, 0)[0] == [0]
+    # equal t on two objects: the loop sees only the first (lower triangle index)
+    m = _planes([F32(2.0), F32(2.0)], [5, 3])
+    assert _modes(m, 1)[0] == [5] and _modes(m, 0)[0] == [3, 5]
+    # iteration cap: the nearest max_iter planes; every iteration traced a hit
+    ids, tr, st = _modes(_planes(0.5 + 0.25 * np.arange(300)), 1, max_iter=100, max_targets=512)
+    assert ids == list(range(100)) and tr["iterations"][0] == 100 and st["hits"] == 100
+    # below the cap the loop ends on a miss: one more iteration than hits
+    ids, tr, st = _modes(_planes(0.5 + 0.25 * np.arange(7)), 1, max_iter=100)
+    assert ids == list(range(7)) and tr["iterations"][0] == 8 and st["max_iterations"] == 8
+
+
+def test_closest_hit_step_loop():
+    # mesh_containment.cu:336-366: next tmin = fl(t + 1e-4)
+    assert _modes(_planes([F32(1.0), F32(1.00005)]), 2)[0] == [0]
+    assert _modes(_planes([F32(1.0), F32(1.0002)]), 2)[0] == [0, 1]
+    # a hit at t = 1e10 ends the loop (the all-hits window includes it)
+    assert _modes(_planes([F32(1e10)]), 2)[0] == [] and _modes(_planes([F32(1e10)]), 0)[0] == [0]
+    # at t = 1000 the add is exact enough: one crossing, contained
+    ids, tr, st = _modes(_planes([F32(1000.0)]), 2)
+    assert ids == [0] and tr["iterations"][0] == 2 and st["hits"] == 1
+    # at t = 3000 fl(t + 1e-4) = t: the loop finds the same hit on every one of its 2000
+    # iterations, toggling it 2000 times (even: not contained)
+    ids, tr, st = _modes(_planes([F32(3000.0)]), 2)
+    assert ids == [] and tr["iterations"][0] == 2000 and st["hits"] == 2000 and st["toggles"] == 1999
+    ids, tr, st = _modes(_planes([F32(3000.0)]), 2, max_iter=1999)
+    assert ids == [0] and st["hits"] == 1999
+
+
+def test_object_ray_tracking_and_origins():
+    # per-source tracking: odd targets with their hit counts, candidate count and overflow
+    zs = [F32(1.0), F32(2.0), F32(3.0), F32(4.0), F32(5.0)]
+    m = _planes(zs, [7, 8, 7, 9, 9])
+    keys, over, tr, st = orc.object_parity_ex(_source_at_origin(), m, 2, False, 0, track=True, max_pair=4)
+    # object 7 crossed twice (even), 8 once (odd); 9 is a third object beyond max_targets = 2
+    assert sorted(int(k & np.uint64(0xffffffff)) for k in keys) == [8]
+    assert list(tr["pair_target_ids"][0]) == [8, -1, -1, -1] and tr["pair_target_hits"][0][0] == 1
+    assert tr["candidate_count"][0] == 2 and tr["candidate_overflow"][0] == 2 and over == 2
+    assert st["additions"] == 2 and st["toggles"] == 1 and st["overflow"] == 2 and st["targets"] == 1
+    # explicit origins: the same answers as the first vertices they replace
+    src = gen.spheres(1, np.array([[0.01, 0.02, 0.03], [5.0, 5.0, 5.0]]), np.array([0.1, 0.1]))
+    tgt = gen.spheres(2, np.zeros((3, 3)), np.array([1.0, 2.0, 3.0]))
+    first = src.vertices[src.indices[[0, 80], 0]]
+    a = orc.object_parity_ex(src, tgt)[0]
+    b = orc.object_parity_ex(None, tgt, origins=first)[0]
+    assert np.array_equal(a, b) and len(a) == 3
+
+
+@pytest.mark.parametrize("seed", [8, 9])
+def test_object_ray_modes_agree_on_closed_meshes(seed):
+    # away from ties and folds (random closed spheres) every form gives the same pairs
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tgt = gen.spheres(2, rng.uniform(0, 10, size=(30, 3)), rng.uniform(1.0, 3.0, size=30), noise=0.05, rng=rng)
+    src = gen.spheres(1, rng.uniform(0, 10, size=(200, 3)), rng.uniform(0.2, 0.8, size=200),
+                      rotations=gen.random_rotations(rng, 200))
+    ref = orc.object_parity_ex(src, tgt, 512, False, 0)[0]
+    assert len(ref) > 20
+    for mode in (1, 2):
+        assert np.array_equal(orc.object_parity_ex(src, tgt, 512, False, mode)[0], ref), mode
