@@ -1,7 +1,10 @@
-// raytracer_containment -- containment join: overlap set from edge rays B->A and A->B,
-// then strict containment of B objects (first vertex) in A objects that do not overlap
-// (query/src/applications/raytracer_containment_main.cpp:96-436). CSV and phase names
-// as the reference; pairs are sorted by (a, b).
+// raytracer_containment -- containment join through MeshContainmentLauncher: overlap set
+// from edge rays B->A and A->B, then strict containment of B objects (first vertex, built
+// on the host) in A objects that do not overlap
+// (query/src/applications/raytracer_containment_main.cpp:96-436). CSV and phase names as
+// the reference; pairs are sorted by (a, b). A table insert that exhausts its probes is
+// detected and the run repeated with twice the slots.
+#include <algorithm>
 #include <fstream>
 #include <iostream>
 
@@ -29,7 +32,7 @@ int main(int argc, char* argv[]) {
         rs3d_cli::MeshPairOptions::meshPairHelp(o);
         rs3d_cli::MeshPairOptions::runHelp(o);
         o.emplace_back("--no-export", "Do not export results to CSV");
-        o.emplace_back("--use-anyhit-point-in-mesh", "All-hits parity for point-in-mesh (the only mode)");
+        o.emplace_back("--use-anyhit-point-in-mesh", "Use AnyHit shader accumulation for point-in-mesh parity (default: enabled)");
         o.emplace_back("--include-overlap-pairs", "Include overlap/touch pairs (union of overlap + containment)");
         rs3d_cli::printHelp(argv[0], "--mesh1 <A.pre> --mesh2 <B.pre> [options]",
                             "Containment join: pairs (a, b) where B object b lies strictly inside A object a.", o);
@@ -63,26 +66,67 @@ int main(int argc, char* argv[]) {
     timer.next("Prepare Kernel Parameters");
     EdgeMeshData ea = PrecomputedEdgeData::uploadFromGeometry(ma, ctx);
     EdgeMeshData eb = PrecomputedEdgeData::uploadFromGeometry(mb, ctx);
-    const rs3d_mesh_view va = ua.view(), vb = ub.view();
+    // first vertex of each B object, on the host (raytracer_containment_main.cpp:195-211)
+    const size_t numBObjects = countObjects(mb);
+    std::vector<Vec3> bFirst(numBObjects, Vec3{0, 0, 0});
+    {
+        std::vector<bool> seen(numBObjects, false);
+        for (size_t t = 0; t < mb.indices.size(); ++t) {
+            const int o = mb.triangleToObject[t];
+            if (o >= 0 && (size_t)o < numBObjects && !seen[o]) {
+                bFirst[o] = mb.vertices[mb.indices[t].x];
+                seen[o] = true;
+            }
+        }
+    }
+    DeviceBuffer dFirst(std::max<size_t>(numBObjects, 1) * sizeof(Vec3));
+    if (numBObjects) dFirst.upload(bFirst.data(), numBObjects * sizeof(Vec3));
+    MeshContainmentLauncher launcher(ctx);
+    uint64_t slots = 16777216ull;  // 16 M slots per table (raytracer_containment_main.cpp:228-229)
     std::vector<uint64_t> contain, overlap;
     auto runOnce = [&]() {
-        uint64_t cap = 1u << 20;
         while (true) {
-            DeviceBuffer c(cap * sizeof(uint64_t)), o(cap * sizeof(uint64_t));
-            uint64_t nc = 0, no = 0, over = 0;
-            const int rc = rs3d_containment_join(ctx.get(), &va, asa.getHandle(), &ea.view, &vb, asb.getHandle(),
-                                                 &eb.view, 0, c.as<uint64_t>(), cap, &nc, o.as<uint64_t>(), cap,
-                                                 &no, &over);
-            if (rc == RS3D_E_CAPACITY) {
-                cap = std::max(nc, no);
+            DeviceHashTable iht, cht;
+            iht.create(slots, false, 1000);
+            cht.create(slots, false, 1000);
+            iht.clear(ctx);
+            cht.clear(ctx);
+            MeshContainmentLaunchParams params;
+            params.intersection_hash_table = &iht.table;
+            params.containment_hash_table = &cht.table;
+            params.b_first_vertices = dFirst.as<float>();
+            params.b_num_objects = (int)numBObjects;
+            params.use_anyhit_point_in_mesh = opt.useAnyhitPointInMesh ? 1 : 0;
+            // phase 1a: B edges against A
+            params.src_edges = &eb.view;
+            params.target = asa.getHandle();
+            params.swap_ids = 0;
+            launcher.launchEdgeCheck(params, eb.num_edges);
+            // phase 1b: A edges against B
+            params.src_edges = &ea.view;
+            params.target = asb.getHandle();
+            params.swap_ids = 1;
+            launcher.launchEdgeCheck(params, ea.num_edges);
+            // phase 2: point in mesh against A
+            params.target = asa.getHandle();
+            launcher.launchPointInMesh(params, (int)numBObjects);
+            if ((iht.failedInserts(ctx) > 0 || cht.failedInserts(ctx) > 0) && slots < (1ull << 30)) {
+                slots *= 2;
                 continue;
             }
-            check(rc, "rs3d_containment_join");
-            contain.resize(nc);
-            overlap.resize(no);
-            c.download(contain.data(), nc * sizeof(uint64_t));
-            o.download(overlap.data(), no * sizeof(uint64_t));
-            if (over) std::cerr << "Warning: " << over << " candidate objects beyond the 512-target cap" << std::endl;
+            if (launcher.lastPointInMeshOverflow())
+                std::cerr << "Warning: " << launcher.lastPointInMeshOverflow()
+                          << " candidate objects beyond the 512-target cap" << std::endl;
+            auto compact = [&](DeviceHashTable& t, std::vector<uint64_t>& keys) {
+                DeviceBuffer out(slots * sizeof(uint64_t));
+                uint64_t n = 0;
+                check(rs3d_ht_compact(ctx.get(), &t.table, out.as<uint64_t>(), slots, &n), "rs3d_ht_compact");
+                keys.resize(n);
+                out.download(keys.data(), n * sizeof(uint64_t));
+                std::sort(keys.begin(), keys.end());
+            };
+            compact(cht, contain);
+            if (opt.includeOverlapPairs) compact(iht, overlap);
             return;
         }
     };

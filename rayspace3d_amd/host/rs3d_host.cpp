@@ -12,6 +12,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <mutex>
 #include <sstream>
 #include <unordered_map>
 
@@ -43,6 +44,10 @@ DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
         o.bytes_ = 0;
     }
     return *this;
+}
+
+void DeviceBuffer::fill(int byteValue) {
+    if (bytes_) hip_check(hipMemset(ptr_, byteValue, bytes_), "hipMemset");
 }
 
 void DeviceBuffer::alloc(size_t bytes) {
@@ -576,6 +581,85 @@ void MeshOverlapEdgesLauncher::launchMesh2ToMesh1(const MeshOverlapEdgesLaunchPa
     launchInternal(p, n);
 }
 
+void MeshIntersectionLauncher::overlap(const MeshIntersectionLaunchParams& p, int launchSize, int swap) {
+    rs3d_edges e = *p.edges;
+    e.num_edges = std::min<uint64_t>((uint64_t)std::max(launchSize, 0), e.num_edges);
+    const int cap = p.overlap_max_iterations > 0 ? p.overlap_max_iterations : 100;
+    if (p.profiling_stats)
+        check(rs3d_edge_profile(ctx_.get(), p.mesh2, &e, RS3D_CHAIN_EPS, cap, p.profiling_stats),
+              "rs3d_edge_profile");
+    if (p.hash_table)
+        check(rs3d_edge_hash_ex(ctx_.get(), p.mesh2, &e, swap, RS3D_CHAIN_EPS, cap, p.hash_table),
+              "rs3d_edge_hash_ex");
+    else if (p.pass == 1)
+        check(rs3d_edge_count_ex(ctx_.get(), p.mesh2, &e, RS3D_CHAIN_EPS, cap, p.collision_counts),
+              "rs3d_edge_count_ex");
+    else
+        check(rs3d_edge_write_ex(ctx_.get(), p.mesh2, &e, RS3D_CHAIN_EPS, cap, p.collision_offsets, swap, p.results),
+              "rs3d_edge_write_ex");
+    ctx_.sync();
+}
+
+void MeshIntersectionLauncher::containment(const MeshIntersectionLaunchParams& p, int launchSize, int swap) {
+    rs3d_mesh_view src = p.mesh1;
+    src.num_objects = std::min<uint64_t>((uint64_t)std::max(launchSize, 0), src.num_objects);
+    rs3d_object_ray_opts o{};
+    o.chain_mode = p.use_anyhit_containment ? RS3D_OBJ_ALL_HITS : RS3D_OBJ_CHAIN_NEXTAFTER;
+    o.max_iterations = p.containment_max_iterations;
+    o.pair_target_ids = p.pair_target_ids;
+    o.pair_target_hits = p.pair_target_hits;
+    o.max_pair_targets = p.max_pair_targets_per_source;
+    o.iterations = p.containment_iterations_per_source;
+    o.candidate_count = p.containment_candidate_count_per_source;
+    o.candidate_overflow = p.containment_candidate_overflow_per_source;
+    o.stats = p.profiling_stats;
+    if (!p.hash_table) {
+        if (p.pass == 1) o.counts = p.collision_counts;
+        else {
+            o.offsets = p.collision_offsets;
+            o.results = p.results;
+        }
+    }
+    containmentOverflow_ = 0;
+    check(rs3d_object_rays(ctx_.get(), &src, p.mesh2, RS3D_OBJ_FALLBACK, swap, p.anyhit_max_pair_targets_per_source,
+                           &o, p.hash_table, nullptr, &containmentOverflow_),
+          "rs3d_object_rays");
+    ctx_.sync();
+}
+
+void MeshIntersectionLauncher::launchOverlapMesh1ToMesh2(const MeshIntersectionLaunchParams& p, int n) { overlap(p, n, 0); }
+void MeshIntersectionLauncher::launchOverlapMesh2ToMesh1(const MeshIntersectionLaunchParams& p, int n) { overlap(p, n, 1); }
+void MeshIntersectionLauncher::launchContainmentMesh1ToMesh2(const MeshIntersectionLaunchParams& p, int n) {
+    containment(p, n, 0);
+}
+void MeshIntersectionLauncher::launchContainmentMesh2ToMesh1(const MeshIntersectionLaunchParams& p, int n) {
+    containment(p, n, 1);
+}
+
+void MeshContainmentLauncher::launchEdgeCheck(const MeshContainmentLaunchParams& p, int numEdges) {
+    rs3d_edges e = *p.src_edges;
+    e.num_edges = std::min<uint64_t>((uint64_t)std::max(numEdges, 0), e.num_edges);
+    // swap_ids 0 stores (hit, source), which is this library's swapped order
+    check(rs3d_edge_hash_ex(ctx_.get(), p.target, &e, p.swap_ids == 0 ? 1 : 0, RS3D_CHAIN_NEXTAFTER, 256,
+                            p.intersection_hash_table),
+          "rs3d_edge_hash_ex");
+    ctx_.sync();
+}
+
+void MeshContainmentLauncher::launchPointInMesh(const MeshContainmentLaunchParams& p, int numBObjects) {
+    rs3d_object_ray_opts o{};
+    o.chain_mode = p.use_anyhit_point_in_mesh ? RS3D_OBJ_ALL_HITS : RS3D_OBJ_CHAIN_STEP;
+    o.max_iterations = 2000;
+    o.origins = p.b_first_vertices;
+    o.num_sources = (uint64_t)std::max(std::min(numBObjects, p.b_num_objects), 0);
+    overflow_ = 0;
+    if (o.num_sources == 0) return;
+    check(rs3d_object_rays(ctx_.get(), nullptr, p.target, RS3D_OBJ_POINT_IN_MESH, 0, p.anyhit_max_unique_a_objects,
+                           &o, p.containment_hash_table, p.intersection_hash_table, &overflow_),
+          "rs3d_object_rays");
+    ctx_.sync();
+}
+
 void DeviceHashTable::create(uint64_t slots, bool bitwise, int maxProbes) {
     buf.alloc(slots * sizeof(uint64_t) + 256);
     table.slots = buf.as<uint64_t>();
@@ -594,8 +678,77 @@ uint64_t DeviceHashTable::failedInserts(DeviceContext& ctx) const {
     return f;
 }
 
+namespace {
+// The library context of the current device for the context-free estimator entry points.
+rs3d_ctx* current_device_ctx() {
+    static std::mutex mu;
+    static rs3d_ctx* ctxs[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ctxs[dev] && rs3d_ctx_create(dev, &ctxs[dev]) != RS3D_OK) ctxs[dev] = nullptr;
+    return ctxs[dev];
+}
+
+float estimate_selectivity(const GridCell* a, const GridCell* b, int numCells, float cellVolume, float epsilon,
+                           float gamma, const char* what) {
+    if (numCells <= 0) return 0.0f;
+    rs3d_ctx* ctx = current_device_ctx();
+    if (!ctx) return -1.0f;
+    const size_t bytes = (size_t)numCells * sizeof(GridCell);
+    void *da = nullptr, *db = nullptr;
+    if (hipMalloc(&da, bytes) != hipSuccess) return -1.0f;
+    if (hipMalloc(&db, bytes) != hipSuccess) {
+        (void)hipFree(da);
+        return -1.0f;
+    }
+    hip_check(hipMemcpy(da, a, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(db, b, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    double total = 0.0;
+    const int rc = rs3d_estimate_raw(ctx, da, db, (uint64_t)numCells, cellVolume, epsilon, gamma, &total);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    check(rc, what);
+    return (float)total;
+}
+}  // namespace
+
+float estimateOverlapSelectivity(const GridCell* h_gridA, const GridCell* h_gridB, int numCells, float cellVolume,
+                                 float epsilon, float gamma) {
+    return estimate_selectivity(h_gridA, h_gridB, numCells, cellVolume, epsilon, gamma, "estimateOverlapSelectivity");
+}
+
+float estimateIntersectionSelectivity(const GridCell* h_gridA, const GridCell* h_gridB, int numCells,
+                                      float cellVolume, float epsilon, float gamma) {
+    return estimate_selectivity(h_gridA, h_gridB, numCells, cellVolume, epsilon, gamma,
+                                "estimateIntersectionSelectivity");
+}
+
+float calculateGlobalAvgSize(const std::vector<GridCell>& cells) {
+    double total = 0.0;
+    long long count = 0;
+    for (const GridCell& c : cells)
+        if (c.TouchCount > 0) {
+            total += (double)c.AvgSizeMean * (double)c.TouchCount;
+            count += c.TouchCount;
+        }
+    return count ? (float)(total / count) : 0.0f;
+}
+
+float calculateGlobalAvgVolRatio(const std::vector<GridCell>& cells) {
+    double total = 0.0;
+    long long count = 0;
+    for (const GridCell& c : cells)
+        if (c.TouchCount > 0) {
+            total += (double)c.VolRatio * (double)c.TouchCount;
+            count += c.TouchCount;
+        }
+    return count ? (float)(total / count) : 1.0f;
+}
+
 long long estimatePairs(DeviceContext& ctx, const GeometryData& a, const GeometryData& b, float eps,
-                        float gamma, bool alpha, bool verbose) {
+                        float gamma, bool alpha, bool verbose, bool intersection) {
+    (void)ctx;
     if (!a.grid.hasGrid || !b.grid.hasGrid) {
         if (verbose) std::cout << "Skipping estimation: Grid data not found." << std::endl;
         return -1;
@@ -605,17 +758,48 @@ long long estimatePairs(DeviceContext& ctx, const GeometryData& a, const Geometr
         if (verbose) std::cerr << "Error: Grid mismatch. Skipping estimation." << std::endl;
         return -1;
     }
-    DeviceBuffer ca(n * sizeof(GridCell)), cb(n * sizeof(GridCell));
-    ca.upload(a.grid.cells.data(), n * sizeof(GridCell));
-    cb.upload(b.grid.cells.data(), n * sizeof(GridCell));
-    const float mn[3] = {a.grid.minBound.x, a.grid.minBound.y, a.grid.minBound.z};
-    const float mx[3] = {a.grid.maxBound.x, a.grid.maxBound.y, a.grid.maxBound.z};
-    int64_t est = 0;
-    check(rs3d_estimate_pairs(ctx.get(), ca.as<void>(), cb.as<void>(), n, mn, mx, a.grid.resolution, eps,
-                              gamma, alpha ? 1 : 0, &est),
-          "rs3d_estimate_pairs");
-    if (verbose)
-        std::cout << "\n=== Selectivity Estimation ===\nFinal Estimated Pairs:     " << est << "\n" << std::endl;
+    // raw estimate on the device, then the replication correction on the host
+    // (query/src/applications/raytracer_overlap_estimated_main.cpp:255-292)
+    const float width = a.grid.maxBound.x - a.grid.minBound.x;
+    const float height = a.grid.maxBound.y - a.grid.minBound.y;
+    const float depth = a.grid.maxBound.z - a.grid.minBound.z;
+    const float csX = width / a.grid.resolution[0], csY = height / a.grid.resolution[1],
+                csZ = depth / a.grid.resolution[2];
+    float cellVolume = csX * csY * csZ;
+    const float raw = intersection
+                          ? estimateIntersectionSelectivity(a.grid.cells.data(), b.grid.cells.data(), (int)n,
+                                                            cellVolume, eps, gamma)
+                          : estimateOverlapSelectivity(a.grid.cells.data(), b.grid.cells.data(), (int)n, cellVolume,
+                                                       eps, gamma);
+    if (raw < 0.0f) {
+        std::cerr << "Error: selectivity estimation could not allocate device memory" << std::endl;
+        return -1;
+    }
+    if (!alpha) return (long long)raw;
+    const float avgSize1 = calculateGlobalAvgSize(a.grid.cells), avgSize2 = calculateGlobalAvgSize(b.grid.cells);
+    const float avgVolRatio1 = calculateGlobalAvgVolRatio(a.grid.cells);
+    const float avgVolRatio2 = calculateGlobalAvgVolRatio(b.grid.cells);
+    const float effectiveSize1 = avgSize1 * std::cbrt(avgVolRatio1);
+    const float effectiveSize2 = avgSize2 * std::cbrt(avgVolRatio2);
+    const float combinedSize = effectiveSize1 + effectiveSize2;
+    const float minkowskiVol = combinedSize * combinedSize * combinedSize;
+    if (cellVolume < 1e-9f) cellVolume = 1e-9f;
+    float alphaF = minkowskiVol / cellVolume;
+    if (alphaF < 1.0f) alphaF = 1.0f;
+    const long long est = (long long)(raw / alphaF);
+    if (verbose) {
+        std::cout << "\n=== Selectivity Estimation ===" << std::endl;
+        std::cout << "Raw Potential Pairs:       " << (long long)raw << std::endl;
+        std::cout << "Avg Object Size (Mesh1):   " << avgSize1 << std::endl;
+        std::cout << "Avg Object Size (Mesh2):   " << avgSize2 << std::endl;
+        std::cout << "Avg VolRatio (Mesh1):      " << avgVolRatio1 << std::endl;
+        std::cout << "Avg VolRatio (Mesh2):      " << avgVolRatio2 << std::endl;
+        std::cout << "Effective Size (Mesh1):    " << effectiveSize1 << std::endl;
+        std::cout << "Effective Size (Mesh2):    " << effectiveSize2 << std::endl;
+        std::cout << "Replication Factor (alpha):" << alphaF << std::endl;
+        std::cout << "Final Estimated Pairs:     " << est << std::endl;
+        std::cout << "==============================\n" << std::endl;
+    }
     return est;
 }
 

@@ -16,6 +16,10 @@
 //   FilterRefinePipeline                       query/src/raytracing/FilterRefine.h:15-48
 //   PrecomputedEdgeData                        query/src/geometry/PrecomputedEdgeData.h:22-26
 //   MeshOverlapEdgesLauncher                   query/src/raytracing/MeshOverlapEdgesLauncher.h:43-56
+//   MeshIntersectionLauncher                   query/src/raytracing/MeshIntersectionLauncher.h:9-60
+//   MeshContainmentLauncher                    query/src/raytracing/MeshContainmentLauncher.h:16-71
+//   estimateOverlapSelectivity                 query/src/cuda/estimated_overlap.h:6-13
+//   estimateIntersectionSelectivity            query/src/cuda/estimated_intersection.h:6-13
 // Device errors are fail-fast (message + exit(1)) as with CUDA_CHECK/OPTIX_CHECK
 // (query/src/optix/OptixHelpers.h:15-29).
 #pragma once
@@ -180,6 +184,7 @@ public:
     void release();
     void upload(const void* src, size_t bytes);
     void download(void* dst, size_t bytes) const;
+    void fill(int byteValue);  // every byte set to byteValue (synchronous)
     template <typename T>
     T* as() const { return static_cast<T*>(ptr_); }
     size_t bytes() const { return bytes_; }
@@ -319,6 +324,91 @@ private:
     DeviceContext& ctx_;
 };
 
+// Intersection-join launches (query/src/raytracing/MeshIntersectionLauncher.h:9-60) over the
+// fields of MeshIntersectionLaunchParams (query/src/cuda/mesh_intersection.h:24-88): mesh1 is
+// the source of the launch (the 2->1 launches take params whose mesh1 fields hold mesh 2),
+// mesh2 the structure it is traced against (mesh2_handle). Every launch is synchronous.
+struct MeshIntersectionLaunchParams {
+    rs3d_mesh_view mesh1{};                 // vertices, indices, triangle_to_object, objects
+    const rs3d_edges* edges = nullptr;      // precomputed source edge rays (overlap phase)
+    rs3d_bvh* mesh2 = nullptr;              // target structure
+    rs3d_hash_table* hash_table = nullptr;  // use_hash_table when non-null
+    int32_t* collision_counts = nullptr;    // two-pass form: pass 1 counts
+    int64_t* collision_offsets = nullptr;   // pass 2 write offsets
+    MeshQueryResult* results = nullptr;     // pass 2 output
+    int pass = 1;
+    int use_anyhit_containment = 1;         // 0: the closest-hit nextafter loop
+    int overlap_max_iterations = 100;
+    int containment_max_iterations = 1024;
+    rs3d_intersection_stats* profiling_stats = nullptr;  // device counters; profiling when set
+    int32_t* pair_target_ids = nullptr;     // pair-hit tracking (device, sources x max)
+    uint32_t* pair_target_hits = nullptr;
+    int max_pair_targets_per_source = 0;
+    uint32_t* containment_iterations_per_source = nullptr;  // containment tracking (device)
+    uint32_t* containment_candidate_count_per_source = nullptr;
+    uint32_t* containment_candidate_overflow_per_source = nullptr;
+    int anyhit_max_pair_targets_per_source = 256;  // distinct targets tracked per source ray
+};
+
+class MeshIntersectionLauncher {
+public:
+    explicit MeshIntersectionLauncher(DeviceContext& ctx) : ctx_(ctx) {}
+    MeshIntersectionLauncher(const MeshIntersectionLauncher&) = delete;
+    MeshIntersectionLauncher& operator=(const MeshIntersectionLauncher&) = delete;
+    void launchOverlapMesh1ToMesh2(const MeshIntersectionLaunchParams& params, int launchSize);
+    void launchOverlapMesh2ToMesh1(const MeshIntersectionLaunchParams& params, int launchSize);
+    void launchContainmentMesh1ToMesh2(const MeshIntersectionLaunchParams& params, int launchSize);
+    void launchContainmentMesh2ToMesh1(const MeshIntersectionLaunchParams& params, int launchSize);
+    void launchMesh1ToMesh2(const MeshIntersectionLaunchParams& params, int launchSize) {
+        launchOverlapMesh1ToMesh2(params, launchSize);
+    }
+    void launchMesh2ToMesh1(const MeshIntersectionLaunchParams& params, int launchSize) {
+        launchOverlapMesh2ToMesh1(params, launchSize);
+    }
+    bool isValid() const { return true; }
+    // Targets beyond anyhit_max_pair_targets_per_source in the last containment launch.
+    uint64_t lastContainmentOverflow() const { return containmentOverflow_; }
+
+private:
+    void overlap(const MeshIntersectionLaunchParams& p, int launchSize, int swap);
+    void containment(const MeshIntersectionLaunchParams& p, int launchSize, int swap);
+    DeviceContext& ctx_;
+    uint64_t containmentOverflow_ = 0;
+};
+
+// Containment-join launches (query/src/raytracing/MeshContainmentLauncher.h:16-71) over the
+// fields of MeshContainmentLaunchParams (query/src/cuda/mesh_containment.h:7-48).
+struct MeshContainmentLaunchParams {
+    const rs3d_edges* src_edges = nullptr;              // phase 1 rays
+    rs3d_bvh* target = nullptr;                          // traced against (target_handle)
+    rs3d_hash_table* intersection_hash_table = nullptr;  // phase 1 output, phase 2 lookup
+    int swap_ids = 0;  // 0: B edges vs A, pairs (A hit, B source); 1: A edges vs B, (A source, B hit)
+    const float* b_first_vertices = nullptr;             // device float3, one per B object
+    int b_num_objects = 0;
+    rs3d_hash_table* containment_hash_table = nullptr;   // phase 2 output
+    int use_anyhit_point_in_mesh = 1;                    // 0: the closest-hit t + 1e-4 loop
+    int anyhit_max_unique_a_objects = 512;
+};
+
+class MeshContainmentLauncher {
+public:
+    explicit MeshContainmentLauncher(DeviceContext& ctx) : ctx_(ctx) {}
+    MeshContainmentLauncher(const MeshContainmentLauncher&) = delete;
+    MeshContainmentLauncher& operator=(const MeshContainmentLauncher&) = delete;
+    // Phase 1: edge rays of src_edges (the first numEdges) against target, nextafter chain
+    // of at most 256 hits, pairs into the intersection table.
+    void launchEdgeCheck(const MeshContainmentLaunchParams& params, int numEdges);
+    // Phase 2: one +z ray per B first vertex against target (A); odd A objects give (a, b)
+    // unless the pair is in the intersection table.
+    void launchPointInMesh(const MeshContainmentLaunchParams& params, int numBObjects);
+    bool isValid() const { return true; }
+    uint64_t lastPointInMeshOverflow() const { return overflow_; }
+
+private:
+    DeviceContext& ctx_;
+    uint64_t overflow_ = 0;
+};
+
 // Hash table owned on the device.
 struct DeviceHashTable {
     DeviceBuffer buf;
@@ -338,7 +428,22 @@ void generateGridStatsOnDevice(DeviceContext& ctx, const GeometryData& geometry,
 
 // Selectivity estimate of two .pre grids (raw / alpha), or -1 when grids are missing.
 long long estimatePairs(DeviceContext& ctx, const GeometryData& a, const GeometryData& b,
-                        float epsilon, float gamma, bool alphaCorrection, bool verbose);
+                        float epsilon, float gamma, bool alphaCorrection, bool verbose,
+                        bool intersection = false);
+
+// Raw selectivity estimate of two grids of numCells cells (host arrays), the sum over
+// cells of TouchA * TouchB * min((AvgA + AvgB + epsilon)^3 / cellVolume * sqrt(RatioA *
+// RatioB)^gamma, 1) (query/src/cuda/estimated_overlap.cu:8-77); -1.0f when device memory
+// cannot be allocated. Runs on the current device.
+float estimateOverlapSelectivity(const GridCell* h_gridA, const GridCell* h_gridB, int numCells,
+                                 float cellVolume, float epsilon = 0.001f, float gamma = 0.8f);
+// The same estimate for the intersection join (query/src/cuda/estimated_intersection.cu).
+float estimateIntersectionSelectivity(const GridCell* h_gridA, const GridCell* h_gridB, int numCells,
+                                      float cellVolume, float epsilon = 0.001f, float gamma = 0.8f);
+// Touch-weighted means of the cells' AvgSizeMean and VolRatio
+// (query/src/applications/raytracer_overlap_estimated_main.cpp:129-158).
+float calculateGlobalAvgSize(const std::vector<GridCell>& cells);
+float calculateGlobalAvgVolRatio(const std::vector<GridCell>& cells);
 
 // Split a comma list of paths (query/src/applications/rayspace_main.cpp:23-37).
 std::vector<std::string> splitPaths(const std::string& input);

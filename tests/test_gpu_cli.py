@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from rayspace3d_amd import gen, prep
+from rayspace3d_amd import gen, prep, preio
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rayspace3d_amd", "bin")
@@ -94,9 +94,40 @@ def test_intersection_cli(datasets):
     d, a, b, _ = datasets
     ref = orc.intersection_pairs(a, b)
     for exe in ("raytracer_intersection_estimated", "raytracer_mesh_intersection"):
-        run(exe, ["--mesh1", "a.pre", "--mesh2", "b.pre"], d)
+        run(exe, ["--mesh1", "a.pre", "--mesh2", "b.pre", "--enable-profiling-stats"], d)
         hdr, rows = read_pairs(d / "intersection_pairs.csv")
         assert hdr == ["a_object_id", "b_object_id"] and set(rows) == keys_to_set(ref)
+    # the tracking CSVs (PairHitTracking.h, ContainmentTracking.h) against the oracle's tracking
+    trk = {}
+    for src, tgt, swap, tag in ((a, b, False, "mesh1_to_mesh2"), (b, a, True, "mesh2_to_mesh1")):
+        _, _, tr, st = orc.object_parity_ex(src, tgt, 256, swap, 0, track=True, max_pair=256, max_out=256)
+        trk[tag] = (tr, st)
+    with open(d / "intersection_pair_hits.csv") as f:
+        rows = list(csv.reader(f))
+    assert rows[0] == ["direction", "source_object_id", "target_object_id", "target_ray_hits"]
+    exp = []
+    for tag in ("mesh1_to_mesh2", "mesh2_to_mesh1"):
+        tr = trk[tag][0]
+        for s_, (ids, hits) in enumerate(zip(tr["pair_target_ids"], tr["pair_target_hits"])):
+            exp += [[tag, str(s_), str(int(t)), str(int(h))] for t, h in zip(ids, hits) if t >= 0]
+    assert rows[1:] == exp and len(exp) > 10
+    with open(d / "intersection_containment_tracking.csv") as f:
+        rows = list(csv.reader(f))
+    assert rows[0] == ["direction", "source_object_id", "iterations", "candidate_count", "candidate_overflow_events"]
+    exp = []
+    for tag in ("mesh1_to_mesh2", "mesh2_to_mesh1"):
+        tr = trk[tag][0]
+        exp += [[tag, str(s_), str(int(i)), str(int(c)), str(int(o))]
+                for s_, (i, c, o) in enumerate(zip(tr["iterations"], tr["candidate_count"], tr["candidate_overflow"]))]
+    assert rows[1:] == exp
+    # profiling counters in the timing JSON (raytracer_intersection_estimated_main.cpp:722-747)
+    cnt = json.load(open(d / "mesh_intersection_timing.json"))["counters"]
+    st = [trk[k][1] for k in trk]
+    for name, key in (("profile_containment_rays_total", "rays"), ("profile_containment_hits_total", "hits"),
+                      ("profile_containment_candidate_toggles", "toggles"),
+                      ("profile_containment_targets_total", "targets")):
+        assert cnt[name] == st[0][key] + st[1][key], name
+    assert cnt["profile_overlap_trace_calls"] > 0 and cnt["profile_actual_intersection_pairs"] == len(ref)
 
 
 def test_containment_cli(datasets):
@@ -189,3 +220,77 @@ def test_mesh_overlap_exact(datasets):
     run("raytracer_mesh_overlap", ["--mesh1", golden, "--mesh2", golden, "--exact", "--warmup-runs", "0"], d)
     hdr, rows = read_pairs(d / "mesh_overlap_results.csv")
     assert len(rows) >= 540 and all((i, i) in set(rows) for i in range(540))
+
+
+def _read_launchers(path):
+    sets, stats, est, flags = {}, {}, {}, {}
+    for line in open(path):
+        w = line.split()
+        if w[0] == "stat":
+            stats[w[1]] = int(w[2])
+        elif w[0] == "estimate":
+            est[w[1]] = float(w[2])
+        elif len(w) == 2:
+            flags[w[0]] = int(w[1])
+        else:
+            n = int(w[1])
+            sets[w[0]] = np.array([int(x) for x in w[2:2 + n]], np.uint64)
+    return sets, stats, est, flags
+
+
+def test_launcher_classes_and_estimators(datasets):
+    """MeshIntersectionLauncher (every method, any-hit and closest-hit containment, hash and
+    two-pass forms, profiling counters), MeshContainmentLauncher (edge check and point in mesh,
+    both point-in-mesh forms) and estimateOverlapSelectivity / estimateIntersectionSelectivity
+    called from C++ (check_launchers), against the oracle."""
+    d, a, b, _ = datasets
+    run("check_launchers", ["--mesh1", "a.pre", "--mesh2", "b.pre", "--out", "launchers.txt"], d)
+    sets, stats, est, flags = _read_launchers(d / "launchers.txt")
+    # intersection launcher
+    assert np.array_equal(sets["intersection_anyhit"], orc.intersection_pairs(a, b))
+    k1 = orc.edge_pairs(a.edges, b.vertices, b.indices, b.tri_to_obj, False)
+    k2 = orc.edge_pairs(b.edges, a.vertices, a.indices, a.tri_to_obj, True)
+    c1 = orc.object_parity_ex(a, b, 256, False, 1, 1024, max_out=256)[0]
+    c2 = orc.object_parity_ex(b, a, 256, True, 1, 1024, max_out=256)[0]
+    assert np.array_equal(sets["intersection_closest"], orc.sort_unique(np.concatenate([k1, k2, c1, c2])))
+    ov12 = orc.sort_unique(k1)
+    assert np.array_equal(sets["overlap12_hash"], ov12) and np.array_equal(sets["overlap12_twopass"], ov12)
+    assert flags["merged_in_ba_order"] == 1 and len(ov12) > 5
+    ct12, _, _, st1 = orc.object_parity_ex(a, b, 256, False, 0, max_out=256)
+    assert np.array_equal(sets["contain12_hash"], orc.sort_unique(ct12))
+    assert np.array_equal(sets["contain12_twopass"], orc.sort_unique(ct12)) and len(ct12) > 0
+    # profiling counters of the any-hit run (both directions)
+    st2 = orc.object_parity_ex(b, a, 256, True, 0, max_out=256)[3]
+    for name, key in (("containment_rays_total", "rays"), ("containment_iterations_total", "iterations"),
+                      ("containment_hits_total", "hits"), ("containment_candidate_additions", "additions"),
+                      ("containment_candidate_toggles", "toggles"), ("containment_targets_total", "targets")):
+        assert stats[name] == st1[key] + st2[key], name
+    calls = iters = hits = mx = 0
+    for e, m in ((a.edges, b), (b.edges, a)):
+        _, cnt = orc.edge_pairs(e, m.vertices, m.indices, m.tri_to_obj, with_counts=True)
+        dv = (e.ends - e.starts).astype(np.float32)
+        ln = np.sqrt((dv[:, 0] * dv[:, 0] + dv[:, 1] * dv[:, 1]) + dv[:, 2] * dv[:, 2]).astype(np.float32)
+        live = (ln >= np.float32(1e-6)) & (e.src_counts > 0)
+        sc = e.src_counts.astype(np.int64)
+        it = np.minimum(np.where(live, cnt // np.maximum(sc, 1), 0) + 1, 100)
+        calls += int(sc[live].sum())
+        iters += int((it * sc)[live].sum())
+        hits += int(cnt[live].sum())
+        mx = max(mx, int(it[live].max()))
+    assert (stats["overlap_trace_calls"], stats["overlap_iterations_total"], stats["overlap_hits_total"],
+            stats["overlap_max_iterations_per_trace"]) == (calls, iters, hits, mx)
+    # containment launcher: any-hit form equals the containment join; the closest-hit form
+    # steps t + 1e-4 from B's first vertices
+    ov, ct = orc.containment_pairs(a, b)
+    assert np.array_equal(sets["containment_overlap"], ov)
+    assert np.array_equal(sets["containment_anyhit"], ct) and len(ct) >= 40
+    first = np.stack([b.vertices[b.indices[np.nonzero(b.tri_to_obj == o)[0][0], 0]] for o in range(b.num_objects)])
+    cand = orc.sort_unique(orc.object_parity_ex(None, a, 512, True, 2, 2000, origins=first)[0])
+    assert np.array_equal(sets["containment_closest"], cand[~np.isin(cand, ov)])
+    # estimators (reference signatures): raw estimate, and with the host alpha correction
+    ga, gb = preio.read_pre(str(d / "a.pre")).grid, preio.read_pre(str(d / "b.pre")).grid
+    cv = float(np.prod((ga.max_bound.astype(np.float32) - ga.min_bound.astype(np.float32)) / np.float32(16)))
+    raw = orc.estimate_raw(ga.cells, gb.cells, cv)
+    assert abs(est["overlap_raw"] - raw) <= 1e-5 * raw and abs(est["intersection_raw"] - raw) <= 1e-5 * raw
+    assert est["empty"] == 0.0
+    assert est["pairs"] == orc.estimate_pairs(ga.cells, gb.cells, ga.min_bound, ga.max_bound, ga.resolution)
