@@ -67,6 +67,9 @@ def main():
                     help="containment: pick leaf size, reordering and run length with the "
                          "autotuner on the workload's own rays first (the defaults are its "
                          "choice on this workload)")
+    ap.add_argument("--legs", type=int, default=1, choices=[0, 1],
+                    help="containment workload: also run the other configurations (single "
+                         "sphere, overlap join, batched, serving) and report them under 'legs'")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--datasets", type=int, default=64, help="serving: resident datasets")
     ap.add_argument("--overlap-objects", type=int, default=50_000,
@@ -294,9 +297,123 @@ def main():
         "bvh_canonical_bytes_per_ray": round((16 * st["rays"] + 64 * st["nodes_visited"] + 64 * st["tris_tested"])
                                              / max(st["rays"], 1), 1),
     }
+    # the same step with the index built cold inside it (BVH + axis grid + one step), as the
+    # reference's filter-refine phase includes its structure builds
+    # (query/src/raytracing/FilterRefine.cpp:81-122)
+    bvh.free()
+    cold = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        b2 = Bvh(ctx, dV, dI, dO, leaf_size=leaf)
+        if args.axis_grid:
+            axis_grid_prepare(ctx, b2, 2)
+        pipe.execute(b2, dP, axis=2, out=out)
+        ctx.sync()
+        cold.append((time.perf_counter() - tc) * 1e3)
+        b2.free()
+    line["build_inclusive"] = {"ms_per_step": round(min(cold), 3),
+                               "mrays_s": round(len(pts) / (min(cold) / 1e3) / 1e6, 1),
+                               "note": "BVH build + axis grid build + one filter-refine step, inputs resident"}
+    ctx.close()
+    del dV, dI, dO, dP, out
+    if args.legs:
+        legs = {}
+        for name, fn in (("single_sphere", leg_single_sphere), ("overlap", leg_overlap),
+                         ("batched", leg_batched), ("serving", leg_serving)):
+            tl = time.time()
+            try:
+                legs[name] = fn(args)
+            except Exception as e:  # a leg that fails is reported, the headline still prints
+                legs[name] = {"error": f"{type(e).__name__}: {e}"}
+            legs[name]["leg_wall_s"] = round(time.time() - tl, 1)
+            torch.cuda.empty_cache()
+            print(f"[bench] leg {name} done in {time.time() - tl:.0f}s", file=sys.stderr, flush=True)
+        line["legs"] = legs
     print(json.dumps(line), flush=True)
+
+
+def leg_single_sphere(args):
+    """Plain point containment: one closed icosphere (level 5, 20,480 triangles, r = 1,
+    vertices radially perturbed by +-0.05), 1,000,000 points uniform in [-1.5, 1.5]^3, one +x
+    ray per point, uint8 inside flag per point; BVH and axis grid resident. Checked against
+    the oracle on every point."""
+    import numpy as np
+    import torch
+
+    from rayspace3d_amd import _lib, gen
+    from rayspace3d_amd.device import Bvh, Context, axis_grid_prepare, axis_grid_stats, to_dev, to_host, trace_points
+
+    mesh, pts = gen.single_sphere_workload(seed=1, n_points=1_000_000, level=5)
+    ctx = Context(0)
+    dV, dI, dO, dP = to_dev(mesh.vertices), to_dev(mesh.indices), to_dev(mesh.tri_to_obj), to_dev(pts)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    bvh = Bvh(ctx, dV, dI, dO)
+    ctx.sync()
+    build_ms = (time.perf_counter() - tb) * 1e3
+    grid = axis_grid_prepare(ctx, bvh, 0)
+    out = torch.empty(len(pts), dtype=torch.uint8, device="cuda")
+    for _ in range(max(args.warmup, 1)):
+        trace_points(ctx, bvh, dP, axis=0, out_kind=_lib.OUT_FLAG_U8, out=out)
+    steps = max(args.steps, 20)
+    ctx.profile(True)
+    ctx.profile_read()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(steps):
+        trace_points(ctx, bvh, dP, axis=0, out_kind=_lib.OUT_FLAG_U8, out=out, sync=False)
+    ctx.sync()
+    t_step = (time.perf_counter() - ts) / steps
+    kern_ms, kern_n = ctx.profile_read()
+    ctx.profile(False)
+    kern_avg = kern_ms / max(kern_n, 1)
+    flags = to_host(out).astype(bool)
+    use_grid = grid["state"] == 1
+    if use_grid:
+        # 12 B origin + 1 B flag + 4 B occupancy word per ray, 16 B per cell-table read,
+        # 64 B per record read
+        g = axis_grid_stats(ctx, bvh, dP, axis=0)
+        alg = 17 * len(pts) + 16 * g["nodes_visited"] + 64 * g["tris_tested"]
+        kernel = "k_points_grid (axis grid, +x)"
+    else:
+        g = None
+        from rayspace3d_amd.device import trace_stats
+        st = trace_stats(ctx, bvh, dP, axis=0)
+        alg = 13 * len(pts) + 64 * st["nodes_visited"] + 64 * st["tris_tested"]
+        kernel = "k_points_refill (BVH walk, +x)"
+    achieved = alg / (kern_avg / 1e3) / 1e9 if kern_avg > 0 else 0.0
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    small = pts[:1000]
+    tc0 = time.perf_counter()
+    orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, small, axis=0, threads=threads)
+    tc1 = time.perf_counter()
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=0, threads=threads)
+    tc2 = time.perf_counter()
+    cpu_rate = (len(pts) - len(small)) / max((tc2 - tc1) - (tc1 - tc0), 1e-9)
+    line = {
+        "config": {"workload": "point containment: 1 icosphere (L5, 20,480 tris, r 1, radial noise 0.05), "
+                               "1,000,000 points uniform in [-1.5,1.5]^3, +x rays, uint8 flags",
+                   "points": len(pts), "triangles": mesh.num_triangles, "seed": 1},
+        "value": round(len(pts) / t_step / 1e6, 3), "unit": "Mrays/s", "ms_per_step": round(t_step * 1e3, 4),
+        "steps": steps,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": kernel,
+                     "kernel_avg_ms": round(kern_avg, 4), "alg_bytes_per_launch": int(alg),
+                     "alg_bytes_per_ray": round(alg / len(pts), 2)},
+        "cpu_baseline": {"value": round(cpu_rate / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                         "sample": "all 1,000,000 points, CPU oracle (binned candidates, same chain rules), "
+                                   "grid build excluded"},
+        "parity_all_points": bool(np.array_equal(flags, ref >= 0)),
+        "inside_fraction": round(float(flags.mean()), 4),
+        "build_ms": round(build_ms, 3), "axis_grid": grid, "axis_grid_stats": g,
+        "build_inclusive_ms": round(build_ms + (grid.get("build_ms") or 0.0) + t_step * 1e3, 3),
+    }
     bvh.free()
     ctx.close()
+    return line
 
 
 def run_copy():
@@ -314,6 +431,10 @@ def run_copy():
 
 
 def run_batched(args):
+    print(json.dumps(leg_batched(args)), flush=True)
+
+
+def leg_batched(args):
     """Independent containment and overlap queries (mesh sizes log-uniform in [1e3, 5e5]
     triangles, point counts log-uniform in [1e3, 1e7]): all queries through one persistent
     launch versus one launch sequence per query, with identical results required."""
@@ -426,12 +547,12 @@ def run_batched(args):
         "pairs": pairs, "pairs_per_s": round(pairs / t_batch, 1),
         "identical_results": same,
     }
-    print(json.dumps(line), flush=True)
     for q in queries:
         for k in ("bvh", "bvh1", "bvh2"):
             if k in q:
                 q[k].free()
     ctx.close()
+    return line
 
 
 def _sub_mesh(gen, M, side):
@@ -572,6 +693,10 @@ def run_trijoin(args):
 
 
 def run_overlap(args):
+    print(json.dumps(leg_overlap(args)), flush=True)
+
+
+def leg_overlap(args):
     """Mesh overlap join: datasets A and B of --overlap-objects icospheres each (levels 2-4
     uniform, per-axis scale 0.5-2, random rotation, r0 = 5, centres uniform in 1000^3).
     One step = both edge directions (A's unique edges against B's BVH and B's against A's)
@@ -690,8 +815,13 @@ def run_overlap(args):
         tc2 = time.perf_counter()
         dt = (tc2 - tc1) - (tc1 - tc0)
         srays = sea.num_edges + seb.num_edges
+        gs = [Bvh.from_mesh(ctx, sa), Bvh.from_mesh(ctx, sb)]
+        gpu_sample = overlap_join(ctx, gs[0], DevEdges(sea), gs[1], DevEdges(seb), mode=1)
+        for g in gs:
+            g.free()
         cpu = {"value": round(srays / max(dt, 1e-9) / 1e6, 4), "unit": "Medge-rays/s", "cores": 1,
-               "kind": "port",
+               "kind": "port", "parity_on_sample": bool(np.array_equal(gpu_sample, ref)),
+               "sample_pairs": int(len(ref)),
                "sample": f"objects of both datasets centred in [0,200]^3 ({srays} edge rays, "
                          f"{len(ref)} pairs), both directions, single-threaded CPU oracle "
                          "(3-D grid candidates, same chain rules), grid build excluded"}
@@ -725,10 +855,10 @@ def run_overlap(args):
         "pairs_sorted_unique": sane,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
     ba.free()
     bb.free()
     ctx.close()
+    return line
 
 
 def run_edges(args):
@@ -814,6 +944,10 @@ def run_edges(args):
 
 
 def run_serving(args):
+    print(json.dumps(leg_serving(args)), flush=True)
+
+
+def leg_serving(args):
     """Poisson streams of mixed containment / overlap / intersection requests against
     resident BVHs of the many-spheres generator, served with continuous batching; p50/p99
     latency and throughput at three arrival rates (fractions of the measured capacity)."""
@@ -845,7 +979,6 @@ def run_serving(args):
         qm.edges = gen.extract_edges(qm)
         queries.append((Bvh.from_mesh(ctx, qm), DevEdges(qm.edges).spatial(ctx)))
     pool = to_dev(gen.uniform_points(rng, 20_000_000, 0.0, 1000.0))
-    out = torch.empty(pool.shape[0], dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     print(f"[bench] setup done in {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
 
@@ -856,8 +989,12 @@ def run_serving(args):
     qsel = rng.integers(0, len(queries), size=n)
     npts = np.exp(rng.uniform(np.log(1e3), np.log(1e6), size=n)).astype(np.int64)
     starts = rng.integers(0, pool.shape[0] - 1_000_000, size=n)
+    # each containment request writes its own range of out (its answers are then comparable
+    # between runs, whatever order concurrent requests finish in)
+    ooff = np.concatenate([[0], np.cumsum(npts)[:-1]]).astype(np.int64)
+    out = torch.empty(int(npts.sum()), dtype=torch.int32, device="cuda")
 
-    def serve(rate, max_slots):
+    def serve(rate, max_slots, record=None):
         srv = Server(ctx, max_slots=max_slots, memory_budget=240 << 30)
         ids = [srv.add_dataset(b, e, grid_res=128) for b, e in srv_datasets]
         qids = [srv.add_dataset(b, e, grid_res=128) for b, e in queries]
@@ -871,7 +1008,7 @@ def run_serving(args):
             while nxt < n and arrivals[nxt] <= now:
                 if kinds[nxt] == 0:
                     p = pool[starts[nxt]: starts[nxt] + npts[nxt]]
-                    o = out[starts[nxt]: starts[nxt] + npts[nxt]]
+                    o = out[ooff[nxt]: ooff[nxt] + npts[nxt]]
                     t = srv.submit(_lib.REQ_CONTAIN, ids[targets[nxt]], points=p, out=o)
                     rays += int(npts[nxt])
                 else:
@@ -887,6 +1024,8 @@ def run_serving(args):
                     if st == 1:
                         lat[tickets[t]] = tnow - arrivals[tickets[t]]
                         pairs += len(keys)
+                        if record is not None:
+                            record[tickets[t]] = keys
                         del tickets[t]
                         done += 1
             elif nxt < n:
@@ -901,10 +1040,31 @@ def run_serving(args):
                 "peak_admitted_bytes": st["peak_memory"]}
 
     serve(None, 64)  # warmup
-    closed = serve(None, 64)
-    single = serve(None, 1)
+    rec_mixed, rec_alone = {}, {}
+    closed = serve(None, 64, rec_mixed)
+    out_mixed = out.clone()
+    single = serve(None, 1, rec_alone)
+    same = bool(torch.equal(out, out_mixed)) and len(rec_mixed) == n and all(
+        np.array_equal(rec_mixed[i], rec_alone[i]) for i in range(n))
     cap = closed["throughput_req_s"]
     rates = [serve(f * cap, 64) for f in (0.3, 0.6, 0.9)]
+    # the plain per-query path: each request through the direct entry points, one after the other
+    from rayspace3d_amd.device import intersection_join, overlap_join, trace_points
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for i in range(n):
+        b, e = srv_datasets[targets[i]]
+        if kinds[i] == 0:
+            trace_points(ctx, b, pool[starts[i]: starts[i] + npts[i]], axis=2,
+                         out=out[ooff[i]: ooff[i] + npts[i]], sync=False)
+        else:
+            qb, qe = queries[qsel[i]]
+            if kinds[i] == 1:
+                overlap_join(ctx, qb, qe, b, e, mode=1)
+            else:
+                intersection_join(ctx, qb, qe, b, e)
+    ctx.sync()
+    plain_s = time.perf_counter() - tp
     line = {
         "metric": "Mrays/s + join pairs/s, batched vs per-query on MI355X (exact); p50/p99 ms",
         "value": closed["mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -916,12 +1076,15 @@ def run_serving(args):
                    "datasets": args.datasets, "requests": n},
         "closed_loop": closed, "one_at_a_time": single,
         "continuous_batching_speedup": round(closed["throughput_req_s"] / single["throughput_req_s"], 3),
+        "per_query_api_req_s": round(n / plain_s, 1),
+        "speedup_vs_per_query_api": round(closed["throughput_req_s"] / (n / plain_s), 3),
+        "identical_results": same,
         "poisson": rates,
     }
-    print(json.dumps(line), flush=True)
     for b, _ in keep + queries:
         b.free()
     ctx.close()
+    return line
 
 
 if __name__ == "__main__":
