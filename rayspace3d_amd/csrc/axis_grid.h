@@ -8,12 +8,21 @@
 
 namespace rs3d {
 
+// Cells are numbered in tiles of 8 (u) x 4 (w): cell (cu, cw) is
+// ((cw / 4) * ntu + cu / 8) * 32 + (cw % 4) * 8 + cu % 8, with ntu = ceil(nu / 8), so one
+// 32-bit occupancy word covers one tile and a tile's cells are consecutive in the cell
+// table and the records. Occupancy has two levels: tsum[g] holds, for tiles 32g..32g+31, a
+// bit per tile that lists records (.x) and the number of such tiles before the group (.y);
+// occ holds the cell bits of those tiles only, in tile order (rank .y + popcount of the
+// lower bits of .x). A point in an empty tile is answered from tsum alone, which is small
+// enough (8 B per 1024 cells) to stay in L2 while records stream through it.
 struct AxisGrid {
-    int axis = 2, nu = 1, nw = 1;
+    int axis = 2, nu = 1, nw = 1, ntu = 1;
     float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
     uint64_t entries = 0;
-    uint2* cells = nullptr;     // nu * nw + 1 entries: first record, box top of that record
-    uint32_t* occ = nullptr;    // one bit per cell: the cell lists records
+    uint2* cells = nullptr;     // tiled cells + 1 entries: first record, box top of that record
+    uint2* tsum = nullptr;      // per 32 tiles: occupied-tile bits, occupied tiles before
+    uint32_t* occ = nullptr;    // per occupied tile: its 32 cell bits
     float4* rec = nullptr;      // 4 float4 per record
     size_t bytes = 0;
     double expected = 0.0;
@@ -22,11 +31,28 @@ struct AxisGrid {
 
 struct AGArgs {
     const uint2* cells;
+    const uint2* tsum;
     const uint32_t* occ;
     const float4* rec;
     float lo_u, lo_w, inv_u, inv_w;
-    int nu, nw;
+    int nu, nw, ntu;
 };
+
+inline AGArgs ag_args(const AxisGrid* g) {
+    return AGArgs{g->cells, g->tsum, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw, g->ntu};
+}
+
+__host__ __device__ __forceinline__ uint32_t ag_cell_id(int cu, int cw, int ntu) {
+    return (((uint32_t)cw >> 2) * (uint32_t)ntu + ((uint32_t)cu >> 3)) * 32u + (((uint32_t)cw & 3u) << 3) +
+           ((uint32_t)cu & 7u);
+}
+
+// Rank of tile t among the occupied tiles, given its group's tsum entry; -1 when t is empty.
+__device__ __forceinline__ int64_t ag_tile_rank(uint2 g, uint32_t t) {
+    const uint32_t b = t & 31u;
+    if (!((g.x >> b) & 1u)) return -1;
+    return (int64_t)g.y + __popc(g.x & ((1u << b) - 1u));
+}
 
 // Cell coordinate along one axis: monotone in x (clamped), NaN goes to cell 0.
 __device__ __forceinline__ int ag_cell(float x, float lo, float inv, int n) {
@@ -94,11 +120,12 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 } else {
                     const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
                     const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
-                    const uint64_t c = (uint64_t)cw * G.nu + cu;
-                    // most rays of a sparse scene fall in empty cells: the occupancy bits
-                    // (L2-resident) answer those without the range fetch
+                    const uint32_t c = ag_cell_id(cu, cw, G.ntu);
+                    // most rays of a sparse scene fall in empty tiles or cells: the occupancy
+                    // levels answer those without the range fetch
                     js = je = 0;
-                    if ((G.occ[c >> 5] >> (c & 31)) & 1u) {
+                    const int64_t tr = ag_tile_rank(G.tsum[c >> 10], c >> 5);
+                    if (tr >= 0 && ((G.occ[tr] >> (c & 31)) & 1u)) {
                         const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
                         if (STATS) ctr.nodes++;
                         // the first record's box top decides as the walk's exit test would
@@ -215,7 +242,7 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             float3 o[4];
             bool ok[4];
             uint32_t cell[4], word[4], pid[4];
-            uint2 rg[4];
+            uint2 rg[4], ts[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint64_t r = base + 64 * k + lane;
@@ -231,10 +258,15 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
                 }
                 const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
                 const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
-                cell[k] = (uint32_t)cw * (uint32_t)G.nu + (uint32_t)cu;
+                cell[k] = ag_cell_id(cu, cw, G.ntu);
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) word[k] = ok[k] ? G.occ[cell[k] >> 5] : 0u;
+            for (int k = 0; k < 4; ++k) ts[k] = ok[k] ? G.tsum[cell[k] >> 10] : make_uint2(0u, 0u);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t tr = ag_tile_rank(ts[k], cell[k] >> 5);
+                word[k] = tr >= 0 ? G.occ[tr] : 0u;
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 ok[k] = (word[k] >> (cell[k] & 31)) & 1u;

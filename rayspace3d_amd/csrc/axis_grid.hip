@@ -33,6 +33,7 @@ void axis_grid_free(AxisGrid* g) {
     if (!g) return;
     if (g->cells) (void)hipFree(g->cells);
     if (g->occ) (void)hipFree(g->occ);
+    if (g->tsum) (void)hipFree(g->tsum);
     if (g->rec) (void)hipFree(g->rec);
     delete g;
 }
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_ag_emit(const TriRec* __restrict__ tris
     uint64_t k = off[i];
     for (int cw = cw0; cw <= cw1; ++cw)
         for (int cu = cu0; cu <= cu1; ++cu) {
-            const uint64_t cell = (uint64_t)cw * G.nu + cu;
+            const uint64_t cell = ag_cell_id(cu, cw, G.ntu);
             keys[k] = (cell << 32) | low;
             vals[k] = (uint32_t)i;
             ++k;
@@ -180,18 +181,38 @@ __global__ __launch_bounds__(256) void k_ag_pad(const uint32_t* __restrict__ cou
     rec[4 * d + 3] = make_float4(nan, nan, nan, nan);
 }
 
-// Occupancy bits from the counts: one cell per lane, a wave's ballot is the two words of
-// its 64 cells.
+// Occupancy words from the counts: one cell per lane, a wave's ballot is the words of its
+// two tiles (nc is a multiple of 32).
 __global__ __launch_bounds__(256) void k_ag_occ(const uint32_t* __restrict__ counts, uint64_t nc,
-                                                uint32_t* __restrict__ occ) {
+                                                uint32_t* __restrict__ words) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t m = __ballot(c < nc && counts[c] != 0u);
-    if ((threadIdx.x & 63) == 0 && c < nc) {
-        const uint64_t w = c / 32;  // c is a multiple of 64 here
-        const uint64_t nwords = (nc + 31) / 32;
-        occ[w] = (uint32_t)m;
-        if (w + 1 < nwords) occ[w + 1] = (uint32_t)(m >> 32);
+    if ((threadIdx.x & 31) == 0 && c < nc) words[c / 32] = (uint32_t)(m >> (threadIdx.x & 32));
+}
+
+// Tile summaries: one tile per lane; lane 32g of the ballot writes group g's tile bits and
+// their count (the scan of the counts gives each group's rank base).
+__global__ __launch_bounds__(256) void k_ag_tsum(const uint32_t* __restrict__ words, uint64_t nt,
+                                                 uint2* __restrict__ tsum, uint32_t* __restrict__ gcount) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t m = __ballot(t < nt && words[t] != 0u);
+    if ((threadIdx.x & 31) == 0 && t < nt) {
+        const uint32_t b = (uint32_t)(m >> (threadIdx.x & 32));
+        tsum[t / 32] = make_uint2(b, 0u);
+        gcount[t / 32] = (uint32_t)__popc(b);
     }
+}
+
+// Compact words of the occupied tiles at their ranks, and each group's rank base.
+__global__ __launch_bounds__(256) void k_ag_fine(const uint32_t* __restrict__ words, uint64_t nt,
+                                                 const uint32_t* __restrict__ gbase, uint2* __restrict__ tsum,
+                                                 uint32_t* __restrict__ occ) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t base = gbase[t / 32];
+    if ((t & 31) == 0) tsum[t / 32].y = base;
+    const uint32_t w = words[t];
+    if (w) occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = w;
 }
 
 // Cell table: (first record, box top of the first record -- the largest of the cell) per
@@ -332,6 +353,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         G.inv_w = ew > 0 ? (float)(nw / ew) : 0.0f;
         G.nu = nu;
         G.nw = nw;
+        G.ntu = (nu + 7) / 8;
         unsigned long long* acc;
         RS3D_TRY(acc_slot(ctx, 0, 0, &acc));
         k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
@@ -343,8 +365,8 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         if (!fine || !std::isfinite(h) || !(h > 0)) return RS3D_OK;  // rejected: too many records
         h = std::min(h * 1.5, std::max(mean_ext, h_min));
     }
-    const uint64_t nc = (uint64_t)nu * nw;
-    if ((double)total / (double)nc > AG_MAX_UNIFORM_LIST) return RS3D_OK;  // rejected: lists too long
+    if ((double)total / ((double)nu * nw) > AG_MAX_UNIFORM_LIST) return RS3D_OK;  // rejected: lists too long
+    const uint64_t nt = (uint64_t)G.ntu * (uint64_t)((nw + 3) / 4), nc = 32 * nt, ng = (nt + 31) / 32;
     RS3D_TRY(scan_u32_exclusive(ctx, span, off, n));
     uint64_t* keys;
     uint32_t *vals, *counts;
@@ -363,7 +385,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     RS3D_TRY(acc_read_sum(ctx, acc_sq, &sq));
     const double weighted = total ? (double)sq / (double)total : 0.0;
     if (weighted > AG_MAX_WEIGHTED_LIST) return RS3D_OK;  // rejected: dense spots too long
-    const double expected = (double)total / (double)nc;
+    const double expected = (double)total / ((double)nu * nw);
     // padded lists (even record counts; see k_ag_pcounts)
     uint32_t *pcounts, *pstart, *start;
     RS3D_TRY(tmp.get(&pcounts, nc));
@@ -379,10 +401,15 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     }
     const uint64_t ptotal = total + odd;
     if (ptotal > 0xffffffffull) return RS3D_OK;  // rejected: too many records
+    uint32_t *words, *gcount, *gbase;
+    RS3D_TRY(tmp.get(&words, nt));
+    RS3D_TRY(tmp.get(&gcount, ng));
+    RS3D_TRY(tmp.get(&gbase, ng));
     AxisGrid* g = new AxisGrid();
     g->axis = AX;
     g->nu = nu;
     g->nw = nw;
+    g->ntu = G.ntu;
     g->lo_u = G.lo_u;
     g->lo_w = G.lo_w;
     g->inv_u = G.inv_u;
@@ -390,19 +417,24 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->entries = total;
     g->expected = expected;
     hipError_t e;
-    const uint64_t nwords = (nc + 31) / 32;
     if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&g->occ, nwords * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->occ, nt * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->tsum, ng * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * 4 * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nwords * sizeof(uint32_t) + ptotal * 4 * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint32_t) + ng * sizeof(uint2) + ptotal * 4 * sizeof(float4);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
-        k_ag_occ<<<nbc, 256, 0, s>>>(counts, nc, g->occ);
+        k_ag_occ<<<nbc, 256, 0, s>>>(counts, nc, words);
+        unsigned nbt;
+        RS3D_TRY(grid_blocks(nt, 256, &nbt));
+        k_ag_tsum<<<nbt, 256, 0, s>>>(words, nt, g->tsum, gcount);
+        RS3D_TRY(scan_u32_exclusive(ctx, gcount, gbase, ng));
+        k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, g->tsum, g->occ);
         if (total) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
@@ -456,7 +488,7 @@ int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
 template <int AX, bool STATS>
 int launch_grid_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
                    int out_kind, void* out, unsigned long long* stats) {
-    const AGArgs G{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const AGArgs G = ag_args(g);
     const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
@@ -478,7 +510,7 @@ int launch_grid(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, 
 template <int AX>
 int launch_grid_filtered_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const FBox& box,
                             int32_t* out, unsigned long long* counts) {
-    const AGArgs G{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+    const AGArgs G = ag_args(g);
     const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
@@ -516,10 +548,11 @@ uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
     // Peak device bytes of a build: records are at most 16 per triangle + 2^20 (the budget
     // the build widens its cells to meet), 64 B each plus up to 1/8 for the padding to even
     // lists, and while the build sorts them 24 B more each (keys, values and the sort's
-    // alternate buffers); cells are within 16 per triangle + 1024, 8 B in the table and
-    // 16 B of counts and offsets during the build; 8 B per triangle of spans and offsets.
+    // alternate buffers); cells are within 16 per triangle + 1024 and the padding of the
+    // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table and 16 B of counts and
+    // offsets during the build; 8 B per triangle of spans and offsets.
     const uint64_t n = bvh->num_tris;
-    const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024;
+    const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024 + (1ull << 19);
     return recs * (72 + 24) + cells * 24 + n * 8 + (64ull << 20);
 }
 

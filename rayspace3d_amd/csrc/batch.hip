@@ -209,7 +209,7 @@ int BatchPlan::add_contain(const rs3d_bvh* b, const float* pts, uint64_t n, int3
         RS3D_TRY(axis_grid_ready(ctx_, const_cast<rs3d_bvh*>(b), axis, &grid, build_structures));
     if (grid) {  // rays in the caller's order on the grid: no reorder keys for this query
         const AxisGrid* g = b->agrid[axis];
-        d.grid = AGArgs{g->cells, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw};
+        d.grid = ag_args(g);
         items.push_back(d);
         return RS3D_OK;
     }
