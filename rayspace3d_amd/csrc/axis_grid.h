@@ -80,6 +80,19 @@ __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict
     }
 }
 
+// One grid record against a point ray: the record's box ends at or above the origin and
+// contains the ray's line in the plane, and the triangle is hit at t in [thr, 1e16).
+template <int AX>
+__device__ __forceinline__ bool grid_rec_hit(const float3 o, const float4& a0, const float4& a1, const float4& a2,
+                                             const float4& a3, float thr, float& t) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float ou = get3(o, U), ow = get3(o, W);
+    if (a2.w < get3(o, AX) || !(ou >= a3.x && ou <= a3.y && ow >= a3.z && ow <= a3.w)) return false;
+    return tri_hit_axis<AX>(o, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                            make_float3(a2.x, a2.y, a2.z), t) &&
+           t >= thr && t < 1e16f;
+}
+
 // Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
 // restarts, chain, output) is the BVH kernels'; only the candidate walk differs: a lane
@@ -92,7 +105,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                                                 int out_kind, void* __restrict__ out, TraceCounters& ctr,
                                                 GridFilter* gf = nullptr, const uint2* qrange = nullptr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    const float eps = 1e-6f, tmax = 1e16f;
+    const float eps = 1e-6f;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint64_t next = lo;
@@ -147,7 +160,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         }
         if (__ballot(active) == 0) break;
         if (!active) continue;
-        const float ou = get3(o, U), ow = get3(o, W), oa = get3(o, AX);
+        const float oa = get3(o, AX);
         if (j < je) {
             const bool two = j + 1 < je;
             const uint64_t b0 = 4 * (uint64_t)j, b1 = 4 * (uint64_t)(two ? j + 1 : j);
@@ -155,17 +168,13 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             const float4 f0 = G.rec[b1], f1 = G.rec[b1 + 1], f2 = G.rec[b1 + 2], f3 = G.rec[b1 + 3];
             // one record: box test in the plane, then the triangle
             auto test = [&](const float4& a0, const float4& a1, const float4& a2, const float4& a3) {
-                if (ou >= a3.x && ou <= a3.y && ow >= a3.z && ow <= a3.w) {
-                    float t;
+                float t;
+                if (grid_rec_hit<AX>(o, a0, a1, a2, a3, thr, t)) {
                     const uint32_t orig = (uint32_t)__float_as_int(a0.w);
-                    if (tri_hit_axis<AX>(o, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                                         make_float3(a2.x, a2.y, a2.z), t) &&
-                        t >= thr && t < tmax) {
-                        ++total;
-                        if (STATS) ctr.hits++;
-                        if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
-                            kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
-                    }
+                    ++total;
+                    if (STATS) ctr.hits++;
+                    if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
+                        kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
                 }
             };
             // records are ordered by box top: from the first that ends below the origin on,
@@ -214,6 +223,151 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
     }
 }
 
+// Item-parallel form of the walk (used by the launches that give it the LDS below): the
+// queued points' lists are cut into record pairs ("items", entry-major); each lane takes an
+// item, tests its two records against its point (every record of the list: those past the
+// walk's exit test fail the same box-top test) and appends the hits to an LDS list in item
+// order, so each entry's hits are consecutive. One lane per entry then runs the hit-buffer
+// chain on its hits, which is the sequential walk's result whenever they all fit the buffer
+// (no restart): entries with more hits than KBUF_CHAIN, or whose hits did not all fit the
+// LDS list, take the sequential walk (points_grid_run) afterwards. A wave's items are
+// independent loads, so no lane idles behind another's long list.
+constexpr uint32_t AG_IQCAP = 256;  // queue entries of one session
+constexpr uint32_t AG_HCAP = 128;   // hits of one session
+constexpr int KBUF_CHAIN = 8;
+
+struct AgItemLds {
+    uint32_t pi[AG_IQCAP];    // point
+    uint2 rg[AG_IQCAP];       // record range (even first, even length)
+    uint32_t pre[AG_IQCAP];   // items of the entries before
+    uint32_t hcnt[AG_IQCAP];  // hits found
+    float ht[AG_HCAP];
+    uint32_t ho[AG_HCAP];     // original triangle
+    int32_t hb[AG_HCAP];      // object
+};
+
+__device__ __forceinline__ uint32_t ag_wave_incl(uint32_t v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+template <int AX>
+__device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* __restrict__ pts, uint32_t nq,
+                                                  AgItemLds& L, int out_kind, void* __restrict__ out,
+                                                  GridFilter* gf, TraceCounters& ctr) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const float eps = 1e-6f;
+    uint32_t items = 0;
+    for (uint32_t b = 0; b < nq; b += 64) {
+        const uint32_t q = b + lane;
+        const uint32_t n = q < nq ? (L.rg[q].y - L.rg[q].x) >> 1 : 0u;
+        const uint32_t inc = ag_wave_incl(n, lane);
+        if (q < nq) {
+            L.pre[q] = items + inc - n;
+            L.hcnt[q] = 0u;
+        }
+        items += __shfl(inc, 63, 64);
+    }
+    __syncthreads();
+    uint32_t hpos = 0;
+    for (uint32_t b = 0; b < items; b += 64) {
+        const uint32_t i = b + lane;
+        bool h0 = false, h1 = false;
+        float t0 = 0.0f, t1 = 0.0f;
+        uint32_t q = 0;
+        float4 e0, e1, f0, f1;
+        if (i < items) {
+            // the last entry whose first item is at or before i
+            uint32_t lo = 0, hi = nq;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (L.pre[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            q = lo;
+            const uint64_t j = (uint64_t)L.rg[q].x + 2 * (uint64_t)(i - L.pre[q]);
+            const float3 o = ld3f(pts, L.pi[q]);
+            e0 = G.rec[4 * j];
+            e1 = G.rec[4 * j + 1];
+            const float4 e2 = G.rec[4 * j + 2], e3 = G.rec[4 * j + 3];
+            f0 = G.rec[4 * j + 4];
+            f1 = G.rec[4 * j + 5];
+            const float4 f2 = G.rec[4 * j + 6], f3 = G.rec[4 * j + 7];
+            h0 = grid_rec_hit<AX>(o, e0, e1, e2, e3, eps, t0);
+            h1 = grid_rec_hit<AX>(o, f0, f1, f2, f3, eps, t1);
+        }
+        const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+        uint32_t pos = hpos + (uint32_t)__popcll(m0 & lt_mask) + (uint32_t)__popcll(m1 & lt_mask);
+        if (h0) {
+            if (pos < AG_HCAP) {
+                L.ht[pos] = t0;
+                L.ho[pos] = (uint32_t)__float_as_int(e0.w);
+                L.hb[pos] = __float_as_int(e1.w);
+            }
+            ++pos;
+        }
+        if (h1 && pos < AG_HCAP) {
+            L.ht[pos] = t1;
+            L.ho[pos] = (uint32_t)__float_as_int(f0.w);
+            L.hb[pos] = __float_as_int(f1.w);
+        }
+        if (h0 || h1) atomicAdd(&L.hcnt[q], (h0 ? 1u : 0u) + (h1 ? 1u : 0u));
+        hpos += (uint32_t)(__popcll(m0) + __popcll(m1));
+    }
+    __syncthreads();
+    // the chain per entry; entries it cannot finish move to the front of pi / rg
+    uint32_t hs = 0, nfb = 0;
+    for (uint32_t b = 0; b < nq; b += 64) {
+        const uint32_t q = b + lane;
+        const bool valid = q < nq;
+        const uint32_t c = valid ? L.hcnt[q] : 0u;
+        const uint32_t pid = valid ? L.pi[q] : 0u;
+        const uint2 rg = valid ? L.rg[q] : make_uint2(0u, 0u);
+        const uint32_t inc = ag_wave_incl(c, lane);
+        const uint32_t s0 = hs + inc - c;
+        hs += __shfl(inc, 63, 64);
+        const bool fb = valid && (c > (uint32_t)KBUF_CHAIN || s0 + c > AG_HCAP);
+        if (valid && !fb) {
+            float bt[KBUF_CHAIN];
+            uint32_t bi[KBUF_CHAIN];
+            int bo[KBUF_CHAIN];
+            kbuf_clear(bt, bi, bo);
+            for (uint32_t k = 0; k < c; ++k) kbuf_insert(bt, bi, bo, L.ht[s0 + k], L.ho[s0 + k], L.hb[s0 + k]);
+            float thr = eps;
+            int count = 0, first_obj = -1;
+#pragma unroll
+            for (int k = 0; k < KBUF_CHAIN; ++k) {
+                if ((uint32_t)k < c && bt[k] >= thr) {
+                    if (count == 0) first_obj = bo[k];
+                    ++count;
+                    float nthr = bt[k] + eps;
+                    if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                    thr = nthr;
+                }
+            }
+            const int res = (count & 1) ? first_obj : -1;
+            grid_write_result(out_kind, out, pid, res);
+            if (gf && res >= 0) gf->inside++;
+        }
+        const uint64_t mf = __ballot(fb);
+        if (fb) {
+            const uint32_t slot = nfb + (uint32_t)__popcll(mf & lt_mask);
+            L.pi[slot] = pid;
+            L.rg[slot] = rg;
+        }
+        nfb += (uint32_t)__popcll(mf);
+    }
+    if (nfb) {
+        __syncthreads();
+        points_grid_run<AX, false, true>(G, pts, L.pi, 0, nfb, out_kind, out, ctr, gf, L.rg);
+    }
+}
+
 // Queue capacity of the two-phase grid form (points; 6 KB of LDS with their ranges).
 constexpr uint32_t AG_QCAP = 512;
 
@@ -226,19 +380,24 @@ constexpr uint32_t AG_QCAP = 512;
 // Phase 2 walks the queue with lane refill (points_grid_run, PRE) whenever phase 1 could
 // not take another 256 points, and at the end. cand counts filter passes (FILT), gf.inside
 // the points found inside. The whole wave calls it with the same lo, hi.
-template <int AX, bool FILT>
+template <int AX, bool FILT, bool ITEMS = false>
 __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const float* __restrict__ pts,
                                                       const uint32_t* __restrict__ imap, uint64_t lo, uint64_t hi,
                                                       int out_kind, void* __restrict__ out, uint32_t* qpi,
                                                       uint2* qrg, uint32_t& cand, GridFilter& gf,
-                                                      TraceCounters& ctr) {
+                                                      TraceCounters& ctr, AgItemLds* L = nullptr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    constexpr uint32_t QCAP = ITEMS ? AG_IQCAP : AG_QCAP;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (ITEMS) {
+        qpi = L->pi;
+        qrg = L->rg;
+    }
     uint64_t base = lo;
     while (true) {
         uint32_t nq = 0;
-        for (; base < hi && nq + 4 * 64 <= AG_QCAP; base += 4 * 64) {
+        for (; base < hi && nq + 4 * 64 <= QCAP; base += 4 * 64) {
             float3 o[4];
             bool ok[4];
             uint32_t cell[4], word[4], pid[4];
@@ -294,7 +453,8 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             }
         }
         __syncthreads();
-        points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
+        if (ITEMS) points_grid_items<AX>(G, pts, nq, *L, out_kind, out, &gf, ctr);
+        else points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
         __syncthreads();
         if (base >= hi) break;
     }

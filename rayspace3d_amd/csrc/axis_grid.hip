@@ -245,11 +245,11 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
         points_grid_run<AX, true>(G, pts, imap, lo, hi, out_kind, out, ctr);
         flush_counters(ctr, stats);
     } else {
-        __shared__ uint32_t qpi[AG_QCAP];
-        __shared__ uint2 qrg[AG_QCAP];
+        __shared__ AgItemLds lds;
         uint32_t cand = 0;
         GridFilter gf{};
-        points_grid_two_phase<AX, false>(G, pts, imap, lo, hi, out_kind, out, qpi, qrg, cand, gf, ctr);
+        points_grid_two_phase<AX, false, true>(G, pts, imap, lo, hi, out_kind, out, nullptr, nullptr, cand, gf, ctr,
+                                               &lds);
     }
 }
 
@@ -261,15 +261,15 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
                                                                    uint64_t n, int32_t* __restrict__ out,
                                                                    uint32_t per_wave, FBox box,
                                                                    unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t qpi[AG_QCAP];
-    __shared__ uint2 qrg[AG_QCAP];
+    __shared__ AgItemLds lds;
     const uint64_t lo = (uint64_t)blockIdx.x * per_wave;
     const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
     const int lane = threadIdx.x & 63;
     uint32_t cand = 0;
     TraceCounters ctr;
     GridFilter gf{box, 0u};
-    points_grid_two_phase<AX, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
+    points_grid_two_phase<AX, true, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf,
+                                          ctr, &lds);
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(acc_stripe(counts), v);
