@@ -223,27 +223,75 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
     }
 }
 
+// One point's walk, one record per step, with the hit buffer and its restarts: the
+// sequential walk's result for list [js, je) (used for the few entries the item-parallel
+// form leaves).
+template <int AX>
+__device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __restrict__ pts, uint32_t pid,
+                                              uint32_t js, uint32_t je, int out_kind, void* __restrict__ out,
+                                              GridFilter* gf) {
+    const float eps = 1e-6f;
+    const float3 o = ld3f(pts, pid);
+    const float oa = get3(o, AX);
+    float thr = eps;
+    int count = 0, first_obj = -1;
+    float bt[KBUF_GRID];
+    uint32_t bi[KBUF_GRID];
+    int bo[KBUF_GRID];
+    while (true) {
+        kbuf_clear(bt, bi, bo);
+        uint32_t total = 0;
+        for (uint32_t j = js; j < je; ++j) {
+            const float4 a0 = G.rec[4 * (uint64_t)j], a1 = G.rec[4 * (uint64_t)j + 1];
+            const float4 a2 = G.rec[4 * (uint64_t)j + 2], a3 = G.rec[4 * (uint64_t)j + 3];
+            if (a2.w < oa) break;
+            float t;
+            if (grid_rec_hit<AX>(o, a0, a1, a2, a3, thr, t)) {
+                const uint32_t orig = (uint32_t)__float_as_int(a0.w);
+                ++total;
+                if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
+                    kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KBUF_GRID; ++k) {
+            if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
+                if (count == 0) first_obj = bo[k];
+                ++count;
+                float nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
+                thr = nthr;
+            }
+        }
+        if (!(total > (uint32_t)KBUF_GRID && count < 10000)) break;
+    }
+    const int res = (count & 1) ? first_obj : -1;
+    grid_write_result(out_kind, out, pid, res);
+    if (gf && res >= 0) gf->inside++;
+}
+
 // Item-parallel form of the walk (used by the launches that give it the LDS below): the
 // queued points' lists are cut into record pairs ("items", entry-major); each lane takes an
 // item, tests its two records against its point (every record of the list: those past the
 // walk's exit test fail the same box-top test) and appends the hits to an LDS list in item
-// order, so each entry's hits are consecutive. One lane per entry then runs the hit-buffer
-// chain on its hits, which is the sequential walk's result whenever they all fit the buffer
-// (no restart): entries with more hits than KBUF_CHAIN, or whose hits did not all fit the
-// LDS list, take the sequential walk (points_grid_run) afterwards. A wave's items are
+// order, so each entry's hits are consecutive and the list is ordered by entry. One lane per
+// entry then runs the chain on its hits, taking them in (t, triangle) order by selection:
+// the hits the sequential walk's buffer would take in turn, so the result is the same.
+// Entries with more than AG_CHAIN_MAX hits, and those from the first whose hits did not
+// all fit the list on, are walked by grid_walk_one afterwards. A wave's items are
 // independent loads, so no lane idles behind another's long list.
-constexpr uint32_t AG_IQCAP = 256;  // queue entries of one session
-constexpr uint32_t AG_HCAP = 128;   // hits of one session
-constexpr int KBUF_CHAIN = 8;
+constexpr uint32_t AG_IQCAP = 256;      // queue entries of one session
+constexpr uint32_t AG_HCAP = 128;       // hits of one session
+constexpr uint32_t AG_CHAIN_MAX = 16;   // hits one entry's chain takes from the list
 
 struct AgItemLds {
-    uint32_t pi[AG_IQCAP];    // point
-    uint2 rg[AG_IQCAP];       // record range (even first, even length)
-    uint32_t pre[AG_IQCAP];   // items of the entries before
-    uint32_t hcnt[AG_IQCAP];  // hits found
+    uint32_t pi[AG_IQCAP];     // point
+    uint32_t first[AG_IQCAP];  // first record of its list (even)
+    uint32_t pre[AG_IQCAP];    // record pairs of the list, then the pairs before it
     float ht[AG_HCAP];
-    uint32_t ho[AG_HCAP];     // original triangle
-    int32_t hb[AG_HCAP];      // object
+    uint32_t ho[AG_HCAP];      // original triangle
+    int32_t hb[AG_HCAP];       // object
+    uint16_t hq[AG_HCAP];      // entry
 };
 
 __device__ __forceinline__ uint32_t ag_wave_incl(uint32_t v, int lane) {
@@ -255,26 +303,34 @@ __device__ __forceinline__ uint32_t ag_wave_incl(uint32_t v, int lane) {
     return v;
 }
 
+// First index in hq[0, n) whose entry is at least q (hq is non-decreasing).
+__device__ __forceinline__ uint32_t ag_hit_lb(const uint16_t* hq, uint32_t n, uint32_t q) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hq[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <int AX>
 __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* __restrict__ pts, uint32_t nq,
                                                   AgItemLds& L, int out_kind, void* __restrict__ out,
-                                                  GridFilter* gf, TraceCounters& ctr) {
+                                                  GridFilter* gf) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const float eps = 1e-6f;
     uint32_t items = 0;
     for (uint32_t b = 0; b < nq; b += 64) {
         const uint32_t q = b + lane;
-        const uint32_t n = q < nq ? (L.rg[q].y - L.rg[q].x) >> 1 : 0u;
+        const uint32_t n = q < nq ? L.pre[q] : 0u;
         const uint32_t inc = ag_wave_incl(n, lane);
-        if (q < nq) {
-            L.pre[q] = items + inc - n;
-            L.hcnt[q] = 0u;
-        }
+        if (q < nq) L.pre[q] = items + inc - n;
         items += __shfl(inc, 63, 64);
     }
     __syncthreads();
-    uint32_t hpos = 0;
+    uint32_t hpos = 0, qdrop = nq;
     for (uint32_t b = 0; b < items; b += 64) {
         const uint32_t i = b + lane;
         bool h0 = false, h1 = false;
@@ -290,7 +346,7 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
                 else hi = mid;
             }
             q = lo;
-            const uint64_t j = (uint64_t)L.rg[q].x + 2 * (uint64_t)(i - L.pre[q]);
+            const uint64_t j = (uint64_t)L.first[q] + 2 * (uint64_t)(i - L.pre[q]);
             const float3 o = ld3f(pts, L.pi[q]);
             e0 = G.rec[4 * j];
             e1 = G.rec[4 * j + 1];
@@ -302,53 +358,69 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
             h1 = grid_rec_hit<AX>(o, f0, f1, f2, f3, eps, t1);
         }
         const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
-        uint32_t pos = hpos + (uint32_t)__popcll(m0 & lt_mask) + (uint32_t)__popcll(m1 & lt_mask);
-        if (h0) {
-            if (pos < AG_HCAP) {
-                L.ht[pos] = t0;
-                L.ho[pos] = (uint32_t)__float_as_int(e0.w);
-                L.hb[pos] = __float_as_int(e1.w);
-            }
-            ++pos;
+        const uint32_t p0 = hpos + (uint32_t)__popcll(m0 & lt_mask) + (uint32_t)__popcll(m1 & lt_mask);
+        const uint32_t p1 = p0 + (h0 ? 1u : 0u);
+        if (h0 && p0 < AG_HCAP) {
+            L.ht[p0] = t0;
+            L.ho[p0] = (uint32_t)__float_as_int(e0.w);
+            L.hb[p0] = __float_as_int(e1.w);
+            L.hq[p0] = (uint16_t)q;
         }
-        if (h1 && pos < AG_HCAP) {
-            L.ht[pos] = t1;
-            L.ho[pos] = (uint32_t)__float_as_int(f0.w);
-            L.hb[pos] = __float_as_int(f1.w);
+        if (h1 && p1 < AG_HCAP) {
+            L.ht[p1] = t1;
+            L.ho[p1] = (uint32_t)__float_as_int(f0.w);
+            L.hb[p1] = __float_as_int(f1.w);
+            L.hq[p1] = (uint16_t)q;
         }
-        if (h0 || h1) atomicAdd(&L.hcnt[q], (h0 ? 1u : 0u) + (h1 ? 1u : 0u));
+        // the entry of the first hit that did not fit
+        const uint64_t md = __ballot((h0 && p0 == AG_HCAP) || (h1 && p1 == AG_HCAP));
+        if (md) qdrop = __shfl(q, __ffsll((unsigned long long)md) - 1, 64);
         hpos += (uint32_t)(__popcll(m0) + __popcll(m1));
     }
+    const uint32_t nh = hpos < AG_HCAP ? hpos : AG_HCAP;
     __syncthreads();
-    // the chain per entry; entries it cannot finish move to the front of pi / rg
-    uint32_t hs = 0, nfb = 0;
+    // the chain per entry; entries it cannot finish move to the front of pi / first / pre
+    // (pre then holds the end of the list)
+    uint32_t nfb = 0;
     for (uint32_t b = 0; b < nq; b += 64) {
         const uint32_t q = b + lane;
         const bool valid = q < nq;
-        const uint32_t c = valid ? L.hcnt[q] : 0u;
-        const uint32_t pid = valid ? L.pi[q] : 0u;
-        const uint2 rg = valid ? L.rg[q] : make_uint2(0u, 0u);
-        const uint32_t inc = ag_wave_incl(c, lane);
-        const uint32_t s0 = hs + inc - c;
-        hs += __shfl(inc, 63, 64);
-        const bool fb = valid && (c > (uint32_t)KBUF_CHAIN || s0 + c > AG_HCAP);
+        uint32_t pid = 0, js = 0, je = 0, s0 = 0, s1 = 0;
+        if (valid) {
+            pid = L.pi[q];
+            js = L.first[q];
+            je = js + 2 * ((q + 1 < nq ? L.pre[q + 1] : items) - L.pre[q]);
+            s0 = ag_hit_lb(L.hq, nh, q);
+            s1 = ag_hit_lb(L.hq, nh, q + 1);
+        }
+        const bool fb = valid && (q >= qdrop || s1 - s0 > AG_CHAIN_MAX);
         if (valid && !fb) {
-            float bt[KBUF_CHAIN];
-            uint32_t bi[KBUF_CHAIN];
-            int bo[KBUF_CHAIN];
-            kbuf_clear(bt, bi, bo);
-            for (uint32_t k = 0; k < c; ++k) kbuf_insert(bt, bi, bo, L.ht[s0 + k], L.ho[s0 + k], L.hb[s0 + k]);
             float thr = eps;
             int count = 0, first_obj = -1;
-#pragma unroll
-            for (int k = 0; k < KBUF_CHAIN; ++k) {
-                if ((uint32_t)k < c && bt[k] >= thr) {
-                    if (count == 0) first_obj = bo[k];
-                    ++count;
-                    float nthr = bt[k] + eps;
-                    if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
-                    thr = nthr;
+            while (true) {
+                // the smallest (t, triangle) at or above thr
+                float bt = INFINITY;
+                uint32_t bi = 0xffffffffu;
+                int bo = -1;
+                bool found = false;
+                for (uint32_t k = s0; k < s1; ++k) {
+                    const float t = L.ht[k];
+                    if (t >= thr) {
+                        const uint32_t tri = L.ho[k];
+                        if (!found || t < bt || (t == bt && tri < bi)) {
+                            bt = t;
+                            bi = tri;
+                            bo = L.hb[k];
+                            found = true;
+                        }
+                    }
                 }
+                if (!found) break;
+                if (count == 0) first_obj = bo;
+                ++count;
+                float nthr = bt + eps;
+                if (nthr <= bt) nthr = nextafterf(bt, INFINITY);
+                thr = nthr;
             }
             const int res = (count & 1) ? first_obj : -1;
             grid_write_result(out_kind, out, pid, res);
@@ -358,13 +430,15 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
         if (fb) {
             const uint32_t slot = nfb + (uint32_t)__popcll(mf & lt_mask);
             L.pi[slot] = pid;
-            L.rg[slot] = rg;
+            L.first[slot] = js;
+            L.pre[slot] = je;
         }
         nfb += (uint32_t)__popcll(mf);
     }
     if (nfb) {
         __syncthreads();
-        points_grid_run<AX, false, true>(G, pts, L.pi, 0, nfb, out_kind, out, ctr, gf, L.rg);
+        for (uint32_t k = lane; k < nfb; k += 64)
+            grid_walk_one<AX>(G, pts, L.pi[k], L.first[k], L.pre[k], out_kind, out, gf);
     }
 }
 
@@ -375,10 +449,11 @@ constexpr uint32_t AG_QCAP = 512;
 // at a time (four independent loads per lane in flight): origin, with FILT the dataset-box
 // filter of gf.box, cell, occupancy bit and the cell-table entries; points that fail the
 // filter, fall in an empty cell or lie above the cell's highest box top are answered -1 at
-// once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS).
+// once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS; with
+// ITEMS the queue is L's, AG_IQCAP entries).
 // Ray r of the list is point imap[r] (r itself without imap); answers go out as out_kind.
-// Phase 2 walks the queue with lane refill (points_grid_run, PRE) whenever phase 1 could
-// not take another 256 points, and at the end. cand counts filter passes (FILT), gf.inside
+// Phase 2 walks the queue (ITEMS: points_grid_items; else lane refill, points_grid_run with
+// PRE) whenever phase 1 could not take another 256 points, and at the end. cand counts filter passes (FILT), gf.inside
 // the points found inside. The whole wave calls it with the same lo, hi.
 template <int AX, bool FILT, bool ITEMS = false>
 __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const float* __restrict__ pts,
@@ -390,10 +465,6 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
     constexpr uint32_t QCAP = ITEMS ? AG_IQCAP : AG_QCAP;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    if (ITEMS) {
-        qpi = L->pi;
-        qrg = L->rg;
-    }
     uint64_t base = lo;
     while (true) {
         uint32_t nq = 0;
@@ -446,14 +517,20 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
                 const uint64_t m = __ballot(ok[k]);
                 if (ok[k]) {
                     const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
-                    qpi[slot] = pid[k];
-                    qrg[slot] = rg[k];
+                    if (ITEMS) {
+                        L->pi[slot] = pid[k];
+                        L->first[slot] = rg[k].x;
+                        L->pre[slot] = (rg[k].y - rg[k].x) >> 1;
+                    } else {
+                        qpi[slot] = pid[k];
+                        qrg[slot] = rg[k];
+                    }
                 }
                 nq += (uint32_t)__popcll(m);
             }
         }
         __syncthreads();
-        if (ITEMS) points_grid_items<AX>(G, pts, nq, *L, out_kind, out, &gf, ctr);
+        if (ITEMS) points_grid_items<AX>(G, pts, nq, *L, out_kind, out, &gf);
         else points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
         __syncthreads();
         if (base >= hi) break;
