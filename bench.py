@@ -200,10 +200,10 @@ def main():
     ncand = int(res.candidate_count)
     if use_grid:
         # the one-pass kernel's bytes: 12 B origin + 4 B answer per point, then per filter
-        # candidate 4 B occupancy word, 16 B per cell-table read (two 8 B entries, occupied
-        # cells) and 64 B per record read (counted on the candidates)
+        # candidate 8 B tile summary, 16 B per cell-table read (two 8 B entries, occupied
+        # cells) and 48 B per record read (counted on the candidates)
         gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
-        alg_bytes = 16 * len(pts) + 4 * gst["rays"] + 16 * gst["nodes_visited"] + 64 * gst["tris_tested"]
+        alg_bytes = 16 * len(pts) + 8 * gst["rays"] + 16 * gst["nodes_visited"] + 48 * gst["tris_tested"]
         bytes_per_ray = alg_bytes / len(pts)
         kernel_name = "k_points_grid_filtered (filter + refine, axis grid)"
     else:
@@ -371,10 +371,10 @@ def leg_single_sphere(args):
     flags = to_host(out).astype(bool)
     use_grid = grid["state"] == 1
     if use_grid:
-        # 12 B origin + 1 B flag + 4 B occupancy word per ray, 16 B per cell-table read,
-        # 64 B per record read
+        # 12 B origin + 1 B flag + 8 B tile summary per ray, 16 B per cell-table read,
+        # 48 B per record read
         g = axis_grid_stats(ctx, bvh, dP, axis=0)
-        alg = 17 * len(pts) + 16 * g["nodes_visited"] + 64 * g["tris_tested"]
+        alg = 21 * len(pts) + 16 * g["nodes_visited"] + 48 * g["tris_tested"]
         kernel = "k_points_grid (axis grid, +x)"
     else:
         g = None

@@ -23,11 +23,15 @@ struct AxisGrid {
     uint2* cells = nullptr;     // tiled cells + 1 entries: first record, box top of that record
     uint2* tsum = nullptr;      // per 32 tiles: occupied-tile bits, occupied tiles before
     uint32_t* occ = nullptr;    // per occupied tile: its 32 cell bits
-    float4* rec = nullptr;      // 4 float4 per record
+    float4* rec = nullptr;      // AG_REC float4 per record
     size_t bytes = 0;
     double expected = 0.0;
     double build_ms = 0.0;
 };
+
+// A record is three float4: (v0, original triangle index), (v1, object id), (v2, upper end
+// of the triangle's padded box along the axis) -- 48 bytes.
+constexpr int AG_REC = 3;
 
 struct AGArgs {
     const uint2* cells;
@@ -80,14 +84,13 @@ __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict
     }
 }
 
-// One grid record against a point ray: the record's box ends at or above the origin and
-// contains the ray's line in the plane, and the triangle is hit at t in [thr, 1e16).
+// One grid record against a point ray: the triangle's box ends at or above the origin and
+// the triangle is hit at t in [thr, 1e16). (The box's extent in the plane is not kept: the
+// padded box contains every hit, so testing it first could only skip triangle tests.)
 template <int AX>
 __device__ __forceinline__ bool grid_rec_hit(const float3 o, const float4& a0, const float4& a1, const float4& a2,
-                                             const float4& a3, float thr, float& t) {
-    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    const float ou = get3(o, U), ow = get3(o, W);
-    if (a2.w < get3(o, AX) || !(ou >= a3.x && ou <= a3.y && ow >= a3.z && ow <= a3.w)) return false;
+                                             float thr, float& t) {
+    if (a2.w < get3(o, AX)) return false;
     return tri_hit_axis<AX>(o, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
                             make_float3(a2.x, a2.y, a2.z), t) &&
            t >= thr && t < 1e16f;
@@ -163,13 +166,13 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         const float oa = get3(o, AX);
         if (j < je) {
             const bool two = j + 1 < je;
-            const uint64_t b0 = 4 * (uint64_t)j, b1 = 4 * (uint64_t)(two ? j + 1 : j);
-            const float4 e0 = G.rec[b0], e1 = G.rec[b0 + 1], e2 = G.rec[b0 + 2], e3 = G.rec[b0 + 3];
-            const float4 f0 = G.rec[b1], f1 = G.rec[b1 + 1], f2 = G.rec[b1 + 2], f3 = G.rec[b1 + 3];
-            // one record: box test in the plane, then the triangle
-            auto test = [&](const float4& a0, const float4& a1, const float4& a2, const float4& a3) {
+            const uint64_t b0 = AG_REC * (uint64_t)j, b1 = AG_REC * (uint64_t)(two ? j + 1 : j);
+            const float4 e0 = G.rec[b0], e1 = G.rec[b0 + 1], e2 = G.rec[b0 + 2];
+            const float4 f0 = G.rec[b1], f1 = G.rec[b1 + 1], f2 = G.rec[b1 + 2];
+            // one record
+            auto test = [&](const float4& a0, const float4& a1, const float4& a2) {
                 float t;
-                if (grid_rec_hit<AX>(o, a0, a1, a2, a3, thr, t)) {
+                if (grid_rec_hit<AX>(o, a0, a1, a2, thr, t)) {
                     const uint32_t orig = (uint32_t)__float_as_int(a0.w);
                     ++total;
                     if (STATS) ctr.hits++;
@@ -184,7 +187,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 j = je;
                 continue;
             }
-            test(e0, e1, e2, e3);
+            test(e0, e1, e2);
             if (!two) {
                 ++j;
                 continue;
@@ -194,7 +197,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 j = je;
                 continue;
             }
-            test(f0, f1, f2, f3);
+            test(f0, f1, f2);
             j += 2;
             continue;
         }
@@ -242,11 +245,11 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __re
         kbuf_clear(bt, bi, bo);
         uint32_t total = 0;
         for (uint32_t j = js; j < je; ++j) {
-            const float4 a0 = G.rec[4 * (uint64_t)j], a1 = G.rec[4 * (uint64_t)j + 1];
-            const float4 a2 = G.rec[4 * (uint64_t)j + 2], a3 = G.rec[4 * (uint64_t)j + 3];
+            const uint64_t b = AG_REC * (uint64_t)j;
+            const float4 a0 = G.rec[b], a1 = G.rec[b + 1], a2 = G.rec[b + 2];
             if (a2.w < oa) break;
             float t;
-            if (grid_rec_hit<AX>(o, a0, a1, a2, a3, thr, t)) {
+            if (grid_rec_hit<AX>(o, a0, a1, a2, thr, t)) {
                 const uint32_t orig = (uint32_t)__float_as_int(a0.w);
                 ++total;
                 if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
@@ -348,14 +351,15 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
             q = lo;
             const uint64_t j = (uint64_t)L.first[q] + 2 * (uint64_t)(i - L.pre[q]);
             const float3 o = ld3f(pts, L.pi[q]);
-            e0 = G.rec[4 * j];
-            e1 = G.rec[4 * j + 1];
-            const float4 e2 = G.rec[4 * j + 2], e3 = G.rec[4 * j + 3];
-            f0 = G.rec[4 * j + 4];
-            f1 = G.rec[4 * j + 5];
-            const float4 f2 = G.rec[4 * j + 6], f3 = G.rec[4 * j + 7];
-            h0 = grid_rec_hit<AX>(o, e0, e1, e2, e3, eps, t0);
-            h1 = grid_rec_hit<AX>(o, f0, f1, f2, f3, eps, t1);
+            const float4* r = G.rec + AG_REC * j;
+            e0 = r[0];
+            e1 = r[1];
+            const float4 e2 = r[2];
+            f0 = r[3];
+            f1 = r[4];
+            const float4 f2 = r[5];
+            h0 = grid_rec_hit<AX>(o, e0, e1, e2, eps, t0);
+            h1 = grid_rec_hit<AX>(o, f0, f1, f2, eps, t1);
         }
         const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
         const uint32_t p0 = hpos + (uint32_t)__popcll(m0 & lt_mask) + (uint32_t)__popcll(m1 & lt_mask);

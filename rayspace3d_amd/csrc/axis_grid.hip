@@ -4,14 +4,14 @@
 // point of the perpendicular plane. The grid covers that plane (the BVH's padded bounds)
 // with square cells; cell c lists every triangle whose padded box -- the BVH's leaf box,
 // computed by the same prim_padded_box -- overlaps c in the plane, as a self-contained
-// 64-byte record (vertices, original index, object, the box's upper end along the axis and
-// its extent in the plane), ordered by that upper end, largest first. A ray reads its
-// cell's entry in the cell table (first record, and that record's box top, so that a ray
-// above it reads no record at all), stops at the first record whose box ends below its
-// origin, and tests the records whose box contains its line. Those are exactly the triangles whose leaf box the
-// BVH walk would accept (ancestor boxes contain the leaf boxes), the padding guarantees
-// that no other triangle can be hit, and the hit buffer, restarts and chain are the BVH
-// kernel's: the results are the same. The gain is the walk: one range lookup and a few
+// 48-byte record (vertices, original index, object and the box's upper end along the axis),
+// ordered by that upper end, largest first. A ray reads its cell's entry in the cell table
+// (first record, and that record's box top, so that a ray above it reads no record at all)
+// and tests the triangles of the records whose box does not end below its origin. They
+// include every triangle whose leaf box the BVH walk would accept (ancestor boxes contain
+// the leaf boxes); the padding guarantees that no triangle outside its box can be hit, so
+// the hits are the BVH walk's, and the hit buffer, restarts and chain are the BVH kernel's:
+// the results are the same. The gain is the walk: one range lookup and a few
 // records instead of ~30 dependent node fetches.
 //
 // Build (per BVH and axis, once): the cell edge is a fraction of the triangles' mean padded
@@ -139,7 +139,6 @@ __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tr
                                                    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ start,
                                                    const uint32_t* __restrict__ pstart, uint64_t m, AGArgs G,
                                                    float4* __restrict__ rec) {
-    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint32_t cell = (uint32_t)(keys[i] >> 32);
@@ -148,16 +147,15 @@ __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tr
     float plo[3], phi[3];
     prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
                     make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
-    rec[4 * d] = r.v0;      // .w: original triangle index
-    rec[4 * d + 1] = r.v1;  // .w: object id
-    rec[4 * d + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
-    rec[4 * d + 3] = make_float4(plo[U], phi[U], plo[W], phi[W]);
+    rec[AG_REC * d] = r.v0;      // .w: original triangle index
+    rec[AG_REC * d + 1] = r.v1;  // .w: object id
+    rec[AG_REC * d + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
 }
 
-// Lists padded to an even number of records, so a two-record step reads one 128-byte line:
-// pcounts = counts rounded up to even; the pad is a sentinel record (NaN vertices and box,
-// box top -inf) after a cell's last record, where the walk's exit test stops every ray
-// (top -inf < origin) and a NaN origin's box test fails.
+// Lists padded to an even number of records (a walk step, or an item, is a record pair):
+// pcounts = counts rounded up to even; the pad is a sentinel record (NaN vertices, box top
+// -inf) after a cell's last record, where the walk's exit test stops every ray (top -inf <
+// origin) and a NaN origin's triangle test fails (t is NaN).
 __global__ __launch_bounds__(256) void k_ag_pcounts(const uint32_t* __restrict__ counts, uint64_t nc,
                                                     uint32_t* __restrict__ pcounts,
                                                     unsigned long long* __restrict__ odd) {
@@ -175,10 +173,9 @@ __global__ __launch_bounds__(256) void k_ag_pad(const uint32_t* __restrict__ cou
     if (c >= nc || !(counts[c] & 1u)) return;
     const uint64_t d = (uint64_t)pstart[c] + counts[c];
     const float nan = __int_as_float(0x7fc00000);
-    rec[4 * d] = make_float4(nan, nan, nan, __int_as_float(-1));
-    rec[4 * d + 1] = make_float4(nan, nan, nan, __int_as_float(-1));
-    rec[4 * d + 2] = make_float4(nan, nan, nan, -INFINITY);
-    rec[4 * d + 3] = make_float4(nan, nan, nan, nan);
+    rec[AG_REC * d] = make_float4(nan, nan, nan, __int_as_float(-1));
+    rec[AG_REC * d + 1] = make_float4(nan, nan, nan, __int_as_float(-1));
+    rec[AG_REC * d + 2] = make_float4(nan, nan, nan, -INFINITY);
 }
 
 // Occupancy words from the counts: one cell per lane, a wave's ballot is the words of its
@@ -228,7 +225,7 @@ __global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ s
         return;
     }
     const uint32_t s = start[c];
-    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? rec[4 * (uint64_t)s + 2].w : -INFINITY));
+    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? rec[AG_REC * (uint64_t)s + 2].w : -INFINITY));
 }
 
 // Point rays on the grid: wave b owns rays [b * per_wave, ...), in the two-phase form; the
@@ -420,12 +417,12 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->occ, nt * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&g->tsum, ng * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * 4 * sizeof(float4))) != hipSuccess) {
+        (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * AG_REC * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint32_t) + ng * sizeof(uint2) + ptotal * 4 * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint32_t) + ng * sizeof(uint2) + ptotal * AG_REC * sizeof(float4);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
@@ -546,14 +543,14 @@ int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready, bool bu
 
 uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
     // Peak device bytes of a build: records are at most 16 per triangle + 2^20 (the budget
-    // the build widens its cells to meet), 64 B each plus up to 1/8 for the padding to even
+    // the build widens its cells to meet), 48 B each plus up to 1/8 for the padding to even
     // lists, and while the build sorts them 24 B more each (keys, values and the sort's
     // alternate buffers); cells are within 16 per triangle + 1024 and the padding of the
     // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table and 16 B of counts and
     // offsets during the build; 8 B per triangle of spans and offsets.
     const uint64_t n = bvh->num_tris;
     const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024 + (1ull << 19);
-    return recs * (72 + 24) + cells * 24 + n * 8 + (64ull << 20);
+    return recs * (54 + 24) + cells * 24 + n * 8 + (64ull << 20);
 }
 
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,

@@ -239,7 +239,7 @@ struct rs3d_ctx {
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
     int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
     int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid
-    int64_t opt_grid_rays_per_wave = 1024;  // run length of grid point launches
+    int64_t opt_grid_rays_per_wave = 256;   // run length of grid point launches
     int64_t opt_grid_cell_pct = 35;         // grid cell edge, % of the mean triangle extent
     int64_t opt_seg_occupancy = 1;          // edge rays test the target's occupancy bits first
     int64_t opt_seg_cell_pct = 200;         // occupancy cell edge, % of the mean triangle extent
