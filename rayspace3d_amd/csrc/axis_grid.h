@@ -96,6 +96,19 @@ __device__ __forceinline__ bool grid_rec_hit(const float3 o, const float4& a0, c
            t >= thr && t < 1e16f;
 }
 
+// Where the item-parallel form puts a point's answer: the session's LDS answers when it
+// buffers them (points sbase.. of a contiguous run), else the output.
+struct GridSink {
+    int out_kind;
+    void* out;
+    int32_t* res;
+    uint32_t sbase;
+    __device__ __forceinline__ void put(uint32_t pid, int r) const {
+        if (res) res[pid - sbase] = r;
+        else grid_write_result(out_kind, out, pid, r);
+    }
+};
+
 // Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
 // restarts, chain, output) is the BVH kernels'; only the candidate walk differs: a lane
@@ -231,8 +244,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
 // form leaves).
 template <int AX>
 __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __restrict__ pts, uint32_t pid,
-                                              uint32_t js, uint32_t je, int out_kind, void* __restrict__ out,
-                                              GridFilter* gf) {
+                                              uint32_t js, uint32_t je, const GridSink& sink, GridFilter* gf) {
     const float eps = 1e-6f;
     const float3 o = ld3f(pts, pid);
     const float oa = get3(o, AX);
@@ -269,7 +281,7 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __re
         if (!(total > (uint32_t)KBUF_GRID && count < 10000)) break;
     }
     const int res = (count & 1) ? first_obj : -1;
-    grid_write_result(out_kind, out, pid, res);
+    sink.put(pid, res);
     if (gf && res >= 0) gf->inside++;
 }
 
@@ -295,6 +307,7 @@ struct AgItemLds {
     uint32_t ho[AG_HCAP];      // original triangle
     int32_t hb[AG_HCAP];       // object
     uint16_t hq[AG_HCAP];      // entry
+    int32_t res[AG_IQCAP];     // answers of the session's points, when it buffers them
 };
 
 __device__ __forceinline__ uint32_t ag_wave_incl(uint32_t v, int lane) {
@@ -319,8 +332,7 @@ __device__ __forceinline__ uint32_t ag_hit_lb(const uint16_t* hq, uint32_t n, ui
 
 template <int AX>
 __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* __restrict__ pts, uint32_t nq,
-                                                  AgItemLds& L, int out_kind, void* __restrict__ out,
-                                                  GridFilter* gf) {
+                                                  AgItemLds& L, const GridSink& sink, GridFilter* gf) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const float eps = 1e-6f;
@@ -427,7 +439,7 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
                 thr = nthr;
             }
             const int res = (count & 1) ? first_obj : -1;
-            grid_write_result(out_kind, out, pid, res);
+            sink.put(pid, res);
             if (gf && res >= 0) gf->inside++;
         }
         const uint64_t mf = __ballot(fb);
@@ -442,7 +454,7 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
     if (nfb) {
         __syncthreads();
         for (uint32_t k = lane; k < nfb; k += 64)
-            grid_walk_one<AX>(G, pts, L.pi[k], L.first[k], L.pre[k], out_kind, out, gf);
+            grid_walk_one<AX>(G, pts, L.pi[k], L.first[k], L.pre[k], sink, gf);
     }
 }
 
@@ -469,10 +481,14 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
     constexpr uint32_t QCAP = ITEMS ? AG_IQCAP : AG_QCAP;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // ITEMS over a contiguous run (no imap): one 256-point round per session, answers
+    // gathered in LDS and written out as whole lines at the session's end
+    const bool buf = ITEMS && imap == nullptr;
     uint64_t base = lo;
     while (true) {
         uint32_t nq = 0;
-        for (; base < hi && nq + 4 * 64 <= QCAP; base += 4 * 64) {
+        const uint64_t sbase = base;
+        for (; base < hi && nq + 4 * 64 <= QCAP && !(buf && base != sbase); base += 4 * 64) {
             float3 o[4];
             bool ok[4];
             uint32_t cell[4], word[4], pid[4];
@@ -517,7 +533,10 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint64_t r = base + 64 * k + lane;
-                if (r < hi && !ok[k]) grid_write_result(out_kind, out, pid[k], -1);
+                if (r < hi && !ok[k]) {
+                    if (buf) L->res[r - sbase] = -1;
+                    else grid_write_result(out_kind, out, pid[k], -1);
+                }
                 const uint64_t m = __ballot(ok[k]);
                 if (ok[k]) {
                     const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
@@ -534,8 +553,17 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             }
         }
         __syncthreads();
-        if (ITEMS) points_grid_items<AX>(G, pts, nq, *L, out_kind, out, &gf);
-        else points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
+        if (ITEMS) {
+            const GridSink sink{out_kind, out, buf ? L->res : nullptr, (uint32_t)sbase};
+            points_grid_items<AX>(G, pts, nq, *L, sink, &gf);
+            if (buf) {
+                __syncthreads();
+                const uint32_t cnt = (uint32_t)((base < hi ? base : hi) - sbase);
+                for (uint32_t k = lane; k < cnt; k += 64) grid_write_result(out_kind, out, (uint32_t)sbase + k, L->res[k]);
+            }
+        } else {
+            points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
+        }
         __syncthreads();
         if (base >= hi) break;
     }
