@@ -125,7 +125,7 @@ __device__ __forceinline__ bool tri_hit(const float3 o, const RayPre& r, const f
 // and returns t = NaN or a miss, and this form returns t = +-inf, NaN or a miss: no window
 // [thr, tmax) with tmax <= 1e16 admits any of them. The hits that count are identical.
 template <int AX>
-__device__ __forceinline__ bool tri_hit_axis(const float3 o, const float3 v0, const float3 v1, const float3 v2,
+__host__ __device__ __forceinline__ bool tri_hit_axis(const float3 o, const float3 v0, const float3 v1, const float3 v2,
                                              float& tout) {
     constexpr int KX = (AX + 1) % 3, KY = (AX + 2) % 3;
     const float Ax = get3(v0, KX) - get3(o, KX), Ay = get3(v0, KY) - get3(o, KY);

@@ -69,7 +69,7 @@ struct FBox {
 };
 
 template <int AX>
-__device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
+__host__ __device__ __forceinline__ bool box_filter_full(const float3 o, const FBox& b) {
     const float3 bv[8] = {make_float3(b.x0, b.y0, b.z0), make_float3(b.x1, b.y0, b.z0),
                           make_float3(b.x1, b.y1, b.z0), make_float3(b.x0, b.y1, b.z0),
                           make_float3(b.x0, b.y0, b.z1), make_float3(b.x1, b.y0, b.z1),
@@ -118,6 +118,49 @@ __device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
         }
     }
     return (count & 1) != 0;
+}
+
+// The same answer without the triangle tests for the points that are clearly placed: with
+// m = 2^-16 of the largest magnitude among the box and the point, plus 1e-5, a point whose
+// in-plane coordinates lie at least m inside the face square, whose distance from the
+// faces' common diagonal (corner lo-lo to hi-hi in the plane) is at least 2^-14 mag^2 in the
+// un-normalised edge function, and whose axis coordinate is at least m from both faces
+// gets exactly the hits the full test would give: every edge function of the face
+// triangles then has its exact sign (the rounding of the differences, products and sum is
+// below 2^-19 mag^2, and the side edges' signs are exact by monotone rounding), so one
+// triangle per face is hit, and t = fl(T / det) lies within 2^-21 relative of the
+// distance to that face, which is at least m: inside the box the top face is hit at
+// t >= 1e-6 and the bottom face behind the origin, one hit (mag < 1e12 keeps the edge
+// functions, T = U*Az + V*Bz + W*Cz and t = T / det below overflow and t below 1e16);
+// above the box no hit; below it two hits at least the box height (> m) apart, which the
+// chain counts twice. A point at least m outside the square in one coordinate and off the
+// diagonal's line by the same margin misses every triangle (a side edge or the diagonal
+// separates it, with exact sign). Everything else -- near a face, an edge or the diagonal,
+// flat or huge boxes, non-finite values -- takes the full test.
+template <int AX>
+__host__ __device__ __forceinline__ bool box_filter_pass(const float3 o, const FBox& b) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    const float lo[3] = {b.x0, b.y0, b.z0}, hi[3] = {b.x1, b.y1, b.z1};
+    const float ulo = lo[U], uhi = hi[U], wlo = lo[W], whi = hi[W], alo = lo[AX], ahi = hi[AX];
+    const float u = get3(o, U), w = get3(o, W), a = get3(o, AX);
+    const float mag = fmaxf(fmaxf(fmaxf(fabsf(ulo), fabsf(uhi)), fmaxf(fabsf(wlo), fabsf(whi))),
+                            fmaxf(fmaxf(fabsf(alo), fabsf(ahi)), fmaxf(fmaxf(fabsf(u), fabsf(w)), fabsf(a))));
+    if (mag < 1e12f) {
+        const float m = mag * 1.52587890625e-5f + 1e-5f;  // 2^-16 mag + 1e-5
+        const float eu = uhi - ulo, ew = whi - wlo;
+        const float dg = (u - ulo) * ew - (w - wlo) * eu;
+        const bool off_diag = fabsf(dg) > mag * mag * 6.103515625e-5f;  // 2^-14 mag^2
+        if (off_diag && eu > 2.0f * m && ew > 2.0f * m && ahi - alo > 2.0f * m) {
+            const bool in_sq = u > ulo + m && u < uhi - m && w > wlo + m && w < whi - m;
+            const bool out_sq = u < ulo - m || u > uhi + m || w < wlo - m || w > whi + m;
+            if (out_sq) return false;
+            if (in_sq) {
+                if (a > alo + m && a < ahi - m) return true;
+                if (a > ahi + m || a < alo - m) return false;
+            }
+        }
+    }
+    return box_filter_full<AX>(o, b);
 }
 
 struct NodeBoxes {

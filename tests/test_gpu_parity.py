@@ -120,6 +120,30 @@ def test_filter_refine(ctx, grid, axis):
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
+def test_filter_box_margins(ctx, axis):
+    # points at and around the thresholds of the filter's fast classification (face
+    # distance m = 2^-16 mag + 1e-5, the faces' diagonal), checked against the oracle's
+    # parity over the whole box mesh
+    import boxcases as bc
+    rng = np.random.Generator(np.random.PCG64(17 + axis))
+    for box in bc.MARGIN_BOXES:
+        pts = bc.margin_points(box, rng)
+        print(f"filter box margins {box}, axis {axis}, {len(pts)} points", flush=True)
+        V, T = bc.box_mesh(box)
+        ref = orc.points_contain(V, T, np.zeros(12, np.int32), pts, axis=axis, use_grid=False) >= 0
+        dp = to_dev(pts)
+        flags = torch.zeros(len(pts), dtype=torch.uint8, device="cuda")
+        hb = (C.c_float * 6)(*[float(np.float32(v)) for v in box])
+        torch.cuda.synchronize()
+        _lib.check(ctx.lib.rs3d_filter_box(ctx.handle, hb, C.c_void_p(dp.data_ptr()), len(pts), axis,
+                                           C.c_void_p(flags.data_ptr())), "filter_box")
+        ctx.sync()
+        got = to_host(flags).astype(bool)
+        assert np.array_equal(got, ref), (box, np.count_nonzero(got != ref))
+        assert 0 < int(ref.sum()) < len(pts)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
 def test_filter_box_flags(ctx, axis):
     # rs3d_filter_box tests the 4 triangles perpendicular to the axis; the oracle parity runs
     # over the whole 12-triangle box mesh, on points on and one ulp off every face plane
