@@ -272,6 +272,18 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
     if (lane == 0 && v) atomicAdd(acc_stripe(counts), v);
 }
 
+// After a filter-refine launch: the stripes' sum into *total (host-coherent memory) and the
+// stripes zeroed for the next launch (one wave; the kernel boundary orders it after the
+// launch's atomics).
+__global__ __launch_bounds__(64) void k_fr_finish(unsigned long long* __restrict__ counts,
+                                                  unsigned long long* __restrict__ total) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long s = counts[(size_t)lane * ACC_PITCH];
+    counts[(size_t)lane * ACC_PITCH] = 0ull;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) *total = s;
+}
+
 namespace {
 
 struct AgTmp {
@@ -531,6 +543,12 @@ int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float*
         case 1: return launch_grid_filtered_ax<1>(ctx, g, pts, n, fb, out, counts);
         default: return launch_grid_filtered_ax<2>(ctx, g, pts, n, fb, out, counts);
     }
+}
+
+int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total) {
+    k_fr_finish<<<1, 64, 0, ctx->stream>>>(counts, total);
+    RS3D_HIP(hipGetLastError());
+    return RS3D_OK;
 }
 
 int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready, bool build) {

@@ -770,6 +770,8 @@ int rs3d_ctx_destroy(rs3d_ctx* ctx) {
     ctx->small.release();
     ctx->accs.release();
     ctx->edge_sel.release();
+    ctx->fr_acc.release();
+    if (ctx->fr_total) (void)hipHostFree(ctx->fr_total);
     ctx->ray_keys.release();
     ctx->ray_vals.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

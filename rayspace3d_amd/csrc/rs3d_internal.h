@@ -244,6 +244,11 @@ struct rs3d_ctx {
     int64_t opt_seg_occupancy = 1;          // edge rays test the target's occupancy bits first
     int64_t opt_seg_cell_pct = 200;         // occupancy cell edge, % of the mean triangle extent
     uint64_t* pinned = nullptr;  // small pinned host buffer for counts
+    // rs3d_filter_refine on a grid: stripes its launches leave zeroed (device), and the word
+    // their counts are written to (host-coherent pinned memory)
+    rs3d::DevBuf fr_acc;
+    unsigned long long* fr_total = nullptr;
+    bool fr_dirty = false;  // a launch may have left the stripes non-zero
     // traversal-kernel timing (rs3d_ctx_profile)
     int profile = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -304,6 +309,9 @@ uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh);
 // accumulator) accumulates candidates << 32 | inside.
 int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n,
                             const float* box, int32_t* out_obj, unsigned long long* counts);
+// One wave after such launches: the stripes of counts added into *total (host-coherent
+// memory) and zeroed again, so that the next launch finds them ready.
+int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total);
 // Segment occupancy bits of a BVH for edge rays (built here on first use when the context
 // allows them); *args gets bits = null when there are none.
 struct OccArgs;

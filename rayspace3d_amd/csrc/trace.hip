@@ -619,6 +619,34 @@ int rs3d_filter_box(rs3d_ctx* ctx, const float* box, const float* points, uint64
     return RS3D_OK;
 }
 
+}  // extern "C"
+
+namespace rs3d {
+// The grid filter-refine counts of a context: ACC_STRIPES stripes, zeroed once here and left
+// zeroed by every launch, and the host-coherent word the launch writes.
+int fr_counts_ready(rs3d_ctx* ctx) {
+    const size_t bytes = (size_t)ACC_STRIPES * ACC_PITCH * sizeof(unsigned long long);
+    if (ctx->fr_total && !ctx->fr_dirty) return RS3D_OK;
+    // first use, or a call that stopped between its launch and the finisher: zero again
+    RS3D_TRY(ctx->fr_acc.ensure(bytes));
+    RS3D_HIP(hipMemsetAsync(ctx->fr_acc.ptr, 0, bytes, ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->fr_dirty = false;
+    if (ctx->fr_total) return RS3D_OK;
+    void* h = nullptr;
+    const hipError_t e = hipHostMalloc(&h, 256, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e != hipSuccess) {
+        set_error("hipHostMalloc of the filter-refine counts failed: %s", hipGetErrorString(e));
+        return RS3D_E_NOMEM;
+    }
+    ctx->fr_total = static_cast<unsigned long long*>(h);
+    *ctx->fr_total = 0;
+    return RS3D_OK;
+}
+}  // namespace rs3d
+
+extern "C" {
+
 int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, uint64_t n,
                        int axis, int32_t* out_obj, uint64_t* candidate_count,
                        uint64_t* inside_count) {
@@ -637,20 +665,24 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
     bool grid = false;
     if (bvh->num_tris) RS3D_TRY(axis_grid_ready(ctx, const_cast<rs3d_bvh*>(bvh), axis, &grid));
     if (grid) {
-        // one pass: filter, trace and count per point (same answers as the three phases)
-        unsigned long long* cnt;
-        RS3D_TRY(acc_slot(ctx, 0, 0, &cnt));
+        // one pass: filter, trace and count per point (same answers as the three phases);
+        // the launch leaves its counts in fr_total and its stripes zeroed for the next call
+        RS3D_TRY(fr_counts_ready(ctx));
+        unsigned long long* cnt = ctx->fr_acc.as<unsigned long long>();
         if (ctx->profile) {
             RS3D_TRY(prof_collect(ctx));
             RS3D_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
         }
+        ctx->fr_dirty = true;
         RS3D_TRY(axis_grid_filter_refine(ctx, const_cast<rs3d_bvh*>(bvh), axis, points, n, box, out_obj, cnt));
         if (ctx->profile) {
             RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
             ctx->prof_pending = true;
         }
-        uint64_t h = 0;
-        RS3D_TRY(acc_read_sum(ctx, cnt, &h));
+        RS3D_TRY(axis_grid_counts_finish(ctx, cnt, ctx->fr_total));
+        ctx->fr_dirty = false;
+        RS3D_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t h = *reinterpret_cast<volatile unsigned long long*>(ctx->fr_total);
         if (candidate_count) *candidate_count = h >> 32;
         if (inside_count) *inside_count = h & 0xffffffffull;
         return RS3D_OK;

@@ -119,6 +119,24 @@ def test_filter_refine(ctx, grid, axis):
     assert np.array_equal(passes[~on_face], in_box[~on_face])
 
 
+def test_filter_refine_counts_repeat(ctx):
+    # the grid launch leaves its counters zeroed for the next call: calls in a row on
+    # different point sets (and sizes) each report their own candidate and inside counts
+    mesh, pts = gen.many_spheres_workload(seed=33, n_objects=400, n_points=300_000, world=80.0)
+    bvh = Bvh.from_mesh(ctx, mesh)
+    pipe = FilterRefinePipeline(ctx)
+    vmin, vmax = mesh.vertices.min(0), mesh.vertices.max(0)
+    for k, m in enumerate((300_000, 1_000, 120_000, 300_000)):
+        p = np.ascontiguousarray(pts[:m] if k != 2 else pts[-m:])
+        print(f"filter-refine repeat {k}: {m} points", flush=True)
+        res = pipe.execute(bvh, to_dev(p), axis=2)
+        ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, p, axis=2)
+        assert np.array_equal(to_host(res.final_results), ref)
+        assert res.inside_count == int((ref >= 0).sum())
+        assert res.candidate_count == int(np.all((p > vmin) & (p < vmax), axis=1).sum())
+    bvh.free()
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_filter_box_margins(ctx, axis):
     # points at and around the thresholds of the filter's fast classification (face
