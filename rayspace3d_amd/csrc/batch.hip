@@ -91,16 +91,16 @@ __device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
         const uint64_t lo = (t - task_off[qa]) * q.task_rays;
         const uint64_t hi = lo + q.task_rays < q.n ? lo + q.task_rays : q.n;
         if ((KINDS & 1) && q.kind == BQ_CONTAIN && q.grid.cells) {
-            // the two-phase grid form; its queue lives in the (idle) traversal stack memory
+            // the two-phase grid form with the item-parallel walk; its LDS is the (idle)
+            // traversal stack memory
             const int ax = AX >= 0 ? AX : q.axis;
-            uint32_t* qpi = reinterpret_cast<uint32_t*>(stack_mem);
-            uint2* qrg = reinterpret_cast<uint2*>(stack_mem + AG_QCAP);
+            AgItemLds* L = reinterpret_cast<AgItemLds*>(stack_mem);
             uint32_t cand = 0;
             GridFilter gf{};
             int32_t* out = static_cast<int32_t*>(q.out);
-            if (ax == 0) points_grid_two_phase<0, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
-            else if (ax == 1) points_grid_two_phase<1, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
-            else points_grid_two_phase<2, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, qpi, qrg, cand, gf, ctr);
+            if (ax == 0) points_grid_two_phase<0, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
+            else if (ax == 1) points_grid_two_phase<1, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
+            else points_grid_two_phase<2, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
             continue;
         }
         if ((KINDS & 1) && q.kind == BQ_CONTAIN) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     const BatchDesc* __restrict__ qs, const uint64_t* __restrict__ task_off,
     const uint32_t* __restrict__ task_q, uint64_t t0, uint64_t t1, unsigned int* __restrict__ heads,
     const uint32_t* __restrict__ perm, HashArgs H, unsigned long long* __restrict__ bad_ids) {
-    static_assert(DEPTH * TR_BLOCK >= 3 * AG_QCAP, "the grid queue lives in the stack memory");
+    static_assert(DEPTH * TR_BLOCK * sizeof(int) >= sizeof(AgItemLds), "the grid walk's LDS is the stack memory");
     __shared__ __align__(16) int stack_mem[DEPTH * TR_BLOCK];
     batch_loop<DEPTH, 1, AX>(qs, task_off, task_q, t0, t1, heads, perm, H, bad_ids, stack_mem);
 }
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_batch(
     const BatchDesc* __restrict__ qs, const uint64_t* __restrict__ task_off,
     const uint32_t* __restrict__ task_q, uint64_t t0, uint64_t t1, unsigned int* __restrict__ heads,
     const uint32_t* __restrict__ perm, HashArgs H, unsigned long long* __restrict__ bad_ids) {
-    static_assert(DEPTH * TR_BLOCK >= 3 * AG_QCAP, "the grid queue lives in the stack memory");
+    static_assert(DEPTH * TR_BLOCK * sizeof(int) >= sizeof(AgItemLds), "the grid walk's LDS is the stack memory");
     __shared__ __align__(16) int stack_mem[DEPTH * TR_BLOCK];
     batch_loop<DEPTH, KINDS, AX>(qs, task_off, task_q, t0, t1, heads, perm, H, bad_ids, stack_mem);
 }

@@ -44,7 +44,7 @@ struct BatchDesc {
 // so the atomics of the persistent waves spread over many addresses; a wave starts on the
 // stripe of its workgroup index and moves on when a stripe runs dry.
 constexpr int TASK_RAYS = 64;
-constexpr int GRID_TASK_RAYS = 1024;  // containment on an axis grid: one lane-refill run per task
+constexpr int GRID_TASK_RAYS = 1024;  // containment on an axis grid: four 256-point sessions per task
 constexpr int NSTRIPE = 32;
 constexpr int STRIPE_PITCH = 64;  // 32-bit words between heads
 
