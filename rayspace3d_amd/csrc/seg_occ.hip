@@ -22,6 +22,7 @@ namespace rs3d {
 struct SegOcc {
     OccArgs args{};
     uint32_t* bits = nullptr;
+    uint32_t* snode = nullptr;  // start node per block of OCC_BLOCK^3 cells
     size_t bytes = 0;
     double cell = 0.0;
     uint64_t occupied = 0;
@@ -31,6 +32,7 @@ struct SegOcc {
 void seg_occ_free(SegOcc* g) {
     if (!g) return;
     if (g->bits) (void)hipFree(g->bits);
+    if (g->snode) (void)hipFree(g->snode);
     delete g;
 }
 
@@ -92,6 +94,87 @@ __global__ __launch_bounds__(256) void k_so_mark(const TriRec* __restrict__ tris
                 const uint32_t m = 1u << (c & 31);
                 if (!(bits[c >> 5] & m)) atomicOr(&bits[c >> 5], m);  // a read first: most bits are set already
             }
+}
+
+// Leaf range per block: for every block holding a cell the triangle's padded box overlaps,
+// the smallest and largest triangle position (Morton order = BVH leaf order). Consecutive
+// lanes with the same block range leave the minimum to the first of them and the maximum
+// to the last (in Morton order most neighbours share their block).
+__global__ __launch_bounds__(256) void k_so_blocks(const TriRec* __restrict__ tris, uint64_t n, OccArgs g,
+                                                   int bnz, uint32_t* __restrict__ bmin,
+                                                   uint32_t* __restrict__ bmax) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1, z0 = 0, z1 = -1;
+    if (valid) {
+        const TriRec r = tris[i];
+        float plo[3], phi[3];
+        prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                        make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
+        x0 = occ_cell(plo[0], g.org_x, g.inv, g.nx) / OCC_BLOCK, x1 = occ_cell(phi[0], g.org_x, g.inv, g.nx) / OCC_BLOCK;
+        y0 = occ_cell(plo[1], g.org_y, g.inv, g.ny) / OCC_BLOCK, y1 = occ_cell(phi[1], g.org_y, g.inv, g.ny) / OCC_BLOCK;
+        z0 = occ_cell(plo[2], g.org_z, g.inv, g.nz) / OCC_BLOCK, z1 = occ_cell(phi[2], g.org_z, g.inv, g.nz) / OCC_BLOCK;
+    }
+    (void)bnz;
+    const int lane = threadIdx.x & 63;
+    auto same_as = [&](int src) {
+        return __shfl(x0, src, 64) == x0 && __shfl(x1, src, 64) == x1 && __shfl(y0, src, 64) == y0 &&
+               __shfl(y1, src, 64) == y1 && __shfl(z0, src, 64) == z0 && __shfl(z1, src, 64) == z1;
+    };
+    const bool same_prev = lane > 0 && same_as(lane - 1);
+    const int nxt = lane < 63 ? lane + 1 : lane;
+    const bool next_valid = __shfl((int)valid, nxt, 64) != 0;
+    const bool same_next = lane < 63 && next_valid && same_as(nxt);
+    if (!valid || (same_prev && same_next)) return;
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) {
+                const uint64_t b = ((uint64_t)z * g.bny + y) * g.bnx + x;
+                if (!same_prev) atomicMin(&bmin[b], (uint32_t)i);
+                if (!same_next) atomicMax(&bmax[b], (uint32_t)i);
+            }
+}
+
+// Start node per block: over the block's 2x2x2 neighbourhood (blocks b .. b+1 per axis)
+// the leaf range [a, b] of the triangles there, and the deepest node whose leaf range holds
+// it, found from the root down (a node covers [F, L]; its left child covers [F, gamma] and
+// is node gamma when internal, its right child [gamma + 1, L]; a leaf child ends the
+// descent at its parent). Blocks with no triangle nearby keep the root.
+__global__ __launch_bounds__(256) void k_so_start(const BNode* __restrict__ nodes, uint64_t n,
+                                                  const uint32_t* __restrict__ bmin,
+                                                  const uint32_t* __restrict__ bmax, int bnx, int bny, int bnz,
+                                                  uint32_t* __restrict__ snode) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nblocks = (uint64_t)bnx * bny * bnz;
+    if (k >= nblocks) return;
+    const int bx = (int)(k % bnx), by = (int)((k / bnx) % bny), bz = (int)(k / ((uint64_t)bnx * bny));
+    uint32_t a = 0xffffffffu, b = 0u;
+    for (int dz = 0; dz < 2 && bz + dz < bnz; ++dz)
+        for (int dy = 0; dy < 2 && by + dy < bny; ++dy)
+            for (int dx = 0; dx < 2 && bx + dx < bnx; ++dx) {
+                const uint64_t q = ((uint64_t)(bz + dz) * bny + (by + dy)) * bnx + (bx + dx);
+                a = min(a, bmin[q]);
+                b = max(b, bmax[q]);
+            }
+    int p = 0;
+    if (n > 1 && a <= b) {
+        uint32_t F = 0;
+        for (int depth = 0; depth < 128; ++depth) {
+            const int4 lk = nodes[p].link;
+            const uint32_t gamma = lk.z > 0 ? F + (uint32_t)lk.z - 1 : (uint32_t)lk.x;
+            if (b <= gamma) {
+                if (lk.z > 0) break;
+                p = lk.x;
+            } else if (a > gamma) {
+                if (lk.w > 0) break;
+                p = lk.y;
+                F = gamma + 1;
+            } else {
+                break;
+            }
+        }
+    }
+    snode[k] = (uint32_t)p;
 }
 
 __global__ __launch_bounds__(256) void k_so_count(const uint32_t* __restrict__ bits, uint64_t nw,
@@ -179,6 +262,37 @@ int build(rs3d_ctx* ctx, rs3d_bvh* bvh, SegOcc** out) {
         uint64_t occ = 0;
         RS3D_TRY(acc_read_sum(ctx, acc, &occ));
         g->occupied = occ;
+        // start nodes of the blocks
+        const int bnz = (a.nz + OCC_BLOCK - 1) / OCC_BLOCK;
+        a.bnx = (a.nx + OCC_BLOCK - 1) / OCC_BLOCK;
+        a.bny = (a.ny + OCC_BLOCK - 1) / OCC_BLOCK;
+        const uint64_t nblocks = (uint64_t)a.bnx * a.bny * bnz;
+        uint32_t *bmin = nullptr, *bmax = nullptr;
+        hipError_t e2;
+        if ((e2 = hipMalloc(&g->snode, nblocks * sizeof(uint32_t))) != hipSuccess ||
+            (e2 = hipMalloc(&bmin, nblocks * sizeof(uint32_t))) != hipSuccess ||
+            (e2 = hipMalloc(&bmax, nblocks * sizeof(uint32_t))) != hipSuccess) {
+            if (bmin) (void)hipFree(bmin);
+            set_error("segment occupancy: hipMalloc of the start nodes failed: %s", hipGetErrorString(e2));
+            return RS3D_E_NOMEM;
+        }
+        const int rb = [&]() -> int {
+            RS3D_TRY(fill_u32(ctx, bmin, nblocks, 0xffffffffu));
+            RS3D_TRY(fill_u32(ctx, bmax, nblocks, 0u));
+            k_so_blocks<<<nb, 256, 0, s>>>(bvh->tris, n, a, bnz, bmin, bmax);
+            RS3D_HIP(hipGetLastError());
+            unsigned nbb;
+            RS3D_TRY(grid_blocks(nblocks, 256, &nbb));
+            k_so_start<<<nbb, 256, 0, s>>>(bvh->nodes, n, bmin, bmax, a.bnx, a.bny, bnz, g->snode);
+            RS3D_HIP(hipGetLastError());
+            RS3D_HIP(hipStreamSynchronize(s));
+            return RS3D_OK;
+        }();
+        (void)hipFree(bmin);
+        (void)hipFree(bmax);
+        RS3D_TRY(rb);
+        a.snode = g->snode;
+        g->bytes += nblocks * sizeof(uint32_t);
         return RS3D_OK;
     }();
     if (rc == RS3D_E_CAPACITY) {  // rejected: edge rays walk the tree
@@ -230,7 +344,9 @@ uint64_t seg_occ_bytes(const rs3d_bvh* bvh) { return bvh->socc ? bvh->socc->byte
 
 uint64_t seg_occ_bytes_bound(const rs3d_bvh* bvh) {
     (void)bvh;
-    return (1ull << 30) / 8 + (16ull << 20);  // bits of at most 2^30 cells, and slack
+    // bits of at most 2^31 cells, 12 B per block of 64 cells while the start nodes are
+    // built (4 B kept), and slack
+    return (1ull << 31) / 8 + (1ull << 31) / 64 * 12 + (16ull << 20);
 }
 
 }  // namespace rs3d

@@ -407,9 +407,12 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
         constexpr int D = decltype(depth_tag)::value;
         constexpr bool ST = decltype(stats_tag)::value;
         unsigned long long* st = ST ? stats : nullptr;
-        const OccArgs none{};
+        // over a selected list the bits are tested already: the walk only looks up its
+        // start node
+        OccArgs walk = occ;
+        if (list) walk.bits = nullptr;
         k_edges<MODE, D, ST><<<list ? nbl : nb, TR_BLOCK, 0, ctx->stream>>>(
-            bvh->nodes, bvh->tris, list ? none : occ, E, list, n_list, swap, chain_rule, max_iter, counts, offsets,
+            bvh->nodes, bvh->tris, walk, E, list, n_list, swap, chain_rule, max_iter, counts, offsets,
             bias, out, H, st);
     };
     auto launch_depth = [&](auto stats_tag) {

@@ -205,7 +205,14 @@ struct OccArgs {
     const uint32_t* bits;  // null: no occupancy test
     float org_x, org_y, org_z, inv;
     int nx, ny, nz;
+    // start nodes (null: every walk starts at the root): for block (bx, by, bz) of
+    // OCC_BLOCK^3 cells, the deepest BVH node whose subtree holds every triangle whose padded
+    // box overlaps a cell of the 2x2x2 blocks from (bx, by, bz) on
+    const uint32_t* snode;
+    int bnx, bny;
 };
+
+constexpr int OCC_BLOCK = 4;
 
 __device__ __forceinline__ int occ_cell(float x, float org, float inv, int n) {
     const float f = fminf(fmaxf((x - org) * inv, 0.0f), (float)(n - 1));
@@ -213,33 +220,46 @@ __device__ __forceinline__ int occ_cell(float x, float org, float inv, int n) {
 }
 
 // Whether the box [lo, hi] covers a set bit, i.e. may meet a padded triangle box of the
-// bits' BVH. Boxes over more than 64 cells are not tested (answer true). words: occupancy
-// words read.
+// bits' BVH (true without bits). Boxes over more than 64 cells are not tested (answer true).
+// words: occupancy words read. With start, a walk for anything inside the box may begin at *start (left
+// unchanged, the root, when the box's cells do not fit one block's 2x2x2 neighbourhood):
+// every triangle whose padded box meets the box overlaps one of the box's cells.
 __device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 lo, const float3 hi,
-                                                uint64_t& words) {
+                                                uint64_t& words, int* start = nullptr) {
     const int x0 = occ_cell(lo.x, g.org_x, g.inv, g.nx), x1 = occ_cell(hi.x, g.org_x, g.inv, g.nx);
     const int y0 = occ_cell(lo.y, g.org_y, g.inv, g.ny), y1 = occ_cell(hi.y, g.org_y, g.inv, g.ny);
     const int z0 = occ_cell(lo.z, g.org_z, g.inv, g.nz), z1 = occ_cell(hi.z, g.org_z, g.inv, g.nz);
     if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 64) return true;
-    for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y)
+    bool any = !g.bits;  // without bits (a walk over selected segments): start node only
+    if (g.bits)
+    for (int z = z0; z <= z1 && !any; ++z)
+        for (int y = y0; y <= y1 && !any; ++y)
             for (int x = x0; x <= x1; ++x) {
                 const uint64_t c = ((uint64_t)z * g.ny + y) * g.nx + x;
                 ++words;
-                if ((g.bits[c >> 5] >> (c & 31)) & 1u) return true;
+                if ((g.bits[c >> 5] >> (c & 31)) & 1u) {
+                    any = true;
+                    break;
+                }
             }
-    return false;
+    if (any && start && g.snode) {
+        const int bx = x0 / OCC_BLOCK, by = y0 / OCC_BLOCK, bz = z0 / OCC_BLOCK;
+        if (x1 < (bx + 2) * OCC_BLOCK && y1 < (by + 2) * OCC_BLOCK && z1 < (bz + 2) * OCC_BLOCK)
+            *start = (int)g.snode[((uint64_t)bz * g.bny + by) * g.bnx + bx];
+    }
+    return any;
 }
 
 // Whether segment a-b may hit a triangle of the bits' BVH: its box, widened by 2^-14 of the
 // coordinates' magnitude, through occ_box_may_hit.
 __device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b,
-                                                    uint64_t& words) {
+                                                    uint64_t& words, int* start = nullptr) {
     const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + 1e-30f;  // 2^-14
     const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + 1e-30f;
     const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + 1e-30f;
     return occ_box_may_hit(g, make_float3(fminf(a.x, b.x) - wa, fminf(a.y, b.y) - wb, fminf(a.z, b.z) - wc),
-                           make_float3(fmaxf(a.x, b.x) + wa, fmaxf(a.y, b.y) + wb, fmaxf(a.z, b.z) + wc), words);
+                           make_float3(fmaxf(a.x, b.x) + wa, fmaxf(a.y, b.y) + wb, fmaxf(a.z, b.z) + wc), words,
+                           start);
 }
 
 // ------------------------------------------------------------------------------------
@@ -392,7 +412,7 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
                                                     const float3 inv, const RayPre& rp, float thr,
                                                     float tmax, int* stack, float (&bt)[KBUF],
                                                     uint32_t (&bi)[KBUF], int (&bo)[KBUF],
-                                                    TraceCounters& ctr) {
+                                                    TraceCounters& ctr, int root = 0) {
     // The computed t of any reported hit lies within the kz extent of its triangle's
     // vertices (scaled by Sz) up to a few ulps; boxes are tested against [thr, tmax] on
     // that slab with a relative margin, and against the full line on all three slabs.
@@ -401,7 +421,7 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
     const float hi_t = tmax + fabsf(tmax) * 9.5367431640625e-7f;
     uint32_t total = 0;
     int sp = 0;
-    int node = 0;
+    int node = root;
     while (true) {
         const NodeBoxes b = load_node(nodes, node);
         if (STATS) ctr.nodes++;
@@ -510,15 +530,17 @@ enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
 // Whether edge s-en may have chain hits: its length reaches 1e-6 and, with occupancy bits,
 // the segment s .. s + tmax d that holds every hit (t in [1e-6, tmax]) covers a set bit
 // (its box is widened by 2^-14 of the coordinates' magnitude inside the test).
-__device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s, const float3 en, uint64_t& words) {
+__device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s, const float3 en, uint64_t& words,
+                                             int* start = nullptr) {
     const float eps = 1e-6f;
     const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
     if (!(len >= eps)) return false;
-    if (!occ.bits) return true;
+    if (!occ.bits && !(start && occ.snode)) return true;
     const float3 d = make_float3(dx / len, dy / len, dz / len);
     const float tmax = len + eps;
-    return occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax), words);
+    return occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax), words,
+                               start);
 }
 
 template <int DEPTH, bool STATS, typename Emit>
@@ -526,7 +548,8 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
                                         const TriRec* __restrict__ tris, const OccArgs& occ,
                                         const float3 s, const float3 en, int chain_rule, int max_iter,
                                         int* stack, TraceCounters& ctr, Emit&& emit) {
-    if (!edge_may_hit(occ, s, en, ctr.occ_words)) return 0;
+    int root = 0;  // the walk starts below the root when the occupancy blocks allow it
+    if (!edge_may_hit(occ, s, en, ctr.occ_words, &root)) return 0;
     if (STATS) ctr.walked++;
     const float eps = 1e-6f;
     const float dx = en.x - s.x, dy = en.y - s.y, dz = en.z - s.z;
@@ -543,7 +566,7 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
     while (true) {
         kbuf_clear(bt, bi, bo);
         const uint32_t total =
-            collect_segment<DEPTH, STATS>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr);
+            collect_segment<DEPTH, STATS>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr, root);
 #pragma unroll
         for (int k = 0; k < KBUF; ++k) {
             if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
