@@ -15,14 +15,17 @@ namespace rs3d {
 // bit per tile that lists records (.x) and the number of such tiles before the group (.y);
 // occ holds the cell bits of those tiles only, in tile order (rank .y + popcount of the
 // lower bits of .x). A point in an empty tile is answered from tsum alone, which is small
-// enough (8 B per 1024 cells) to stay in L2 while records stream through it.
+// enough (8 B per 1024 cells) to stay in L2 while records stream through it. Beside each
+// occupied tile's cell bits, occ keeps the highest box top of the tile's records (the
+// largest first-record top of its cells; +inf when one is NaN): a point above it has
+// nothing to test in any cell of the tile and is answered without the cell-table read.
 struct AxisGrid {
     int axis = 2, nu = 1, nw = 1, ntu = 1;
     float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
     uint64_t entries = 0;
     uint2* cells = nullptr;     // tiled cells + 1 entries: first record, box top of that record
     uint2* tsum = nullptr;      // per 32 tiles: occupied-tile bits, occupied tiles before
-    uint32_t* occ = nullptr;    // per occupied tile: its 32 cell bits
+    uint2* occ = nullptr;       // per occupied tile: its 32 cell bits, highest box top (bits)
     float4* rec = nullptr;      // AG_REC float4 per record
     size_t bytes = 0;
     double expected = 0.0;
@@ -36,7 +39,7 @@ constexpr int AG_REC = 3;
 struct AGArgs {
     const uint2* cells;
     const uint2* tsum;
-    const uint32_t* occ;
+    const uint2* occ;
     const float4* rec;
     float lo_u, lo_w, inv_u, inv_w;
     int nu, nw, ntu;
@@ -154,7 +157,8 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                     // levels answer those without the range fetch
                     js = je = 0;
                     const int64_t tr = ag_tile_rank(G.tsum[c >> 10], c >> 5);
-                    if (tr >= 0 && ((G.occ[tr] >> (c & 31)) & 1u)) {
+                    const uint2 oc = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
+                    if (((oc.x >> (c & 31)) & 1u) && !(__uint_as_float(oc.y) < get3(o, AX))) {
                         const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
                         if (STATS) ctr.nodes++;
                         // the first record's box top decides as the walk's exit test would
@@ -464,8 +468,8 @@ constexpr uint32_t AG_QCAP = 512;
 // Point rays [lo, hi) on the grid in two alternating phases. Phase 1 streams the points 256
 // at a time (four independent loads per lane in flight): origin, with FILT the dataset-box
 // filter of gf.box, cell, occupancy bit and the cell-table entries; points that fail the
-// filter, fall in an empty cell or lie above the cell's highest box top are answered -1 at
-// once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS; with
+// filter, fall in an empty cell or lie above the tile's or the cell's highest box top are
+// answered -1 at once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS; with
 // ITEMS the queue is L's, AG_IQCAP entries).
 // Ray r of the list is point imap[r] (r itself without imap); answers go out as out_kind.
 // Phase 2 walks the queue (ITEMS: points_grid_items; else lane refill, points_grid_run with
@@ -491,8 +495,8 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
         for (; base < hi && nq + 4 * 64 <= QCAP && !(buf && base != sbase); base += 4 * 64) {
             float3 o[4];
             bool ok[4];
-            uint32_t cell[4], word[4], pid[4];
-            uint2 rg[4], ts[4];
+            uint32_t cell[4], pid[4];
+            uint2 rg[4], ts[4], oc[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint64_t r = base + 64 * k + lane;
@@ -515,11 +519,12 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int64_t tr = ag_tile_rank(ts[k], cell[k] >> 5);
-                word[k] = tr >= 0 ? G.occ[tr] : 0u;
+                oc[k] = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                ok[k] = (word[k] >> (cell[k] & 31)) & 1u;
+                // an occupied cell, and the point not above the tile's highest box top
+                ok[k] = ((oc[k].x >> (cell[k] & 31)) & 1u) && !(__uint_as_float(oc[k].y) < get3(o[k], AX));
                 uint2 c0 = make_uint2(0u, 0u), c1 = make_uint2(0u, 0u);
                 if (ok[k]) {
                     c0 = G.cells[cell[k]];

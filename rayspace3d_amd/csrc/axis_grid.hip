@@ -200,16 +200,24 @@ __global__ __launch_bounds__(256) void k_ag_tsum(const uint32_t* __restrict__ wo
     }
 }
 
-// Compact words of the occupied tiles at their ranks, and each group's rank base.
+// Compact words of the occupied tiles at their ranks, each with the highest box top of the
+// tile's records (the cells' first-record tops, from the cell table: +inf when one is NaN,
+// so that no point is ever skipped on it), and each group's rank base.
 __global__ __launch_bounds__(256) void k_ag_fine(const uint32_t* __restrict__ words, uint64_t nt,
-                                                 const uint32_t* __restrict__ gbase, uint2* __restrict__ tsum,
-                                                 uint32_t* __restrict__ occ) {
+                                                 const uint32_t* __restrict__ gbase, const uint2* __restrict__ cells,
+                                                 uint2* __restrict__ tsum, uint2* __restrict__ occ) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     const uint32_t base = gbase[t / 32];
     if ((t & 31) == 0) tsum[t / 32].y = base;
     const uint32_t w = words[t];
-    if (w) occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = w;
+    if (!w) return;
+    float top = -INFINITY;
+    for (uint32_t m = w; m; m &= m - 1u) {
+        const float c = __uint_as_float(cells[32 * t + (uint32_t)__ffs(m) - 1u].y);
+        top = c != c ? INFINITY : (c > top ? c : top);
+    }
+    occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = make_uint2(w, __float_as_uint(top));
 }
 
 // Cell table: (first record, box top of the first record -- the largest of the cell) per
@@ -427,14 +435,14 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->expected = expected;
     hipError_t e;
     if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&g->occ, nt * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->occ, nt * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->tsum, ng * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * AG_REC * sizeof(float4))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint32_t) + ng * sizeof(uint2) + ptotal * AG_REC * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint2) + ng * sizeof(uint2) + ptotal * AG_REC * sizeof(float4);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
@@ -443,7 +451,6 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         RS3D_TRY(grid_blocks(nt, 256, &nbt));
         k_ag_tsum<<<nbt, 256, 0, s>>>(words, nt, g->tsum, gcount);
         RS3D_TRY(scan_u32_exclusive(ctx, gcount, gbase, ng));
-        k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, g->tsum, g->occ);
         if (total) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
@@ -454,6 +461,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         unsigned nbc1;
         RS3D_TRY(grid_blocks(nc + 1, 256, &nbc1));
         k_ag_cells<<<nbc1, 256, 0, s>>>(pstart, counts, g->rec, nc, (uint32_t)ptotal, g->cells);
+        k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, g->cells, g->tsum, g->occ);
         RS3D_HIP(hipGetLastError());
         RS3D_HIP(hipStreamSynchronize(s));
         return RS3D_OK;
@@ -564,11 +572,12 @@ uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
     // the build widens its cells to meet), 48 B each plus up to 1/8 for the padding to even
     // lists, and while the build sorts them 24 B more each (keys, values and the sort's
     // alternate buffers); cells are within 16 per triangle + 1024 and the padding of the
-    // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table and 16 B of counts and
-    // offsets during the build; 8 B per triangle of spans and offsets.
+    // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table, 16 B of counts and
+    // offsets during the build and under 1 B of tile words, tile entries and summaries; 8 B
+    // per triangle of spans and offsets.
     const uint64_t n = bvh->num_tris;
     const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024 + (1ull << 19);
-    return recs * (54 + 24) + cells * 24 + n * 8 + (64ull << 20);
+    return recs * (54 + 24) + cells * 25 + n * 8 + (64ull << 20);
 }
 
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,
