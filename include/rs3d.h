@@ -62,7 +62,8 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * triangles (see rs3d_axis_grid_prepare) where one is accepted for the ray axis; 0 keeps
  * them on the BVH. Results are unchanged either way. Grid launches take the rays in the
  * caller's order (no reordering) with RS3D_OPT_GRID_RAYS_PER_WAVE rays per wave run
- * (default 256; 64..2^20). RS3D_OPT_GRID_CELL_PCT: cell edge of grids built afterwards,
+ * (default 256; 64..2^20), halved down to 64 for launches that would otherwise have fewer
+ * than 16,384 runs. RS3D_OPT_GRID_CELL_PCT: cell edge of grids built afterwards,
  * in percent of the triangles' mean extent (default 35; 10..1000; widened while the grid
  * would hold more than 16 records per triangle). Results are unchanged.
  * RS3D_OPT_SEG_OCCUPANCY: 1 (default) lets edge rays first test the target's segment

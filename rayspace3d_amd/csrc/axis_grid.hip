@@ -502,11 +502,22 @@ int ensure_grid(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis) {
     return RS3D_OK;
 }
 
+// Run length of a grid launch over n rays: the context's RS3D_OPT_GRID_RAYS_PER_WAVE,
+// halved (down to 64) while the launch would have fewer than 16,384 waves -- about two
+// waves per resident wave slot of the chip (256 CUs x 28 at this kernel's registers), which
+// small launches need to hide the latency of their dependent loads (1 M points: 230 us at
+// 256 per wave, 196 us at 64).
+uint32_t grid_run_length(const rs3d_ctx* ctx, uint64_t n) {
+    uint64_t pw = (uint64_t)ctx->opt_grid_rays_per_wave;
+    while (pw > 64 && n / pw < 16384) pw = pw / 2 > 64 ? pw / 2 : 64;
+    return (uint32_t)pw;
+}
+
 template <int AX, bool STATS>
 int launch_grid_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const uint32_t* imap,
                    int out_kind, void* out, unsigned long long* stats) {
     const AGArgs G = ag_args(g);
-    const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
+    const uint32_t pw = grid_run_length(ctx, n);
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
     k_points_grid<AX, STATS><<<nb, TR_BLOCK, 0, ctx->stream>>>(G, pts, n, imap, out_kind, out, pw, stats);
@@ -528,7 +539,7 @@ template <int AX>
 int launch_grid_filtered_ax(rs3d_ctx* ctx, const AxisGrid* g, const float* pts, uint64_t n, const FBox& box,
                             int32_t* out, unsigned long long* counts) {
     const AGArgs G = ag_args(g);
-    const uint32_t pw = (uint32_t)ctx->opt_grid_rays_per_wave;
+    const uint32_t pw = grid_run_length(ctx, n);
     unsigned nb;
     RS3D_TRY(grid_blocks(n, pw, &nb));
     k_points_grid_filtered<AX><<<nb, TR_BLOCK, 0, ctx->stream>>>(G, pts, n, out, pw, box, counts);
