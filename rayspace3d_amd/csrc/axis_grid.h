@@ -54,6 +54,12 @@ __host__ __device__ __forceinline__ uint32_t ag_cell_id(int cu, int cw, int ntu)
            ((uint32_t)cu & 7u);
 }
 
+// Pins a loaded value in a register at this point. The grid kernels issue the loads of a
+// wave's four rays together and use the values only for some lanes; without the pin the
+// compiler moves each load into the branch that uses it and waits for it there, which
+// serialises the four loads. (An empty statement: it emits no instruction.)
+__device__ __forceinline__ void ag_pin(uint32_t& x) { asm volatile("" : "+v"(x)); }
+
 // Rank of tile t among the occupied tiles, given its group's tsum entry; -1 when t is empty.
 __device__ __forceinline__ int64_t ag_tile_rank(uint2 g, uint32_t t) {
     const uint32_t b = t & 31u;
@@ -115,14 +121,13 @@ struct GridSink {
 // Point rays [lo, run_hi) of a ray list on the grid, lane-refill form: the 64 lanes of the
 // calling wave take rays from the run as they finish. The per-ray program (hit buffer,
 // restarts, chain, output) is the BVH kernels'; only the candidate walk differs: a lane
-// reads two records per step (the second is in flight while the first is tested). With
-// gf, the lane counts the points it finds inside. With PRE, the list holds points whose
-// record range was looked up already: qrange[r] is ray imap[r]'s (first, end) record.
-template <int AX, bool STATS, bool PRE = false>
+// reads two records per step (the second is in flight while the first is tested). This
+// sequential form is the counting pass (STATS: cell-table reads, records read, hits,
+// restarts) that gives the grid launches' algorithmic bytes.
+template <int AX, bool STATS>
 __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __restrict__ pts,
                                                 const uint32_t* __restrict__ imap, uint64_t lo, uint64_t run_hi,
-                                                int out_kind, void* __restrict__ out, TraceCounters& ctr,
-                                                GridFilter* gf = nullptr, const uint2* qrange = nullptr) {
+                                                int out_kind, void* __restrict__ out, TraceCounters& ctr) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
     const float eps = 1e-6f;
     const int lane = threadIdx.x & 63;
@@ -145,27 +150,21 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             if (!active && r < run_hi) {
                 pi = imap ? imap[r] : (uint32_t)r;
                 o = ld3f(pts, pi);
-                if (PRE) {
-                    const uint2 rg = qrange[r];
-                    js = rg.x;
-                    je = rg.y;
-                } else {
-                    const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
-                    const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
-                    const uint32_t c = ag_cell_id(cu, cw, G.ntu);
-                    // most rays of a sparse scene fall in empty tiles or cells: the occupancy
-                    // levels answer those without the range fetch
-                    js = je = 0;
-                    const int64_t tr = ag_tile_rank(G.tsum[c >> 10], c >> 5);
-                    const uint2 oc = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
-                    if (((oc.x >> (c & 31)) & 1u) && !(__uint_as_float(oc.y) < get3(o, AX))) {
-                        const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
-                        if (STATS) ctr.nodes++;
-                        // the first record's box top decides as the walk's exit test would
-                        if (!(__uint_as_float(c0.y) < get3(o, AX))) {
-                            js = c0.x;
-                            je = c1.x;
-                        }
+                const int cu = ag_cell(get3(o, U), G.lo_u, G.inv_u, G.nu);
+                const int cw = ag_cell(get3(o, W), G.lo_w, G.inv_w, G.nw);
+                const uint32_t c = ag_cell_id(cu, cw, G.ntu);
+                // most rays of a sparse scene fall in empty tiles or cells: the occupancy
+                // levels answer those without the range fetch
+                js = je = 0;
+                const int64_t tr = ag_tile_rank(G.tsum[c >> 10], c >> 5);
+                const uint2 oc = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
+                if (((oc.x >> (c & 31)) & 1u) && !(__uint_as_float(oc.y) < get3(o, AX))) {
+                    const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
+                    if (STATS) ctr.nodes++;
+                    // the first record's box top decides as the walk's exit test would
+                    if (!(__uint_as_float(c0.y) < get3(o, AX))) {
+                        js = c0.x;
+                        je = c1.x;
                     }
                 }
                 j = js;
@@ -238,7 +237,6 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         }
         const int res = (count & 1) ? first_obj : -1;
         grid_write_result(out_kind, out, pi, res);
-        if (gf && res >= 0) gf->inside++;
         active = false;
     }
 }
@@ -247,10 +245,9 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
 // sequential walk's result for list [js, je) (used for the few entries the item-parallel
 // form leaves).
 template <int AX>
-__device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __restrict__ pts, uint32_t pid,
-                                              uint32_t js, uint32_t je, const GridSink& sink, GridFilter* gf) {
+__device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float3 o, uint32_t pid, uint32_t js,
+                                              uint32_t je, const GridSink& sink, GridFilter* gf) {
     const float eps = 1e-6f;
-    const float3 o = ld3f(pts, pid);
     const float oa = get3(o, AX);
     float thr = eps;
     int count = 0, first_obj = -1;
@@ -298,20 +295,27 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float* __re
 // the hits the sequential walk's buffer would take in turn, so the result is the same.
 // Entries with more than AG_CHAIN_MAX hits, and those from the first whose hits did not
 // all fit the list on, are walked by grid_walk_one afterwards. A wave's items are
-// independent loads, so no lane idles behind another's long list.
-constexpr uint32_t AG_IQCAP = 256;      // queue entries of one session
-constexpr uint32_t AG_HCAP = 128;       // hits of one session
+// independent loads, so no lane idles behind another's long list. An entry carries its
+// point's origin, so the items read it from LDS and not again from memory.
+constexpr uint32_t AG_SESSION = 256;    // rays of one session (four per lane)
+constexpr uint32_t AG_IQCAP = 128;      // queue entries of one items pass
+constexpr uint32_t AG_HCAP = 128;       // hits of one items pass
 constexpr uint32_t AG_CHAIN_MAX = 16;   // hits one entry's chain takes from the list
+constexpr int32_t AG_DEFER = INT32_MIN; // res mark: queued by a later pass of the session
 
 struct AgItemLds {
     uint32_t pi[AG_IQCAP];     // point
     uint32_t first[AG_IQCAP];  // first record of its list (even)
     uint32_t pre[AG_IQCAP];    // record pairs of the list, then the pairs before it
+    float ox[AG_IQCAP];        // the point's origin
+    float oy[AG_IQCAP];
+    float oz[AG_IQCAP];
     float ht[AG_HCAP];
     uint32_t ho[AG_HCAP];      // original triangle
     int32_t hb[AG_HCAP];       // object
     uint16_t hq[AG_HCAP];      // entry
-    int32_t res[AG_IQCAP];     // answers of the session's points, when it buffers them
+    int32_t res[AG_SESSION];   // per ray of the session: its answer when the session buffers
+                               // them, else -1; AG_DEFER while it waits for a later pass
 };
 
 __device__ __forceinline__ uint32_t ag_wave_incl(uint32_t v, int lane) {
@@ -335,8 +339,8 @@ __device__ __forceinline__ uint32_t ag_hit_lb(const uint16_t* hq, uint32_t n, ui
 }
 
 template <int AX>
-__device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* __restrict__ pts, uint32_t nq,
-                                                  AgItemLds& L, const GridSink& sink, GridFilter* gf) {
+__device__ __forceinline__ void points_grid_items(const AGArgs& G, uint32_t nq, AgItemLds& L, const GridSink& sink,
+                                                  GridFilter* gf) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const float eps = 1e-6f;
@@ -366,7 +370,7 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
             }
             q = lo;
             const uint64_t j = (uint64_t)L.first[q] + 2 * (uint64_t)(i - L.pre[q]);
-            const float3 o = ld3f(pts, L.pi[q]);
+            const float3 o = make_float3(L.ox[q], L.oy[q], L.oz[q]);
             const float4* r = G.rec + AG_REC * j;
             e0 = r[0];
             e1 = r[1];
@@ -448,129 +452,159 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, const float* 
         }
         const uint64_t mf = __ballot(fb);
         if (fb) {
+            // (every lane's reads of this batch precede its writes, and later batches' entries
+            // lie beyond every slot written so far)
             const uint32_t slot = nfb + (uint32_t)__popcll(mf & lt_mask);
+            const float ox = L.ox[q], oy = L.oy[q], oz = L.oz[q];
             L.pi[slot] = pid;
             L.first[slot] = js;
             L.pre[slot] = je;
+            L.ox[slot] = ox;
+            L.oy[slot] = oy;
+            L.oz[slot] = oz;
         }
         nfb += (uint32_t)__popcll(mf);
     }
     if (nfb) {
         __syncthreads();
         for (uint32_t k = lane; k < nfb; k += 64)
-            grid_walk_one<AX>(G, pts, L.pi[k], L.first[k], L.pre[k], sink, gf);
+            grid_walk_one<AX>(G, make_float3(L.ox[k], L.oy[k], L.oz[k]), L.pi[k], L.first[k], L.pre[k], sink, gf);
     }
 }
 
-// Queue capacity of the two-phase grid form (points; 6 KB of LDS with their ranges).
-constexpr uint32_t AG_QCAP = 512;
-
-// Point rays [lo, hi) on the grid in two alternating phases. Phase 1 streams the points 256
-// at a time (four independent loads per lane in flight): origin, with FILT the dataset-box
-// filter of gf.box, cell, occupancy bit and the cell-table entries; points that fail the
-// filter, fall in an empty cell or lie above the tile's or the cell's highest box top are
-// answered -1 at once, the others are queued (qpi: point, qrg: record range; AG_QCAP entries of LDS; with
-// ITEMS the queue is L's, AG_IQCAP entries).
-// Ray r of the list is point imap[r] (r itself without imap); answers go out as out_kind.
-// Phase 2 walks the queue (ITEMS: points_grid_items; else lane refill, points_grid_run with
-// PRE) whenever phase 1 could not take another 256 points, and at the end. cand counts filter passes (FILT), gf.inside
-// the points found inside. The whole wave calls it with the same lo, hi.
-template <int AX, bool FILT, bool ITEMS = false>
+// Point rays [lo, hi) on the grid in sessions of AG_SESSION rays (four per lane). Phase 1
+// loads, four independent loads per lane in flight: origin, with FILT the dataset-box
+// filter of gf.box, cell, tile summary, occupancy word (cell bits, tile box top) and, for an
+// occupied cell below the tile's top, the cell-table entries; rays that fail the filter,
+// fall in an empty tile or cell, or lie above the tile's or the cell's highest box top are
+// answered -1 at once, the others are queued in L with their origin and record range.
+// Phase 2 walks the queue (points_grid_items). A session whose queue would exceed
+// AG_IQCAP marks the rest AG_DEFER and queues them in further passes (each pass reloads only
+// its marked rays). Ray r of the list is point imap[r] (r itself without imap); answers go
+// out as out_kind, over a contiguous run (no imap) gathered in L.res and written as whole
+// lines at the session's end. cand counts filter passes (FILT), gf.inside the points found
+// inside. The whole wave calls it with the same lo, hi.
+template <int AX, bool FILT>
 __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const float* __restrict__ pts,
                                                       const uint32_t* __restrict__ imap, uint64_t lo, uint64_t hi,
-                                                      int out_kind, void* __restrict__ out, uint32_t* qpi,
-                                                      uint2* qrg, uint32_t& cand, GridFilter& gf,
-                                                      TraceCounters& ctr, AgItemLds* L = nullptr) {
+                                                      int out_kind, void* __restrict__ out, uint32_t& cand,
+                                                      GridFilter& gf, AgItemLds& L) {
     constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
-    constexpr uint32_t QCAP = ITEMS ? AG_IQCAP : AG_QCAP;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // ITEMS over a contiguous run (no imap): one 256-point round per session, answers
-    // gathered in LDS and written out as whole lines at the session's end
-    const bool buf = ITEMS && imap == nullptr;
-    uint64_t base = lo;
-    while (true) {
-        uint32_t nq = 0;
-        const uint64_t sbase = base;
-        for (; base < hi && nq + 4 * 64 <= QCAP && !(buf && base != sbase); base += 4 * 64) {
+    const bool buf = imap == nullptr;
+    for (uint64_t sbase = lo; sbase < hi; sbase += AG_SESSION) {
+        const uint64_t send = hi - sbase > AG_SESSION ? sbase + AG_SESSION : hi;
+        const GridSink sink{out_kind, out, buf ? L.res : nullptr, (uint32_t)sbase};
+        bool first_pass = true;
+        while (true) {
             float3 o[4];
-            bool ok[4];
+            bool act[4], ok[4];
             uint32_t cell[4], pid[4];
-            uint2 rg[4], ts[4], oc[4];
+            uint2 ts[4], oc[4];
+            // Every load below is issued by every lane, inactive ones at an index known to be
+            // valid (the session's first ray, tile group and occupied tile 0, cell 0) with the
+            // value then ignored: a load under a branch is waited for inside that branch, which
+            // would serialise the four rays' loads of each step.
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint64_t r = base + 64 * k + lane;
-                ok[k] = r < hi;
-                pid[k] = ok[k] ? (imap ? imap[r] : (uint32_t)r) : 0u;
-                o[k] = ok[k] ? ld3f(pts, pid[k]) : make_float3(0.0f, 0.0f, 0.0f);
+                const uint64_t r = sbase + 64 * k + lane;
+                act[k] = r < send && (first_pass || L.res[r - sbase] == AG_DEFER);
+                const uint64_t rr = act[k] ? r : sbase;
+                pid[k] = imap ? imap[rr] : (uint32_t)rr;
+                o[k] = ld3f(pts, pid[k]);
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                if (FILT) {
-                    ok[k] = ok[k] && box_filter_pass<AX>(o[k], gf.box);
-                    cand += ok[k] ? 1u : 0u;
-                }
+                // a deferred ray passed the filter in the first pass (and was counted there)
+                ok[k] = act[k] && (!FILT || !first_pass || box_filter_pass<AX>(o[k], gf.box));
+                if (FILT && first_pass) cand += ok[k] ? 1u : 0u;
                 const int cu = ag_cell(get3(o[k], U), G.lo_u, G.inv_u, G.nu);
                 const int cw = ag_cell(get3(o[k], W), G.lo_w, G.inv_w, G.nw);
                 cell[k] = ag_cell_id(cu, cw, G.ntu);
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) ts[k] = ok[k] ? G.tsum[cell[k] >> 10] : make_uint2(0u, 0u);
+            for (int k = 0; k < 4; ++k) ts[k] = G.tsum[cell[k] >> 10];  // (cells are clamped into the grid)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ag_pin(ts[k].x);
+                ag_pin(ts[k].y);
+                if (!ok[k]) ts[k] = make_uint2(0u, 0u);
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int64_t tr = ag_tile_rank(ts[k], cell[k] >> 5);
-                oc[k] = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
+                oc[k] = G.occ[tr >= 0 ? tr : 0];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                // an occupied cell, and the point not above the tile's highest box top
-                ok[k] = ((oc[k].x >> (cell[k] & 31)) & 1u) && !(__uint_as_float(oc[k].y) < get3(o[k], AX));
-                uint2 c0 = make_uint2(0u, 0u), c1 = make_uint2(0u, 0u);
-                if (ok[k]) {
-                    c0 = G.cells[cell[k]];
-                    c1 = G.cells[cell[k] + 1];
-                }
-                // a point above the cell's highest box top has nothing to test (the walk's
-                // exit test on the first record)
-                ok[k] = ok[k] && !(__uint_as_float(c0.y) < get3(o[k], AX));
-                rg[k] = make_uint2(c0.x, c1.x);
+                ag_pin(oc[k].x);
+                ag_pin(oc[k].y);
+                if (ag_tile_rank(ts[k], cell[k] >> 5) < 0) oc[k] = make_uint2(0u, 0u);
             }
+            // queue the points in an occupied cell that do not lie above the tile's highest box
+            // top, with their origins (which are then no longer held in registers)
+            uint32_t nq = 0, slot[4];
+            float oa[4];
+            bool deferred = false;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint64_t r = base + 64 * k + lane;
-                if (r < hi && !ok[k]) {
-                    if (buf) L->res[r - sbase] = -1;
-                    else grid_write_result(out_kind, out, pid[k], -1);
+                oa[k] = get3(o[k], AX);
+                ok[k] = ((oc[k].x >> (cell[k] & 31)) & 1u) && !(__uint_as_float(oc[k].y) < oa[k]);
+                const uint32_t rs = 64u * (uint32_t)k + (uint32_t)lane;
+                if (act[k] && !ok[k]) {
+                    L.res[rs] = -1;
+                    if (!buf) grid_write_result(out_kind, out, pid[k], -1);
                 }
                 const uint64_t m = __ballot(ok[k]);
+                slot[k] = nq + (uint32_t)__popcll(m & lt_mask);
                 if (ok[k]) {
-                    const uint32_t slot = nq + (uint32_t)__popcll(m & lt_mask);
-                    if (ITEMS) {
-                        L->pi[slot] = pid[k];
-                        L->first[slot] = rg[k].x;
-                        L->pre[slot] = (rg[k].y - rg[k].x) >> 1;
+                    if (slot[k] < AG_IQCAP) {
+                        L.pi[slot[k]] = pid[k];
+                        L.ox[slot[k]] = o[k].x;
+                        L.oy[slot[k]] = o[k].y;
+                        L.oz[slot[k]] = o[k].z;
+                        L.res[rs] = -1;  // the walk writes the answer (buffered or out)
                     } else {
-                        qpi[slot] = pid[k];
-                        qrg[slot] = rg[k];
+                        L.res[rs] = AG_DEFER;
+                        deferred = true;
+                        ok[k] = false;
                     }
                 }
                 nq += (uint32_t)__popcll(m);
             }
-        }
-        __syncthreads();
-        if (ITEMS) {
-            const GridSink sink{out_kind, out, buf ? L->res : nullptr, (uint32_t)sbase};
-            points_grid_items<AX>(G, pts, nq, *L, sink, &gf);
-            if (buf) {
-                __syncthreads();
-                const uint32_t cnt = (uint32_t)((base < hi ? base : hi) - sbase);
-                for (uint32_t k = lane; k < cnt; k += 64) grid_write_result(out_kind, out, (uint32_t)sbase + k, L->res[k]);
+            // their record ranges; a point above its cell's highest box top (the walk's exit
+            // test on the first record) gets an empty list
+            uint2 c0[4];
+            uint32_t c1[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t c = ok[k] ? cell[k] : 0u;
+                c0[k] = G.cells[c];
+                c1[k] = G.cells[c + 1].x;
             }
-        } else {
-            points_grid_run<AX, false, true>(G, pts, qpi, 0, nq, out_kind, out, ctr, &gf, qrg);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ag_pin(c0[k].x);
+                ag_pin(c0[k].y);
+                ag_pin(c1[k]);
+                if (ok[k]) {
+                    const bool below = !(__uint_as_float(c0[k].y) < oa[k]);
+                    L.first[slot[k]] = c0[k].x;
+                    L.pre[slot[k]] = below ? (c1[k] - c0[k].x) >> 1 : 0u;
+                }
+            }
+            __syncthreads();
+            points_grid_items<AX>(G, nq < AG_IQCAP ? nq : AG_IQCAP, L, sink, &gf);
+            __syncthreads();
+            if (!__ballot(deferred)) break;
+            first_pass = false;
         }
-        __syncthreads();
-        if (base >= hi) break;
+        if (buf) {
+            const uint32_t cnt = (uint32_t)(send - sbase);
+            for (uint32_t k = lane; k < cnt; k += 64) grid_write_result(out_kind, out, (uint32_t)sbase + k, L.res[k]);
+            __syncthreads();
+        }
     }
 }
 

@@ -253,8 +253,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid(AGArgs G, const float*
         __shared__ AgItemLds lds;
         uint32_t cand = 0;
         GridFilter gf{};
-        points_grid_two_phase<AX, false, true>(G, pts, imap, lo, hi, out_kind, out, nullptr, nullptr, cand, gf, ctr,
-                                               &lds);
+        points_grid_two_phase<AX, false>(G, pts, imap, lo, hi, out_kind, out, cand, gf, lds);
     }
 }
 
@@ -271,10 +270,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
     const uint64_t hi = lo + per_wave < n ? lo + per_wave : n;
     const int lane = threadIdx.x & 63;
     uint32_t cand = 0;
-    TraceCounters ctr;
     GridFilter gf{box, 0u};
-    points_grid_two_phase<AX, true, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf,
-                                          ctr, &lds);
+    points_grid_two_phase<AX, true>(G, pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, cand, gf, lds);
     unsigned long long v = ((unsigned long long)cand << 32) | gf.inside;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(acc_stripe(counts), v);

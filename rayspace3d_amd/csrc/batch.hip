@@ -98,9 +98,9 @@ __device__ __forceinline__ void batch_loop(const BatchDesc* __restrict__ qs,
             uint32_t cand = 0;
             GridFilter gf{};
             int32_t* out = static_cast<int32_t*>(q.out);
-            if (ax == 0) points_grid_two_phase<0, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
-            else if (ax == 1) points_grid_two_phase<1, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
-            else points_grid_two_phase<2, false, true>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, nullptr, nullptr, cand, gf, ctr, L);
+            if (ax == 0) points_grid_two_phase<0, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, cand, gf, *L);
+            else if (ax == 1) points_grid_two_phase<1, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, cand, gf, *L);
+            else points_grid_two_phase<2, false>(q.grid, q.pts, nullptr, lo, hi, RS3D_OUT_OBJ_I32, out, cand, gf, *L);
             continue;
         }
         if ((KINDS & 1) && q.kind == BQ_CONTAIN) {

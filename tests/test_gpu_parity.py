@@ -9,7 +9,7 @@ import torch
 import oracle as orc
 from rayspace3d_amd import _lib, gen
 from rayspace3d_amd.device import axis_grid_prepare, axis_grid_stats
-from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, containment_join,
+from rayspace3d_amd.device import (Bvh, DevEdges, FilterRefinePipeline, batch_execute, containment_join,
                                    edge_count, intersection_join, overlap_join, sort_unique_u64,
                                    to_dev, to_host, trace_points, trace_stats)
 
@@ -980,3 +980,39 @@ def test_index_map_launch(ctx, grid):
         b.free()
     finally:
         ctx.set_option(_lib.OPT_AXIS_GRID, saved)
+
+
+def test_grid_sessions_with_deferred_passes(ctx):
+    # nearly every point lies over the sphere and below its top, so a 256-point session
+    # queues far more than the 128 entries of one walk pass: the rest wait for later passes
+    # (contiguous launches, index-map launches, filter-refine and the batch engine)
+    mesh, _ = gen.single_sphere_workload(seed=13, n_points=0, level=5)
+    rng = np.random.Generator(np.random.PCG64(13))
+    n = 150_000
+    pts = np.empty((n, 3), np.float32)
+    pts[:, 0] = rng.uniform(-0.6, 0.6, n)
+    pts[:, 1] = rng.uniform(-0.6, 0.6, n)
+    pts[:, 2] = rng.uniform(-1.5, 0.9, n)
+    ref = orc.points_contain(mesh.vertices, mesh.indices, mesh.tri_to_obj, pts, axis=2)
+    assert 0.3 < (ref >= 0).mean() < 0.9
+    print("dense sessions: single sphere L5, 150k points over the sphere", flush=True)
+    b = Bvh.from_mesh(ctx, mesh)
+    assert axis_grid_prepare(ctx, b, 2)["state"] == 1
+    st = axis_grid_stats(ctx, b, to_dev(pts), axis=2)
+    assert st["nodes_visited"] > 0.9 * n  # cell-table reads: nearly every point walks
+    dp = to_dev(pts)
+    got = to_host(trace_points(ctx, b, dp, axis=2))
+    assert np.array_equal(got, ref), f"contiguous: {np.count_nonzero(got != ref)} differ"
+    perm = rng.permutation(n).astype(np.int32)
+    out = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    got = to_host(trace_points(ctx, b, dp, axis=2, index_map=to_dev(perm), out=out))
+    assert np.array_equal(got, ref), f"index map: {np.count_nonzero(got != ref)} differ"
+    fr = FilterRefinePipeline(ctx).execute(b, dp, axis=2)
+    assert np.array_equal(to_host(fr.final_results), ref)
+    vmin, vmax = mesh.vertices.min(0), mesh.vertices.max(0)
+    assert fr.candidate_count == int(np.all((pts > vmin) & (pts < vmax), axis=1).sum())
+    assert fr.inside_count == int((ref >= 0).sum())
+    bo = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    batch_execute(ctx, [{"kind": "contain", "bvh": b, "points": dp, "out": bo, "axis": 2}])
+    assert np.array_equal(to_host(bo), ref)
+    b.free()
