@@ -5,20 +5,22 @@
 // overlap and intersection: a query dataset against a resident dataset) queue FIFO. Each
 // call of rs3d_server_step is one scheduling iteration: it admits queued requests while
 // a slot is free and their estimated device memory fits the budget (pair tables are
-// sized from the grid-histogram estimate of the dataset pair), runs every admitted
+// sized from the grid-histogram estimate of the dataset pair; the step computes the
+// estimates of the requests submitted since the last one in one launch), runs every admitted
 // request in one persistent launch of the batch engine, and completes them. Requests that
 // arrive while a step runs are admitted at the next iteration. Each request's result is
 // identical to running it alone (the batch engine runs the same per-ray programs and
 // keeps one pair set per request).
 //
 // The memory budget covers what the server keeps resident -- per dataset the first
-// triangle of each object and the grid histogram, and the axis grids and segment
+// triangle of each object, the grid histogram and its non-empty cells, and the axis grids and segment
 // occupancy bits of the datasets' BVHs -- plus the per-step memory of the admitted
 // requests. Grids and bits only speed requests up (results are identical on the BVH
 // walk), so the server builds one only when its upper-bound size fits what the budget
 // leaves, before the step's launch; otherwise the request runs on the walk. The BVHs and
 // mesh arrays themselves belong to the caller and are not counted.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <set>
@@ -38,6 +40,21 @@ struct ServerDataset {
     DevBuf grid;        // GridCell[res^3]
     int res = 0;
     float gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0};
+    // the histogram's non-empty cells (device, increasing index) and its touch-weighted
+    // means (for the estimate's alpha correction): a request's estimate visits only the
+    // non-empty cells of the sparser of its two histograms
+    DevBuf nz;
+    uint64_t nnz = 0;
+    float g_size = 0.0f, g_ratio = 1.0f;
+};
+
+// One pending estimate of rs3d_server_step's batched estimate kernel.
+struct SvEstPair {
+    const GridCellRec* a;  // query dataset's histogram
+    const GridCellRec* b;  // target dataset's histogram
+    const uint32_t* idx;   // non-empty cells of the sparser one
+    uint64_t n;
+    float cell_volume;
 };
 
 struct ServerRequest {
@@ -59,7 +76,7 @@ struct rs3d_server {
     std::vector<rs3d::ServerRequest> reqs;
     std::deque<int64_t> queue;
     rs3d::BatchPlan plan{nullptr};
-    rs3d::DevBuf keys, scratch;
+    rs3d::DevBuf keys, scratch, est;
     uint64_t peak_mem = 0;
     uint64_t steps = 0;
 };
@@ -71,13 +88,94 @@ int batch_keys_to_pairs(rs3d_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_
 
 using namespace rs3d;
 
+__global__ __launch_bounds__(256) void k_sv_nz_flags(const GridCellRec* __restrict__ cells, uint64_t n,
+                                                     uint8_t* __restrict__ flags) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = cells[i].touch > 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_sv_gather(const GridCellRec* __restrict__ cells,
+                                                   const uint32_t* __restrict__ idx, uint64_t n,
+                                                   GridCellRec* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cells[idx[i]];
+}
+
+// The raw estimate (rs3d_estimate_raw's sum) of each pending pair over the listed cells:
+// block (x, y) sums its share of pair y's cells into partial[y * gridDim.x + x].
+__global__ __launch_bounds__(256) void k_sv_estimate(const SvEstPair* __restrict__ pairs, float eps, float gamma,
+                                                     double* __restrict__ partial) {
+    __shared__ double red[256];
+    const SvEstPair P = pairs[blockIdx.y];
+    double s = 0.0;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.n;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = P.idx[k];
+        const GridCellRec a = P.a[i], b = P.b[i];
+        if (a.touch == 0 || b.touch == 0) continue;
+        const float cs = a.avg + b.avg + eps;
+        const float mink = cs * cs * cs;
+        float prob = mink / P.cell_volume;
+        const float ratio = sqrtf(a.ratio * b.ratio);
+        prob *= powf(ratio, gamma);
+        prob = fminf(prob, 1.0f);
+        s += (double)((float)a.touch * (float)b.touch * prob);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// The dataset's non-empty histogram cells (kept on the device) and touch-weighted means.
+static int sparse_histogram(rs3d_ctx* ctx, ServerDataset* d) {
+    const uint64_t nc = (uint64_t)d->res * d->res * d->res;
+    DevBuf tmp;
+    const size_t fl = (nc + 255) & ~(size_t)255;
+    RS3D_TRY(tmp.ensure(fl + nc * sizeof(uint32_t) + nc * sizeof(GridCellRec)));
+    uint8_t* flags = tmp.as<uint8_t>();
+    uint32_t* idx = reinterpret_cast<uint32_t*>(flags + fl);
+    GridCellRec* cells = reinterpret_cast<GridCellRec*>(idx + nc);
+    const unsigned nb = (unsigned)((nc + 255) / 256);
+    k_sv_nz_flags<<<nb, 256, 0, ctx->stream>>>(d->grid.as<GridCellRec>(), nc, flags);
+    int rc = rs3d_compact_flags(ctx, flags, nc, idx, &d->nnz);
+    if (rc == RS3D_OK) rc = d->nz.ensure(std::max<uint64_t>(d->nnz, 1) * sizeof(uint32_t));
+    std::vector<GridCellRec> h(d->nnz);
+    if (rc == RS3D_OK && d->nnz) {
+        k_sv_gather<<<(unsigned)((d->nnz + 255) / 256), 256, 0, ctx->stream>>>(d->grid.as<GridCellRec>(), idx,
+                                                                              d->nnz, cells);
+        hipError_t e = hipMemcpyAsync(d->nz.ptr, idx, d->nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h.data(), cells, d->nnz * sizeof(GridCellRec), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            set_error("server: histogram copy failed: %s", hipGetErrorString(e));
+            rc = RS3D_E_HIP;
+        }
+    }
+    tmp.release();
+    if (rc != RS3D_OK) return rc;
+    double sz = 0.0, ra = 0.0, cnt = 0.0;
+    for (const GridCellRec& c : h) {
+        sz += (double)c.avg * (double)c.touch;
+        ra += (double)c.ratio * (double)c.touch;
+        cnt += (double)c.touch;
+    }
+    d->g_size = cnt > 0 ? (float)(sz / cnt) : 0.0f;
+    d->g_ratio = cnt > 0 ? (float)(ra / cnt) : 1.0f;
+    return RS3D_OK;
+}
+
 // Device bytes resident for the server's datasets: its own per-dataset arrays and the
 // accelerating structures that exist for their BVHs (each BVH counted once).
 static uint64_t resident_bytes(const rs3d_server* s) {
     uint64_t b = 0;
     std::set<const rs3d_bvh*> seen;
     for (const ServerDataset* d : s->ds) {
-        b += d->first_tri.bytes + d->grid.bytes;
+        b += d->first_tri.bytes + d->grid.bytes + d->nz.bytes;
         if (!seen.insert(d->bvh).second) continue;
         for (int ax = 0; ax < 3; ++ax) {
             rs3d_axis_grid_info gi;
@@ -92,6 +190,68 @@ static uint64_t pow2_ge(uint64_t v) {
     uint64_t p = 1024;
     while (p < v) p <<= 1;
     return p;
+}
+
+// Device memory of an overlap or intersection request, from its estimate: its pair table
+// share at <= 1/4 load, compaction and sort; intersection adds its per-object parity state.
+static uint64_t request_mem(const ServerRequest& q, const ServerDataset* a, const ServerDataset* b) {
+    uint64_t m = pow2_ge(4 * (uint64_t)q.estimate) * 8 * 3;
+    if (q.req.kind == RS3D_REQ_INTERSECT) m += (a->mesh.num_objects + b->mesh.num_objects) * 256ull * 5;
+    return m;
+}
+
+// The pending estimates of the first queued requests (up to 4 x max_slots), in one launch:
+// rs3d_estimate_pairs (alpha-corrected; the target's bounds set the cell volume) over the
+// non-empty cells of the sparser histogram of each pair.
+static int estimate_pending(rs3d_server* s) {
+    std::vector<int64_t> pend;
+    std::vector<SvEstPair> pairs;
+    const size_t look = std::min<size_t>({s->queue.size(), (size_t)s->max_slots * 4, (size_t)65535});
+    uint64_t most = 0;
+    for (size_t k = 0; k < look; ++k) {
+        const ServerRequest& q = s->reqs[s->queue[k]];
+        if (q.estimate >= 0) continue;
+        const ServerDataset* a = s->ds[q.req.dataset];
+        const ServerDataset* b = s->ds[q.req.query_dataset];
+        const ServerDataset* sm = a->nnz <= b->nnz ? a : b;
+        const float w = a->gmax[0] - a->gmin[0], h = a->gmax[1] - a->gmin[1], d = a->gmax[2] - a->gmin[2];
+        const float r = (float)a->res;
+        pairs.push_back(SvEstPair{b->grid.as<GridCellRec>(), a->grid.as<GridCellRec>(), sm->nz.as<uint32_t>(),
+                                  sm->nnz, (w / r) * (h / r) * (d / r)});
+        pend.push_back(s->queue[k]);
+        most = std::max(most, sm->nnz);
+    }
+    if (pend.empty()) return RS3D_OK;
+    const unsigned bpp = (unsigned)std::min<uint64_t>(std::max<uint64_t>((most + 8191) / 8192, 1), 64);
+    const size_t pb = (pairs.size() * sizeof(SvEstPair) + 255) & ~(size_t)255;
+    RS3D_TRY(s->est.ensure(pb + pairs.size() * bpp * sizeof(double)));
+    SvEstPair* dpairs = s->est.as<SvEstPair>();
+    double* dpart = reinterpret_cast<double*>(s->est.as<uint8_t>() + pb);
+    RS3D_HIP(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(SvEstPair), hipMemcpyHostToDevice,
+                            s->ctx->stream));
+    k_sv_estimate<<<dim3(bpp, (unsigned)pairs.size()), 256, 0, s->ctx->stream>>>(dpairs, 0.001f, 0.8f, dpart);
+    RS3D_HIP(hipGetLastError());
+    std::vector<double> part(pairs.size() * bpp);
+    RS3D_HIP(hipMemcpyAsync(part.data(), dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+    RS3D_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (size_t k = 0; k < pend.size(); ++k) {
+        ServerRequest& q = s->reqs[pend[k]];
+        const ServerDataset* a = s->ds[q.req.dataset];
+        const ServerDataset* b = s->ds[q.req.query_dataset];
+        double raw = 0.0;
+        for (unsigned j = 0; j < bpp; ++j) raw += part[k * bpp + j];
+        const float rawf = (float)raw;
+        const float e1 = b->g_size * std::cbrt(b->g_ratio), e2 = a->g_size * std::cbrt(a->g_ratio);
+        const float comb = e1 + e2;
+        const float mink = comb * comb * comb;
+        float cv = pairs[k].cell_volume;
+        if (cv < 1e-9f) cv = 1e-9f;
+        float alpha = mink / cv;
+        if (alpha < 1.0f) alpha = 1.0f;
+        q.estimate = std::max<int64_t>((int64_t)(rawf / alpha), 256);
+        q.mem = request_mem(q, a, b);
+    }
+    return RS3D_OK;
 }
 
 extern "C" {
@@ -115,9 +275,11 @@ int rs3d_server_destroy(rs3d_server* s) {
     for (auto* d : s->ds) {
         d->first_tri.release();
         d->grid.release();
+        d->nz.release();
         delete d;
     }
     s->keys.release();
+    s->est.release();
     s->scratch.release();
     s->plan.release();
     delete s;
@@ -156,10 +318,12 @@ int rs3d_server_add_dataset(rs3d_server* s, const rs3d_mesh_view* mesh, const rs
                 d->gmin[k] = b[k];
                 d->gmax[k] = b[3 + k];
             }
+        if (rc == RS3D_OK) rc = sparse_histogram(s->ctx, d);
     }
     if (rc != RS3D_OK) {
         d->first_tri.release();
         d->grid.release();
+        d->nz.release();
         delete d;
         return rc;
     }
@@ -192,18 +356,14 @@ int rs3d_server_submit(rs3d_server* s, const rs3d_request* r, int64_t* ticket) {
             set_error("rs3d_server_submit: overlap and intersection requests need edge payloads");
             return RS3D_E_ARG;
         }
-        // pair table share from the grid-histogram estimate (both grids must exist and match)
-        int64_t est = 0;
+        // pair table share from the grid-histogram estimate (both grids must exist and match);
+        // the next step computes the pending estimates of its queue in one launch
         if (a->res && a->res == b->res) {
-            const uint32_t res[3] = {(uint32_t)a->res, (uint32_t)a->res, (uint32_t)a->res};
-            const uint64_t nc = (uint64_t)a->res * a->res * a->res;
-            RS3D_TRY(rs3d_estimate_pairs(s->ctx, b->grid.ptr, a->grid.ptr, nc, a->gmin, a->gmax, res,
-                                         0.001f, 0.8f, 1, &est));
+            q.estimate = -1;
+        } else {
+            q.estimate = 256;
+            q.mem = request_mem(q, a, b);
         }
-        q.estimate = std::max<int64_t>(est, 256);
-        q.mem = pow2_ge(4 * (uint64_t)q.estimate) * 8 * 3;  // table at <= 1/4 load + compaction + sort
-        if (r->kind == RS3D_REQ_INTERSECT)
-            q.mem += (a->mesh.num_objects + b->mesh.num_objects) * 256ull * 5;
     } else {
         set_error("rs3d_server_submit: unknown request kind %d", r->kind);
         return RS3D_E_ARG;
@@ -222,6 +382,7 @@ int rs3d_server_step(rs3d_server* s, int* admitted, int* completed) {
     if (admitted) *admitted = 0;
     if (completed) *completed = 0;
     if (s->queue.empty()) return RS3D_OK;
+    RS3D_TRY(estimate_pending(s));
     // admission: FIFO, slots and the memory budget (the head request is always admitted).
     // A request whose dataset lacks a grid (containment) or occupancy bits (overlap,
     // intersection) plans their build when the bound of their size fits the budget next to
