@@ -65,7 +65,8 @@ void* rs3d_ctx_stream(rs3d_ctx* ctx);
  * (default 256; 64..2^20), halved down to 64 for launches that would otherwise have fewer
  * than 16,384 runs. RS3D_OPT_GRID_CELL_PCT: cell edge of grids built afterwards,
  * in percent of the triangles' mean extent (default 35; 10..1000; widened while the grid
- * would hold more than 16 records per triangle). Results are unchanged.
+ * would hold more than 16 records per triangle; without a caller's value, grids under 2^19
+ * records take finer cells, down to 10 %, while they stay that small). Results are unchanged.
  * RS3D_OPT_SEG_OCCUPANCY: 1 (default) lets edge rays first test the target's segment
  * occupancy bits (see rs3d_seg_occupancy_prepare); 0 always walks the BVH. Results are
  * unchanged. RS3D_OPT_SEG_CELL_PCT: their cell edge for bits built afterwards, in percent

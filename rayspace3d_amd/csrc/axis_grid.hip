@@ -42,6 +42,8 @@ constexpr uint64_t AG_MAX_CELLS = 1ull << 26;
 constexpr int AG_MAX_DIM = 32768;
 constexpr double AG_MAX_UNIFORM_LIST = 64.0;    // records per cell, averaged over cells
 constexpr double AG_MAX_WEIGHTED_LIST = 256.0;  // averaged over records (dense spots)
+constexpr uint64_t AG_SMALL_RECORDS = 1ull << 19;  // grids this small take finer cells
+constexpr double AG_MIN_CELL_PCT = 10.0;
 
 // Unsigned key whose order is the float order.
 __device__ __forceinline__ uint32_t ag_ord(float f) {
@@ -356,11 +358,13 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     RS3D_TRY(grid_blocks(n, 256, &nb));
     uint64_t total = 0;
     int nu = 1, nw = 1;
-    for (;;) {
+    // the grid of cell edge e: its geometry into G, nu, nw and the records per triangle
+    // into span (total: their sum)
+    auto span_pass = [&](double e) -> int {
         nu = nw = 1;
-        if (h > 0 && std::isfinite(h)) {
-            nu = (int)std::min<double>(std::max<double>(std::ceil(eu / h), 1.0), AG_MAX_DIM);
-            nw = (int)std::min<double>(std::max<double>(std::ceil(ew / h), 1.0), AG_MAX_DIM);
+        if (e > 0 && std::isfinite(e)) {
+            nu = (int)std::min<double>(std::max<double>(std::ceil(eu / e), 1.0), AG_MAX_DIM);
+            nw = (int)std::min<double>(std::max<double>(std::ceil(ew / e), 1.0), AG_MAX_DIM);
         }
         G.lo_u = (float)lo_u;
         G.lo_w = (float)lo_w;
@@ -373,12 +377,35 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         RS3D_TRY(acc_slot(ctx, 0, 0, &acc));
         k_ag_span<AX><<<nb, 256, 0, s>>>(bvh->tris, n, G, span, acc);
         RS3D_HIP(hipGetLastError());
-        RS3D_TRY(acc_read_sum(ctx, acc, &total));
+        return acc_read_sum(ctx, acc, &total);
+    };
+    for (;;) {
+        RS3D_TRY(span_pass(h));
         const bool fine = h < mean_ext;
         const uint64_t budget = (fine ? 16 : 6) * n + (1ull << 20);
         if (total <= budget && total <= 0xffffffffull) break;
         if (!fine || !std::isfinite(h) || !(h > 0)) return RS3D_OK;  // rejected: too many records
         h = std::min(h * 1.5, std::max(mean_ext, h_min));
+    }
+    // A small grid (under AG_SMALL_RECORDS records, a few MB: it stays in the caches) takes
+    // finer cells, in 1.5x steps down to AG_MIN_CELL_PCT of the mean extent, while it stays
+    // that small: its rays then read shorter lists (one 20,480-triangle sphere, 1 M points:
+    // 6.1 M records read at 35 %, 4.5 M at 20 %). Not for an edge the caller set explicitly.
+    if (!ctx->opt_grid_cell_pct_set && total < AG_SMALL_RECORDS) {
+        const double h_floor = std::max(mean_ext * AG_MIN_CELL_PCT / 100.0, h_min);
+        while (h / 1.5 >= h_floor) {
+            const uint64_t prev = total;
+            RS3D_TRY(span_pass(h / 1.5));
+            if (total > AG_SMALL_RECORDS) {
+                RS3D_TRY(span_pass(h));  // back to the last edge that fit
+                if (total != prev) {
+                    set_error("axis grid: span total changed between passes");
+                    return RS3D_E_HIP;
+                }
+                break;
+            }
+            h /= 1.5;
+        }
     }
     if ((double)total / ((double)nu * nw) > AG_MAX_UNIFORM_LIST) return RS3D_OK;  // rejected: lists too long
     const uint64_t nt = (uint64_t)G.ntu * (uint64_t)((nw + 3) / 4), nc = 32 * nt, ng = (nt + 31) / 32;

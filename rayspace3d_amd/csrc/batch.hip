@@ -693,6 +693,7 @@ int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_qu
     p.opt_axis_grid = ctx->opt_axis_grid;
     p.opt_grid_rays_per_wave = ctx->opt_grid_rays_per_wave;
     p.opt_grid_cell_pct = ctx->opt_grid_cell_pct;
+    p.opt_grid_cell_pct_set = ctx->opt_grid_cell_pct_set;
     p.opt_seg_occupancy = ctx->opt_seg_occupancy;
     p.opt_seg_cell_pct = ctx->opt_seg_cell_pct;
     if (hipHostMalloc((void**)&p.pinned, CTX_PINNED_BYTES, hipHostMallocDefault) != hipSuccess) {

@@ -840,6 +840,7 @@ int rs3d_ctx_set_option(rs3d_ctx* ctx, int option, int64_t value) {
             return RS3D_E_ARG;
         }
         ctx->opt_grid_cell_pct = value;
+        ctx->opt_grid_cell_pct_set = true;
         return RS3D_OK;
     }
     if (option == RS3D_OPT_AXIS_GRID) {

@@ -241,6 +241,7 @@ struct rs3d_ctx {
     int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid
     int64_t opt_grid_rays_per_wave = 256;   // run length of grid point launches
     int64_t opt_grid_cell_pct = 35;         // grid cell edge, % of the mean triangle extent
+    bool opt_grid_cell_pct_set = false;     // set by the caller (small grids keep it as given)
     int64_t opt_seg_occupancy = 1;          // edge rays test the target's occupancy bits first
     int64_t opt_seg_cell_pct = 200;         // occupancy cell edge, % of the mean triangle extent
     uint64_t* pinned = nullptr;  // small pinned host buffer for counts
