@@ -156,15 +156,20 @@ def main():
     for _ in range(args.warmup):
         res = pipe.execute(bvh, dP, axis=2, out=out)
     torch.cuda.synchronize()
-    ctx.profile(True)
-    ctx.profile_read()
-    torch.cuda.synchronize()
     ts = time.perf_counter()
     for _ in range(args.steps):
         res = pipe.execute(bvh, dP, axis=2, out=out, sync=False)
     ctx.sync()
     torch.cuda.synchronize()
     te = time.perf_counter()
+    # the kernel's device time: HIP events around each launch, on the library's stream, over
+    # a second run of the same steps (the events cost the host ~10 us per call, so the
+    # throughput steps above run without them)
+    ctx.profile(True)
+    ctx.profile_read()
+    for _ in range(args.steps):
+        pipe.execute(bvh, dP, axis=2, out=out, sync=False)
+    ctx.sync()
     kern_ms, kern_n = ctx.profile_read()
     ctx.profile(False)
     ms_per_step = (te - ts) * 1e3 / args.steps
@@ -357,14 +362,18 @@ def leg_single_sphere(args):
     for _ in range(max(args.warmup, 1)):
         trace_points(ctx, bvh, dP, axis=0, out_kind=_lib.OUT_FLAG_U8, out=out)
     steps = max(args.steps, 20)
-    ctx.profile(True)
-    ctx.profile_read()
     torch.cuda.synchronize()
     ts = time.perf_counter()
     for _ in range(steps):
         trace_points(ctx, bvh, dP, axis=0, out_kind=_lib.OUT_FLAG_U8, out=out, sync=False)
     ctx.sync()
     t_step = (time.perf_counter() - ts) / steps
+    # kernel device time from HIP events, over a second run of the same steps
+    ctx.profile(True)
+    ctx.profile_read()
+    for _ in range(steps):
+        trace_points(ctx, bvh, dP, axis=0, out_kind=_lib.OUT_FLAG_U8, out=out, sync=False)
+    ctx.sync()
     kern_ms, kern_n = ctx.profile_read()
     ctx.profile(False)
     kern_avg = kern_ms / max(kern_n, 1)
