@@ -511,6 +511,11 @@ int rs3d_server_result(rs3d_server* server, int64_t ticket, int* done, uint64_t*
                        uint64_t max_keys, uint64_t* n_keys);
 int rs3d_server_stats(rs3d_server* server, uint64_t* queued, uint64_t* peak_memory,
                       uint64_t* steps);
+/* The grid-histogram pair estimate that sized an overlap or intersection request's table
+ * (rs3d_estimate_pairs with alpha correction over the two datasets' histograms, the target's
+ * bounds setting the cell volume; at least 256): -1 until a step has computed it, 0 for a
+ * containment request. No reference counterpart (for monitoring and tests). */
+int rs3d_server_request_estimate(rs3d_server* server, int64_t ticket, int64_t* estimate);
 
 /* ---- selectivity estimation (replaces estimateOverlapSelectivity,
  *      query/src/cuda/estimated_overlap.cu:43-77) --------------------------------------

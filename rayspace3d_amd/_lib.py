@@ -238,6 +238,7 @@ SIGNATURES = {
     "rs3d_server_step": (I32, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "rs3d_server_result": (I32, [P, C.c_int64, C.POINTER(C.c_int), P, U64, PU64]),
     "rs3d_server_stats": (I32, [P, PU64, PU64, PU64]),
+    "rs3d_server_request_estimate": (I32, [P, C.c_int64, C.POINTER(C.c_int64)]),
     "rs3d_estimate_raw": (I32, [P, P, P, U64, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_double)]),
     "rs3d_grid_stats": (I32, [P, P, U64, P, U64, P, U64, I32, C.c_float, P, C.POINTER(C.c_float)]),
     "rs3d_estimate_pairs": (I32, [P, P, P, U64, C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -511,6 +511,15 @@ int rs3d_server_result(rs3d_server* s, int64_t ticket, int* done, uint64_t* keys
     return RS3D_OK;
 }
 
+int rs3d_server_request_estimate(rs3d_server* s, int64_t ticket, int64_t* estimate) {
+    if (!s || ticket < 0 || ticket >= (int64_t)s->reqs.size() || !estimate) {
+        set_error("rs3d_server_request_estimate: bad arguments");
+        return RS3D_E_ARG;
+    }
+    *estimate = s->reqs[ticket].estimate;
+    return RS3D_OK;
+}
+
 int rs3d_server_stats(rs3d_server* s, uint64_t* queued, uint64_t* peak_memory, uint64_t* steps) {
     if (!s) {
         set_error("rs3d_server_stats: null server");

@@ -602,6 +602,13 @@ class Server:
                   "rs3d_server_result")
         return done.value, keys[:n.value]
 
+    def estimate(self, ticket: int) -> int:
+        """The pair estimate that sized an overlap/intersection request (-1 before a step)."""
+        e = C.c_int64()
+        check(self.ctx.lib.rs3d_server_request_estimate(self.handle, ticket, C.byref(e)),
+              "rs3d_server_request_estimate")
+        return e.value
+
     def stats(self):
         q, m, s = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(self.ctx.lib.rs3d_server_stats(self.handle, C.byref(q), C.byref(m), C.byref(s)),

@@ -1016,3 +1016,33 @@ def test_grid_sessions_with_deferred_passes(ctx):
     batch_execute(ctx, [{"kind": "contain", "bvh": b, "points": dp, "out": bo, "axis": 2}])
     assert np.array_equal(to_host(bo), ref)
     b.free()
+
+
+def test_server_estimates_match_oracle(ctx):
+    # the server computes the pending requests' pair estimates in one launch at the step's
+    # start, over the non-empty cells of the sparser histogram: rs3d_estimate_pairs' formula
+    # (alpha-corrected, the target's bounds set the cell volume), up to the float rounding of
+    # the device-built histograms (tolerance: sizing only)
+    from rayspace3d_amd.device import Server
+    a = gen.overlap_dataset(3, n_objects=3000, world=100.0, levels=(2,))
+    b = gen.overlap_dataset(4, n_objects=3000, world=100.0, levels=(2,))
+    c = gen.overlap_dataset(5, n_objects=20, world=100.0, levels=(2, 3))
+    for m in (a, b, c):
+        m.edges = gen.extract_edges(m)
+    bvhs = [Bvh.from_mesh(ctx, m) for m in (a, b, c)]
+    srv = Server(ctx, max_slots=8)
+    ids = [srv.add_dataset(bv, DevEdges(m.edges), grid_res=16) for bv, m in zip(bvhs, (a, b, c))]
+    reqs = [(_lib.REQ_OVERLAP, 0, 1), (_lib.REQ_INTERSECT, 1, 0), (_lib.REQ_OVERLAP, 0, 2), (_lib.REQ_OVERLAP, 2, 2)]
+    tickets = [srv.submit(k, ids[d], ids[q]) for k, d, q in reqs]
+    assert all(srv.estimate(t) == -1 for t in tickets)
+    srv.step()
+    grids = [orc.grid_stats(m.vertices, m.indices, m.tri_to_obj, 16) for m in (a, b, c)]
+    for t, (_, d, q) in zip(tickets, reqs):
+        cd, mn, mx = grids[d]
+        ref = max(orc.estimate_pairs(grids[q][0], cd, mn, mx, [16, 16, 16]), 256)
+        est = srv.estimate(t)
+        assert abs(est - ref) <= max(2, 1e-3 * ref), (d, q, est, ref)
+    assert orc.estimate_pairs(grids[1][0], grids[0][0], grids[0][1], grids[0][2], [16, 16, 16]) > 1000
+    srv.close()
+    for bv in bvhs:
+        bv.free()
