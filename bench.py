@@ -206,9 +206,9 @@ def main():
     if use_grid:
         # the one-pass kernel's bytes: 12 B origin + 4 B answer per point, then per filter
         # candidate 8 B tile summary, 16 B per cell-table read (two 8 B entries, occupied
-        # cells) and 48 B per record read (counted on the candidates)
+        # cells) and 40 B per record read (counted on the candidates)
         gst = axis_grid_stats(ctx, bvh, to_dev(pts[cand]), axis=2)
-        alg_bytes = 16 * len(pts) + 8 * gst["rays"] + 16 * gst["nodes_visited"] + 48 * gst["tris_tested"]
+        alg_bytes = 16 * len(pts) + 8 * gst["rays"] + 16 * gst["nodes_visited"] + 40 * gst["tris_tested"]
         bytes_per_ray = alg_bytes / len(pts)
         kernel_name = "k_points_grid_filtered (filter + refine, axis grid)"
     else:
@@ -381,9 +381,9 @@ def leg_single_sphere(args):
     use_grid = grid["state"] == 1
     if use_grid:
         # 12 B origin + 1 B flag + 8 B tile summary per ray, 16 B per cell-table read,
-        # 48 B per record read
+        # 40 B per record read
         g = axis_grid_stats(ctx, bvh, dP, axis=0)
-        alg = 21 * len(pts) + 16 * g["nodes_visited"] + 48 * g["tris_tested"]
+        alg = 21 * len(pts) + 16 * g["nodes_visited"] + 40 * g["tris_tested"]
         kernel = "k_points_grid (axis grid, +x)"
     else:
         g = None

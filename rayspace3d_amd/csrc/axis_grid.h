@@ -26,27 +26,38 @@ struct AxisGrid {
     uint2* cells = nullptr;     // tiled cells + 1 entries: first record, box top of that record
     uint2* tsum = nullptr;      // per 32 tiles: occupied-tile bits, occupied tiles before
     uint2* occ = nullptr;       // per occupied tile: its 32 cell bits, highest box top (bits)
-    float4* rec = nullptr;      // AG_REC float4 per record
+    float* rec = nullptr;       // AG_RECF floats per record
+    int32_t* obj = nullptr;     // object of each original triangle
     size_t bytes = 0;
     double expected = 0.0;
     double build_ms = 0.0;
 };
 
-// A record is three float4: (v0, original triangle index), (v1, object id), (v2, upper end
-// of the triangle's padded box along the axis) -- 48 bytes.
-constexpr int AG_REC = 3;
+// A record is ten floats: v0, v1, v2 and the original triangle index (its bits) -- 40
+// bytes, so a record pair at an even index is 80 bytes, five 16-byte loads. A list is
+// ordered by its triangles' largest vertex coordinate along the axis ("top"), largest
+// first: a ray whose origin lies above a triangle's top cannot hit it (every vertex is at or
+// below the origin, so T <= 0 and t <= 0 in tri_hit_axis), hence nothing from the first
+// such record on. The object of a triangle is looked up (obj) only for a point found inside.
+constexpr int AG_RECF = 10;
+
+struct AgRec {
+    float3 v0, v1, v2;
+    uint32_t orig;
+};
 
 struct AGArgs {
     const uint2* cells;
     const uint2* tsum;
     const uint2* occ;
-    const float4* rec;
+    const float* rec;
+    const int32_t* obj;
     float lo_u, lo_w, inv_u, inv_w;
     int nu, nw, ntu;
 };
 
 inline AGArgs ag_args(const AxisGrid* g) {
-    return AGArgs{g->cells, g->tsum, g->occ, g->rec, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw, g->ntu};
+    return AGArgs{g->cells, g->tsum, g->occ, g->rec, g->obj, g->lo_u, g->lo_w, g->inv_u, g->inv_w, g->nu, g->nw, g->ntu};
 }
 
 __host__ __device__ __forceinline__ uint32_t ag_cell_id(int cu, int cw, int ntu) {
@@ -93,16 +104,52 @@ __device__ __forceinline__ void grid_write_result(int out_kind, void* __restrict
     }
 }
 
-// One grid record against a point ray: the triangle's box ends at or above the origin and
-// the triangle is hit at t in [thr, 1e16). (The box's extent in the plane is not kept: the
-// padded box contains every hit, so testing it first could only skip triangle tests.)
+// Record j (8-byte aligned: five 8-byte loads).
+__device__ __forceinline__ AgRec ag_load_one(const float* __restrict__ rec, uint64_t j) {
+    const float2* p = reinterpret_cast<const float2*>(rec + AG_RECF * j);
+    const float2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    AgRec r;
+    r.v0 = make_float3(a.x, a.y, b.x);
+    r.v1 = make_float3(b.y, c.x, c.y);
+    r.v2 = make_float3(d.x, d.y, e.x);
+    r.orig = __float_as_uint(e.y);
+    return r;
+}
+
+// Records j and j + 1 for an even j (16-byte aligned: five 16-byte loads).
+__device__ __forceinline__ void ag_load_pair(const float* __restrict__ rec, uint64_t j, AgRec& r0, AgRec& r1) {
+    const float4* p = reinterpret_cast<const float4*>(rec + AG_RECF * j);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    r0.v0 = make_float3(a.x, a.y, a.z);
+    r0.v1 = make_float3(a.w, b.x, b.y);
+    r0.v2 = make_float3(b.z, b.w, c.x);
+    r0.orig = __float_as_uint(c.y);
+    r1.v0 = make_float3(c.z, c.w, d.x);
+    r1.v1 = make_float3(d.y, d.z, d.w);
+    r1.v2 = make_float3(e.x, e.y, e.z);
+    r1.orig = __float_as_uint(e.w);
+}
+
+// The record's top: its largest vertex coordinate along the axis (NaN only where every
+// coordinate is NaN, as in the pad record).
 template <int AX>
-__device__ __forceinline__ bool grid_rec_hit(const float3 o, const float4& a0, const float4& a1, const float4& a2,
-                                             float thr, float& t) {
-    if (a2.w < get3(o, AX)) return false;
-    return tri_hit_axis<AX>(o, make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                            make_float3(a2.x, a2.y, a2.z), t) &&
-           t >= thr && t < 1e16f;
+__device__ __forceinline__ float ag_top(const AgRec& r) {
+    return fmaxf(fmaxf(get3(r.v0, AX), get3(r.v1, AX)), get3(r.v2, AX));
+}
+
+// One grid record against a point ray: the triangle reaches the origin's height and is hit
+// at t in [thr, 1e16). (The record keeps no box: the padded box contains every hit, so
+// testing it first could only skip triangle tests.)
+template <int AX>
+__device__ __forceinline__ bool grid_rec_hit(const float3 o, const AgRec& r, float thr, float& t) {
+    if (ag_top<AX>(r) < get3(o, AX)) return false;
+    return tri_hit_axis<AX>(o, r.v0, r.v1, r.v2, t) && t >= thr && t < 1e16f;
+}
+
+// The answer of a point whose chain counted `count` hits, the first on original triangle
+// first_orig: that triangle's object when the count is odd, else -1.
+__device__ __forceinline__ int ag_answer(const AGArgs& G, int count, int first_orig) {
+    return (count & 1) ? G.obj[(uint32_t)first_orig] : -1;
 }
 
 // Where the item-parallel form puts a point's answer: the session's LDS answers when it
@@ -137,7 +184,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
     uint32_t pi = 0, j = 0, je = 0, js = 0;
     float3 o = make_float3(0.0f, 0.0f, 0.0f);
     float thr = eps;
-    int count = 0, first_obj = -1;
+    int count = 0, first_orig = -1;
     uint32_t total = 0;
     float bt[KBUF_GRID];
     uint32_t bi[KBUF_GRID];
@@ -170,7 +217,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 j = js;
                 thr = eps;
                 count = 0;
-                first_obj = -1;
+                first_orig = -1;
                 total = 0;
                 kbuf_clear(bt, bi, bo);
                 active = true;
@@ -181,39 +228,38 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
         if (!active) continue;
         const float oa = get3(o, AX);
         if (j < je) {
+            // (lists have even lengths and start at even records: j is even here)
             const bool two = j + 1 < je;
-            const uint64_t b0 = AG_REC * (uint64_t)j, b1 = AG_REC * (uint64_t)(two ? j + 1 : j);
-            const float4 e0 = G.rec[b0], e1 = G.rec[b0 + 1], e2 = G.rec[b0 + 2];
-            const float4 f0 = G.rec[b1], f1 = G.rec[b1 + 1], f2 = G.rec[b1 + 2];
-            // one record
-            auto test = [&](const float4& a0, const float4& a1, const float4& a2) {
+            AgRec e, f;
+            if (two) ag_load_pair(G.rec, j, e, f);
+            else e = f = ag_load_one(G.rec, j);
+            // one record (the buffer keeps the original triangle as the hit's tag)
+            auto test = [&](const AgRec& r) {
                 float t;
-                if (grid_rec_hit<AX>(o, a0, a1, a2, thr, t)) {
-                    const uint32_t orig = (uint32_t)__float_as_int(a0.w);
+                if (grid_rec_hit<AX>(o, r, thr, t)) {
                     ++total;
                     if (STATS) ctr.hits++;
-                    if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
-                        kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
+                    if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && r.orig < bi[KBUF_GRID - 1]))
+                        kbuf_insert(bt, bi, bo, t, r.orig, (int)r.orig);
                 }
             };
-            // records are ordered by box top: from the first that ends below the origin on,
-            // none can be hit
+            // records are ordered by top: from the first below the origin on, none can be hit
             if (STATS) ctr.tris++;
-            if (e2.w < oa) {
+            if (ag_top<AX>(e) < oa) {
                 j = je;
                 continue;
             }
-            test(e0, e1, e2);
+            test(e);
             if (!two) {
                 ++j;
                 continue;
             }
             if (STATS) ctr.tris++;
-            if (f2.w < oa) {
+            if (ag_top<AX>(f) < oa) {
                 j = je;
                 continue;
             }
-            test(f0, f1, f2);
+            test(f);
             j += 2;
             continue;
         }
@@ -221,7 +267,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
 #pragma unroll
         for (int k = 0; k < KBUF_GRID; ++k) {
             if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
-                if (count == 0) first_obj = bo[k];
+                if (count == 0) first_orig = bo[k];
                 ++count;
                 float nthr = bt[k] + eps;
                 if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
@@ -235,8 +281,7 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
             if (STATS) ctr.restarts++;
             continue;
         }
-        const int res = (count & 1) ? first_obj : -1;
-        grid_write_result(out_kind, out, pi, res);
+        grid_write_result(out_kind, out, pi, ag_answer(G, count, first_orig));
         active = false;
     }
 }
@@ -250,7 +295,7 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float3 o, u
     const float eps = 1e-6f;
     const float oa = get3(o, AX);
     float thr = eps;
-    int count = 0, first_obj = -1;
+    int count = 0, first_orig = -1;
     float bt[KBUF_GRID];
     uint32_t bi[KBUF_GRID];
     int bo[KBUF_GRID];
@@ -258,21 +303,19 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float3 o, u
         kbuf_clear(bt, bi, bo);
         uint32_t total = 0;
         for (uint32_t j = js; j < je; ++j) {
-            const uint64_t b = AG_REC * (uint64_t)j;
-            const float4 a0 = G.rec[b], a1 = G.rec[b + 1], a2 = G.rec[b + 2];
-            if (a2.w < oa) break;
+            const AgRec r = ag_load_one(G.rec, j);
+            if (ag_top<AX>(r) < oa) break;
             float t;
-            if (grid_rec_hit<AX>(o, a0, a1, a2, thr, t)) {
-                const uint32_t orig = (uint32_t)__float_as_int(a0.w);
+            if (grid_rec_hit<AX>(o, r, thr, t)) {
                 ++total;
-                if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && orig < bi[KBUF_GRID - 1]))
-                    kbuf_insert(bt, bi, bo, t, orig, __float_as_int(a1.w));
+                if (t < bt[KBUF_GRID - 1] || (t == bt[KBUF_GRID - 1] && r.orig < bi[KBUF_GRID - 1]))
+                    kbuf_insert(bt, bi, bo, t, r.orig, (int)r.orig);
             }
         }
 #pragma unroll
         for (int k = 0; k < KBUF_GRID; ++k) {
             if ((uint32_t)k < total && count < 10000 && bt[k] >= thr) {
-                if (count == 0) first_obj = bo[k];
+                if (count == 0) first_orig = bo[k];
                 ++count;
                 float nthr = bt[k] + eps;
                 if (nthr <= bt[k]) nthr = nextafterf(bt[k], INFINITY);
@@ -281,7 +324,7 @@ __device__ __forceinline__ void grid_walk_one(const AGArgs& G, const float3 o, u
         }
         if (!(total > (uint32_t)KBUF_GRID && count < 10000)) break;
     }
-    const int res = (count & 1) ? first_obj : -1;
+    const int res = ag_answer(G, count, first_orig);
     sink.put(pid, res);
     if (gf && res >= 0) gf->inside++;
 }
@@ -312,7 +355,6 @@ struct AgItemLds {
     float oz[AG_IQCAP];
     float ht[AG_HCAP];
     uint32_t ho[AG_HCAP];      // original triangle
-    int32_t hb[AG_HCAP];       // object
     uint16_t hq[AG_HCAP];      // entry
     int32_t res[AG_SESSION];   // per ray of the session: its answer when the session buffers
                                // them, else -1; AG_DEFER while it waits for a later pass
@@ -359,7 +401,7 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, uint32_t nq, 
         bool h0 = false, h1 = false;
         float t0 = 0.0f, t1 = 0.0f;
         uint32_t q = 0;
-        float4 e0, e1, f0, f1;
+        AgRec e, f;
         if (i < items) {
             // the last entry whose first item is at or before i
             uint32_t lo = 0, hi = nq;
@@ -371,29 +413,21 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, uint32_t nq, 
             q = lo;
             const uint64_t j = (uint64_t)L.first[q] + 2 * (uint64_t)(i - L.pre[q]);
             const float3 o = make_float3(L.ox[q], L.oy[q], L.oz[q]);
-            const float4* r = G.rec + AG_REC * j;
-            e0 = r[0];
-            e1 = r[1];
-            const float4 e2 = r[2];
-            f0 = r[3];
-            f1 = r[4];
-            const float4 f2 = r[5];
-            h0 = grid_rec_hit<AX>(o, e0, e1, e2, eps, t0);
-            h1 = grid_rec_hit<AX>(o, f0, f1, f2, eps, t1);
+            ag_load_pair(G.rec, j, e, f);
+            h0 = grid_rec_hit<AX>(o, e, eps, t0);
+            h1 = grid_rec_hit<AX>(o, f, eps, t1);
         }
         const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
         const uint32_t p0 = hpos + (uint32_t)__popcll(m0 & lt_mask) + (uint32_t)__popcll(m1 & lt_mask);
         const uint32_t p1 = p0 + (h0 ? 1u : 0u);
         if (h0 && p0 < AG_HCAP) {
             L.ht[p0] = t0;
-            L.ho[p0] = (uint32_t)__float_as_int(e0.w);
-            L.hb[p0] = __float_as_int(e1.w);
+            L.ho[p0] = e.orig;
             L.hq[p0] = (uint16_t)q;
         }
         if (h1 && p1 < AG_HCAP) {
             L.ht[p1] = t1;
-            L.ho[p1] = (uint32_t)__float_as_int(f0.w);
-            L.hb[p1] = __float_as_int(f1.w);
+            L.ho[p1] = f.orig;
             L.hq[p1] = (uint16_t)q;
         }
         // the entry of the first hit that did not fit
@@ -420,12 +454,11 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, uint32_t nq, 
         const bool fb = valid && (q >= qdrop || s1 - s0 > AG_CHAIN_MAX);
         if (valid && !fb) {
             float thr = eps;
-            int count = 0, first_obj = -1;
+            int count = 0, first_orig = -1;
             while (true) {
                 // the smallest (t, triangle) at or above thr
                 float bt = INFINITY;
                 uint32_t bi = 0xffffffffu;
-                int bo = -1;
                 bool found = false;
                 for (uint32_t k = s0; k < s1; ++k) {
                     const float t = L.ht[k];
@@ -434,19 +467,18 @@ __device__ __forceinline__ void points_grid_items(const AGArgs& G, uint32_t nq, 
                         if (!found || t < bt || (t == bt && tri < bi)) {
                             bt = t;
                             bi = tri;
-                            bo = L.hb[k];
                             found = true;
                         }
                     }
                 }
                 if (!found) break;
-                if (count == 0) first_obj = bo;
+                if (count == 0) first_orig = (int)bi;
                 ++count;
                 float nthr = bt + eps;
                 if (nthr <= bt) nthr = nextafterf(bt, INFINITY);
                 thr = nthr;
             }
-            const int res = (count & 1) ? first_obj : -1;
+            const int res = ag_answer(G, count, first_orig);
             sink.put(pid, res);
             if (gf && res >= 0) gf->inside++;
         }

@@ -4,21 +4,21 @@
 // point of the perpendicular plane. The grid covers that plane (the BVH's padded bounds)
 // with square cells; cell c lists every triangle whose padded box -- the BVH's leaf box,
 // computed by the same prim_padded_box -- overlaps c in the plane, as a self-contained
-// 48-byte record (vertices, original index, object and the box's upper end along the axis),
-// ordered by that upper end, largest first. A ray reads its cell's entry in the cell table
-// (first record, and that record's box top, so that a ray above it reads no record at all)
-// and tests the triangles of the records whose box does not end below its origin. They
-// include every triangle whose leaf box the BVH walk would accept (ancestor boxes contain
-// the leaf boxes); the padding guarantees that no triangle outside its box can be hit, so
-// the hits are the BVH walk's, and the hit buffer, restarts and chain are the BVH kernel's:
-// the results are the same. The gain is the walk: one range lookup and a few
-// records instead of ~30 dependent node fetches.
+// 40-byte record (vertices and original index), ordered by the triangle's top (its largest
+// vertex coordinate along the axis), largest first. A ray reads its cell's entry in the
+// cell table (first record, and that record's top, so that a ray above it reads no record
+// at all) and tests the triangles of the records that reach its origin. They include every
+// triangle whose leaf box the BVH walk would accept (ancestor boxes contain the leaf
+// boxes); the padding guarantees that no triangle outside its box can be hit, so the hits
+// are the BVH walk's, and the hit buffer, restarts and chain are the BVH kernel's: the
+// results are the same. The gain is the walk: one range lookup and a few records instead of
+// ~30 dependent node fetches. The object of a point's answer comes from a per-triangle map.
 //
 // Build (per BVH and axis, once): the cell edge is a fraction of the triangles' mean padded
 // extent in the plane (RS3D_OPT_GRID_CELL_PCT, widened while the records exceed their
 // budget); records per triangle from the cell span of its box, budget and list-length checks
 // (rejected grids leave the launches on the BVH), one
-// (cell << 32 | descending box top) key per record, a stable radix sort, per-cell counts
+// (cell << 32 | descending top) key per record, a stable radix sort, per-cell counts
 // and their scan as the cell ranges, and a gather of the records.
 #include <algorithm>
 #include <chrono>
@@ -35,6 +35,7 @@ void axis_grid_free(AxisGrid* g) {
     if (g->occ) (void)hipFree(g->occ);
     if (g->tsum) (void)hipFree(g->tsum);
     if (g->rec) (void)hipFree(g->rec);
+    if (g->obj) (void)hipFree(g->obj);
     delete g;
 }
 
@@ -109,8 +110,11 @@ __global__ __launch_bounds__(256) void k_ag_emit(const TriRec* __restrict__ tris
     if (i >= n) return;
     float plo[3], phi[3];
     int cu0, cu1, cw0, cw1;
-    ag_cells<AX>(tris[i], G, plo, phi, cu0, cu1, cw0, cw1);
-    const uint64_t low = 0xffffffffu - ag_ord(phi[AX]);  // box top, descending
+    const TriRec r = tris[i];
+    ag_cells<AX>(r, G, plo, phi, cu0, cu1, cw0, cw1);
+    const float top = fmaxf(fmaxf(get3(make_float3(r.v0.x, r.v0.y, r.v0.z), AX), get3(make_float3(r.v1.x, r.v1.y, r.v1.z), AX)),
+                            get3(make_float3(r.v2.x, r.v2.y, r.v2.z), AX));
+    const uint64_t low = 0xffffffffu - ag_ord(top);  // the record's top (ag_top), descending
     uint64_t k = off[i];
     for (int cw = cw0; cw <= cw1; ++cw)
         for (int cu = cu0; cu <= cu1; ++cu) {
@@ -136,28 +140,37 @@ __global__ __launch_bounds__(256) void k_ag_sq(const uint32_t* __restrict__ coun
 
 // Record i of the sorted (cell, top) keys goes to its cell's padded list: pstart[cell] +
 // its rank in the cell (i - start[cell]).
-template <int AX>
 __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tris, const uint32_t* __restrict__ vals,
                                                    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ start,
-                                                   const uint32_t* __restrict__ pstart, uint64_t m, AGArgs G,
-                                                   float4* __restrict__ rec) {
+                                                   const uint32_t* __restrict__ pstart, uint64_t m,
+                                                   float* __restrict__ rec) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint32_t cell = (uint32_t)(keys[i] >> 32);
     const uint64_t d = (uint64_t)pstart[cell] + (i - start[cell]);
     const TriRec r = tris[vals[i]];
-    float plo[3], phi[3];
-    prim_padded_box(make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
-                    make_float3(r.v2.x, r.v2.y, r.v2.z), plo, phi);
-    rec[AG_REC * d] = r.v0;      // .w: original triangle index
-    rec[AG_REC * d + 1] = r.v1;  // .w: object id
-    rec[AG_REC * d + 2] = make_float4(r.v2.x, r.v2.y, r.v2.z, phi[AX]);
+    float2* p = reinterpret_cast<float2*>(rec + AG_RECF * d);
+    p[0] = make_float2(r.v0.x, r.v0.y);
+    p[1] = make_float2(r.v0.z, r.v1.x);
+    p[2] = make_float2(r.v1.y, r.v1.z);
+    p[3] = make_float2(r.v2.x, r.v2.y);
+    p[4] = make_float2(r.v2.z, r.v0.w);  // .w of v0: the original triangle index
+}
+
+// The object of each original triangle (the records keep only the triangle's index).
+__global__ __launch_bounds__(256) void k_ag_objmap(const TriRec* __restrict__ tris, uint64_t n,
+                                                   int32_t* __restrict__ obj) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TriRec r = tris[i];
+    const uint32_t orig = __float_as_uint(r.v0.w);
+    if (orig < n) obj[orig] = __float_as_int(r.v1.w);
 }
 
 // Lists padded to an even number of records (a walk step, or an item, is a record pair):
-// pcounts = counts rounded up to even; the pad is a sentinel record (NaN vertices, box top
-// -inf) after a cell's last record, where the walk's exit test stops every ray (top -inf <
-// origin) and a NaN origin's triangle test fails (t is NaN).
+// pcounts = counts rounded up to even; the pad is a sentinel record (NaN vertices, original
+// index 0xffffffff) after a cell's last record, whose triangle test fails for every ray (t
+// is NaN), so no chain can count it.
 __global__ __launch_bounds__(256) void k_ag_pcounts(const uint32_t* __restrict__ counts, uint64_t nc,
                                                     uint32_t* __restrict__ pcounts,
                                                     unsigned long long* __restrict__ odd) {
@@ -170,14 +183,14 @@ __global__ __launch_bounds__(256) void k_ag_pcounts(const uint32_t* __restrict__
 
 __global__ __launch_bounds__(256) void k_ag_pad(const uint32_t* __restrict__ counts,
                                                 const uint32_t* __restrict__ pstart, uint64_t nc,
-                                                float4* __restrict__ rec) {
+                                                float* __restrict__ rec) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc || !(counts[c] & 1u)) return;
     const uint64_t d = (uint64_t)pstart[c] + counts[c];
     const float nan = __int_as_float(0x7fc00000);
-    rec[AG_REC * d] = make_float4(nan, nan, nan, __int_as_float(-1));
-    rec[AG_REC * d + 1] = make_float4(nan, nan, nan, __int_as_float(-1));
-    rec[AG_REC * d + 2] = make_float4(nan, nan, nan, -INFINITY);
+    float2* p = reinterpret_cast<float2*>(rec + AG_RECF * d);
+    for (int k = 0; k < 4; ++k) p[k] = make_float2(nan, nan);
+    p[4] = make_float2(nan, __int_as_float(-1));
 }
 
 // Occupancy words from the counts: one cell per lane, a wave's ballot is the words of its
@@ -222,11 +235,12 @@ __global__ __launch_bounds__(256) void k_ag_fine(const uint32_t* __restrict__ wo
     occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = make_uint2(w, __float_as_uint(top));
 }
 
-// Cell table: (first record, box top of the first record -- the largest of the cell) per
-// cell, and (record count, -inf) after the last cell.
+// Cell table: (first record, top of the first record -- the largest of the cell) per cell,
+// and (record count, -inf) after the last cell.
+template <int AX>
 __global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ start,
                                                   const uint32_t* __restrict__ counts,
-                                                  const float4* __restrict__ rec, uint64_t nc, uint32_t total,
+                                                  const float* __restrict__ rec, uint64_t nc, uint32_t total,
                                                   uint2* __restrict__ cells) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c > nc) return;
@@ -235,7 +249,7 @@ __global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ s
         return;
     }
     const uint32_t s = start[c];
-    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? rec[AG_REC * (uint64_t)s + 2].w : -INFINITY));
+    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? ag_top<AX>(ag_load_one(rec, s)) : -INFINITY));
 }
 
 // Point rays on the grid: wave b owns rays [b * per_wave, ...), in the two-phase form; the
@@ -462,12 +476,14 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->occ, nt * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->tsum, ng * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * AG_REC * sizeof(float4))) != hipSuccess) {
+        (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * AG_RECF * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&g->obj, n * sizeof(int32_t))) != hipSuccess) {
         set_error("axis grid: hipMalloc failed: %s", hipGetErrorString(e));
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint2) + ng * sizeof(uint2) + ptotal * AG_REC * sizeof(float4);
+    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint2) + ng * sizeof(uint2) + ptotal * AG_RECF * sizeof(float) +
+               n * sizeof(int32_t);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
@@ -480,12 +496,13 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
             RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
             RS3D_TRY(grid_blocks(total, 256, &nbe));
-            k_ag_gather<AX><<<nbe, 256, 0, s>>>(bvh->tris, vals, keys, start, pstart, total, G, g->rec);
+            k_ag_gather<<<nbe, 256, 0, s>>>(bvh->tris, vals, keys, start, pstart, total, g->rec);
             k_ag_pad<<<nbc, 256, 0, s>>>(counts, pstart, nc, g->rec);
         }
         unsigned nbc1;
         RS3D_TRY(grid_blocks(nc + 1, 256, &nbc1));
-        k_ag_cells<<<nbc1, 256, 0, s>>>(pstart, counts, g->rec, nc, (uint32_t)ptotal, g->cells);
+        k_ag_cells<AX><<<nbc1, 256, 0, s>>>(pstart, counts, g->rec, nc, (uint32_t)ptotal, g->cells);
+        k_ag_objmap<<<nb, 256, 0, s>>>(bvh->tris, n, g->obj);
         k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, g->cells, g->tsum, g->occ);
         RS3D_HIP(hipGetLastError());
         RS3D_HIP(hipStreamSynchronize(s));
@@ -605,15 +622,15 @@ int axis_grid_ready(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, bool* ready, bool bu
 
 uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
     // Peak device bytes of a build: records are at most 16 per triangle + 2^20 (the budget
-    // the build widens its cells to meet), 48 B each plus up to 1/8 for the padding to even
+    // the build widens its cells to meet), 40 B each plus up to 1/8 for the padding to even
     // lists, and while the build sorts them 24 B more each (keys, values and the sort's
     // alternate buffers); cells are within 16 per triangle + 1024 and the padding of the
     // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table, 16 B of counts and
-    // offsets during the build and under 1 B of tile words, tile entries and summaries; 8 B
-    // per triangle of spans and offsets.
+    // offsets during the build and under 1 B of tile words, tile entries and summaries; per
+    // triangle 8 B of spans and offsets and 4 B of its object.
     const uint64_t n = bvh->num_tris;
     const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024 + (1ull << 19);
-    return recs * (54 + 24) + cells * 25 + n * 8 + (64ull << 20);
+    return recs * (45 + 24) + cells * 25 + n * 12 + (64ull << 20);
 }
 
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,
