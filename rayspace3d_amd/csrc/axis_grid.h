@@ -10,22 +10,24 @@ namespace rs3d {
 
 // Cells are numbered in tiles of 8 (u) x 4 (w): cell (cu, cw) is
 // ((cw / 4) * ntu + cu / 8) * 32 + (cw % 4) * 8 + cu % 8, with ntu = ceil(nu / 8), so one
-// 32-bit occupancy word covers one tile and a tile's cells are consecutive in the cell
-// table and the records. Occupancy has two levels: tsum[g] holds, for tiles 32g..32g+31, a
-// bit per tile that lists records (.x) and the number of such tiles before the group (.y);
-// occ holds the cell bits of those tiles only, in tile order (rank .y + popcount of the
-// lower bits of .x). A point in an empty tile is answered from tsum alone, which is small
-// enough (8 B per 1024 cells) to stay in L2 while records stream through it. Beside each
-// occupied tile's cell bits, occ keeps the highest box top of the tile's records (the
-// largest first-record top of its cells; +inf when one is NaN): a point above it has
-// nothing to test in any cell of the tile and is answered without the cell-table read.
+// 32-bit occupancy word covers one tile and a tile's cells are consecutive in the records.
+// Occupancy has two levels: tsum[g] holds, for tiles 32g..32g+31, a bit per tile that lists
+// records (.x) and the number of such tiles before the group (.y); occ holds, for those
+// tiles only, in tile order (rank .y + popcount of the lower bits of .x): the tile's cell
+// bits, the highest top of its records (the largest first-record top of its cells; +inf
+// when one is NaN) and the rank of its first occupied cell. A point in an empty tile is
+// answered from tsum alone, which is small enough (8 B per 1024 cells) to stay in L2 while
+// records stream through it; a point above its tile's highest top has nothing to test in
+// any of its cells and is answered without the cell-table read. The cell table lists the
+// occupied cells only (rank = occ .z + popcount of the lower cell bits), a quarter of the
+// dense table's size or less, which keeps it in the caches far better.
 struct AxisGrid {
     int axis = 2, nu = 1, nw = 1, ntu = 1;
     float lo_u = 0.0f, lo_w = 0.0f, inv_u = 0.0f, inv_w = 0.0f;
     uint64_t entries = 0;
-    uint2* cells = nullptr;     // tiled cells + 1 entries: first record, box top of that record
+    uint2* cells = nullptr;     // occupied cells + 1 entries: first record, top of that record
     uint2* tsum = nullptr;      // per 32 tiles: occupied-tile bits, occupied tiles before
-    uint2* occ = nullptr;       // per occupied tile: its 32 cell bits, highest box top (bits)
+    uint4* occ = nullptr;       // per occupied tile: cell bits, highest top (bits), first cell rank
     float* rec = nullptr;       // AG_RECF floats per record
     int32_t* obj = nullptr;     // object of each original triangle
     size_t bytes = 0;
@@ -49,7 +51,7 @@ struct AgRec {
 struct AGArgs {
     const uint2* cells;
     const uint2* tsum;
-    const uint2* occ;
+    const uint4* occ;
     const float* rec;
     const int32_t* obj;
     float lo_u, lo_w, inv_u, inv_w;
@@ -204,9 +206,10 @@ __device__ __forceinline__ void points_grid_run(const AGArgs& G, const float* __
                 // levels answer those without the range fetch
                 js = je = 0;
                 const int64_t tr = ag_tile_rank(G.tsum[c >> 10], c >> 5);
-                const uint2 oc = tr >= 0 ? G.occ[tr] : make_uint2(0u, 0u);
+                const uint4 oc = tr >= 0 ? G.occ[tr] : make_uint4(0u, 0u, 0u, 0u);
                 if (((oc.x >> (c & 31)) & 1u) && !(__uint_as_float(oc.y) < get3(o, AX))) {
-                    const uint2 c0 = G.cells[c], c1 = G.cells[c + 1];
+                    const uint32_t rk = oc.z + (uint32_t)__popc(oc.x & ((1u << (c & 31)) - 1u));
+                    const uint2 c0 = G.cells[rk], c1 = G.cells[rk + 1];
                     if (STATS) ctr.nodes++;
                     // the first record's box top decides as the walk's exit test would
                     if (!(__uint_as_float(c0.y) < get3(o, AX))) {
@@ -533,7 +536,8 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             float3 o[4];
             bool act[4], ok[4];
             uint32_t cell[4], pid[4];
-            uint2 ts[4], oc[4];
+            uint2 ts[4];
+            uint4 oc[4];
             // Every load below is issued by every lane, inactive ones at an index known to be
             // valid (the session's first ray, tile group and occupied tile 0, cell 0) with the
             // value then ignored: a load under a branch is waited for inside that branch, which
@@ -572,7 +576,8 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             for (int k = 0; k < 4; ++k) {
                 ag_pin(oc[k].x);
                 ag_pin(oc[k].y);
-                if (ag_tile_rank(ts[k], cell[k] >> 5) < 0) oc[k] = make_uint2(0u, 0u);
+                ag_pin(oc[k].z);
+                if (ag_tile_rank(ts[k], cell[k] >> 5) < 0) oc[k] = make_uint4(0u, 0u, 0u, 0u);
             }
             // queue the points in an occupied cell that do not lie above the tile's highest box
             // top, with their origins (which are then no longer held in registers)
@@ -611,7 +616,8 @@ __device__ __forceinline__ void points_grid_two_phase(const AGArgs& G, const flo
             uint32_t c1[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t c = ok[k] ? cell[k] : 0u;
+                // the cell's rank among the occupied cells (0: a valid entry for the others)
+                const uint32_t c = ok[k] ? oc[k].z + (uint32_t)__popc(oc[k].x & ((1u << (cell[k] & 31)) - 1u)) : 0u;
                 c0[k] = G.cells[c];
                 c1[k] = G.cells[c + 1].x;
             }

@@ -215,41 +215,56 @@ __global__ __launch_bounds__(256) void k_ag_tsum(const uint32_t* __restrict__ wo
     }
 }
 
-// Compact words of the occupied tiles at their ranks, each with the highest box top of the
+// Compact entries of the occupied tiles at their ranks: the cell bits, the highest top of the
 // tile's records (the cells' first-record tops, from the cell table: +inf when one is NaN,
-// so that no point is ever skipped on it), and each group's rank base.
+// so that no point is ever skipped on it) and the rank of the tile's first occupied cell
+// (crank of its first cell); and each group's rank base.
 __global__ __launch_bounds__(256) void k_ag_fine(const uint32_t* __restrict__ words, uint64_t nt,
-                                                 const uint32_t* __restrict__ gbase, const uint2* __restrict__ cells,
-                                                 uint2* __restrict__ tsum, uint2* __restrict__ occ) {
+                                                 const uint32_t* __restrict__ gbase, const uint32_t* __restrict__ crank,
+                                                 const uint2* __restrict__ cells, uint2* __restrict__ tsum,
+                                                 uint4* __restrict__ occ) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     const uint32_t base = gbase[t / 32];
     if ((t & 31) == 0) tsum[t / 32].y = base;
     const uint32_t w = words[t];
     if (!w) return;
+    const uint32_t cb = crank[32 * t];
     float top = -INFINITY;
-    for (uint32_t m = w; m; m &= m - 1u) {
-        const float c = __uint_as_float(cells[32 * t + (uint32_t)__ffs(m) - 1u].y);
+    for (uint32_t k = 0; k < (uint32_t)__popc(w); ++k) {
+        const float c = __uint_as_float(cells[cb + k].y);
         top = c != c ? INFINITY : (c > top ? c : top);
     }
-    occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = make_uint2(w, __float_as_uint(top));
+    occ[base + __popc(tsum[t / 32].x & ((1u << (t & 31)) - 1u))] = make_uint4(w, __float_as_uint(top), cb, 0u);
 }
 
-// Cell table: (first record, top of the first record -- the largest of the cell) per cell,
-// and (record count, -inf) after the last cell.
+// Occupied-cell flags (for their ranks) and their number.
+__global__ __launch_bounds__(256) void k_ag_oflag(const uint32_t* __restrict__ counts, uint64_t nc,
+                                                  uint32_t* __restrict__ flag, unsigned long long* __restrict__ total) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t f = c < nc && counts[c] != 0u ? 1u : 0u;
+    if (c < nc) flag[c] = f;
+    const unsigned long long v = block_sum_256(f);
+    if (threadIdx.x == 0 && v) atomicAdd(acc_stripe(total), v);
+}
+
+// Cell table over the occupied cells, in cell order (occupied cell c at crank[c]): (first
+// record, top of the first record -- the largest of the cell), and (record count, -inf)
+// after the last; an occupied cell's list ends where the next one's starts.
 template <int AX>
 __global__ __launch_bounds__(256) void k_ag_cells(const uint32_t* __restrict__ start,
                                                   const uint32_t* __restrict__ counts,
-                                                  const float* __restrict__ rec, uint64_t nc, uint32_t total,
+                                                  const uint32_t* __restrict__ crank, const float* __restrict__ rec,
+                                                  uint64_t nc, uint32_t nocc, uint32_t total,
                                                   uint2* __restrict__ cells) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > nc) return;
-    if (c == nc) {
-        cells[c] = make_uint2(total, __float_as_uint(-INFINITY));
-        return;
+    if (c == 0) {
+        cells[nocc] = make_uint2(total, __float_as_uint(-INFINITY));
+        cells[nocc + 1] = make_uint2(total, __float_as_uint(-INFINITY));
     }
+    if (c >= nc || !counts[c]) return;
     const uint32_t s = start[c];
-    cells[c] = make_uint2(s, __float_as_uint(counts[c] ? ag_top<AX>(ag_load_one(rec, s)) : -INFINITY));
+    cells[crank[c]] = make_uint2(s, __float_as_uint(ag_top<AX>(ag_load_one(rec, s))));
 }
 
 // Point rays on the grid: wave b owns rays [b * per_wave, ...), in the two-phase form; the
@@ -457,10 +472,20 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     }
     const uint64_t ptotal = total + odd;
     if (ptotal > 0xffffffffull) return RS3D_OK;  // rejected: too many records
-    uint32_t *words, *gcount, *gbase;
+    uint32_t *words, *gcount, *gbase, *oflag, *crank;
     RS3D_TRY(tmp.get(&words, nt));
     RS3D_TRY(tmp.get(&gcount, ng));
     RS3D_TRY(tmp.get(&gbase, ng));
+    RS3D_TRY(tmp.get(&oflag, nc));
+    RS3D_TRY(tmp.get(&crank, nc));
+    uint64_t nocc = 0;
+    {
+        unsigned long long* acc_occ;
+        RS3D_TRY(acc_slot(ctx, 3, 0, &acc_occ));
+        k_ag_oflag<<<nbc, 256, 0, s>>>(counts, nc, oflag, acc_occ);
+        RS3D_HIP(hipGetLastError());
+        RS3D_TRY(acc_read_sum(ctx, acc_occ, &nocc));
+    }
     AxisGrid* g = new AxisGrid();
     g->axis = AX;
     g->nu = nu;
@@ -473,8 +498,9 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     g->entries = total;
     g->expected = expected;
     hipError_t e;
-    if ((e = hipMalloc(&g->cells, (nc + 1) * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&g->occ, nt * sizeof(uint2))) != hipSuccess ||
+    // (one entry more than the table: a lane with no occupied cell reads entries 0 and 1)
+    if ((e = hipMalloc(&g->cells, (nocc + 2) * sizeof(uint2))) != hipSuccess ||
+        (e = hipMalloc(&g->occ, nt * sizeof(uint4))) != hipSuccess ||
         (e = hipMalloc(&g->tsum, ng * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&g->rec, std::max<uint64_t>(ptotal, 1) * AG_RECF * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&g->obj, n * sizeof(int32_t))) != hipSuccess) {
@@ -482,11 +508,12 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         axis_grid_free(g);
         return RS3D_E_NOMEM;
     }
-    g->bytes = (nc + 1) * sizeof(uint2) + nt * sizeof(uint2) + ng * sizeof(uint2) + ptotal * AG_RECF * sizeof(float) +
-               n * sizeof(int32_t);
+    g->bytes = (nocc + 2) * sizeof(uint2) + nt * sizeof(uint4) + ng * sizeof(uint2) +
+               ptotal * AG_RECF * sizeof(float) + n * sizeof(int32_t);
     const int rc = [&]() -> int {
         RS3D_TRY(scan_u32_exclusive(ctx, counts, start, nc));
         RS3D_TRY(scan_u32_exclusive(ctx, pcounts, pstart, nc));
+        RS3D_TRY(scan_u32_exclusive(ctx, oflag, crank, nc));
         k_ag_occ<<<nbc, 256, 0, s>>>(counts, nc, words);
         unsigned nbt;
         RS3D_TRY(grid_blocks(nt, 256, &nbt));
@@ -499,11 +526,10 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
             k_ag_gather<<<nbe, 256, 0, s>>>(bvh->tris, vals, keys, start, pstart, total, g->rec);
             k_ag_pad<<<nbc, 256, 0, s>>>(counts, pstart, nc, g->rec);
         }
-        unsigned nbc1;
-        RS3D_TRY(grid_blocks(nc + 1, 256, &nbc1));
-        k_ag_cells<AX><<<nbc1, 256, 0, s>>>(pstart, counts, g->rec, nc, (uint32_t)ptotal, g->cells);
+        k_ag_cells<AX><<<nbc, 256, 0, s>>>(pstart, counts, crank, g->rec, nc, (uint32_t)nocc, (uint32_t)ptotal,
+                                           g->cells);
         k_ag_objmap<<<nb, 256, 0, s>>>(bvh->tris, n, g->obj);
-        k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, g->cells, g->tsum, g->occ);
+        k_ag_fine<<<nbt, 256, 0, s>>>(words, nt, gbase, crank, g->cells, g->tsum, g->occ);
         RS3D_HIP(hipGetLastError());
         RS3D_HIP(hipStreamSynchronize(s));
         return RS3D_OK;
@@ -625,12 +651,12 @@ uint64_t axis_grid_bytes_bound(const rs3d_bvh* bvh) {
     // the build widens its cells to meet), 40 B each plus up to 1/8 for the padding to even
     // lists, and while the build sorts them 24 B more each (keys, values and the sort's
     // alternate buffers); cells are within 16 per triangle + 1024 and the padding of the
-    // edge tiles (under 2^19 at 32768 cells a side), 8 B in the table, 16 B of counts and
-    // offsets during the build and under 1 B of tile words, tile entries and summaries; per
-    // triangle 8 B of spans and offsets and 4 B of its object.
+    // edge tiles (under 2^19 at 32768 cells a side), at most 8 B in the table, 24 B of
+    // counts, offsets and ranks during the build and under 1 B of tile words, tile entries
+    // and summaries; per triangle 8 B of spans and offsets and 4 B of its object.
     const uint64_t n = bvh->num_tris;
     const uint64_t recs = 16 * n + (1ull << 20), cells = 16 * n + 1024 + (1ull << 19);
-    return recs * (45 + 24) + cells * 25 + n * 12 + (64ull << 20);
+    return recs * (45 + 24) + cells * 33 + n * 12 + (64ull << 20);
 }
 
 int axis_grid_launch(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float* pts, uint64_t n, const uint32_t* imap,
