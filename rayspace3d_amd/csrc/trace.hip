@@ -109,7 +109,7 @@ __device__ __forceinline__ void edge_one(const BNode* __restrict__ nodes, const 
     const int soff = E.src_off[e];
     int64_t cursor = 0;
     if (MODE == EMODE_WRITE_PAIRS || MODE == EMODE_WRITE_KEYS) cursor = offsets[e] + offset_bias;
-    const int hits = edge_ray<DEPTH, STATS>(
+    const int hits = edge_ray<DEPTH, STATS, (DEPTH < EDGE_LDS_DEPTH ? DEPTH : EDGE_LDS_DEPTH)>(
         nodes, tris, occ, ld3f(E.starts, e), ld3f(E.ends, e), chain_rule, max_iter, stack, ctr, [&](int tgt) {
             for (int q = 0; q < nsrc; ++q) {
                 const int src = E.src_obj[soff + q];
@@ -146,9 +146,11 @@ __device__ __forceinline__ void flush_edge_counters(const TraceCounters& ctr, un
 }
 
 // Edge rays (A.2), one per lane: segment from start towards end, hits in [thr, len + 1e-6].
-// With list, lane i runs edge list[i] of n_list; otherwise edge i of E.
+// With list, lane i runs edge list[i] of n_list; otherwise edge i of E. The walk waits on
+// chains of dependent node fetches: six waves per SIMD (a few registers spilled, the stack
+// mostly in LDS) hide more of that latency than five without spills.
 template <int MODE, int DEPTH, bool STATS>
-__global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ nodes,
+__global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_edges(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, OccArgs occ, EdgeArgs E,
                                                     const uint32_t* __restrict__ list, uint64_t n_list,
                                                     int swap, int chain_rule, int max_iter,
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_edges(const BNode* __restrict__ no
                                                     const int64_t* __restrict__ offsets,
                                                     int64_t offset_bias, void* __restrict__ out,
                                                     HashArgs H, unsigned long long* __restrict__ stats) {
-    __shared__ int stack_mem[DEPTH * TR_BLOCK];
+    __shared__ int stack_mem[(DEPTH < EDGE_LDS_DEPTH ? DEPTH : EDGE_LDS_DEPTH) * TR_BLOCK];
     const uint64_t i = (uint64_t)blockIdx.x * TR_BLOCK + threadIdx.x;
     TraceCounters ctr;
     if (i < (list ? n_list : E.num_edges))

@@ -12,6 +12,10 @@ constexpr int KBUF = 8;
 // Hit buffer of the axis-grid walk: a grid ray meets few triangles (a sparse scene's rays
 // cross a handful of surfaces), so a smaller buffer saves registers; more hits restart.
 constexpr int KBUF_GRID = 4;
+// Hit buffer of a segment walk: an edge meets few triangles within its length; more restart.
+constexpr int KBUF_EDGE = 4;
+// LDS part of a segment walk's stack (entries per lane).
+constexpr int EDGE_LDS_DEPTH = 16;
 
 __device__ __forceinline__ float3 ld3f(const float* p, uint64_t i) {
     return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
@@ -229,19 +233,32 @@ __device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 l
     const int x0 = occ_cell(lo.x, g.org_x, g.inv, g.nx), x1 = occ_cell(hi.x, g.org_x, g.inv, g.nx);
     const int y0 = occ_cell(lo.y, g.org_y, g.inv, g.ny), y1 = occ_cell(hi.y, g.org_y, g.inv, g.ny);
     const int z0 = occ_cell(lo.z, g.org_z, g.inv, g.nz), z1 = occ_cell(hi.z, g.org_z, g.inv, g.nz);
-    if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 64) return true;
+    const int ex = x1 - x0, ey = y1 - y0 + 1, ez = z1 - z0 + 1;
+    if ((ex + 1) * ey * ez > 64) return true;
     bool any = !g.bits;  // without bits (a walk over selected segments): start node only
-    if (g.bits)
-    for (int z = z0; z <= z1 && !any; ++z)
-        for (int y = y0; y <= y1 && !any; ++y)
-            for (int x = x0; x <= x1; ++x) {
-                const uint64_t c = ((uint64_t)z * g.ny + y) * g.nx + x;
-                ++words;
-                if ((g.bits[c >> 5] >> (c & 31)) & 1u) {
-                    any = true;
-                    break;
-                }
+    if (g.bits && ex >= 0 && ey > 0 && ez > 0) {
+        // The box's cells of one (y, z) row are consecutive bits: one or two words per row
+        // (three or more only for a row of over 32 cells). Rows go in groups of four whose
+        // loads are all issued before the test; padding rows repeat the group's first row.
+        const int rows = ey * ez;
+        for (int r0 = 0; r0 < rows && !any; r0 += 4) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int r = r0 + k < rows ? r0 + k : r0;
+                const int zq = r / ey;
+                const uint64_t c0 = ((uint64_t)(z0 + zq) * g.ny + (y0 + r - zq * ey)) * g.nx + x0;
+                const uint64_t c1 = c0 + ex;
+                const uint64_t w0 = c0 >> 5, w1 = c1 >> 5;
+                const uint32_t mlo = ~0u << (c0 & 31), mhi = ~0u >> (31 - (c1 & 31));
+                const uint32_t a = g.bits[w0], b = g.bits[w1];
+                acc |= w0 == w1 ? (a & mlo & mhi) : ((a & mlo) | (b & mhi));
+                for (uint64_t w = w0 + 1; w < w1; ++w) acc |= g.bits[w];
+                if (r0 + k < rows) words += w1 - w0 + 1;
             }
+            any = acc != 0;
+        }
+    }
     if (any && start && g.snode) {
         const int bx = x0 / OCC_BLOCK, by = y0 / OCC_BLOCK, bz = z0 / OCC_BLOCK;
         if (x1 < (bx + 2) * OCC_BLOCK && y1 < (by + 2) * OCC_BLOCK && z1 < (bz + 2) * OCC_BLOCK)
@@ -406,12 +423,15 @@ __device__ __forceinline__ bool ht_contains(const HashArgs& h, uint64_t key) {
     return false;
 }
 
-template <int DEPTH, bool STATS>
+// The stack's first DL entries live in LDS (stride TR_BLOCK), deeper ones in a per-lane
+// private array: a walk from a start node rarely nests deep, and a small LDS stack leaves
+// room for more waves per CU.
+template <int DEPTH, bool STATS, int KB, int DL>
 __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, const float3 o,
                                                     const float3 inv, const RayPre& rp, float thr,
-                                                    float tmax, int* stack, float (&bt)[KBUF],
-                                                    uint32_t (&bi)[KBUF], int (&bo)[KBUF],
+                                                    float tmax, int* stack, float (&bt)[KB],
+                                                    uint32_t (&bi)[KB], int (&bo)[KB],
                                                     TraceCounters& ctr, int root = 0) {
     // The computed t of any reported hit lies within the kz extent of its triangle's
     // vertices (scaled by Sz) up to a few ulps; boxes are tested against [thr, tmax] on
@@ -422,6 +442,7 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
     uint32_t total = 0;
     int sp = 0;
     int node = root;
+    int deep[DEPTH > DL ? DEPTH - DL : 1];
     while (true) {
         const NodeBoxes b = load_node(nodes, node);
         if (STATS) ctr.nodes++;
@@ -457,7 +478,7 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
                     if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t <= tmax) {
                         ++total;
                         if (STATS) ctr.hits++;
-                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                        if (t < bt[KB - 1] || (t == bt[KB - 1] && orig < bi[KB - 1]))
                             kbuf_insert(bt, bi, bo, t, orig, obj);
                     }
                 }
@@ -466,7 +487,8 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
         const bool in0 = h[0] && b.n0 == 0;
         const bool in1 = h[1] && b.n1 == 0;
         if (in0 && in1) {
-            stack[sp * TR_BLOCK] = b.l1;
+            if (DL >= DEPTH || sp < DL) stack[sp * TR_BLOCK] = b.l1;
+            else deep[sp - DL] = b.l1;
             ++sp;
             node = b.l0;
         } else if (in0) {
@@ -476,7 +498,7 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
         } else {
             if (sp == 0) break;
             --sp;
-            node = stack[sp * TR_BLOCK];
+            node = (DL >= DEPTH || sp < DL) ? stack[sp * TR_BLOCK] : deep[sp - DL];
         }
     }
     return total;
@@ -543,7 +565,7 @@ __device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s,
                                start);
 }
 
-template <int DEPTH, bool STATS, typename Emit>
+template <int DEPTH, bool STATS, int DL = DEPTH, typename Emit>
 __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
                                         const TriRec* __restrict__ tris, const OccArgs& occ,
                                         const float3 s, const float3 en, int chain_rule, int max_iter,
@@ -560,17 +582,20 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
     const float3 inv = make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float thr = eps;
     int iter = 0;
-    float bt[KBUF];
-    uint32_t bi[KBUF];
-    int bo[KBUF];
+    float bt[KBUF_EDGE];
+    uint32_t bi[KBUF_EDGE];
+    int bo[KBUF_EDGE];
     while (true) {
         kbuf_clear(bt, bi, bo);
         const uint32_t total =
-            collect_segment<DEPTH, STATS>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr, root);
+            collect_segment<DEPTH, STATS, KBUF_EDGE, DL>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr,
+                                                         root);
+        // the chain over the buffer first, then one emission site for its members in order
+        uint32_t chain = 0;
 #pragma unroll
-        for (int k = 0; k < KBUF; ++k) {
+        for (int k = 0; k < KBUF_EDGE; ++k) {
             if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
-                emit(bo[k]);
+                chain |= 1u << k;
                 ++iter;
                 float nthr;
                 if (chain_rule == CHAIN_EPS) {
@@ -583,7 +608,16 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
                 thr = nthr;
             }
         }
-        if (total <= (uint32_t)KBUF || iter >= max_iter || thr > tmax) break;
+        while (chain) {
+            const int k = __builtin_ctz(chain);
+            chain &= chain - 1;
+            int obj = bo[0];
+#pragma unroll
+            for (int q = 1; q < KBUF_EDGE; ++q)
+                if (k == q) obj = bo[q];
+            emit(obj);
+        }
+        if (total <= (uint32_t)KBUF_EDGE || iter >= max_iter || thr > tmax) break;
         if (STATS) ctr.restarts++;
     }
     return iter;
