@@ -235,7 +235,7 @@ struct rs3d_ctx {
     rs3d::DevBuf sort_keys, sort_vals, sort_hist, sort_csum, scan_tmp, tmp_a, tmp_b, tmp_c, small;
     rs3d::DevBuf ray_keys, ray_vals;  // ray reordering (coherence) keys and permutation
     rs3d::DevBuf accs;                // striped accumulators (rs3d::acc_slot)
-    rs3d::DevBuf edge_sel;            // edge launches: selection flags and index list
+    rs3d::DevBuf edge_sel;            // edge launches: selection count or flags, and index list
     int64_t opt_ray_sort_min = 1 << 16;  // reorder rays for launches of at least this many
     int64_t opt_rays_per_wave = 64;      // lane-refill run length of point launches (0: off)
     int64_t opt_axis_grid = 1;           // point launches may use an accepted axis grid

@@ -167,14 +167,46 @@ __global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))
     flush_edge_counters<STATS>(ctr, stats);
 }
 
-// Edge selection against occupancy bits: sel[e] = 1 where edge e may hit (edge_may_hit).
-// A light kernel (no LDS stack, few registers) keeps many waves of these random bit reads
-// in flight; the walk then runs only over the selected edges, in full waves.
-__global__ __launch_bounds__(256) void k_edge_select(EdgeArgs E, OccArgs occ, uint8_t* __restrict__ sel) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E.num_edges) return;
-    uint64_t words = 0;
-    sel[i] = edge_may_hit(occ, ld3f(E.starts, i), ld3f(E.ends, i), words) ? 1 : 0;
+// Selection and compaction of the edges to walk, in one pass: block b tests edges
+// [b * SEL_BLOCK_EDGES, (b + 1) * SEL_BLOCK_EDGES) against the occupancy bits (edge_may_hit)
+// and writes the indices of those that may hit, in increasing order, at positions it
+// claims with one atomic add on *count. Blocks claim in any order: the list holds each
+// block's stretch of the (spatially ordered) edges in order.
+constexpr int SEL_EPT = 4;  // edges per thread
+constexpr int SEL_BLOCK_EDGES = 256 * SEL_EPT;
+__global__ __launch_bounds__(256) void k_edge_select_list(EdgeArgs E, OccArgs occ, uint32_t* __restrict__ list,
+                                                          uint32_t* __restrict__ count) {
+    __shared__ uint32_t wcnt[SEL_EPT * 4];
+    __shared__ uint32_t base_s;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t t0 = (uint64_t)blockIdx.x * SEL_BLOCK_EDGES;
+    uint64_t ball[SEL_EPT];
+#pragma unroll
+    for (int k = 0; k < SEL_EPT; ++k) {
+        const uint64_t i = t0 + (uint64_t)k * 256 + tid;
+        uint64_t words = 0;
+        const bool f = i < E.num_edges && edge_may_hit(occ, ld3f(E.starts, i), ld3f(E.ends, i), words);
+        ball[k] = __ballot(f);
+        if (lane == 0) wcnt[k * 4 + w] = (uint32_t)__popcll(ball[k]);
+    }
+    __syncthreads();
+    if (tid == 0) {  // exclusive offsets in (item, wave) order, and the block's claim
+        uint32_t run = 0;
+        for (int q = 0; q < SEL_EPT * 4; ++q) {
+            const uint32_t c = wcnt[q];
+            wcnt[q] = run;
+            run += c;
+        }
+        base_s = run ? atomicAdd(count, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = base_s;
+#pragma unroll
+    for (int k = 0; k < SEL_EPT; ++k)
+        if ((ball[k] >> lane) & 1ull)
+            list[base + wcnt[k * 4 + w] + (uint32_t)__popcll(ball[k] & lt)] =
+                (uint32_t)(t0 + (uint64_t)k * 256 + tid);
 }
 
 __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ v, uint64_t n,
@@ -399,9 +431,10 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     RS3D_TRY(pick_depth(bvh, &depth));
     OccArgs occ;
     RS3D_TRY(seg_occ_args(ctx, const_cast<rs3d_bvh*>(bvh), &occ));
-    // One ray per lane. With occupancy bits (large timed launches): a selection pass, a
-    // stable compaction of the selected edges, and the walk over them alone (the counting
-    // pass keeps the one-kernel form, which counts the words read).
+    // One ray per lane. With occupancy bits (large timed launches): a selection pass that
+    // also compacts the selected edges (in stretches of SEL_BLOCK_EDGES, each in order), and
+    // the walk over them alone (the counting pass keeps the one-kernel form, which counts
+    // the words read).
     const uint32_t* list = nullptr;
     uint64_t n_list = 0;
     unsigned nbl = nb;
@@ -438,16 +471,20 @@ static int launch_edges(rs3d_ctx* ctx, const rs3d_bvh* bvh, const rs3d_edges* ed
     // save: the walking kernel tests the bits itself)
     if (occ.bits && edges->num_edges >= (1ull << 20) && edges->num_edges <= 0xffffffffull) {
         const uint64_t n = edges->num_edges;
-        const size_t sel_bytes = (n + 255) & ~(size_t)255;
-        RS3D_TRY(ctx->edge_sel.ensure(sel_bytes + n * sizeof(uint32_t)));
-        uint8_t* sel = ctx->edge_sel.as<uint8_t>();
-        uint32_t* idx = reinterpret_cast<uint32_t*>(sel + sel_bytes);
+        RS3D_TRY(ctx->edge_sel.ensure(256 + n * sizeof(uint32_t)));  // count, then the list
+        uint32_t* cnt = ctx->edge_sel.as<uint32_t>();
+        uint32_t* idx = reinterpret_cast<uint32_t*>(ctx->edge_sel.as<uint8_t>() + 256);
+        RS3D_TRY(fill_u32(ctx, cnt, 1, 0u));
         unsigned nbs;
-        RS3D_TRY(grid_blocks(n, 256, &nbs));
-        k_edge_select<<<nbs, 256, 0, ctx->stream>>>(E, occ, sel);
+        RS3D_TRY(grid_blocks(n, SEL_BLOCK_EDGES, &nbs));
+        k_edge_select_list<<<nbs, 256, 0, ctx->stream>>>(E, occ, idx, cnt);
         RS3D_HIP(hipGetLastError());
         if (MODE == EMODE_COUNT) RS3D_TRY(fill_u32(ctx, counts, n, 0u));  // unselected edges: no hits
-        RS3D_TRY(rs3d_compact_flags(ctx, sel, n, idx, &n_list));
+        RS3D_HIP(hipMemcpyAsync(ctx->pinned, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        RS3D_HIP(hipStreamSynchronize(ctx->stream));
+        uint32_t h;
+        memcpy(&h, ctx->pinned, sizeof h);
+        n_list = h;
         list = idx;
         RS3D_TRY(grid_blocks(n_list ? n_list : 1, TR_BLOCK, &nbl));
     }

@@ -12,8 +12,9 @@ constexpr int KBUF = 8;
 // Hit buffer of the axis-grid walk: a grid ray meets few triangles (a sparse scene's rays
 // cross a handful of surfaces), so a smaller buffer saves registers; more hits restart.
 constexpr int KBUF_GRID = 4;
-// Hit buffer of a segment walk: an edge meets few triangles within its length; more restart.
-constexpr int KBUF_EDGE = 4;
+// Hit buffer of a segment walk: one hit. A walked edge meets 0.1 triangles on average, and an
+// edge with k hits takes k passes; the registers saved buy a seventh wave per SIMD.
+constexpr int KBUF_EDGE = 1;
 // LDS part of a segment walk's stack (entries per lane).
 constexpr int EDGE_LDS_DEPTH = 16;
 
