@@ -241,22 +241,31 @@ __device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 l
         // The box's cells of one (y, z) row are consecutive bits: one or two words per row
         // (three or more only for a row of over 32 cells). Rows go in groups of four whose
         // loads are all issued before the test; padding rows repeat the group's first row.
+        // (cell numbers fit 32 bits: a grid holds at most 2^30 cells)
         const int rows = ey * ez;
+        int yb = y0, zb = z0;  // first row of the group
         for (int r0 = 0; r0 < rows && !any; r0 += 4) {
             uint32_t acc = 0;
+            int y = yb, z = zb;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int r = r0 + k < rows ? r0 + k : r0;
-                const int zq = r / ey;
-                const uint64_t c0 = ((uint64_t)(z0 + zq) * g.ny + (y0 + r - zq * ey)) * g.nx + x0;
-                const uint64_t c1 = c0 + ex;
-                const uint64_t w0 = c0 >> 5, w1 = c1 >> 5;
+                const bool real = r0 + k < rows;
+                const uint32_t c0 = ((uint32_t)(real ? z : zb) * (uint32_t)g.ny + (uint32_t)(real ? y : yb)) *
+                                        (uint32_t)g.nx + (uint32_t)x0;
+                const uint32_t c1 = c0 + (uint32_t)ex;
+                const uint32_t w0 = c0 >> 5, w1 = c1 >> 5;
                 const uint32_t mlo = ~0u << (c0 & 31), mhi = ~0u >> (31 - (c1 & 31));
                 const uint32_t a = g.bits[w0], b = g.bits[w1];
                 acc |= w0 == w1 ? (a & mlo & mhi) : ((a & mlo) | (b & mhi));
-                for (uint64_t w = w0 + 1; w < w1; ++w) acc |= g.bits[w];
-                if (r0 + k < rows) words += w1 - w0 + 1;
+                for (uint32_t w = w0 + 1; w < w1; ++w) acc |= g.bits[w];
+                if (real) words += w1 - w0 + 1;
+                if (++y > y1) {
+                    y = y0;
+                    ++z;
+                }
             }
+            yb = y;
+            zb = z;
             any = acc != 0;
         }
     }
