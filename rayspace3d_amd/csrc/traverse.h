@@ -453,63 +453,60 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
     int sp = 0;
     int node = root;
     int deep[DEPTH > DL ? DEPTH - DL : 1];
+    // box test of one child (its six bounds as values: no array or pointer selects, which
+    // led the compiler to keep node fields in scratch memory)
+    auto box = [&](float lx, float ly, float lz, float hx, float hy, float hz) -> bool {
+        const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+        const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+        const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+        const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        const float klo = rp.kz == 0 ? lx : (rp.kz == 1 ? ly : lz);
+        const float khi = rp.kz == 0 ? hx : (rp.kz == 1 ? hy : hz);
+        const float k0 = (klo - okz) * rp.Sz, k1 = (khi - okz) * rp.Sz;
+        const float kmin = fminf(k0, k1), kmax = fmaxf(k0, k1);
+        return (tn <= tf + fabsf(tf) * 9.5367431640625e-7f) && kmax >= lo_t && kmin <= hi_t && lx <= hx;
+    };
+    auto leaf = [&](int first, int cnt) {
+        for (int j = first; j < first + cnt; ++j) {
+            float3 v0, v1, v2;
+            uint32_t orig;
+            int obj;
+            load_tri(tris, j, v0, v1, v2, orig, obj);
+            if (STATS) ctr.tris++;
+            float t;
+            if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t <= tmax) {
+                ++total;
+                if (STATS) ctr.hits++;
+                if (t < bt[KB - 1] || (t == bt[KB - 1] && orig < bi[KB - 1])) kbuf_insert(bt, bi, bo, t, orig, obj);
+            }
+        }
+    };
     while (true) {
         const NodeBoxes b = load_node(nodes, node);
         if (STATS) ctr.nodes++;
-        bool h[2];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const float* lo = c == 0 ? b.lo0 : b.lo1;
-            const float* hi = c == 0 ? b.hi0 : b.hi1;
-            const float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
-            const float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
-            const float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
-            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-            const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-            const float klo = rp.kz == 0 ? lo[0] : (rp.kz == 1 ? lo[1] : lo[2]);
-            const float khi = rp.kz == 0 ? hi[0] : (rp.kz == 1 ? hi[1] : hi[2]);
-            const float k0 = (klo - okz) * rp.Sz, k1 = (khi - okz) * rp.Sz;
-            const float kmin = fminf(k0, k1), kmax = fmaxf(k0, k1);
-            h[c] = (tn <= tf + fabsf(tf) * 9.5367431640625e-7f) && kmax >= lo_t && kmin <= hi_t &&
-                   lo[0] <= hi[0];
-        }
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int cnt = c == 0 ? b.n0 : b.n1;
-            const int first = c == 0 ? b.l0 : b.l1;
-            if (h[c] && cnt > 0) {
-                for (int j = first; j < first + cnt; ++j) {
-                    float3 v0, v1, v2;
-                    uint32_t orig;
-                    int obj;
-                    load_tri(tris, j, v0, v1, v2, orig, obj);
-                    if (STATS) ctr.tris++;
-                    float t;
-                    if (tri_hit(o, rp, v0, v1, v2, t) && t >= thr && t <= tmax) {
-                        ++total;
-                        if (STATS) ctr.hits++;
-                        if (t < bt[KB - 1] || (t == bt[KB - 1] && orig < bi[KB - 1]))
-                            kbuf_insert(bt, bi, bo, t, orig, obj);
-                    }
-                }
-            }
-        }
-        const bool in0 = h[0] && b.n0 == 0;
-        const bool in1 = h[1] && b.n1 == 0;
+        const int l0 = b.l0, l1 = b.l1, n0 = b.n0, n1 = b.n1;
+        const bool h0 = box(b.lo0[0], b.lo0[1], b.lo0[2], b.hi0[0], b.hi0[1], b.hi0[2]);
+        const bool h1 = box(b.lo1[0], b.lo1[1], b.lo1[2], b.hi1[0], b.hi1[1], b.hi1[2]);
+        if (h0 && n0 > 0) leaf(l0, n0);
+        if (h1 && n1 > 0) leaf(l1, n1);
+        // Next node without branches around the choice: descend (pushing the second child
+        // when both are inner nodes) or pop. Lanes that descend and lanes that pop then stay
+        // in one loop; a branch per case lets the compiler nest a descend loop inside a pop
+        // loop, where a lane that must pop waits for every lane of the wave to pop.
+        const bool in0 = h0 && n0 == 0;
+        const bool in1 = h1 && n1 == 0;
+        const bool desc = in0 || in1;
+        if (!desc && sp == 0) break;
+        const int top = sp - 1;  // the entry a pop takes (read before this step's push)
+        int popv = stack[(DL >= DEPTH || top < DL ? (top > 0 ? top : 0) : DL - 1) * TR_BLOCK];
+        if (DL < DEPTH && !desc && top >= DL) popv = *(volatile int*)&deep[top - DL];
         if (in0 && in1) {
-            if (DL >= DEPTH || sp < DL) stack[sp * TR_BLOCK] = b.l1;
-            else deep[sp - DL] = b.l1;
-            ++sp;
-            node = b.l0;
-        } else if (in0) {
-            node = b.l0;
-        } else if (in1) {
-            node = b.l1;
-        } else {
-            if (sp == 0) break;
-            --sp;
-            node = (DL >= DEPTH || sp < DL) ? stack[sp * TR_BLOCK] : deep[sp - DL];
+            if (DL >= DEPTH || sp < DL) stack[sp * TR_BLOCK] = l1;
+            else deep[sp - DL] = l1;
         }
+        node = desc ? (in0 ? l0 : l1) : popv;
+        sp += (in0 && in1) ? 1 : (desc ? 0 : -1);
     }
     return total;
 }
