@@ -305,54 +305,46 @@ __device__ __forceinline__ uint32_t collect_axis(const BNode* __restrict__ nodes
     uint32_t total = 0;
     int sp = 0;
     int node = 0;
+    auto leaf = [&](int first, int cnt) {
+        for (int j = first; j < first + cnt; ++j) {
+            float3 v0, v1, v2;
+            uint32_t orig;
+            int obj;
+            load_tri(tris, j, v0, v1, v2, orig, obj);
+            if (STATS) ctr.tris++;
+            float t;
+            if (tri_hit_axis<AX>(o, v0, v1, v2, t) && (t > thr || (t == thr && orig >= thr_tri)) && t < tmax_excl) {
+                ++total;
+                if (STATS) ctr.hits++;
+                if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
+                    kbuf_insert(bt, bi, bo, t, orig, obj);
+            }
+        }
+    };
     while (true) {
         const NodeBoxes b = load_node(nodes, node);
         if (STATS) ctr.nodes++;
+        const int l0 = b.l0, l1 = b.l1, n0 = b.n0, n1 = b.n1;
         // A triangle entirely behind the origin (all vertices <= o along AX) yields t <= 0,
         // so boxes are culled only on the perpendicular slab and on hi < o along AX.
         const bool h0 = ou >= b.lo0[U] && ou <= b.hi0[U] && ow >= b.lo0[W] && ow <= b.hi0[W] &&
                         b.hi0[AX] >= oa;
         const bool h1 = ou >= b.lo1[U] && ou <= b.hi1[U] && ow >= b.lo1[W] && ow <= b.hi1[W] &&
                         b.hi1[AX] >= oa;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const bool hit = c == 0 ? h0 : h1;
-            const int cnt = c == 0 ? b.n0 : b.n1;
-            const int first = c == 0 ? b.l0 : b.l1;
-            if (hit && cnt > 0) {
-                for (int j = first; j < first + cnt; ++j) {
-                    float3 v0, v1, v2;
-                    uint32_t orig;
-                    int obj;
-                    load_tri(tris, j, v0, v1, v2, orig, obj);
-                    if (STATS) ctr.tris++;
-                    float t;
-                    if (tri_hit_axis<AX>(o, v0, v1, v2, t) && (t > thr || (t == thr && orig >= thr_tri)) &&
-                        t < tmax_excl) {
-                        ++total;
-                        if (STATS) ctr.hits++;
-                        if (t < bt[KBUF - 1] || (t == bt[KBUF - 1] && orig < bi[KBUF - 1]))
-                            kbuf_insert(bt, bi, bo, t, orig, obj);
-                    }
-                }
-            }
-        }
-        const bool in0 = h0 && b.n0 == 0;
-        const bool in1 = h1 && b.n1 == 0;
-        if (in0 && in1) {
-            const bool first0 = b.lo0[AX] <= b.lo1[AX];
-            stack[sp * TR_BLOCK] = first0 ? b.l1 : b.l0;
-            ++sp;
-            node = first0 ? b.l0 : b.l1;
-        } else if (in0) {
-            node = b.l0;
-        } else if (in1) {
-            node = b.l1;
-        } else {
-            if (sp == 0) break;
-            --sp;
-            node = stack[sp * TR_BLOCK];
-        }
+        if (h0 && n0 > 0) leaf(l0, n0);
+        if (h1 && n1 > 0) leaf(l1, n1);
+        // next node without branches around the choice (as in collect_segment): the nearer
+        // child along AX first when both are inner nodes, the other pushed; else pop
+        const bool in0 = h0 && n0 == 0;
+        const bool in1 = h1 && n1 == 0;
+        const bool desc = in0 || in1;
+        if (!desc && sp == 0) break;
+        const int popv = stack[(sp > 0 ? sp - 1 : 0) * TR_BLOCK];  // read before this step's push
+        const bool both = in0 && in1;
+        const bool first0 = b.lo0[AX] <= b.lo1[AX];
+        if (both) stack[sp * TR_BLOCK] = first0 ? l1 : l0;
+        node = both ? (first0 ? l0 : l1) : (desc ? (in0 ? l0 : l1) : popv);
+        sp += both ? 1 : (desc ? 0 : -1);
     }
     return total;
 }

@@ -98,21 +98,15 @@ __global__ __launch_bounds__(TR_BLOCK) void k_tri_join(const BNode* __restrict__
                     ht_insert(H, ((uint64_t)x << 32) | y);
                 }
             }
+            // next node without branches around the choice (descend, push, pop in one loop)
             const bool in0 = h0 && b.z == 0;
             const bool in1 = h1 && b.w == 0;
-            if (in0 && in1) {
-                stack[sp * TR_BLOCK] = b.y;
-                ++sp;
-                node = b.x;
-            } else if (in0) {
-                node = b.x;
-            } else if (in1) {
-                node = b.y;
-            } else {
-                if (sp == 0) break;
-                --sp;
-                node = stack[sp * TR_BLOCK];
-            }
+            const bool desc = in0 || in1;
+            if (!desc && sp == 0) break;
+            const int popv = stack[(sp > 0 ? sp - 1 : 0) * TR_BLOCK];  // read before this step's push
+            if (in0 && in1) stack[sp * TR_BLOCK] = b.y;
+            node = desc ? (in0 ? b.x : b.y) : popv;
+            sp += (in0 && in1) ? 1 : (desc ? 0 : -1);
         }
     }
     if (STATS) flush_counters(ctr, stats);
