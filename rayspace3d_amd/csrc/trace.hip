@@ -150,7 +150,7 @@ __device__ __forceinline__ void flush_edge_counters(const TraceCounters& ctr, un
 // chains of dependent node fetches: six waves per SIMD (a few registers spilled, the stack
 // mostly in LDS) hide more of that latency than five without spills.
 template <int MODE, int DEPTH, bool STATS>
-__global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_edges(const BNode* __restrict__ nodes,
+__global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_edges(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, OccArgs occ, EdgeArgs E,
                                                     const uint32_t* __restrict__ list, uint64_t n_list,
                                                     int swap, int chain_rule, int max_iter,
@@ -181,12 +181,20 @@ __global__ __launch_bounds__(256) void k_edge_select_list(EdgeArgs E, OccArgs oc
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t t0 = (uint64_t)blockIdx.x * SEL_BLOCK_EDGES;
+    // the block's endpoints from a block base (32-bit offsets within the block)
+    const char* sb = reinterpret_cast<const char*>(E.starts + 3 * t0);
+    const char* eb = reinterpret_cast<const char*>(E.ends + 3 * t0);
+    auto ld = [](const char* p, uint32_t j) {
+        const float* q = reinterpret_cast<const float*>(p + 12u * j);
+        return make_float3(q[0], q[1], q[2]);
+    };
     uint64_t ball[SEL_EPT];
 #pragma unroll
     for (int k = 0; k < SEL_EPT; ++k) {
-        const uint64_t i = t0 + (uint64_t)k * 256 + tid;
+        const uint32_t j = (uint32_t)(k * 256 + tid);
+        const uint64_t i = t0 + j;
         uint64_t words = 0;
-        const bool f = i < E.num_edges && edge_may_hit(occ, ld3f(E.starts, i), ld3f(E.ends, i), words);
+        const bool f = i < E.num_edges && edge_may_hit(occ, ld(sb, j), ld(eb, j), words);
         ball[k] = __ballot(f);
         if (lane == 0) wcnt[k * 4 + w] = (uint32_t)__popcll(ball[k]);
     }

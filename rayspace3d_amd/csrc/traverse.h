@@ -243,29 +243,33 @@ __device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 l
         // loads are all issued before the test; padding rows repeat the group's first row.
         // (cell numbers fit 32 bits: a grid holds at most 2^30 cells)
         const int rows = ey * ez;
-        int yb = y0, zb = z0;  // first row of the group
+        const uint32_t nx = (uint32_t)g.nx;
+        const uint32_t wrap_step = (uint32_t)g.ny * nx - (uint32_t)(ey - 1) * nx;  // last y of a plane -> next z
+        uint32_t cb = ((uint32_t)z0 * (uint32_t)g.ny + (uint32_t)y0) * nx + (uint32_t)x0;  // group's first row
+        int yb = y0;
         for (int r0 = 0; r0 < rows && !any; r0 += 4) {
             uint32_t acc = 0;
-            int y = yb, z = zb;
+            uint32_t c = cb;
+            int y = yb;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const bool real = r0 + k < rows;
-                const uint32_t c0 = ((uint32_t)(real ? z : zb) * (uint32_t)g.ny + (uint32_t)(real ? y : yb)) *
-                                        (uint32_t)g.nx + (uint32_t)x0;
+                const uint32_t c0 = real ? c : cb;
                 const uint32_t c1 = c0 + (uint32_t)ex;
                 const uint32_t w0 = c0 >> 5, w1 = c1 >> 5;
                 const uint32_t mlo = ~0u << (c0 & 31), mhi = ~0u >> (31 - (c1 & 31));
-                const uint32_t a = g.bits[w0], b = g.bits[w1];
+                // (byte offsets below 2^27: 32-bit, so the loads take the scalar base)
+                const uint32_t a = *(const uint32_t*)((const char*)g.bits + (w0 << 2));
+                const uint32_t b = *(const uint32_t*)((const char*)g.bits + (w1 << 2));
                 acc |= w0 == w1 ? (a & mlo & mhi) : ((a & mlo) | (b & mhi));
                 for (uint32_t w = w0 + 1; w < w1; ++w) acc |= g.bits[w];
                 if (real) words += w1 - w0 + 1;
-                if (++y > y1) {
-                    y = y0;
-                    ++z;
-                }
+                const bool wrap = ++y > y1;
+                c += wrap ? wrap_step : nx;
+                y = wrap ? y0 : y;
             }
+            cb = c;
             yb = y;
-            zb = z;
             any = acc != 0;
         }
     }
@@ -278,12 +282,12 @@ __device__ __forceinline__ bool occ_box_may_hit(const OccArgs& g, const float3 l
 }
 
 // Whether segment a-b may hit a triangle of the bits' BVH: its box, widened by 2^-14 of the
-// coordinates' magnitude, through occ_box_may_hit.
-__device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b,
+// coordinates' magnitude plus pad on each axis, through occ_box_may_hit.
+__device__ __forceinline__ bool occ_segment_may_hit(const OccArgs& g, const float3 a, const float3 b, float pad,
                                                     uint64_t& words, int* start = nullptr) {
-    const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + 1e-30f;  // 2^-14
-    const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + 1e-30f;
-    const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + 1e-30f;
+    const float wa = (fabsf(a.x) + fabsf(b.x)) * 6.103515625e-5f + pad;  // 2^-14
+    const float wb = (fabsf(a.y) + fabsf(b.y)) * 6.103515625e-5f + pad;
+    const float wc = (fabsf(a.z) + fabsf(b.z)) * 6.103515625e-5f + pad;
     return occ_box_may_hit(g, make_float3(fminf(a.x, b.x) - wa, fminf(a.y, b.y) - wb, fminf(a.z, b.z) - wc),
                            make_float3(fmaxf(a.x, b.x) + wa, fmaxf(a.y, b.y) + wb, fmaxf(a.z, b.z) + wc), words,
                            start);
@@ -549,8 +553,10 @@ enum { CHAIN_EPS = 0, CHAIN_NEXT = 1 };
 // A.2 edge ray from s towards en: calls emit(target object) once per chain hit; returns
 // the number of chain hits.
 // Whether edge s-en may have chain hits: its length reaches 1e-6 and, with occupancy bits,
-// the segment s .. s + tmax d that holds every hit (t in [1e-6, tmax]) covers a set bit
-// (its box is widened by 2^-14 of the coordinates' magnitude inside the test).
+// the box of s-en covers a set bit. Every hit lies on s + t d, d the unit direction, t in
+// [1e-6, len + 1e-6]: within 1e-6 beyond en on each axis, which the absolute pad of 2e-6
+// covers; the rounding of the computed hit, of d and of en is covered by the 2^-14 relative
+// widening (and the padded triangle boxes of the bits).
 __device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s, const float3 en, uint64_t& words,
                                              int* start = nullptr) {
     const float eps = 1e-6f;
@@ -558,10 +564,7 @@ __device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s,
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
     if (!(len >= eps)) return false;
     if (!occ.bits && !(start && occ.snode)) return true;
-    const float3 d = make_float3(dx / len, dy / len, dz / len);
-    const float tmax = len + eps;
-    return occ_segment_may_hit(occ, s, make_float3(s.x + d.x * tmax, s.y + d.y * tmax, s.z + d.z * tmax), words,
-                               start);
+    return occ_segment_may_hit(occ, s, en, 2e-6f, words, start);
 }
 
 template <int DEPTH, bool STATS, int DL = DEPTH, typename Emit>
