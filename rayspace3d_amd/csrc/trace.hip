@@ -147,8 +147,8 @@ __device__ __forceinline__ void flush_edge_counters(const TraceCounters& ctr, un
 
 // Edge rays (A.2), one per lane: segment from start towards end, hits in [thr, len + 1e-6].
 // With list, lane i runs edge list[i] of n_list; otherwise edge i of E. The walk waits on
-// chains of dependent node fetches: six waves per SIMD (a few registers spilled, the stack
-// mostly in LDS) hide more of that latency than five without spills.
+// chains of dependent node fetches: eight waves per SIMD (64 registers, a few spilled; the
+// stack mostly in LDS) hide more of that latency than seven without spills.
 template <int MODE, int DEPTH, bool STATS>
 __global__ __launch_bounds__(TR_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_edges(const BNode* __restrict__ nodes,
                                                     const TriRec* __restrict__ tris, OccArgs occ, EdgeArgs E,
