@@ -429,26 +429,24 @@ __device__ __forceinline__ bool ht_contains(const HashArgs& h, uint64_t key) {
     return false;
 }
 
-// The stack's first DL entries live in LDS (stride TR_BLOCK), deeper ones in a per-lane
-// private array: a walk from a start node rarely nests deep, and a small LDS stack leaves
-// room for more waves per CU.
+// One node visit of a segment walk (ray o + t d, inv = 1/d, rp its shear, okz = o along
+// rp.kz) over hits in [thr, tmax]: tests both children of node, offers the hits of leaf
+// children to the buffer (total counts them) and moves node to the next node to visit.
+// Returns false when the pass is complete. The stack's first DL entries live in LDS
+// (stride TR_BLOCK), deeper ones in deep, a per-lane private array: a walk from a start
+// node rarely nests deep, and a small LDS stack leaves room for more waves per CU.
+// The computed t of any reported hit lies within the kz extent of its triangle's vertices
+// (scaled by Sz) up to a few ulps; boxes are tested against [lo_t, hi_t] (thr and tmax with
+// a relative margin) on that slab, and against the full line on all three slabs.
+template <int DEPTH, int DL>
+using SegDeep = int[DEPTH > DL ? DEPTH - DL : 1];
+
 template <int DEPTH, bool STATS, int KB, int DL>
-__device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ nodes,
-                                                    const TriRec* __restrict__ tris, const float3 o,
-                                                    const float3 inv, const RayPre& rp, float thr,
-                                                    float tmax, int* stack, float (&bt)[KB],
-                                                    uint32_t (&bi)[KB], int (&bo)[KB],
-                                                    TraceCounters& ctr, int root = 0) {
-    // The computed t of any reported hit lies within the kz extent of its triangle's
-    // vertices (scaled by Sz) up to a few ulps; boxes are tested against [thr, tmax] on
-    // that slab with a relative margin, and against the full line on all three slabs.
-    const float okz = get3(o, rp.kz);
-    const float lo_t = thr - fabsf(thr) * 9.5367431640625e-7f;    // 2^-20
-    const float hi_t = tmax + fabsf(tmax) * 9.5367431640625e-7f;
-    uint32_t total = 0;
-    int sp = 0;
-    int node = root;
-    int deep[DEPTH > DL ? DEPTH - DL : 1];
+__device__ __forceinline__ bool seg_visit(const BNode* __restrict__ nodes, const TriRec* __restrict__ tris,
+                                          const float3 o, const float3 inv, const RayPre& rp, float okz, float thr,
+                                          float lo_t, float tmax, float hi_t, int& node, int& sp, int* stack,
+                                          SegDeep<DEPTH, DL>& deep, float (&bt)[KB], uint32_t (&bi)[KB],
+                                          int (&bo)[KB], uint32_t& total, TraceCounters& ctr) {
     // box test of one child (its six bounds as values: no array or pointer selects, which
     // led the compiler to keep node fields in scratch memory)
     auto box = [&](float lx, float ly, float lz, float hx, float hy, float hz) -> bool {
@@ -478,31 +476,54 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
             }
         }
     };
-    while (true) {
-        const NodeBoxes b = load_node(nodes, node);
-        if (STATS) ctr.nodes++;
-        const int l0 = b.l0, l1 = b.l1, n0 = b.n0, n1 = b.n1;
-        const bool h0 = box(b.lo0[0], b.lo0[1], b.lo0[2], b.hi0[0], b.hi0[1], b.hi0[2]);
-        const bool h1 = box(b.lo1[0], b.lo1[1], b.lo1[2], b.hi1[0], b.hi1[1], b.hi1[2]);
-        if (h0 && n0 > 0) leaf(l0, n0);
-        if (h1 && n1 > 0) leaf(l1, n1);
-        // Next node without branches around the choice: descend (pushing the second child
-        // when both are inner nodes) or pop. Lanes that descend and lanes that pop then stay
-        // in one loop; a branch per case lets the compiler nest a descend loop inside a pop
-        // loop, where a lane that must pop waits for every lane of the wave to pop.
-        const bool in0 = h0 && n0 == 0;
-        const bool in1 = h1 && n1 == 0;
-        const bool desc = in0 || in1;
-        if (!desc && sp == 0) break;
-        const int top = sp - 1;  // the entry a pop takes (read before this step's push)
-        int popv = stack[(DL >= DEPTH || top < DL ? (top > 0 ? top : 0) : DL - 1) * TR_BLOCK];
-        if (DL < DEPTH && !desc && top >= DL) popv = *(volatile int*)&deep[top - DL];
-        if (in0 && in1) {
-            if (DL >= DEPTH || sp < DL) stack[sp * TR_BLOCK] = l1;
-            else deep[sp - DL] = l1;
-        }
-        node = desc ? (in0 ? l0 : l1) : popv;
-        sp += (in0 && in1) ? 1 : (desc ? 0 : -1);
+    const NodeBoxes b = load_node(nodes, node);
+    if (STATS) ctr.nodes++;
+    const int l0 = b.l0, l1 = b.l1, n0 = b.n0, n1 = b.n1;
+    const bool h0 = box(b.lo0[0], b.lo0[1], b.lo0[2], b.hi0[0], b.hi0[1], b.hi0[2]);
+    const bool h1 = box(b.lo1[0], b.lo1[1], b.lo1[2], b.hi1[0], b.hi1[1], b.hi1[2]);
+    if (h0 && n0 > 0) leaf(l0, n0);
+    if (h1 && n1 > 0) leaf(l1, n1);
+    // Next node without branches around the choice: descend (pushing the second child when
+    // both are inner nodes) or pop. Lanes that descend and lanes that pop then stay in one
+    // loop; a branch per case lets the compiler nest a descend loop inside a pop loop,
+    // where a lane that must pop waits for every lane of the wave to pop.
+    const bool in0 = h0 && n0 == 0;
+    const bool in1 = h1 && n1 == 0;
+    const bool desc = in0 || in1;
+    if (!desc && sp == 0) return false;
+    const int top = sp - 1;  // the entry a pop takes (read before this step's push)
+    int popv = stack[(DL >= DEPTH || top < DL ? (top > 0 ? top : 0) : DL - 1) * TR_BLOCK];
+    if (DL < DEPTH && !desc && top >= DL) popv = *(volatile int*)&deep[top - DL];
+    if (in0 && in1) {
+        if (DL >= DEPTH || sp < DL) stack[sp * TR_BLOCK] = l1;
+        else deep[sp - DL] = l1;
+    }
+    node = desc ? (in0 ? l0 : l1) : popv;
+    sp += (in0 && in1) ? 1 : (desc ? 0 : -1);
+    return true;
+}
+
+// Margins of the kz-slab test for hits in [thr, tmax].
+__device__ __forceinline__ float seg_lo_t(float thr) { return thr - fabsf(thr) * 9.5367431640625e-7f; }  // 2^-20
+__device__ __forceinline__ float seg_hi_t(float tmax) { return tmax + fabsf(tmax) * 9.5367431640625e-7f; }
+
+// All hits of one pass of a segment walk at or above thr, from node root down, into the
+// buffer; returns their number.
+template <int DEPTH, bool STATS, int KB, int DL>
+__device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ nodes,
+                                                    const TriRec* __restrict__ tris, const float3 o,
+                                                    const float3 inv, const RayPre& rp, float thr,
+                                                    float tmax, int* stack, float (&bt)[KB],
+                                                    uint32_t (&bi)[KB], int (&bo)[KB],
+                                                    TraceCounters& ctr, int root = 0) {
+    const float okz = get3(o, rp.kz);
+    const float lo_t = seg_lo_t(thr), hi_t = seg_hi_t(tmax);
+    uint32_t total = 0;
+    int sp = 0;
+    int node = root;
+    SegDeep<DEPTH, DL> deep;
+    while (seg_visit<DEPTH, STATS, KB, DL>(nodes, tris, o, inv, rp, okz, thr, lo_t, tmax, hi_t, node, sp, stack,
+                                           deep, bt, bi, bo, total, ctr)) {
     }
     return total;
 }
@@ -567,6 +588,43 @@ __device__ __forceinline__ bool edge_may_hit(const OccArgs& occ, const float3 s,
     return occ_segment_may_hit(occ, s, en, 2e-6f, words, start);
 }
 
+// The chain of a segment walk inside the sorted buffer of one pass (total hits at or above
+// thr were offered): the members' buffer positions as a bit mask, in order; iter counts
+// them and thr moves past each by the chain rule.
+template <int KB>
+__device__ __forceinline__ uint32_t seg_chain(const float (&bt)[KB], uint32_t total, int chain_rule, int max_iter,
+                                              float tmax, int& iter, float& thr) {
+    const float eps = 1e-6f;
+    uint32_t chain = 0;
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
+            chain |= 1u << k;
+            ++iter;
+            float nthr;
+            if (chain_rule == CHAIN_EPS) {
+                nthr = bt[k] + eps;
+                if (nthr <= bt[k]) nthr = bt[k] + 2e-4f;
+            } else {
+                nthr = nextafterf(bt[k], tmax);
+                if (nthr <= bt[k]) nthr = bt[k] + eps;
+            }
+            thr = nthr;
+        }
+    }
+    return chain;
+}
+
+// bo[k] for a k known only at run time (selects, so the buffer stays in registers).
+template <int KB>
+__device__ __forceinline__ int kbuf_at(const int (&bo)[KB], int k) {
+    int v = bo[0];
+#pragma unroll
+    for (int q = 1; q < KB; ++q)
+        if (k == q) v = bo[q];
+    return v;
+}
+
 template <int DEPTH, bool STATS, int DL = DEPTH, typename Emit>
 __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
                                         const TriRec* __restrict__ tris, const OccArgs& occ,
@@ -593,32 +651,8 @@ __device__ __forceinline__ int edge_ray(const BNode* __restrict__ nodes,
             collect_segment<DEPTH, STATS, KBUF_EDGE, DL>(nodes, tris, s, inv, rp, thr, tmax, stack, bt, bi, bo, ctr,
                                                          root);
         // the chain over the buffer first, then one emission site for its members in order
-        uint32_t chain = 0;
-#pragma unroll
-        for (int k = 0; k < KBUF_EDGE; ++k) {
-            if ((uint32_t)k < total && iter < max_iter && thr <= tmax && bt[k] >= thr) {
-                chain |= 1u << k;
-                ++iter;
-                float nthr;
-                if (chain_rule == CHAIN_EPS) {
-                    nthr = bt[k] + eps;
-                    if (nthr <= bt[k]) nthr = bt[k] + 2e-4f;
-                } else {
-                    nthr = nextafterf(bt[k], tmax);
-                    if (nthr <= bt[k]) nthr = bt[k] + eps;
-                }
-                thr = nthr;
-            }
-        }
-        while (chain) {
-            const int k = __builtin_ctz(chain);
-            chain &= chain - 1;
-            int obj = bo[0];
-#pragma unroll
-            for (int q = 1; q < KBUF_EDGE; ++q)
-                if (k == q) obj = bo[q];
-            emit(obj);
-        }
+        for (uint32_t chain = seg_chain(bt, total, chain_rule, max_iter, tmax, iter, thr); chain; chain &= chain - 1)
+            emit(kbuf_at(bo, __builtin_ctz(chain)));
         if (total <= (uint32_t)KBUF_EDGE || iter >= max_iter || thr > tmax) break;
         if (STATS) ctr.restarts++;
     }
