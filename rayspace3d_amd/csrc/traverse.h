@@ -429,10 +429,10 @@ __device__ __forceinline__ bool ht_contains(const HashArgs& h, uint64_t key) {
     return false;
 }
 
-// One node visit of a segment walk (ray o + t d, inv = 1/d, rp its shear, okz = o along
-// rp.kz) over hits in [thr, tmax]: tests both children of node, offers the hits of leaf
-// children to the buffer (total counts them) and moves node to the next node to visit.
-// Returns false when the pass is complete. The stack's first DL entries live in LDS
+// One node visit of a segment walk (ray o + t d, inv = 1/d, rp its shear) over hits in
+// [thr, tmax]: tests both children of node, offers the hits of leaf children to the
+// buffer (total counts them) and moves node to the next node to visit. Returns false when
+// the pass is complete. The stack's first DL entries live in LDS
 // (stride TR_BLOCK), deeper ones in deep, a per-lane private array: a walk from a start
 // node rarely nests deep, and a small LDS stack leaves room for more waves per CU.
 // The computed t of any reported hit lies within the kz extent of its triangle's vertices
@@ -443,22 +443,24 @@ using SegDeep = int[DEPTH > DL ? DEPTH - DL : 1];
 
 template <int DEPTH, bool STATS, int KB, int DL>
 __device__ __forceinline__ bool seg_visit(const BNode* __restrict__ nodes, const TriRec* __restrict__ tris,
-                                          const float3 o, const float3 inv, const RayPre& rp, float okz, float thr,
+                                          const float3 o, const float3 inv, const RayPre& rp, float thr,
                                           float lo_t, float tmax, float hi_t, int& node, int& sp, int* stack,
                                           SegDeep<DEPTH, DL>& deep, float (&bt)[KB], uint32_t (&bi)[KB],
                                           int (&bo)[KB], uint32_t& total, TraceCounters& ctr) {
-    // box test of one child (its six bounds as values: no array or pointer selects, which
-    // led the compiler to keep node fields in scratch memory)
+    // Box test of one child (its six bounds as values: no array or pointer selects, which
+    // led the compiler to keep node fields in scratch memory). The kz slab's values
+    // (bound - o[kz]) * Sz are the slab values of axis kz, bit for bit: Sz and inv[kz] are
+    // both 1.0f / d[kz], and d[kz] is never 0 (|d[kz]| is the largest component).
     auto box = [&](float lx, float ly, float lz, float hx, float hy, float hz) -> bool {
         const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
         const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
         const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
-        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-        const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        const float klo = rp.kz == 0 ? lx : (rp.kz == 1 ? ly : lz);
-        const float khi = rp.kz == 0 ? hx : (rp.kz == 1 ? hy : hz);
-        const float k0 = (klo - okz) * rp.Sz, k1 = (khi - okz) * rp.Sz;
-        const float kmin = fminf(k0, k1), kmax = fmaxf(k0, k1);
+        const float nx = fminf(tx0, tx1), ny = fminf(ty0, ty1), nz = fminf(tz0, tz1);
+        const float fx = fmaxf(tx0, tx1), fy = fmaxf(ty0, ty1), fz = fmaxf(tz0, tz1);
+        const float tn = fmaxf(fmaxf(nx, ny), nz);
+        const float tf = fminf(fminf(fx, fy), fz);
+        const float kmin = rp.kz == 0 ? nx : (rp.kz == 1 ? ny : nz);
+        const float kmax = rp.kz == 0 ? fx : (rp.kz == 1 ? fy : fz);
         return (tn <= tf + fabsf(tf) * 9.5367431640625e-7f) && kmax >= lo_t && kmin <= hi_t && lx <= hx;
     };
     auto leaf = [&](int first, int cnt) {
@@ -516,13 +518,12 @@ __device__ __forceinline__ uint32_t collect_segment(const BNode* __restrict__ no
                                                     float tmax, int* stack, float (&bt)[KB],
                                                     uint32_t (&bi)[KB], int (&bo)[KB],
                                                     TraceCounters& ctr, int root = 0) {
-    const float okz = get3(o, rp.kz);
     const float lo_t = seg_lo_t(thr), hi_t = seg_hi_t(tmax);
     uint32_t total = 0;
     int sp = 0;
     int node = root;
     SegDeep<DEPTH, DL> deep;
-    while (seg_visit<DEPTH, STATS, KB, DL>(nodes, tris, o, inv, rp, okz, thr, lo_t, tmax, hi_t, node, sp, stack,
+    while (seg_visit<DEPTH, STATS, KB, DL>(nodes, tris, o, inv, rp, thr, lo_t, tmax, hi_t, node, sp, stack,
                                            deep, bt, bi, bo, total, ctr)) {
     }
     return total;
