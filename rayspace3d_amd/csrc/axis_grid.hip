@@ -3,23 +3,25 @@
 // Point containment casts rays along one axis, so every ray is a vertical half-line over a
 // point of the perpendicular plane. The grid covers that plane (the BVH's padded bounds)
 // with square cells; cell c lists every triangle whose padded box -- the BVH's leaf box,
-// computed by the same prim_padded_box -- overlaps c in the plane, as a self-contained
-// 40-byte record (vertices and original index), ordered by the triangle's top (its largest
-// vertex coordinate along the axis), largest first. A ray reads its cell's entry in the
-// cell table (first record, and that record's top, so that a ray above it reads no record
-// at all) and tests the triangles of the records that reach its origin. They include every
-// triangle whose leaf box the BVH walk would accept (ancestor boxes contain the leaf
-// boxes); the padding guarantees that no triangle outside its box can be hit, so the hits
-// are the BVH walk's, and the hit buffer, restarts and chain are the BVH kernel's: the
-// results are the same. The gain is the walk: one range lookup and a few records instead of
-// ~30 dependent node fetches. The object of a point's answer comes from a per-triangle map.
+// computed by the same prim_padded_box -- overlaps c in the plane and whose projected
+// footprint, widened by the fp32 test's error bound, meets c (ag_tri_meets_cell), as a
+// self-contained 40-byte record (vertices and original index), ordered by the triangle's
+// top (its largest vertex coordinate along the axis), largest first. A ray reads its cell's
+// entry in the cell table (first record, and that record's top, so that a ray above it
+// reads no record at all) and tests the triangles of the records that reach its origin.
+// They include every triangle the fp32 test can report for a ray of the cell: the padding
+// guarantees that no triangle is hit from outside its box, the footprint test that none is
+// hit from a cell its widened footprint misses. So the hits are the BVH walk's, and the
+// hit buffer, restarts and chain are the BVH kernel's: the results are the same. The gain
+// is the walk: one range lookup and a few records instead of ~30 dependent node fetches.
+// The object of a point's answer comes from a per-triangle map.
 //
 // Build (per BVH and axis, once): the cell edge is a fraction of the triangles' mean padded
 // extent in the plane (RS3D_OPT_GRID_CELL_PCT, widened while the records exceed their
-// budget); records per triangle from the cell span of its box, budget and list-length checks
-// (rejected grids leave the launches on the BVH), one
-// (cell << 32 | descending top) key per record, a stable radix sort, per-cell counts
-// and their scan as the cell ranges, and a gather of the records.
+// budget); records per triangle from the cells of its box that its footprint meets, budget
+// and list-length checks (rejected grids leave the launches on the BVH), one
+// (cell << 32 | descending top) key per record, a stable radix sort, per-cell counts and
+// their scan as the cell ranges, and a gather of the records.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -45,6 +47,9 @@ constexpr double AG_MAX_UNIFORM_LIST = 64.0;    // records per cell, averaged ov
 constexpr double AG_MAX_WEIGHTED_LIST = 256.0;  // averaged over records (dense spots)
 constexpr uint64_t AG_SMALL_RECORDS = 1ull << 19;  // grids this small take finer cells
 constexpr double AG_MIN_CELL_PCT = 10.0;
+// Triangles whose box spans more cells than this are listed in every cell of the box (the
+// footprint test would cost more than it saves).
+constexpr uint64_t AG_FOOTPRINT_MAX_SPAN = 4096;
 
 // Unsigned key whose order is the float order.
 __device__ __forceinline__ uint32_t ag_ord(float f) {
@@ -62,6 +67,73 @@ __device__ __forceinline__ void ag_cells(const TriRec& r, const AGArgs& G, float
     cu1 = ag_cell(phi[U], G.lo_u, G.inv_u, G.nu);
     cw0 = ag_cell(plo[W], G.lo_w, G.inv_w, G.nw);
     cw1 = ag_cell(phi[W], G.lo_w, G.inv_w, G.nw);
+}
+
+// Whether the square of cell (cu, cw) can hold a ray that the fp32 axis triangle test
+// (tri_hit_axis) reports for the triangle: false only when the square lies strictly outside
+// one edge line of the triangle's projection by more than that edge's error bound, so
+// listing the triangle only in cells that pass loses no hit.
+//  * The square is widened by 2^-7 of a cell: the cell mapping ag_cell rounds (x - lo) and
+//    its product with inv, a relative error under 2^-23 of at most 32,768 cells (2^-8 of a
+//    cell). Boundary cells, which take every ray beyond the grid, are never pruned.
+//  * The test decides the side of an edge BC by the sign of fl(Cx By) - fl(Cy Bx), C and B
+//    the vertices less the origin (each rounded once): a wrong sign needs |U| <= 6 2^-24
+//    |C - o| |B - o|, i.e. a distance under 2^-21.4 |C - o| |B - o| / |BC| from the line.
+//    A pruned square is inside the padded box, so |C - o|, |B - o| <= D = Lmax + 4 pad
+//    (Lmax the longest projected edge, pad the box padding): the edge's bound is taken as
+//    2^-16 D^2 / |BC|. Edges shorter than Lmax / 256, and degenerate projections, prune
+//    nothing. All of it in double.
+template <int AX>
+__device__ __forceinline__ bool ag_tri_meets_cell(const TriRec& r, int cu, int cw, const AGArgs& G) {
+    constexpr int U = (AX + 1) % 3, W = (AX + 2) % 3;
+    if (cu <= 0 || cw <= 0 || cu >= G.nu - 1 || cw >= G.nw - 1) return true;
+    const float3 v[3] = {make_float3(r.v0.x, r.v0.y, r.v0.z), make_float3(r.v1.x, r.v1.y, r.v1.z),
+                         make_float3(r.v2.x, r.v2.y, r.v2.z)};
+    double pu[3], pw[3];
+    float mag = 0.0f, ext = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        pu[k] = (double)get3(v[k], U);
+        pw[k] = (double)get3(v[k], W);
+    }
+    {
+        const float lo[3] = {fminf(fminf(v[0].x, v[1].x), v[2].x), fminf(fminf(v[0].y, v[1].y), v[2].y),
+                             fminf(fminf(v[0].z, v[1].z), v[2].z)};
+        const float hi[3] = {fmaxf(fmaxf(v[0].x, v[1].x), v[2].x), fmaxf(fmaxf(v[0].y, v[1].y), v[2].y),
+                             fmaxf(fmaxf(v[0].z, v[1].z), v[2].z)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ext = fmaxf(ext, hi[k] - lo[k]);
+            mag = fmaxf(mag, fmaxf(fabsf(lo[k]), fabsf(hi[k])));
+        }
+    }
+    const double area2 = (pu[1] - pu[0]) * (pw[2] - pw[0]) - (pw[1] - pw[0]) * (pu[2] - pu[0]);
+    double l2max = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double du = pu[(k + 1) % 3] - pu[k], dw = pw[(k + 1) % 3] - pw[k];
+        l2max = fmax(l2max, du * du + dw * dw);
+    }
+    if (!(fabs(area2) > l2max * 0x1p-40) || !(l2max < 1e300)) return true;  // degenerate or not finite
+    const double d = sqrt(l2max) + 4.0 * (double)box_pad(mag, ext);
+    const double hu = 1.0 / (double)G.inv_u, hw = 1.0 / (double)G.inv_w;
+    const double u0 = (double)G.lo_u + cu * hu - hu * 0x1p-7, u1 = (double)G.lo_u + (cu + 1) * hu + hu * 0x1p-7;
+    const double w0 = (double)G.lo_w + cw * hw - hw * 0x1p-7, w1 = (double)G.lo_w + (cw + 1) * hw + hw * 0x1p-7;
+    const double sgn = area2 > 0.0 ? 1.0 : -1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double au = pu[k], aw = pw[k];
+        const double eu = pu[(k + 1) % 3] - au, ew = pw[(k + 1) % 3] - aw;
+        const double len2 = eu * eu + ew * ew;
+        if (len2 < l2max * 0x1p-16) continue;
+        // the corner farthest to the triangle's side of the line (sgn * cross(e, q - a) > 0 inside)
+        const double c00 = sgn * (eu * (w0 - aw) - ew * (u0 - au)), c10 = sgn * (eu * (w0 - aw) - ew * (u1 - au));
+        const double c01 = sgn * (eu * (w1 - aw) - ew * (u0 - au)), c11 = sgn * (eu * (w1 - aw) - ew * (u1 - au));
+        const double best = fmax(fmax(c00, c10), fmax(c01, c11));
+        // best / |e| is the corner's distance from the line; the bound 2^-16 d^2 / |e| times |e|
+        if (best < -0x1p-16 * d * d) return false;
+    }
+    return true;
 }
 
 // Sum of the triangles' padded extents in the plane (the larger of the two), for the cell size.
@@ -94,8 +166,14 @@ __global__ __launch_bounds__(256) void k_ag_span(const TriRec* __restrict__ tris
     if (i < n) {
         float plo[3], phi[3];
         int cu0, cu1, cw0, cw1;
-        ag_cells<AX>(tris[i], G, plo, phi, cu0, cu1, cw0, cw1);
+        const TriRec r = tris[i];
+        ag_cells<AX>(r, G, plo, phi, cu0, cu1, cw0, cw1);
         v = (unsigned long long)(cu1 - cu0 + 1) * (unsigned long long)(cw1 - cw0 + 1);
+        if (v <= AG_FOOTPRINT_MAX_SPAN) {  // the cells of the box that the footprint meets
+            v = 0;
+            for (int cw = cw0; cw <= cw1; ++cw)
+                for (int cu = cu0; cu <= cu1; ++cu) v += ag_tri_meets_cell<AX>(r, cu, cw, G) ? 1 : 0;
+        }
         span[i] = v > 0xffffffffull ? 0xffffffffu : (uint32_t)v;
     }
     v = block_sum_256(v);
@@ -116,8 +194,10 @@ __global__ __launch_bounds__(256) void k_ag_emit(const TriRec* __restrict__ tris
                             get3(make_float3(r.v2.x, r.v2.y, r.v2.z), AX));
     const uint64_t low = 0xffffffffu - ag_ord(top);  // the record's top (ag_top), descending
     uint64_t k = off[i];
+    const bool test = (uint64_t)(cu1 - cu0 + 1) * (uint64_t)(cw1 - cw0 + 1) <= AG_FOOTPRINT_MAX_SPAN;
     for (int cw = cw0; cw <= cw1; ++cw)
         for (int cu = cu0; cu <= cu1; ++cu) {
+            if (test && !ag_tri_meets_cell<AX>(r, cu, cw, G)) continue;  // as counted by k_ag_span
             const uint64_t cell = ag_cell_id(cu, cw, G.ntu);
             keys[k] = (cell << 32) | low;
             vals[k] = (uint32_t)i;

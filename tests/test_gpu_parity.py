@@ -860,6 +860,62 @@ def test_axis_grid_cell_size_invariant(ctx, pct):
     b.free()
 
 
+@pytest.mark.parametrize("axis,pct,centre", [(2, 35, 0.0), (0, 10, 400.0), (1, 100, 400.0), (2, 20, -3000.0)])
+def test_axis_grid_footprint_edges_and_slivers(ctx, axis, pct, centre):
+    # Cells list a triangle only where its projected footprint, widened by the fp32 edge
+    # test's error bound, meets them. Triangles spanning many cells (slivers and ordinary
+    # ones, near the origin and far from it), and rays on, a few ulps beside and 1e-6
+    # beside their projected edges and at their vertices, must give the walk's and the
+    # oracle's answers.
+    rng = np.random.Generator(np.random.PCG64(40 + axis + pct))
+    tris = []
+    for k in range(2400):
+        c = centre + rng.uniform(-20.0, 20.0, 3)
+        size = 10.0 ** rng.uniform(-0.5, 1.0)
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        if k % 3 == 0:  # sliver: the third vertex close to the first edge
+            b = a * rng.uniform(0.0, 1.0) + rng.normal(size=3) * 10.0 ** rng.uniform(-6.0, -2.0)
+        tris.append([c, c + a * size, c + b * size])
+    V = np.asarray(tris, np.float64).reshape(-1, 3).astype(np.float32)
+    I = np.arange(len(V), dtype=np.uint32).reshape(-1, 3)
+    O = (np.arange(len(I)) % 97).astype(np.int32)
+    mesh = gen.Mesh(V, I, O, num_objects=97)
+    U, W = (axis + 1) % 3, (axis + 2) % 3
+    pts = []
+    for t in range(len(I)):
+        P = V[I[t]]
+        below = np.float32(P[:, axis].min() - 1.0)
+        for e in range(3):
+            A, B = P[e], P[(e + 1) % 3]
+            for s in np.float32([0.0, 0.25, 0.5, 1.0 / 3.0, 0.999]):
+                q = np.empty(3, np.float32)
+                q[axis] = below
+                for ax in (U, W):
+                    q[ax] = np.float32(A[ax] + s * (B[ax] - A[ax]))
+                pts.append(q.copy())
+                for d in (np.float32(1e-6), np.float32(-1e-6)):  # beside the edge, both ways
+                    r = q.copy()
+                    r[U] = np.float32(r[U] + d * max(1.0, abs(float(r[U]))))
+                    pts.append(r)
+                r = q.copy()
+                r[W] = np.nextafter(np.nextafter(r[W], np.float32(np.inf)), np.float32(np.inf))
+                pts.append(r)
+    pts = np.asarray(pts, np.float32)
+    saved = ctx.get_option(_lib.OPT_GRID_CELL_PCT)
+    ctx.set_option(_lib.OPT_GRID_CELL_PCT, pct)
+    try:
+        b, info, g, v = _grid_and_bvh(ctx, mesh, pts, axis)
+    finally:
+        ctx.set_option(_lib.OPT_GRID_CELL_PCT, saved)
+    print(f"footprint: {len(I)} triangles, {len(pts)} points, axis {axis}, pct {pct}: {info}", flush=True)
+    assert info["state"] == 1 and info["entries"] > (2 if pct >= 100 else 4) * len(I)  # over several cells
+    ref = orc.points_contain(V, I, O, pts, axis=axis, threads=8)
+    assert np.array_equal(g, v), np.count_nonzero(g != v)
+    assert np.array_equal(g, ref), np.count_nonzero(g != ref)
+    assert (g >= 0).sum() > 100
+    b.free()
+
+
 def test_axis_grid_rejected_falls_back(ctx):
     # 150 stacked copies of one large square: every ray's cell would list 300 records, so
     # the grid is rejected and launches stay on the BVH with the same results
