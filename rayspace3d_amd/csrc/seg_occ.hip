@@ -60,8 +60,73 @@ __global__ __launch_bounds__(256) void k_so_size(const TriRec* __restrict__ tris
 
 constexpr int SO_MAX_SPAN = 4096;  // cells of one triangle box; beyond, the bits are rejected
 
-// One triangle per thread: set the bits of every cell its padded box overlaps. A box over
-// more than SO_MAX_SPAN cells counts in *big and marks nothing (the caller drops the bits).
+// Whether cell (x, y, z) can hold a point that an fp32 test reports on the triangle: the
+// separating-axis test (box axes, triangle normal, the nine edge-axis products) of the
+// triangle against the cell widened by the triangle's box padding (a reported hit lies
+// within it of the triangle: box_pad bounds the fp32 tests' error, as for the padded box
+// itself) plus 2^-20 (nx + ny + nz + 1024) of a cell (occ_cell, which places a segment's
+// points, rounds (x - org) inv with a relative error under 2^-23, at most 2^-23 n cells on
+// an axis of n cells), all in double. Non-finite triangles meet every cell.
+__device__ __forceinline__ bool so_tri_meets_cell(const TriRec& r, int x, int y, int z, const OccArgs& g) {
+    const double v[3][3] = {{r.v0.x, r.v0.y, r.v0.z}, {r.v1.x, r.v1.y, r.v1.z}, {r.v2.x, r.v2.y, r.v2.z}};
+    float mag = 0.0f, ext = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float lo = fminf(fminf((float)v[0][k], (float)v[1][k]), (float)v[2][k]);
+        const float hi = fmaxf(fmaxf((float)v[0][k], (float)v[1][k]), (float)v[2][k]);
+        ext = fmaxf(ext, hi - lo);
+        mag = fmaxf(mag, fmaxf(fabsf(lo), fabsf(hi)));
+    }
+    if (!(mag < 1e30f)) return true;
+    const double h = 1.0 / (double)g.inv;
+    const double half = 0.5 * h + h * 0x1p-20 * (double)(g.nx + g.ny + g.nz + 1024) + (double)box_pad(mag, ext);
+    const double c[3] = {(double)g.org_x + (x + 0.5) * h, (double)g.org_y + (y + 0.5) * h,
+                         (double)g.org_z + (z + 0.5) * h};
+    double p[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p[i][k] = v[i][k] - c[k];
+    // box axes
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (fmin(fmin(p[0][k], p[1][k]), p[2][k]) > half || fmax(fmax(p[0][k], p[1][k]), p[2][k]) < -half)
+            return false;
+    }
+    const double e[3][3] = {{p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]},
+                            {p[2][0] - p[1][0], p[2][1] - p[1][1], p[2][2] - p[1][2]},
+                            {p[0][0] - p[2][0], p[0][1] - p[2][1], p[0][2] - p[2][2]}};
+    // triangle normal
+    {
+        const double n0 = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+        const double n1 = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+        const double n2 = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+        const double d = n0 * p[0][0] + n1 * p[0][1] + n2 * p[0][2];
+        if (fabs(d) > half * (fabs(n0) + fabs(n1) + fabs(n2))) return false;
+    }
+    // the nine products of a box axis with a triangle edge
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            // axis = unit k x e[j]
+            double a[3] = {0.0, 0.0, 0.0};
+            const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+            a[k1] = -e[j][k2];
+            a[k2] = e[j][k1];
+            const double q0 = a[0] * p[0][0] + a[1] * p[0][1] + a[2] * p[0][2];
+            const double q1 = a[0] * p[1][0] + a[1] * p[1][1] + a[2] * p[1][2];
+            const double q2 = a[0] * p[2][0] + a[1] * p[2][1] + a[2] * p[2][2];
+            const double rad = half * (fabs(a[0]) + fabs(a[1]) + fabs(a[2]));
+            if (fmin(fmin(q0, q1), q2) > rad || fmax(fmax(q0, q1), q2) < -rad) return false;
+        }
+    }
+    return true;
+}
+
+// One triangle per thread: set the bits of every cell its padded box overlaps and
+// so_tri_meets_cell accepts. A box over more than SO_MAX_SPAN cells counts in *big and marks
+// nothing (the caller drops the bits).
 __global__ __launch_bounds__(256) void k_so_mark(const TriRec* __restrict__ tris, uint64_t n, OccArgs g,
                                                  uint32_t* __restrict__ bits, unsigned long long* __restrict__ big) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,20 +141,16 @@ __global__ __launch_bounds__(256) void k_so_mark(const TriRec* __restrict__ tris
         y0 = occ_cell(plo[1], g.org_y, g.inv, g.ny), y1 = occ_cell(phi[1], g.org_y, g.inv, g.ny);
         z0 = occ_cell(plo[2], g.org_z, g.inv, g.nz), z1 = occ_cell(phi[2], g.org_z, g.inv, g.nz);
     }
-    // triangles in Morton order: a lane whose cell range equals the previous lane's leaves
-    // the marking to that lane (same cells)
-    const int lane = threadIdx.x & 63;
-    const bool same = lane > 0 && __shfl_up(x0, 1, 64) == x0 && __shfl_up(x1, 1, 64) == x1 &&
-                      __shfl_up(y0, 1, 64) == y0 && __shfl_up(y1, 1, 64) == y1 &&
-                      __shfl_up(z0, 1, 64) == z0 && __shfl_up(z1, 1, 64) == z1;
-    if (!valid || same) return;
+    if (!valid) return;
     if ((uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) * (uint64_t)(z1 - z0 + 1) > (uint64_t)SO_MAX_SPAN) {
         atomicAdd(acc_stripe(big), 1ull);
         return;
     }
+    const TriRec r = tris[i];
     for (int z = z0; z <= z1; ++z)
         for (int y = y0; y <= y1; ++y)
             for (int x = x0; x <= x1; ++x) {
+                if (!so_tri_meets_cell(r, x, y, z, g)) continue;
                 const uint64_t c = ((uint64_t)z * g.ny + y) * g.nx + x;
                 const uint32_t m = 1u << (c & 31);
                 if (!(bits[c >> 5] & m)) atomicOr(&bits[c >> 5], m);  // a read first: most bits are set already
