@@ -145,10 +145,11 @@ int rs3d_trace_points_stats(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* poi
 
 /* Axis grid (no counterpart in the reference, whose rays run on RT cores): for rays along
  * +axis, a uniform 2-D grid over the plane perpendicular to the axis lists, per cell, the
- * triangles whose padded box (the BVH's leaf box) overlaps the cell, ordered by the box's
- * upper end along the axis. A ray reads its cell and tests the listed triangles whose box
- * contains its line and reaches its origin -- the same candidates as the BVH walk, so the
- * same hits and results -- in one lookup instead of a walk down the tree. The cell edge is
+ * triangles whose padded box (the BVH's leaf box) overlaps the cell and whose projected
+ * footprint, widened by the fp32 triangle test's error bound, meets it, ordered by the
+ * triangle's top along the axis. A ray reads its cell and tests the listed triangles that
+ * reach its origin -- every triangle the test can report for the ray, so the same hits and
+ * results as the BVH walk -- in one lookup instead of a walk down the tree. The cell edge is
  * RS3D_OPT_GRID_CELL_PCT percent of the triangles' mean extent in the plane (default 35),
  * widened while the records would exceed 16 per triangle + 2^20. A grid is built once per
  * (BVH, axis), lazily by the first point launch along that axis or here, and rejected (the
@@ -248,9 +249,9 @@ int rs3d_edge_write(rs3d_ctx* ctx, const rs3d_bvh* target, const rs3d_edges* edg
 
 /* Segment occupancy bits (no counterpart in the reference, whose rays run on RT cores): one
  * bit per cube cell of a uniform 3-D grid over the target BVH's padded bounds, set where a
- * padded triangle box (the BVH's leaf box) overlaps the cell. An edge ray whose segment
- * box (widened by 2^-14 of its coordinates) covers no set bit cannot hit the target and
- * skips the walk; results are unchanged. Built once per BVH by the first edge launch
+ * triangle, widened by its box padding, meets the cell. An edge ray whose segment box
+ * (widened by 2^-14 of its coordinates and 2e-6) covers no set bit cannot hit the target
+ * and skips the walk; results are unchanged. Built once per BVH by the first edge launch
  * against it, or here; freed with the BVH. state: 1 built, 2 rejected (a triangle box over
  * more than 4096 cells, or non-finite bounds: edge rays walk the tree), 3 refused for lack
  * of device memory (edge rays walk the tree; this call tries again). */
