@@ -237,6 +237,43 @@ __global__ __launch_bounds__(256) void k_ag_gather(const TriRec* __restrict__ tr
     p[4] = make_float2(r.v2.z, r.v0.w);  // .w of v0: the original triangle index
 }
 
+// Sort of the (cell << 32 | descending top, triangle) records by cell, then descending top,
+// then triangle (the order of a stable sort of the keys in emission order), in two passes:
+// k_ag_bucket places every record in its cell's range (in any order), k_ag_rank moves each
+// to its rank among its cell's records. Lists are short (their mean weighted by length is
+// bounded by the acceptance checks), so a rank costs one pass over the list.
+__global__ __launch_bounds__(256) void k_ag_bucket(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                   uint64_t m, const uint32_t* __restrict__ start,
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ bkey,
+                                                   uint32_t* __restrict__ bval, uint32_t* __restrict__ bcell) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t k = keys[i];
+    const uint32_t cell = (uint32_t)(k >> 32);
+    const uint32_t pos = start[cell] + atomicAdd(&cursor[cell], 1u);
+    bkey[pos] = (uint32_t)k;
+    bval[pos] = vals[i];
+    bcell[pos] = cell;
+}
+
+__global__ __launch_bounds__(256) void k_ag_rank(const uint32_t* __restrict__ bkey, const uint32_t* __restrict__ bval,
+                                                 const uint32_t* __restrict__ bcell, uint64_t m,
+                                                 const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ counts, uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t cell = bcell[i], key = bkey[i], val = bval[i];
+    const uint32_t s0 = start[cell], cnt = counts[cell];
+    uint32_t rank = 0;
+    for (uint32_t j = s0; j < s0 + cnt; ++j) {
+        const uint32_t kj = bkey[j], vj = bval[j];
+        rank += (kj < key || (kj == key && vj < val)) ? 1u : 0u;
+    }
+    keys[s0 + rank] = ((uint64_t)cell << 32) | key;
+    vals[s0 + rank] = val;
+}
+
 // The object of each original triangle (the records keep only the triangle's index).
 __global__ __launch_bounds__(256) void k_ag_objmap(const TriRec* __restrict__ tris, uint64_t n,
                                                    int32_t* __restrict__ obj) {
@@ -403,15 +440,22 @@ __global__ __launch_bounds__(64) void k_fr_finish(unsigned long long* __restrict
 
 namespace {
 
+// Build temporaries, from the context's stream-ordered pool when it has one: freed on the
+// build's stream into the pool, which keeps them for the next build instead of unmapping
+// them (hipFree of the temporaries took ~2 ms of a 9 ms build).
 struct AgTmp {
+    rs3d_ctx* ctx;
     std::vector<void*> ptrs;
+    explicit AgTmp(rs3d_ctx* c) : ctx(c) {}
     ~AgTmp() {
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : ptrs) (void)(ctx->tmp_pool ? hipFreeAsync(p, ctx->stream) : hipFree(p));
     }
     template <typename T>
     int get(T** out, size_t count) {
         void* p = nullptr;
-        const hipError_t e = hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+        const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+        const hipError_t e = ctx->tmp_pool ? hipMallocFromPoolAsync(&p, bytes, ctx->tmp_pool, ctx->stream)
+                                           : hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             set_error("axis grid: hipMalloc(%zu) failed: %s", count * sizeof(T), hipGetErrorString(e));
             return RS3D_E_NOMEM;
@@ -459,7 +503,7 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
     if (eu > 0 && ew > 0) h_min = std::sqrt(eu * ew / max_cells);
     h_min = std::max({h_min, eu / AG_MAX_DIM, ew / AG_MAX_DIM});
     double h = std::max(mean_ext * (double)ctx->opt_grid_cell_pct / 100.0, h_min);
-    AgTmp tmp;
+    AgTmp tmp(ctx);
     uint32_t *span, *off;
     RS3D_TRY(tmp.get(&span, n));
     RS3D_TRY(tmp.get(&off, n));
@@ -600,9 +644,16 @@ int build_ax(rs3d_ctx* ctx, rs3d_bvh* bvh, AxisGrid** out) {
         k_ag_tsum<<<nbt, 256, 0, s>>>(words, nt, g->tsum, gcount);
         RS3D_TRY(scan_u32_exclusive(ctx, gcount, gbase, ng));
         if (total) {
-            RS3D_TRY(radix_sort_u64_pairs(ctx, keys, vals, total));
             unsigned nbe;
             RS3D_TRY(grid_blocks(total, 256, &nbe));
+            uint32_t *cursor, *bkey, *bval, *bcell;
+            RS3D_TRY(tmp.get(&cursor, nc));
+            RS3D_TRY(tmp.get(&bkey, total));
+            RS3D_TRY(tmp.get(&bval, total));
+            RS3D_TRY(tmp.get(&bcell, total));
+            RS3D_TRY(fill_u32(ctx, cursor, nc, 0u));
+            k_ag_bucket<<<nbe, 256, 0, s>>>(keys, vals, total, start, cursor, bkey, bval, bcell);
+            k_ag_rank<<<nbe, 256, 0, s>>>(bkey, bval, bcell, total, start, counts, keys, vals);
             k_ag_gather<<<nbe, 256, 0, s>>>(bvh->tris, vals, keys, start, pstart, total, g->rec);
             k_ag_pad<<<nbc, 256, 0, s>>>(counts, pstart, nc, g->rec);
         }

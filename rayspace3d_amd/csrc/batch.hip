@@ -687,6 +687,7 @@ int rs3d_batch_create(rs3d_ctx* ctx, const rs3d_batch_query* queries, int num_qu
     rs3d_ctx& p = b->priv;
     p.device = ctx->device;
     p.stream = ctx->stream;
+    p.tmp_pool = ctx->tmp_pool;  // borrowed: the caller's context owns it
     p.num_cus = ctx->num_cus;
     p.opt_ray_sort_min = ctx->opt_ray_sort_min;
     p.opt_rays_per_wave = ctx->opt_rays_per_wave;

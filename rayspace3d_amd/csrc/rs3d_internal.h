@@ -230,6 +230,9 @@ struct DevBuf {
 struct rs3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // stream-ordered pool for build temporaries (axis grids), keeping up to
+    // CTX_TMP_POOL_KEEP bytes cached between builds (null: plain hipMalloc / hipFree)
+    hipMemPool_t tmp_pool = nullptr;
     int num_cus = 0;
     // Scratch, reused across calls on this context (calls on one context are serial).
     rs3d::DevBuf sort_keys, sort_vals, sort_hist, sort_csum, scan_tmp, tmp_a, tmp_b, tmp_c, small;
@@ -349,6 +352,9 @@ constexpr int ACC_PITCH = 16;  // 8-byte words from one stripe to the next
 constexpr int ACC_SLOTS = 8;   // accumulators per context (6 and 7: the u64 radix sort)
 // Size of a context's pinned host buffer (`pinned`): one accumulator's stripes.
 constexpr size_t CTX_PINNED_BYTES = (size_t)ACC_STRIPES * ACC_PITCH * sizeof(uint64_t);
+// Bytes of freed build temporaries a context's pool keeps for later builds (the rest goes
+// back to the device); the pool is destroyed with the context.
+constexpr uint64_t CTX_TMP_POOL_KEEP = 4ull << 30;
 // Slot k (< ACC_SLOTS) of the context, every stripe set to init (fill kernel, graph-safe).
 int acc_slot(rs3d_ctx* ctx, int k, uint64_t init, unsigned long long** acc);
 // The stripes of an accumulator, copied to host (synchronises the stream).
