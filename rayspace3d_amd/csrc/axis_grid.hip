@@ -20,8 +20,8 @@
 // extent in the plane (RS3D_OPT_GRID_CELL_PCT, widened while the records exceed their
 // budget); records per triangle from the cells of its box that its footprint meets, budget
 // and list-length checks (rejected grids leave the launches on the BVH), one
-// (cell << 32 | descending top) key per record, a stable radix sort, per-cell counts and
-// their scan as the cell ranges, and a gather of the records.
+// (cell << 32 | descending top) key per record, per-cell counts and their scan as the cell
+// ranges, a bucket and a rank pass that order each list, and a gather of the records.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
