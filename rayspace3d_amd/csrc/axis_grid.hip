@@ -430,19 +430,13 @@ __global__ __launch_bounds__(TR_BLOCK) void k_points_grid_filtered(AGArgs G, con
 // stripes zeroed for the next launch (one wave; the kernel boundary orders it after the
 // launch's atomics).
 __global__ __launch_bounds__(64) void k_fr_finish(unsigned long long* __restrict__ counts,
-                                                  unsigned long long* __restrict__ total,
-                                                  unsigned long long seq) {
+                                                  unsigned long long* __restrict__ total) {
     const int lane = threadIdx.x & 63;
     unsigned long long s = counts[(size_t)lane * ACC_PITCH];
     counts[(size_t)lane * ACC_PITCH] = 0ull;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) {
-        // the sum, then (after a system-scope fence) the call's sequence number, which the
-        // host waits for: seeing it means the sum before it has landed too
-        *reinterpret_cast<volatile unsigned long long*>(total) = s;
-        __threadfence_system();
-        *reinterpret_cast<volatile unsigned long long*>(total + 1) = seq;
-    }
+    // one aligned 8-byte store over the all-ones word the host left there, which it polls
+    if (lane == 0) *reinterpret_cast<volatile unsigned long long*>(total) = s;
 }
 
 namespace {
@@ -770,9 +764,8 @@ int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float*
     }
 }
 
-int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total,
-                            unsigned long long seq) {
-    k_fr_finish<<<1, 64, 0, ctx->stream>>>(counts, total, seq);
+int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total) {
+    k_fr_finish<<<1, 64, 0, ctx->stream>>>(counts, total);
     RS3D_HIP(hipGetLastError());
     return RS3D_OK;
 }

@@ -251,8 +251,7 @@ struct rs3d_ctx {
     // rs3d_filter_refine on a grid: stripes its launches leave zeroed (device), and the word
     // their counts are written to (host-coherent pinned memory)
     rs3d::DevBuf fr_acc;
-    unsigned long long* fr_total = nullptr;  // host-coherent: [0] the counts, [1] the call number
-    unsigned long long fr_seq = 0;           // number of the last filter-refine call
+    unsigned long long* fr_total = nullptr;  // host-coherent: the counts (all ones until they land)
     bool fr_dirty = false;  // a launch may have left the stripes non-zero
     // traversal-kernel timing (rs3d_ctx_profile)
     int profile = 0;
@@ -316,8 +315,7 @@ int axis_grid_filter_refine(rs3d_ctx* ctx, rs3d_bvh* bvh, int axis, const float*
                             const float* box, int32_t* out_obj, unsigned long long* counts);
 // One wave after such launches: the stripes of counts added into *total (host-coherent
 // memory) and zeroed again, so that the next launch finds them ready.
-int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total,
-                            unsigned long long seq);
+int axis_grid_counts_finish(rs3d_ctx* ctx, unsigned long long* counts, unsigned long long* total);
 // Segment occupancy bits of a BVH for edge rays (built here on first use when the context
 // allows them); *args gets bits = null when there are none.
 struct OccArgs;

@@ -692,20 +692,20 @@ int fr_counts_ready(rs3d_ctx* ctx) {
         return RS3D_E_NOMEM;
     }
     ctx->fr_total = static_cast<unsigned long long*>(h);
-    ctx->fr_total[0] = 0;
-    ctx->fr_total[1] = ctx->fr_seq;
+    *ctx->fr_total = 0;
     return RS3D_OK;
 }
 
-// Waits for the finisher of filter-refine call `seq`: polls the call number it writes to
-// host-coherent memory after the counts, for up to 2 ms (a short launch is seen as soon as
-// it lands, without the stream synchronize's wake-up), then falls back to the stream
-// synchronize, which also reports a launch that failed.
-int fr_wait(rs3d_ctx* ctx, unsigned long long seq) {
+// Waits for the counts of a filter-refine call, whose word the host set to all ones before
+// the launch: polls it for up to 2 ms (a short launch is seen as soon as its finisher's one
+// 8-byte store lands, without the stream synchronize's wake-up), then falls back to the
+// stream synchronize, which also reports a launch that failed. (All ones are also the counts
+// of 2^32 - 1 candidates all inside: such a call waits the 2 ms and then synchronizes.)
+int fr_wait(rs3d_ctx* ctx) {
     volatile unsigned long long* w = ctx->fr_total;
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned i = 1;; ++i) {
-        if (w[1] == seq) {
+        if (*w != ~0ull) {
             std::atomic_thread_fence(std::memory_order_acquire);
             return RS3D_OK;
         }
@@ -751,10 +751,10 @@ int rs3d_filter_refine(rs3d_ctx* ctx, const rs3d_bvh* bvh, const float* points, 
             RS3D_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
             ctx->prof_pending = true;
         }
-        const unsigned long long seq = ++ctx->fr_seq;
-        RS3D_TRY(axis_grid_counts_finish(ctx, cnt, ctx->fr_total, seq));
+        *reinterpret_cast<volatile unsigned long long*>(ctx->fr_total) = ~0ull;
+        RS3D_TRY(axis_grid_counts_finish(ctx, cnt, ctx->fr_total));
         ctx->fr_dirty = false;
-        RS3D_TRY(fr_wait(ctx, seq));
+        RS3D_TRY(fr_wait(ctx));
         const uint64_t h = *reinterpret_cast<volatile unsigned long long*>(ctx->fr_total);
         if (candidate_count) *candidate_count = h >> 32;
         if (inside_count) *inside_count = h & 0xffffffffull;
